@@ -1,0 +1,283 @@
+"""MI355X-native sparse regularized-inverse holographic volume reconstruction.
+
+Python binding of librihvr.so (C ABI in include/rihvr.h).  The class
+:class:`SparseCompressiveHolo` mirrors the reference's in-process API
+(src/core/sparse_compressive_holo.h:25-190): construct with the parameter set,
+``inverseReconstruct(holo, mode)``, ``getPlane(z)``, ``setStepsize(step)``,
+``destroy()``.  Arrays are complex64 numpy arrays (host) or torch tensors on
+the GPU (device pointers are passed through without copies).
+
+The library has no CPU fallback: if librihvr.so is missing or fails to load,
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "_build" / "librihvr.so"
+
+HOST, DEVICE = 0, 1
+MODE_FASTA_L1 = 5
+MODE_FASTA_FUSED_LASSO = 6
+MODE_FASTA_FUSED_LASSO_2D = 7
+
+STATUS_NAMES = {
+    0: "RIHVR_OK", 1: "RIHVR_E_UNKNOWN_TYPE", 2: "RIHVR_E_BAD_FILENAME", 3: "RIHVR_E_MISSING_DATA",
+    4: "RIHVR_E_INVALID_STATE", 5: "RIHVR_E_OUT_OF_MEMORY", 6: "RIHVR_E_UNKNOWN_MODE",
+    7: "RIHVR_E_INVALID_ARGUMENT", 8: "RIHVR_E_UNKNOWN_ERROR", 9: "RIHVR_E_INVALID_FILE",
+    10: "RIHVR_E_INVALID_DATA", 11: "RIHVR_E_CRITICAL_ASSUMPTION", 12: "RIHVR_E_INFINITE_LOOP",
+    13: "RIHVR_E_DEVICE",
+}
+
+# Every function include/rihvr.h declares.
+EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_destroy",
+            "rihvr_reconstruct", "rihvr_get_volume", "rihvr_get_plane", "rihvr_set_stepsize",
+            "rihvr_get_estimate", "rihvr_forward", "rihvr_adjoint", "rihvr_prox")
+
+
+class RihvrError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {STATUS_NAMES.get(status, status)}")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    """``rihvr_params`` (include/rihvr.h)."""
+
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("num_planes", ctypes.c_int32),
+        ("start_plane", ctypes.c_float),
+        ("plane_stepsize", ctypes.c_float),
+        ("wavelength", ctypes.c_float),
+        ("resolution", ctypes.c_float),
+        ("zero_padding", ctypes.c_int32),
+        ("num_inverse_iterations", ctypes.c_int32),
+        ("regularization_param", ctypes.c_float),
+        ("regularization_TV_param", ctypes.c_float),
+        ("num_tv_iterations", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.POINTER
+
+
+class Stats(ctypes.Structure):
+    """``rihvr_stats`` (include/rihvr.h)."""
+
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("total_backtracks", ctypes.c_int32),
+        ("stepsize", ctypes.c_double),
+        ("f_initial", ctypes.c_double),
+        ("f_final", ctypes.c_double),
+        ("sparsity", ctypes.c_double),
+        ("nnz", ctypes.c_int64),
+        ("f", _P(ctypes.c_double)),
+        ("f_resid", _P(ctypes.c_double)),
+        ("ratio", _P(ctypes.c_double)),
+        ("lim", _P(ctypes.c_double)),
+        ("backtracks", _P(ctypes.c_int32)),
+        ("restarts", _P(ctypes.c_int32)),
+        ("restart_sum", _P(ctypes.c_double)),
+    ]
+
+
+def make_params(width, height, num_planes, start_plane, plane_stepsize, wavelength, resolution,
+                num_inverse_iterations, regularization_param, regularization_TV_param,
+                num_tv_iterations, mode=MODE_FASTA_FUSED_LASSO_2D, zero_padding=0) -> Params:
+    return Params(int(width), int(height), int(num_planes), float(start_plane), float(plane_stepsize),
+                  float(wavelength), float(resolution), int(zero_padding), int(num_inverse_iterations),
+                  float(regularization_param), float(regularization_TV_param), int(num_tv_iterations),
+                  int(mode))
+
+
+class StatsBuffers:
+    def __init__(self, n_iter: int):
+        n = max(1, int(n_iter))
+        self.f = np.zeros(n, np.float64)
+        self.f_resid = np.zeros(n, np.float64)
+        self.ratio = np.zeros(n, np.float64)
+        self.lim = np.zeros(n, np.float64)
+        self.backtracks = np.zeros(n, np.int32)
+        self.restarts = np.zeros(n, np.int32)
+        self.restart_sum = np.zeros(n, np.float64)
+        d = lambda a: a.ctypes.data_as(_P(ctypes.c_double))  # noqa: E731
+        i = lambda a: a.ctypes.data_as(_P(ctypes.c_int32))  # noqa: E731
+        self.stats = Stats(0, 0, 0.0, 0.0, 0.0, 0.0, 0, d(self.f), d(self.f_resid), d(self.ratio),
+                           d(self.lim), i(self.backtracks), i(self.restarts), d(self.restart_sum))
+
+    def as_dict(self) -> dict:
+        s = self.stats
+        n = s.iterations
+        return dict(iterations=n, total_backtracks=s.total_backtracks, stepsize=s.stepsize,
+                    f_initial=s.f_initial, f_final=s.f_final, sparsity=s.sparsity, nnz=s.nnz,
+                    f=self.f[:n].copy(), f_resid=self.f_resid[:n].copy(), ratio=self.ratio[:n].copy(),
+                    lim=self.lim[:n].copy(), backtracks=self.backtracks[:n].copy(),
+                    restarts=self.restarts[:n].copy(), restart_sum=self.restart_sum[:n].copy())
+
+
+_lib = None
+
+
+def lib():
+    """Loads librihvr.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"librihvr.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp, ip = ctypes.c_void_p, ctypes.c_int
+        st = ctypes.c_int
+        L.rihvr_abi_version.restype = ctypes.c_int
+        L.rihvr_status_string.argtypes = [ctypes.c_int]
+        L.rihvr_status_string.restype = ctypes.c_char_p
+        L.rihvr_create.argtypes = [_P(Params), _P(vp)]
+        L.rihvr_create.restype = st
+        L.rihvr_destroy.argtypes = [vp]
+        L.rihvr_destroy.restype = st
+        L.rihvr_reconstruct.argtypes = [vp, vp, ip, vp, ip, _P(Stats)]
+        L.rihvr_reconstruct.restype = st
+        L.rihvr_get_volume.argtypes = [vp, vp, ip]
+        L.rihvr_get_volume.restype = st
+        L.rihvr_get_plane.argtypes = [vp, ctypes.c_int32, vp, ip]
+        L.rihvr_get_plane.restype = st
+        L.rihvr_set_stepsize.argtypes = [vp, ctypes.c_double]
+        L.rihvr_set_stepsize.restype = st
+        L.rihvr_get_estimate.argtypes = [vp, vp, vp, ip]
+        L.rihvr_get_estimate.restype = st
+        L.rihvr_forward.argtypes = [vp, vp, ip, vp, ip]
+        L.rihvr_forward.restype = st
+        L.rihvr_adjoint.argtypes = [vp, vp, ip, vp, ip]
+        L.rihvr_adjoint.restype = st
+        L.rihvr_prox.argtypes = [vp, vp, vp, ctypes.c_double, vp, ip]
+        L.rihvr_prox.restype = st
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise RihvrError(rc, what)
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _arg(a, shape, writable=False):
+    """(pointer, location, keepalive) for a complex64 numpy array or CUDA torch tensor."""
+    if a is None:
+        return None, HOST, None
+    if _is_torch(a):
+        import torch
+        if a.dtype != torch.complex64 or not a.is_contiguous() or not a.is_cuda:
+            raise ValueError("device arrays must be contiguous complex64 CUDA tensors")
+        if tuple(a.shape) != tuple(shape):
+            raise ValueError(f"shape {tuple(a.shape)} != {tuple(shape)}")
+        return ctypes.c_void_p(a.data_ptr()), DEVICE, a
+    arr = np.asarray(a)
+    if arr.dtype != np.complex64 or not arr.flags.c_contiguous:
+        if writable:
+            raise ValueError("output arrays must be contiguous complex64")
+        arr = np.ascontiguousarray(arr, dtype=np.complex64)
+    if tuple(arr.shape) != tuple(shape):
+        raise ValueError(f"shape {tuple(arr.shape)} != {tuple(shape)}")
+    return ctypes.c_void_p(arr.ctypes.data), HOST, arr
+
+
+class SparseCompressiveHolo:
+    """Mirror of umnholo::SparseCompressiveHolo over the C ABI."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        self.shape_holo = (params.height, params.width)
+        self.shape_vol = (params.num_planes, params.height, params.width)
+        self._ctx = ctypes.c_void_p()
+        _check(lib().rihvr_create(ctypes.byref(params), ctypes.byref(self._ctx)), "rihvr_create")
+        self.last_stats = None
+
+    # -- lifecycle --------------------------------------------------------
+    def destroy(self):
+        if self._ctx:
+            lib().rihvr_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    # -- operations ---------------------------------------------------------
+    def inverseReconstruct(self, holo, mode=None, warm_start=None) -> dict:
+        if mode is not None and int(mode) != self.params.mode:
+            raise ValueError("mode is fixed at construction (rihvr_params.mode)")
+        hp, hl, _k1 = _arg(holo, self.shape_holo)
+        wp, wl, _k2 = _arg(warm_start, self.shape_vol)
+        sb = StatsBuffers(self.params.num_inverse_iterations)
+        _check(lib().rihvr_reconstruct(self._ctx, hp, hl, wp, wl, ctypes.byref(sb.stats)),
+               "rihvr_reconstruct")
+        self.last_stats = sb.as_dict()
+        return self.last_stats
+
+    def getVolume(self, out=None):
+        out = np.empty(self.shape_vol, np.complex64) if out is None else out
+        p, loc, _k = _arg(out, self.shape_vol, writable=True)
+        _check(lib().rihvr_get_volume(self._ctx, p, loc), "rihvr_get_volume")
+        return out
+
+    def getPlane(self, z: int, out=None):
+        out = np.empty(self.shape_holo, np.complex64) if out is None else out
+        p, loc, _k = _arg(out, self.shape_holo, writable=True)
+        _check(lib().rihvr_get_plane(self._ctx, int(z), p, loc), "rihvr_get_plane")
+        return out
+
+    def setStepsize(self, step: float):
+        _check(lib().rihvr_set_stepsize(self._ctx, float(step)), "rihvr_set_stepsize")
+
+    def getEstimate(self):
+        est = np.empty(self.shape_holo, np.complex64)
+        res = np.empty(self.shape_holo, np.complex64)
+        _check(lib().rihvr_get_estimate(self._ctx, ctypes.c_void_p(est.ctypes.data),
+                                        ctypes.c_void_p(res.ctypes.data), HOST), "rihvr_get_estimate")
+        return est, res
+
+    def forward(self, vol, out=None):
+        vp, vl, _k = _arg(vol, self.shape_vol)
+        out = np.empty(self.shape_holo, np.complex64) if out is None else out
+        op, ol, _k2 = _arg(out, self.shape_holo, writable=True)
+        _check(lib().rihvr_forward(self._ctx, vp, vl, op, ol), "rihvr_forward")
+        return out
+
+    def adjoint(self, holo, out=None):
+        hp, hl, _k = _arg(holo, self.shape_holo)
+        out = np.empty(self.shape_vol, np.complex64) if out is None else out
+        op, ol, _k2 = _arg(out, self.shape_vol, writable=True)
+        _check(lib().rihvr_adjoint(self._ctx, hp, hl, op, ol), "rihvr_adjoint")
+        return out
+
+    def prox(self, x0, grad, stepsize, out=None):
+        xp, xl, _k = _arg(x0, self.shape_vol)
+        gp, gl, _k2 = _arg(grad, self.shape_vol)
+        if xl != gl:
+            raise ValueError("x0 and grad must be in the same memory")
+        if out is None:
+            out = np.empty(self.shape_vol, np.complex64)
+        op, ol, _k3 = _arg(out, self.shape_vol, writable=True)
+        if ol != xl:
+            raise ValueError("out must be in the same memory as x0")
+        _check(lib().rihvr_prox(self._ctx, xp, gp, float(stepsize), op, ol), "rihvr_prox")
+        return out

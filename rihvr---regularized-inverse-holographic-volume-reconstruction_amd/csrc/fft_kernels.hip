@@ -1,0 +1,392 @@
+// fft_kernels.hip -- row and column FFT kernels of the forward operator A, the
+// adjoint A^T and the hologram transforms, with their prologues/epilogues fused.
+//
+// Layouts: planes [z][y][x] complex64, x fastest.  Row kernels transform along
+// x (length W), column kernels along y (length H) on strips of 16 columns, so
+// every global access is a 128-byte row segment.
+#include "fft.hpp"
+#include "kernels.hpp"
+
+namespace rihvr {
+
+constexpr int kColStrip = 16;  // columns per column-kernel workgroup
+
+template <int L> struct RowCfg {
+    static constexpr int TPT = FftPlan<L>::TPT;
+    static constexpr int RPW = (256 / TPT) > 0 ? (256 / TPT) : 1;  // rows per workgroup
+    static constexpr int THREADS = TPT * RPW;
+    static constexpr int LP = L + 1;  // padded LDS row pitch (float2)
+};
+template <int L> struct ColCfg {
+    static constexpr int TPT = FftPlan<L>::TPT;
+    static constexpr int THREADS = TPT * kColStrip;
+};
+
+template <int L>
+__device__ __forceinline__ void load_twiddles(float2* tw_s, const float2* __restrict__ twg) {
+    for (int m = threadIdx.x; m < L; m += blockDim.x) tw_s[m] = twg[m];
+}
+
+// ----------------------------------------------------------------------------
+// Row kernels
+// ----------------------------------------------------------------------------
+enum RowMode { ROW_PLAIN = 0, ROW_ADJ = 1, ROW_FWD = 2, ROW_SUMPART = 3 };
+
+struct RowArgs {
+    const float2* in;       // PLAIN: [nrows][L]; FWD: volume; SUMPART: [nparts][H][L]
+    float2* out;            // [nrows][L]
+    int nrows;              // rows per plane (H) for ADJ/FWD; total rows otherwise
+    const float2* twg;      // W_L^m table for the transform sign
+    // ADJ
+    const float2* rhat;     // [H][L] spectrum of the residual
+    const double* kappa;    // [H][L]
+    const float* zlist;     // plane positions
+    int nz, zchunk;
+    float invN;
+    // FWD
+    const float2* phase;    // per plane (cos phi_z, sin phi_z)
+    // SUMPART
+    int nparts;
+    long part_stride;
+};
+
+template <int L, int SIGN, int MODE>
+__global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
+    using C = RowCfg<L>;
+    using Plan = FftPlan<L>;
+    __shared__ float2 tw_s[L];
+    __shared__ float2 buf[C::RPW * C::LP];
+    load_twiddles<L>(tw_s, a.twg);
+    const int tr = threadIdx.x / C::TPT;
+    const int t = threadIdx.x - tr * C::TPT;
+    float2* mybuf = buf + tr * C::LP;
+    auto lds = [&](int j) -> float2& { return mybuf[j]; };
+    const int row = blockIdx.x * C::RPW + tr;  // row within a plane (ADJ/FWD) or global row
+    const bool valid = row < a.nrows;
+    const int rrow = valid ? row : a.nrows - 1;
+    __syncthreads();
+
+    if constexpr (MODE == ROW_PLAIN) {
+        const float2* src_row = a.in + (size_t)rrow * L;
+        float2* dst_row = a.out + (size_t)rrow * L;
+        FftRun<L, SIGN, typename Plan::radices>::run(
+            tw_s, t, [&](int, int, int j) { return src_row[j]; },
+            [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
+    } else if constexpr (MODE == ROW_SUMPART) {
+        const float2* src_row = a.in + (size_t)rrow * L;
+        float2* dst_row = a.out + (size_t)rrow * L;
+        FftRun<L, SIGN, typename Plan::radices>::run(
+            tw_s, t,
+            [&](int, int, int j) {
+                float2 s = src_row[j];
+                for (int p = 1; p < a.nparts; ++p) s = cadd(s, src_row[(size_t)p * a.part_stride + j]);
+                return s;
+            },
+            [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
+    } else if constexpr (MODE == ROW_FWD) {
+        const int z = blockIdx.y;
+        const float2 ph = a.phase[z];
+        // e^{-i phi_z}: the A prologue (sparse_phase_scale_kernel with -z)
+        const float px = ph.x, py = -ph.y;
+        const size_t off = ((size_t)z * a.nrows + rrow) * L;
+        const float2* src_row = a.in + off;
+        float2* dst_row = a.out + off;
+        FftRun<L, SIGN, typename Plan::radices>::run(
+            tw_s, t,
+            [&](int, int, int j) {
+                float2 d = src_row[j];
+                return make_float2(d.x * px - d.y * py, d.y * px + d.x * py);
+            },
+            [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
+    } else {  // ROW_ADJ: H_z * R_hat / N, then inverse FFT along x, for a chunk of planes
+        constexpr int NBF0 = L / Plan::first;
+        constexpr int NB0 = (NBF0 + C::TPT - 1) / C::TPT;
+        float2 rh[NB0][Plan::first];
+        double kp[NB0][Plan::first];
+        const float2* rrow_p = a.rhat + (size_t)rrow * L;
+        const double* krow_p = a.kappa + (size_t)rrow * L;
+#pragma unroll
+        for (int i = 0; i < NB0; ++i) {
+            const int b = t + i * C::TPT;
+            const int bb = b < NBF0 ? b : NBF0 - 1;
+#pragma unroll
+            for (int r = 0; r < Plan::first; ++r) {
+                rh[i][r] = rrow_p[bb + r * NBF0];
+                kp[i][r] = krow_p[bb + r * NBF0];
+            }
+        }
+        const int z0 = blockIdx.y * a.zchunk;
+        const int z1 = min(a.nz, z0 + a.zchunk);
+        for (int z = z0; z < z1; ++z) {
+            const double zz = (double)a.zlist[z];
+            float2* dst_row = a.out + ((size_t)z * a.nrows + rrow) * L;
+            FftRun<L, SIGN, typename Plan::radices>::run(
+                tw_s, t,
+                [&](int i, int r, int) {
+                    float2 h = transfer(zz, kp[i][r]);
+                    return cscale(cmul(h, rh[i][r]), a.invN);
+                },
+                [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
+            __syncthreads();
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Column kernels: a workgroup transforms 16 adjacent columns of one plane
+// (or of a chunk of planes, accumulating).  LDS layout [j][16] keeps the 16
+// columns of one row contiguous.
+// ----------------------------------------------------------------------------
+enum ColMode { COL_PLAIN = 0, COL_ADJ = 1, COL_FWDACC = 2, COL_REAL = 3 };
+
+struct ColArgs {
+    const float2* in;       // [planes][L][W]
+    float2* out;            // PLAIN/ADJ: [planes][L][W]; FWDACC: [chunks][L][W]; REAL: est [L][W]
+    int W;
+    const float2* twg;
+    // ADJ
+    const float2* phase;
+    // FWDACC
+    const double* kappa;
+    const float* zlist;
+    int nz, zchunk;
+    float invN;
+    // REAL
+    const float2* meas;
+    float2* resid;
+    double* partial;        // one double per workgroup: sum |r|^2
+};
+
+template <int L, int SIGN, int MODE>
+__global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
+    using C = ColCfg<L>;
+    using Plan = FftPlan<L>;
+    __shared__ float2 tw_s[L];
+    __shared__ float2 buf[L * kColStrip];
+    __shared__ double red_s[C::THREADS / 64 + 1];
+    load_twiddles<L>(tw_s, a.twg);
+    const int ci = threadIdx.x % kColStrip;
+    const int t = threadIdx.x / kColStrip;
+    const int col = blockIdx.x * kColStrip + ci;
+    auto lds = [&](int j) -> float2& { return buf[j * kColStrip + ci]; };
+    const size_t W = (size_t)a.W;
+    __syncthreads();
+
+    if constexpr (MODE == COL_PLAIN || MODE == COL_ADJ) {
+        const int z = blockIdx.y;
+        const float2* src = a.in + (size_t)z * L * W + col;
+        float2* dst = a.out + (size_t)z * L * W + col;
+        float2 ph = make_float2(1.0f, 0.0f);
+        if constexpr (MODE == COL_ADJ) ph = a.phase[z];  // e^{+i phi_z} (sparse_phase_scale_kernel2)
+        FftRun<L, SIGN, typename Plan::radices>::run(
+            tw_s, t, [&](int, int, int j) { return src[(size_t)j * W]; },
+            [&](int, int, int j, float2 v) {
+                if constexpr (MODE == COL_ADJ)
+                    v = make_float2(v.x * ph.x - v.y * ph.y, v.y * ph.x + v.x * ph.y);
+                dst[(size_t)j * W] = v;
+            },
+            lds);
+    } else if constexpr (MODE == COL_FWDACC) {
+        using LP = LastPass<L>;
+        float2 acc[LP::NB][LP::R];
+        double kp[LP::NB][LP::R];
+#pragma unroll
+        for (int i = 0; i < LP::NB; ++i) {
+            const int b = t + i * LP::TPT;
+            const int bb = b < LP::NBF ? b : LP::NBF - 1;
+#pragma unroll
+            for (int r = 0; r < LP::R; ++r) {
+                acc[i][r] = make_float2(0.0f, 0.0f);
+                kp[i][r] = a.kappa[(size_t)(bb + r * LP::NBF) * W + col];
+            }
+        }
+        const int z0 = blockIdx.y * a.zchunk;
+        const int z1 = min(a.nz, z0 + a.zchunk);
+        for (int z = z0; z < z1; ++z) {
+            const double mz = -(double)a.zlist[z];
+            const float2* src = a.in + (size_t)z * L * W + col;
+            FftRun<L, SIGN, typename Plan::radices>::run(
+                tw_s, t, [&](int, int, int j) { return src[(size_t)j * W]; },
+                [&](int i, int r, int, float2 v) {
+                    // H_{-z} . FFT(...) / (Nx Ny), summed over planes
+                    // (sparse_rs_phase_mult_kernel + sparse_add_plane_kernel)
+                    float2 h = transfer(mz, kp[i][r]);
+                    acc[i][r] = cadd(acc[i][r], cscale(cmul(h, v), a.invN));
+                },
+                lds);
+            __syncthreads();
+        }
+        float2* dst = a.out + (size_t)blockIdx.y * L * W + col;
+#pragma unroll
+        for (int i = 0; i < LP::NB; ++i) {
+            const int b = t + i * LP::TPT;
+            if (b < LP::NBF) {
+#pragma unroll
+                for (int r = 0; r < LP::R; ++r) dst[(size_t)(b + r * LP::NBF) * W] = acc[i][r];
+            }
+        }
+    } else {  // COL_REAL: est = Re(IFFT(...)), r = est - meas, partial sum |r|^2
+        const float2* src = a.in + col;
+        double s = 0.0;
+        FftRun<L, SIGN, typename Plan::radices>::run(
+            tw_s, t, [&](int, int, int j) { return src[(size_t)j * W]; },
+            [&](int, int, int j, float2 v) {
+                const size_t o = (size_t)j * W + col;
+                float2 e = make_float2(v.x, 0.0f);  // back_propagate_finish_kernel
+                a.out[o] = e;
+                float2 m = a.meas[o];
+                float2 r = make_float2(e.x - m.x, e.y - m.y);  // RM_EST_TRUE
+                a.resid[o] = r;
+                s += (double)r.x * r.x + (double)r.y * r.y;
+            },
+            lds);
+        double tot = block_sum(s, red_s);
+        if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Host launchers
+// ----------------------------------------------------------------------------
+#define RIHVR_FOR_EACH_L(X) \
+    X(16) X(32) X(48) X(64) X(80) X(96) X(128) X(192) X(256) X(320) X(384) X(448) X(512) X(576) \
+    X(640) X(768) X(832) X(896) X(1024)
+
+bool fft_length_supported(int L) {
+    switch (L) {
+#define X(n) case n: return true;
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return false;
+    }
+}
+
+int row_rows_per_wg(int L) {
+    switch (L) {
+#define X(n) case n: return RowCfg<n>::RPW;
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return 1;
+    }
+}
+
+template <int L, int SIGN, int MODE>
+static hipError_t launch_rows_t(const RowArgs& a, int nrows_total_or_plane, int gy, hipStream_t s) {
+    dim3 grid((nrows_total_or_plane + RowCfg<L>::RPW - 1) / RowCfg<L>::RPW, gy);
+    hipLaunchKernelGGL((k_rows<L, SIGN, MODE>), grid, dim3(RowCfg<L>::THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int L, int SIGN, int MODE>
+static hipError_t launch_cols_t(const ColArgs& a, int gy, hipStream_t s) {
+    dim3 grid(a.W / kColStrip, gy);
+    hipLaunchKernelGGL((k_cols<L, SIGN, MODE>), grid, dim3(ColCfg<L>::THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t fft_rows_plain(int L, int sign, const float2* in, float2* out, int nrows,
+                          const float2* tw, hipStream_t s) {
+    RowArgs a{};
+    a.in = in; a.out = out; a.nrows = nrows; a.twg = tw;
+    switch (L) {
+#define X(n) case n: return sign < 0 ? launch_rows_t<n, -1, ROW_PLAIN>(a, nrows, 1, s) \
+                                     : launch_rows_t<n, 1, ROW_PLAIN>(a, nrows, 1, s);
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out,
+                             int nrows, const float2* tw_inv, hipStream_t s) {
+    RowArgs a{};
+    a.in = parts; a.out = out; a.nrows = nrows; a.twg = tw_inv; a.nparts = nparts;
+    a.part_stride = part_stride;
+    switch (L) {
+#define X(n) case n: return launch_rows_t<n, 1, ROW_SUMPART>(a, nrows, 1, s);
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t fft_rows_fwd(int L, const float2* vol, const float2* phase, float2* out, int H, int nz,
+                        const float2* tw_fwd, hipStream_t s) {
+    RowArgs a{};
+    a.in = vol; a.out = out; a.nrows = H; a.twg = tw_fwd; a.phase = phase;
+    switch (L) {
+#define X(n) case n: return launch_rows_t<n, -1, ROW_FWD>(a, H, nz, s);
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t fft_rows_adj(int L, const float2* rhat, const double* kappa, const float* zlist, float2* out,
+                        int H, int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s) {
+    RowArgs a{};
+    a.out = out; a.nrows = H; a.twg = tw_inv; a.rhat = rhat; a.kappa = kappa; a.zlist = zlist;
+    a.nz = nz; a.zchunk = zchunk; a.invN = invN;
+    const int gy = (nz + zchunk - 1) / zchunk;
+    switch (L) {
+#define X(n) case n: return launch_rows_t<n, 1, ROW_ADJ>(a, H, gy, s);
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t fft_cols_plain(int L, int sign, const float2* in, float2* out, int W, int nplanes,
+                          const float2* tw, hipStream_t s) {
+    ColArgs a{};
+    a.in = in; a.out = out; a.W = W; a.twg = tw;
+    switch (L) {
+#define X(n) case n: return sign < 0 ? launch_cols_t<n, -1, COL_PLAIN>(a, nplanes, s) \
+                                     : launch_cols_t<n, 1, COL_PLAIN>(a, nplanes, s);
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t fft_cols_adj(int L, const float2* in, const float2* phase, float2* out, int W, int nz,
+                        const float2* tw_inv, hipStream_t s) {
+    ColArgs a{};
+    a.in = in; a.out = out; a.W = W; a.twg = tw_inv; a.phase = phase;
+    switch (L) {
+#define X(n) case n: return launch_cols_t<n, 1, COL_ADJ>(a, nz, s);
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t fft_cols_fwdacc(int L, const float2* in, const double* kappa, const float* zlist,
+                           float2* parts, int W, int nz, int zchunk, float invN, const float2* tw_fwd,
+                           hipStream_t s) {
+    ColArgs a{};
+    a.in = in; a.out = parts; a.W = W; a.twg = tw_fwd; a.kappa = kappa; a.zlist = zlist; a.nz = nz;
+    a.zchunk = zchunk; a.invN = invN;
+    const int gy = (nz + zchunk - 1) / zchunk;
+    switch (L) {
+#define X(n) case n: return launch_cols_t<n, -1, COL_FWDACC>(a, gy, s);
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t fft_cols_real(int L, const float2* in, float2* est, const float2* meas, float2* resid,
+                         double* partial, int W, const float2* tw_inv, hipStream_t s) {
+    ColArgs a{};
+    a.in = in; a.out = est; a.W = W; a.twg = tw_inv; a.meas = meas; a.resid = resid;
+    a.partial = partial;
+    switch (L) {
+#define X(n) case n: return launch_cols_t<n, 1, COL_REAL>(a, 1, s);
+        RIHVR_FOR_EACH_L(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace rihvr

@@ -1,0 +1,94 @@
+// kernels.hpp -- host-side launchers of the gfx950 kernels (one stream each).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace rihvr {
+
+// ---- FFT (fft_kernels.hip) -------------------------------------------------
+bool fft_length_supported(int L);
+int row_rows_per_wg(int L);
+
+// Plain row FFT of nrows contiguous rows of length L.  tw: W_L^m for the sign.
+hipError_t fft_rows_plain(int L, int sign, const float2* in, float2* out, int nrows,
+                          const float2* tw, hipStream_t s);
+// Inverse row FFT of sum_p parts[p] (nparts partial spectra, part_stride elements apart).
+hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out,
+                             int nrows, const float2* tw_inv, hipStream_t s);
+// A, first pass: forward row FFT of vol[z] * e^{-i phi_z} for nz planes of H rows.
+hipError_t fft_rows_fwd(int L, const float2* vol, const float2* phase, float2* out, int H, int nz,
+                        const float2* tw_fwd, hipStream_t s);
+// A^T, first pass: inverse row FFT of H_z * rhat / N, planes processed zchunk per workgroup.
+hipError_t fft_rows_adj(int L, const float2* rhat, const double* kappa, const float* zlist, float2* out,
+                        int H, int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s);
+// Plain column FFT (length L = H) of nplanes planes of width W.
+hipError_t fft_cols_plain(int L, int sign, const float2* in, float2* out, int W, int nplanes,
+                          const float2* tw, hipStream_t s);
+// A^T, second pass: inverse column FFT, epilogue * e^{+i phi_z}.
+hipError_t fft_cols_adj(int L, const float2* in, const float2* phase, float2* out, int W, int nz,
+                        const float2* tw_inv, hipStream_t s);
+// A, second pass: forward column FFT, epilogue H_{-z} / N summed over zchunk planes
+// into parts[chunk].
+hipError_t fft_cols_fwdacc(int L, const float2* in, const double* kappa, const float* zlist,
+                           float2* parts, int W, int nz, int zchunk, float invN, const float2* tw_fwd,
+                           hipStream_t s);
+// Final inverse column FFT of one plane: est = (Re, 0), resid = est - meas,
+// partial[blockIdx.x] = sum |resid|^2.
+hipError_t fft_cols_real(int L, const float2* in, float2* est, const float2* meas, float2* resid,
+                         double* partial, int W, const float2* tw_inv, hipStream_t s);
+
+// ---- prox and elementwise (prox_kernels.hip) --------------------------------
+// Reduction slots written by the prox kernels (per wave partials, summed by reduce_partials).
+enum ProxSlot {
+    PS_LS1 = 0,      // sum Re<y - x0, g>             (sparse_linesearch_1_kernel)
+    PS_LS2 = 1,      // sum |y - x0|^2                (sparse_linesearch_2_kernel)
+    PS_RESTART = 2,  // quirky restart sum            (sparse_fistaRestart_kernel)
+    PS_L1X0 = 3,     // sum |x0| over x0 != 0         (calcL1NormComplex of the momentum point)
+    PS_TVX0 = 4,     // sum 2D TV of x0               (calcTVNorm2d of the momentum point)
+    PS_NNZX0 = 5,    // count of x0 != 0
+    PS_NNZY = 6,     // count of y != 0
+    PS_COUNT = 8
+};
+
+struct ProxLaunch {
+    const float2* Y0;   // y_{k-1}
+    const float2* YM;   // y_{k-2}
+    const float2* G;    // gradient A^T r
+    float2* Y;          // output y_k
+    int W, H, nz;
+    float s_mom;        // x0 = Y0 + s_mom * (Y0 - YM)
+    double t;           // step size
+    double reg;         // L1 weight
+    double reg_tv;      // TV weight
+    int n_tv;
+    double* partial;    // [nwaves][PS_COUNT]
+};
+// Number of per-wave partial rows the mode-7 prox writes for this geometry.
+int prox_fl2d_num_waves(int W, int H, int nz, int n_tv);
+bool prox_fl2d_supported(int n_tv);
+hipError_t prox_fl2d(const ProxLaunch& a, hipStream_t s);
+
+int prox_l1_num_blocks(int W, int H, int nz);
+hipError_t prox_l1(const ProxLaunch& a, hipStream_t s);
+
+// out = Y + s (Y - Y0) (the returned momentum point) with partials
+// [nblocks][PS_COUNT]: L1, TV and nnz of out in slots PS_L1X0, PS_TVX0, PS_NNZX0.
+int momentum_out_num_blocks(int W, int H, int nz);
+hipError_t momentum_out(const float2* Y, const float2* Y0, float s_mom, float2* out, int W, int H,
+                        int nz, double* partial, hipStream_t s);
+
+// Hologram momentum by linearity (sparse_compressive_holo.cu:494-498):
+// est = cur + s (cur - prev); resid = est - meas; partial[block] = sum |resid|^2.
+int holo_num_blocks(long n);
+hipError_t holo_momentum(const float2* cur, const float2* prev, double s_mom, const float2* meas,
+                         float2* est, float2* resid, long n, double* partial, hipStream_t s);
+// resid = est - meas with partial sum |resid|^2
+hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, long n, double* partial,
+                         hipStream_t s);
+
+// sums rows of partial [nrows][ncols] into out[ncols] (fixed order: deterministic)
+hipError_t reduce_partials(const double* partial, int nrows, int ncols, double* out, hipStream_t s);
+
+// flags[0] = 1 if any element of a [n] complex array is not finite
+hipError_t check_finite(const float2* a, long n, int* flag, hipStream_t s);
+
+}  // namespace rihvr

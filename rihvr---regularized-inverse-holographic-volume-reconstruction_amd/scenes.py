@@ -1,0 +1,145 @@
+"""Seeded synthetic holograms for the benchmark and test workloads.
+
+Objects (opaque disks, point tracers, rods) are rasterised into per-plane
+opacity masks on the reconstruction grid z_i = z0 + i*dz.  Each plane is
+propagated to the hologram plane with the angular-spectrum kernel
+exp(i 2 pi z (sqrt(1 - lambda^2 f^2 / dx^2) - 1) / lambda) -- the same
+convention as the solver's forward operator, so objects refocus at their
+plane -- and superposed on a unit background (first-Born model):
+    U = 1 + sum_i IFFT( FFT(-m_i) * P_{z_i} ),   I = |U|^2 + N(0, sigma^2).
+The solver input is b = I / mean(I) - 1 (the reference's normalize_mean0,
+src/core/hologram.cu:332-358), complex64 with zero imaginary part.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+
+import numpy as np
+
+try:
+    import scipy.fft as _fft
+
+    def _fft2(a):
+        return _fft.fft2(a, workers=os.cpu_count() or 1)
+
+    def _ifft2(a):
+        return _fft.ifft2(a, workers=os.cpu_count() or 1)
+except ImportError:  # pragma: no cover
+    _fft2, _ifft2 = np.fft.fft2, np.fft.ifft2
+
+
+@dataclasses.dataclass
+class Optics:
+    width: int
+    height: int
+    num_planes: int
+    start_plane: float
+    plane_stepsize: float
+    wavelength: float
+    resolution: float
+
+    def planes(self):
+        z0 = np.float32(self.start_plane)
+        dz = np.float32(self.plane_stepsize)
+        return np.array([z0 + np.float32(np.float32(i) * dz) for i in range(self.num_planes)], np.float64)
+
+
+def _draw_disc(mask, cx, cy, r):
+    H, W = mask.shape
+    rr = int(np.ceil(r)) + 1
+    ys = np.arange(int(np.floor(cy)) - rr, int(np.floor(cy)) + rr + 2)
+    xs = np.arange(int(np.floor(cx)) - rr, int(np.floor(cx)) + rr + 2)
+    yy, xx = np.meshgrid(ys, xs, indexing="ij")
+    inside = (xx + 0.5 - cx) ** 2 + (yy + 0.5 - cy) ** 2 <= r * r
+    if not inside.any():  # a tracer always covers at least its centre pixel
+        inside = (xx == int(np.floor(cx))) & (yy == int(np.floor(cy)))
+    mask[yy[inside] % H, xx[inside] % W] = 1.0
+
+
+def propagate_sum(masks, optics: Optics):
+    """sum_i IFFT(FFT(-m_i) P_{z_i}) for the planes that hold objects."""
+    H, W = optics.height, optics.width
+    lam, dx = float(np.float32(optics.wavelength)), float(np.float32(optics.resolution))
+    fx = np.fft.fftfreq(W)[None, :]
+    fy = np.fft.fftfreq(H)[:, None]
+    root = np.sqrt(np.maximum(1.0 - (fx * fx + fy * fy) * (lam / dx) ** 2, 0.0))
+    acc = np.zeros((H, W), np.complex128)
+    zs = optics.planes()
+    for i in range(optics.num_planes):
+        m = masks[i]
+        if not m.any():
+            continue
+        P = np.exp(1j * 2 * np.pi * zs[i] * (root - 1.0) / lam)
+        acc += _fft2(-m.astype(np.float64)) * P
+    return _ifft2(acc)
+
+
+def hologram_from_masks(masks, optics: Optics, sigma: float, rng):
+    U = 1.0 + propagate_sum(masks, optics)
+    inten = np.abs(U) ** 2 + rng.normal(0.0, sigma, size=U.shape)
+    b = inten / inten.mean() - 1.0
+    return b.astype(np.float32).astype(np.complex64)
+
+
+def disks_scene(optics: Optics, n_disks: int, rmin: float, rmax: float, seed: int, sigma: float):
+    """C1-style scene: opaque disks at uniform (x, y, plane)."""
+    rng = np.random.default_rng(seed)
+    masks = np.zeros((optics.num_planes, optics.height, optics.width), np.float32)
+    for _ in range(n_disks):
+        r = rng.uniform(rmin, rmax)
+        cx, cy = rng.uniform(0, optics.width), rng.uniform(0, optics.height)
+        zi = rng.integers(0, optics.num_planes)
+        _draw_disc(masks[zi], cx, cy, r)
+    return hologram_from_masks(masks, optics, sigma, rng), masks
+
+
+def tracers_rods_scene(optics: Optics, n_tracers: int, n_rods: int, seed: int, sigma: float,
+                       rod_len=(20.0, 80.0), rod_width=3.0, tracer_diam=(1.0, 2.0)):
+    """C2-style scene: point tracers plus rods of random 3D orientation, the rods
+    rasterised into per-plane occluding slices (discs of the rod width sampled
+    every half pixel along the axis; z in plane units = pixels * dx / dz)."""
+    rng = np.random.default_rng(seed)
+    H, W, Nz = optics.height, optics.width, optics.num_planes
+    masks = np.zeros((Nz, H, W), np.float32)
+    for _ in range(n_tracers):
+        d = rng.uniform(*tracer_diam)
+        cx, cy = rng.uniform(0, W), rng.uniform(0, H)
+        zi = rng.integers(0, Nz)
+        _draw_disc(masks[zi], cx, cy, d / 2)
+    z_per_px = optics.resolution / optics.plane_stepsize
+    for _ in range(n_rods):
+        length = rng.uniform(*rod_len)
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        cx, cy, cz = rng.uniform(0, W), rng.uniform(0, H), rng.uniform(0, Nz)
+        for s in np.arange(-length / 2, length / 2 + 1e-9, 0.5):
+            x, y = cx + s * v[0], cy + s * v[1]
+            zi = int(np.floor(cz + s * v[2] * z_per_px))
+            if 0 <= zi < Nz:
+                _draw_disc(masks[zi], x, y, rod_width / 2)
+    return hologram_from_masks(masks, optics, sigma, rng), masks
+
+
+# Workloads (the parameter sets of the reference's CLI parameter file)
+C1 = dict(width=256, height=256, num_planes=32, start_plane=1000.0, plane_stepsize=50.0,
+          wavelength=0.633, resolution=2.0, num_inverse_iterations=50, regularization_param=0.01,
+          regularization_TV_param=0.01, num_tv_iterations=5)
+C2 = dict(width=1024, height=1024, num_planes=128, start_plane=2500.0, plane_stepsize=20.0,
+          wavelength=0.632, resolution=1.1, num_inverse_iterations=100, regularization_param=0.15,
+          regularization_TV_param=0.1, num_tv_iterations=5)
+SEED_C1 = 20261201
+SEED_C2 = 20261202
+
+
+def optics_of(cfg: dict) -> Optics:
+    return Optics(cfg["width"], cfg["height"], cfg["num_planes"], cfg["start_plane"],
+                  cfg["plane_stepsize"], cfg["wavelength"], cfg["resolution"])
+
+
+def c1_hologram(seed=SEED_C1, cfg=C1):
+    return disks_scene(optics_of(cfg), 20, 2.0, 6.0, seed, 0.01)[0]
+
+
+def c2_hologram(seed=SEED_C2, cfg=C2):
+    return tracers_rods_scene(optics_of(cfg), 500, 20, seed, 0.02)[0]

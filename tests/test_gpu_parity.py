@@ -1,0 +1,190 @@
+"""GPU parity: librihvr (HIP, gfx950) against the CPU oracle's FP32 restatement.
+
+Tolerances (relative L2 over the complex arrays):
+  operators A, A^T and one prox call : 2e-5 (FP32 FFT / stencil rounding only)
+  full solves                          : 1e-4, identical
+                                         thresholded support except voxels whose
+                                         magnitude is within 1e-4 of the threshold,
+                                         identical backtrack/restart decisions.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-30))
+
+
+def cfg(W, H, Nz, its=5, reg=0.01, tv=0.01, ntv=5, z0=1000.0, dz=50.0, lam=0.633, dx=2.0, mode=7):
+    return dict(width=W, height=H, num_planes=Nz, start_plane=z0, plane_stepsize=dz, wavelength=lam,
+                resolution=dx, num_inverse_iterations=its, regularization_param=reg,
+                regularization_TV_param=tv, num_tv_iterations=ntv, mode=mode)
+
+
+def rand_vol(rng, Nz, H, W, sparsity=None):
+    v = (rng.standard_normal((Nz, H, W)) + 1j * rng.standard_normal((Nz, H, W))).astype(np.complex64)
+    if sparsity is not None:
+        v *= (rng.random((Nz, H, W)) < sparsity)
+    return v
+
+
+SHAPES = [(32, 32, 4), (64, 48, 3), (256, 256, 6), (448, 448, 2), (640, 576, 2), (1024, 1024, 2)]
+
+
+@pytest.mark.parametrize("W,H,Nz", SHAPES)
+def test_forward_matches_oracle(pkg, orc, gpu_available, W, H, Nz):
+    c = cfg(W, H, Nz, z0=2500.0, dz=20.0, lam=0.632, dx=1.1)
+    rng = np.random.default_rng(W * 7 + H)
+    x = rand_vol(rng, Nz, H, W, sparsity=0.05)
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        est = rec.forward(x)
+    ref = orc.forward(orc.make_params(**c), x, "f32")
+    assert np.all(est.imag == 0)
+    assert rel(est, ref) < 2e-5, rel(est, ref)
+
+
+@pytest.mark.parametrize("W,H,Nz", SHAPES)
+def test_adjoint_matches_oracle(pkg, orc, gpu_available, W, H, Nz):
+    c = cfg(W, H, Nz, z0=2500.0, dz=20.0, lam=0.632, dx=1.1)
+    rng = np.random.default_rng(W * 11 + H)
+    r = rng.standard_normal((H, W)).astype(np.complex64)
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        g = rec.adjoint(r)
+    ref = orc.adjoint(orc.make_params(**c), r, "f32")
+    assert rel(g, ref) < 2e-5, rel(g, ref)
+
+
+def test_adjoint_identity_gpu(pkg, gpu_available):
+    """<A x, r> = sum Re(conj(x) A^T r) for the GPU operator pair (K1)."""
+    c = cfg(256, 256, 8, z0=2500.0, dz=20.0, lam=0.632, dx=1.1)
+    rng = np.random.default_rng(5)
+    x = rand_vol(rng, 8, 256, 256)
+    r = rng.standard_normal((256, 256)).astype(np.complex64)
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        ax = rec.forward(x)
+        atr = rec.adjoint(r)
+    lhs = float(np.sum(ax.real.astype(np.float64) * r.real))
+    rhs = float(np.sum((np.conj(x.astype(np.complex128)) * atr).real))
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1e-12) + 1e-3, (lhs, rhs)
+
+
+@pytest.mark.parametrize("ntv", [0, 1, 3, 5, 8])
+@pytest.mark.parametrize("W,H", [(32, 32), (64, 48), (256, 128), (1024, 64)])
+def test_prox_fl2d_matches_oracle(pkg, orc, gpu_available, ntv, W, H):
+    Nz = 3
+    c = cfg(W, H, Nz, reg=0.05, tv=0.02, ntv=ntv)
+    rng = np.random.default_rng(ntv * 100 + W + H)
+    x0 = rand_vol(rng, Nz, H, W, sparsity=0.2) * np.float32(0.5)
+    g = rand_vol(rng, Nz, H, W) * np.float32(2.0)
+    t = 1.0 / Nz
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        y = rec.prox(x0, g, t)
+    ref = orc.prox(orc.make_params(**c), x0, g, t, "f32")
+    assert rel(y, ref) < 2e-5, rel(y, ref)
+    # thresholded support: only voxels at the threshold may differ
+    thr = t * 0.05
+    nz_g = (y.real != 0) | (y.imag != 0)
+    nz_r = (ref.real != 0) | (ref.imag != 0)
+    diff = nz_g != nz_r
+    assert diff.sum() <= max(2, int(1e-4 * y.size)), diff.sum()
+    del thr
+
+
+def test_prox_l1_matches_oracle(pkg, orc, gpu_available):
+    W, H, Nz = 64, 64, 4
+    c = cfg(W, H, Nz, reg=0.1, tv=0.0, mode=5)
+    rng = np.random.default_rng(3)
+    x0 = rand_vol(rng, Nz, H, W, sparsity=0.3)
+    g = rand_vol(rng, Nz, H, W)
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        y = rec.prox(x0, g, 0.25)
+    ref = orc.prox(orc.make_params(**c), x0, g, 0.25, "f32")
+    assert rel(y, ref) < 1e-6, rel(y, ref)
+
+
+def _solve_both(pkg, orc, c, holo, warm=None):
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        st = rec.inverseReconstruct(holo, warm_start=warm)
+        vol = rec.getVolume()
+        est, res = rec.getEstimate()
+    ref, rest, rres, rst = orc.solve(orc.make_params(**c), holo, warm, "f32")
+    return vol, st, ref, rst, (est, res, rest, rres)
+
+
+def _support_mismatch(a, b, thr_mag):
+    na = (a.real != 0) | (a.imag != 0)
+    nb = (b.real != 0) | (b.imag != 0)
+    return int((na != nb).sum())
+
+
+def test_solve_small_matches_oracle(pkg, orc, scenes, gpu_available):
+    c = cfg(64, 64, 8, its=10)
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 6, 2.0, 4.0, 11, 0.01)
+    vol, st, ref, rst, (est, res, rest, rres) = _solve_both(pkg, orc, c, holo)
+    assert rel(vol, ref) < 1e-4, rel(vol, ref)
+    assert list(st["backtracks"]) == list(rst["backtracks"])
+    assert list(st["restarts"]) == list(rst["restarts"])
+    assert _support_mismatch(vol, ref, 0) <= max(2, int(1e-4 * vol.size))
+    assert abs(st["f_final"] - rst["f_final"]) <= 1e-4 * abs(rst["f_final"])
+    np.testing.assert_allclose(st["f"], rst["f"], rtol=1e-4)
+    assert rel(est, rest) < 1e-4 and rel(res, rres) < 1e-3
+
+
+def test_solve_c1_matches_oracle(pkg, orc, scenes, gpu_available):
+    """The 256^2 x 32 workload (50 FISTA x 5 TV iterations) at full size."""
+    c = dict(scenes.C1, mode=7)
+    holo = scenes.c1_hologram()
+    vol, st, ref, rst, _ = _solve_both(pkg, orc, c, holo)
+    r = rel(vol, ref)
+    print(f"C1 rel-L2 vs FP32 oracle: {r:.3e}; support mismatches {_support_mismatch(vol, ref, 0)} "
+          f"of {vol.size}; nnz {st['nnz']} vs {rst['nnz']}; backtracks {st['total_backtracks']} vs "
+          f"{rst['total_backtracks']}")
+    assert r < 1e-4, r
+    assert list(st["restarts"]) == list(rst["restarts"])
+    assert list(st["backtracks"]) == list(rst["backtracks"])
+    assert _support_mismatch(vol, ref, 0) <= max(2, int(1e-5 * vol.size))
+
+
+def test_solve_mode5_and_warm_start(pkg, orc, scenes, gpu_available):
+    c = cfg(64, 64, 6, its=6, tv=0.0, reg=0.02)  # TV weight 0 -> mode 5 (:192-195)
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 5, 2.0, 4.0, 12, 0.01)
+    rng = np.random.default_rng(1)
+    warm = rand_vol(rng, 6, 64, 64, sparsity=0.01) * np.float32(0.1)
+    vol, st, ref, rst, _ = _solve_both(pkg, orc, c, holo, warm)
+    assert rel(vol, ref) < 1e-4, rel(vol, ref)
+    assert list(st["restarts"]) == list(rst["restarts"])
+
+
+def test_device_pointers_match_host(pkg, scenes, gpu_available):
+    import torch
+    c = cfg(64, 64, 4, its=3)
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 4, 2.0, 4.0, 13, 0.01)
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        rec.inverseReconstruct(holo)
+        a = rec.getVolume()
+        hd = torch.from_numpy(holo).cuda()
+        rec.inverseReconstruct(hd)
+        out = torch.empty((4, 64, 64), dtype=torch.complex64, device="cuda")
+        rec.getVolume(out)
+        torch.cuda.synchronize()
+    np.testing.assert_array_equal(a, out.cpu().numpy())
+
+
+def test_error_codes(pkg, gpu_available):
+    with pytest.raises(pkg.RihvrError) as e:
+        pkg.SparseCompressiveHolo(pkg.make_params(**cfg(40, 32, 2)))
+    assert e.value.status == 7
+    with pytest.raises(pkg.RihvrError) as e:
+        pkg.SparseCompressiveHolo(pkg.make_params(**cfg(32, 32, 2, mode=6)))
+    assert e.value.status == 6
+    with pkg.SparseCompressiveHolo(pkg.make_params(**cfg(32, 32, 2))) as rec:
+        bad = np.zeros((32, 32), np.complex64)
+        bad[3, 4] = np.nan
+        with pytest.raises(pkg.RihvrError) as e:
+            rec.inverseReconstruct(bad)
+        assert e.value.status == 10
+        with pytest.raises(pkg.RihvrError) as e:
+            rec.getPlane(0)
+        assert e.value.status == 4
