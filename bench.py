@@ -1,0 +1,187 @@
+"""Benchmark: FISTA iterations per second of the plain FASTA/FISTA + 2D-TV solve
+on the 1024x1024x128 FP32 volume (the reference's default mode 7), on one MI355X.
+
+A step is one FISTA iteration of SparseCompressiveHolo::inverseReconstruct
+(prox with A^T inside, A, residual, FFT(r), reductions, momentum; backtracks that
+occur are inside the step).  The hologram is generated synthetically (500 tracers +
+20 rods, seeded), copied to HBM and the solve set up (rihvr_begin) before the timed
+region; W warm-up iterations run, then exactly K iterations are timed between
+device synchronisations.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__ as entry  # noqa: E402
+
+METRIC = "FISTA iter/s @1024^2x128 FP32 (%HBM roof); s/frame to tol, warm-start vs plain"
+HBM_PEAK_GBPS = 8000.0          # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3        # MI355X FP32 vector (vendor figure)
+
+
+def algorithmic_bytes(name: str, W: int, H: int, Nz: int, nparts: int) -> float:
+    """Compulsory HBM bytes of one launch of each solver kernel (complex64 = 8 B)."""
+    N, V = W * H, W * H * Nz
+    return {
+        # reads y_{k-1}, y_{k-2}, g; writes y_k
+        "prox": 32.0 * V,
+        # reads y_k, writes the row-transformed planes
+        "fft_rows_fwd": 16.0 * V,
+        # reads the row-transformed planes, writes nparts partial spectra
+        "fft_cols_fwdacc": 8.0 * V + 8.0 * N * nparts,
+        # reads R-hat and kappa (16 B / pixel), writes the row-transformed planes
+        "fft_rows_adj": 8.0 * V + 16.0 * N,
+        # reads the row-transformed planes, writes g
+        "fft_cols_adj": 16.0 * V,
+    }.get(name, float("nan"))
+
+
+def iteration_bytes(W, H, Nz):
+    """B = 3V + 8H per FISTA iteration (read y_{k-1}, y_{k-2}, write y_k; 8 holograms)."""
+    N = W * H
+    return 3 * 8.0 * N * Nz + 8 * 8.0 * N
+
+
+def iteration_flops(W, H, Nz):
+    """F = 10 Nz N log2 N + 10 N log2 N + 200 N Nz."""
+    N = W * H
+    return 10 * Nz * N * math.log2(N) + 10 * N * math.log2(N) + 200 * N * Nz
+
+
+def cpu_baseline(cfg: dict, holo, iters_hi=2, iters_lo=1):
+    """FP32 oracle (CPU restatement of the reference) on the host cores.  Per-iteration
+    time = (t(iters_hi) - t(iters_lo)) / (iters_hi - iters_lo), which cancels the
+    solve set-up."""
+    import oracle
+    times = {}
+    for n in (iters_lo, iters_hi):
+        p = oracle.make_params(**dict(cfg, num_inverse_iterations=n))
+        t0 = time.perf_counter()
+        oracle.solve(p, holo, None, "f32")
+        times[n] = time.perf_counter() - t0
+    per_it = (times[iters_hi] - times[iters_lo]) / (iters_hi - iters_lo)
+    return dict(value=1.0 / per_it, unit="iter/s", cores=oracle.get_threads(), kind="port",
+                sample=(f"{cfg['width']}x{cfg['height']}x{cfg['num_planes']} FP32 oracle, "
+                        f"{iters_hi - iters_lo} FISTA iterations (difference of a {iters_hi}- and a "
+                        f"{iters_lo}-iteration solve, {times[iters_hi] + times[iters_lo]:.1f} s total)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus_requested": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+
+    pkg = entry.load_package()
+    scenes = entry._load_submodule("scenes")
+    cfg = dict(scenes.C2 if args.workload == "c2" else scenes.C1, mode=7)
+    holo = scenes.c2_hologram() if args.workload == "c2" else scenes.c1_hologram()
+    W, H, Nz = cfg["width"], cfg["height"], cfg["num_planes"]
+
+    rec = pkg.SparseCompressiveHolo(pkg.make_params(**cfg))
+    holo_dev = torch.from_numpy(holo).cuda()
+    torch.cuda.synchronize()
+    rec.begin(holo_dev)
+    if args.warmup > 0:
+        rec.iterate(args.warmup)
+    torch.cuda.synchronize()
+    rec.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = rec.iterate(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing = rec.get_timing()
+    rec.set_timing(False)
+    fin = rec.finish()
+    rec.destroy()
+
+    elapsed = t1 - t0
+    ips = args.steps / elapsed
+    nparts = (Nz + 7) // 8
+    # dominant kernel: the largest share of GPU time in the timed region
+    dom = max(timing.items(), key=lambda kv: kv[1][1])
+    dom_name, (dom_launches, dom_ms) = dom
+    dom_avg_ms = dom_ms / max(1, dom_launches)
+    dom_bytes = algorithmic_bytes(dom_name, W, H, Nz, nparts)
+    achieved = dom_bytes / (dom_avg_ms * 1e-3) / 1e9
+    it_bytes = iteration_bytes(W, H, Nz)
+    it_gbps = it_bytes * ips / 1e9
+    gpu_ms_total = sum(v[1] for v in timing.values())
+
+    out = {
+        "metric": METRIC,
+        "value": ips,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 500 tracers + 20 rods, angular-spectrum forward model, sigma 0.02)",
+        "config": {
+            "workload": f"plain FISTA+TV (mode 7), {W}x{H}x{Nz} FP32, z0={cfg['start_plane']} dz={cfg['plane_stepsize']} "
+                        f"dx={cfg['resolution']} lambda={cfg['wavelength']}, L1 {cfg['regularization_param']}, "
+                        f"TV {cfg['regularization_TV_param']}, {cfg['num_tv_iterations']} TV iterations",
+            "width": W, "height": H, "num_planes": Nz,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "avg_launch_ms": dom_avg_ms,
+            "algorithmic_bytes_per_launch": dom_bytes,
+        },
+        "iteration_roofline": {
+            "bytes_per_iteration": it_bytes,
+            "achieved_GBps": it_gbps,
+            "hbm_frac": it_gbps / HBM_PEAK_GBPS,
+            "fp32_tflops": iteration_flops(W, H, Nz) * ips / 1e12,
+            "fp32_frac": iteration_flops(W, H, Nz) * ips / 1e12 / FP32_PEAK_TFLOPS,
+        },
+        "kernels_ms_per_iteration": {k: v[1] / args.steps for k, v in sorted(timing.items(), key=lambda kv: -kv[1][1])},
+        "gpu_busy_frac": gpu_ms_total / (1e3 * elapsed),
+        "solver": {"backtracks_timed": int(np.sum(st["backtracks"])), "restarts_timed": int(np.sum(st["restarts"])),
+                   "stepsize": st["stepsize"], "f_final": fin["f_final"], "sparsity": fin["sparsity"]},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(cfg, holo)
+        except Exception as e:  # reported, never substituted for the GPU value
+            out["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -133,6 +133,38 @@ rihvr_status rihvr_reconstruct(rihvr_ctx* ctx, const float* holo, int holo_loc,
                                const float* warm_start, int warm_loc,
                                rihvr_stats* stats);
 
+/* Stepwise form of rihvr_reconstruct, split where the reference's
+ * inverseReconstruct splits (sparse_compressive_holo.cu:173-633):
+ *   rihvr_begin   -- the set-up before the loop (:238-296): start point, initial
+ *                    residual, objective (*f_initial, may be NULL) and gradient;
+ *   rihvr_iterate -- n_iter passes of the loop body (:299-607), continuing the
+ *                    FISTA state of the previous call; per-iteration arrays of
+ *                    `stats` are indexed from 0 for this call; asking for
+ *                    stats->f or stats->f_resid costs one extra reduction pass;
+ *   rihvr_finish  -- forms the returned momentum point (:483-489) and the final
+ *                    objective (:608-633).
+ * rihvr_reconstruct(h, w, st) == begin(h, w) + iterate(num_inverse_iterations) + finish.
+ * rihvr_iterate / rihvr_finish without an active begin: RIHVR_E_INVALID_STATE. */
+rihvr_status rihvr_begin(rihvr_ctx* ctx, const float* holo, int holo_loc,
+                         const float* warm_start, int warm_loc, double* f_initial);
+rihvr_status rihvr_iterate(rihvr_ctx* ctx, int32_t n_iter, rihvr_stats* stats);
+rihvr_status rihvr_finish(rihvr_ctx* ctx, rihvr_stats* stats);
+
+/* Per-kernel GPU timing with HIP events on the context's stream: the
+ * counterpart of the reference's DECLARE_TIMING / SAVE_TIMING phase timers
+ * (src/core/umnholo.h:79-124, compiled out by default there).
+ * rihvr_set_timing(ctx, 1) resets the counters and starts recording; 0 stops.
+ * rihvr_get_timing writes up to `capacity` records and sets *count to the number
+ * of timed kernels. */
+typedef struct rihvr_kernel_time {
+    char    name[32];
+    int64_t launches;
+    double  total_ms;
+} rihvr_kernel_time;
+rihvr_status rihvr_set_timing(rihvr_ctx* ctx, int32_t enable);
+rihvr_status rihvr_get_timing(rihvr_ctx* ctx, rihvr_kernel_time* out, int32_t capacity,
+                              int32_t* count);
+
 /* Copies the reconstructed volume ([Nz][H][W] complex64) out.  Replaces reading
  * x1 through SparseVolume::getPlane for every plane / saveSparse (:874-879). */
 rihvr_status rihvr_get_volume(rihvr_ctx* ctx, float* out, int out_loc);

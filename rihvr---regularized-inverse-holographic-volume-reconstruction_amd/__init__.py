@@ -35,7 +35,8 @@ STATUS_NAMES = {
 
 # Every function include/rihvr.h declares.
 EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_destroy",
-            "rihvr_reconstruct", "rihvr_get_volume", "rihvr_get_plane", "rihvr_set_stepsize",
+            "rihvr_reconstruct", "rihvr_begin", "rihvr_iterate", "rihvr_finish", "rihvr_set_timing",
+            "rihvr_get_timing", "rihvr_get_volume", "rihvr_get_plane", "rihvr_set_stepsize",
             "rihvr_get_estimate", "rihvr_forward", "rihvr_adjoint", "rihvr_prox")
 
 
@@ -87,6 +88,12 @@ class Stats(ctypes.Structure):
         ("restarts", _P(ctypes.c_int32)),
         ("restart_sum", _P(ctypes.c_double)),
     ]
+
+
+class KernelTime(ctypes.Structure):
+    """``rihvr_kernel_time`` (include/rihvr.h)."""
+
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double)]
 
 
 def make_params(width, height, num_planes, start_plane, plane_stepsize, wavelength, resolution,
@@ -144,6 +151,16 @@ def lib():
         L.rihvr_destroy.restype = st
         L.rihvr_reconstruct.argtypes = [vp, vp, ip, vp, ip, _P(Stats)]
         L.rihvr_reconstruct.restype = st
+        L.rihvr_begin.argtypes = [vp, vp, ip, vp, ip, _P(ctypes.c_double)]
+        L.rihvr_begin.restype = st
+        L.rihvr_iterate.argtypes = [vp, ctypes.c_int32, _P(Stats)]
+        L.rihvr_iterate.restype = st
+        L.rihvr_finish.argtypes = [vp, _P(Stats)]
+        L.rihvr_finish.restype = st
+        L.rihvr_set_timing.argtypes = [vp, ctypes.c_int32]
+        L.rihvr_set_timing.restype = st
+        L.rihvr_get_timing.argtypes = [vp, _P(KernelTime), ctypes.c_int32, _P(ctypes.c_int32)]
+        L.rihvr_get_timing.restype = st
         L.rihvr_get_volume.argtypes = [vp, vp, ip]
         L.rihvr_get_volume.restype = st
         L.rihvr_get_plane.argtypes = [vp, ctypes.c_int32, vp, ip]
@@ -232,6 +249,39 @@ class SparseCompressiveHolo:
                "rihvr_reconstruct")
         self.last_stats = sb.as_dict()
         return self.last_stats
+
+    # -- stepwise solve (rihvr_begin / rihvr_iterate / rihvr_finish) -------
+    def begin(self, holo, warm_start=None) -> float:
+        hp, hl, _k1 = _arg(holo, self.shape_holo)
+        wp, wl, _k2 = _arg(warm_start, self.shape_vol)
+        f0 = ctypes.c_double(0.0)
+        _check(lib().rihvr_begin(self._ctx, hp, hl, wp, wl, ctypes.byref(f0)), "rihvr_begin")
+        return f0.value
+
+    def iterate(self, n: int, with_objective: bool = False) -> dict:
+        sb = StatsBuffers(n)
+        if not with_objective:
+            sb.stats.f = _P(ctypes.c_double)()
+            sb.stats.f_resid = _P(ctypes.c_double)()
+        _check(lib().rihvr_iterate(self._ctx, int(n), ctypes.byref(sb.stats)), "rihvr_iterate")
+        return sb.as_dict()
+
+    def finish(self) -> dict:
+        sb = StatsBuffers(1)
+        _check(lib().rihvr_finish(self._ctx, ctypes.byref(sb.stats)), "rihvr_finish")
+        d = sb.as_dict()
+        return dict(f_final=d["f_final"], nnz=d["nnz"], sparsity=d["sparsity"], stepsize=d["stepsize"],
+                    total_backtracks=d["total_backtracks"], f_initial=d["f_initial"])
+
+    def set_timing(self, enable: bool):
+        _check(lib().rihvr_set_timing(self._ctx, 1 if enable else 0), "rihvr_set_timing")
+
+    def get_timing(self) -> dict:
+        n = ctypes.c_int32(0)
+        _check(lib().rihvr_get_timing(self._ctx, None, 0, ctypes.byref(n)), "rihvr_get_timing")
+        arr = (KernelTime * n.value)()
+        _check(lib().rihvr_get_timing(self._ctx, arr, n.value, ctypes.byref(n)), "rihvr_get_timing")
+        return {k.name.decode(): (int(k.launches), float(k.total_ms)) for k in arr}
 
     def getVolume(self, out=None):
         out = np.empty(self.shape_vol, np.complex64) if out is None else out

@@ -338,7 +338,7 @@ k_momentum_out(const float2* __restrict__ Y, const float2* __restrict__ Y0, floa
         const int p = (int)(i - zo);
         const int y = p / W, x = p - y * W;
         const float2 v = mom(i);
-        out[i] = v;
+        if (out) out[i] = v;
         if (v.x != 0.0f || v.y != 0.0f) {
             l1 += (double)sqrtf(v.x * v.x + v.y * v.y);
             nnz += 1.0;

@@ -43,6 +43,38 @@ enum Slot { SL_PROX = 0, SL_RESID_A = 8, SL_RESID_M = 9, SL_COUNT = 16 };
         if (st_ != RIHVR_OK) return st_;      \
     } while (0)
 
+// Kernels timed with HIP events when timing is enabled (the reference's
+// DECLARE_TIMING / SAVE_TIMING phases, src/core/umnholo.h:79-124).
+enum KernelId {
+    K_ROWS_FWD = 0, K_COLS_FWDACC, K_ROWS_SUMPARTS, K_COLS_REAL, K_PROX, K_HOLO_MOM, K_ROWS_PLAIN,
+    K_COLS_PLAIN, K_ROWS_ADJ, K_COLS_ADJ, K_REDUCE, K_MOM_OUT, K_COUNT
+};
+const char* const kKernelNames[K_COUNT] = {
+    "fft_rows_fwd", "fft_cols_fwdacc", "fft_rows_sumparts", "fft_cols_real", "prox", "holo_momentum",
+    "fft_rows_plain", "fft_cols_plain", "fft_rows_adj", "fft_cols_adj", "reduce_partials", "momentum_out"};
+
+struct Timer {
+    bool on = false;
+    struct Pending { int id; hipEvent_t a, b; };
+    std::vector<hipEvent_t> pool;
+    std::vector<Pending> pending;
+    double total_ms[K_COUNT] = {};
+    int64_t launches[K_COUNT] = {};
+};
+
+// state carried between rihvr_begin / rihvr_iterate / rihvr_finish
+struct SolveState {
+    bool active = false;
+    int mode = RIHVR_MODE_FASTA_FUSED_LASSO_2D;
+    double t = 0;             // step size
+    double alpha1 = 1.0;      // FISTA alpha
+    float s_mom = 0.0f;       // momentum step of the last update
+    double prev_summag = 0;   // sum |r|^2 at the current momentum point
+    double f_initial = 0;
+    int total_bt = 0;
+    int iters = 0;
+};
+
 }  // namespace
 
 struct rihvr_ctx {
@@ -76,6 +108,8 @@ struct rihvr_ctx {
     bool have_result = false;
     bool force_step = false;
     double forced_step = 0;
+    Timer tm;
+    SolveState ss;
 };
 
 namespace {
@@ -131,33 +165,73 @@ rihvr_status copy_out(rihvr_ctx* c, float* dst, const float2* src, int loc, size
     return RIHVR_OK;
 }
 
+// Launches through `launch` (returns hipError_t), bracketed by events when timing is on.
+template <class F> rihvr_status timed(rihvr_ctx* c, int id, F&& launch) {
+    if (!c->tm.on) {
+        RIHVR_HIP(launch());
+        return RIHVR_OK;
+    }
+    hipEvent_t ev[2];
+    for (hipEvent_t& e : ev) {
+        if (c->tm.pool.empty()) {
+            RIHVR_HIP(hipEventCreate(&e));
+        } else {
+            e = c->tm.pool.back();
+            c->tm.pool.pop_back();
+        }
+    }
+    RIHVR_HIP(hipEventRecord(ev[0], c->stream));
+    RIHVR_HIP(launch());
+    RIHVR_HIP(hipEventRecord(ev[1], c->stream));
+    c->tm.pending.push_back({id, ev[0], ev[1]});
+    return RIHVR_OK;
+}
+
+// Accumulates the durations of completed event pairs (call after a stream sync).
+rihvr_status resolve_timing(rihvr_ctx* c) {
+    for (auto& p : c->tm.pending) {
+        float ms = 0;
+        RIHVR_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        c->tm.total_ms[p.id] += ms;
+        c->tm.launches[p.id] += 1;
+        c->tm.pool.push_back(p.a);
+        c->tm.pool.push_back(p.b);
+    }
+    c->tm.pending.clear();
+    return RIHVR_OK;
+}
+
 rihvr_status fetch_scalars(rihvr_ctx* c) {
     RIHVR_HIP(hipMemcpyAsync(c->h_scal, c->scal, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     RIHVR_HIP(hipStreamSynchronize(c->stream));
-    return RIHVR_OK;
+    return resolve_timing(c);
 }
 
 // est = A vol, resid = est - meas, scal[SL_RESID_A] = sum |resid|^2
 // (backPropagate :1935-2077 + calcResidual :2676-2723 + calcSumMagnitude)
 rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
     const float invN = (float)(1.0 / (double)c->N);
-    RIHVR_HIP(fft_rows_fwd(c->W, vol, c->phase, c->T, c->H, c->Nz, c->twfW, c->stream));
-    RIHVR_HIP(fft_cols_fwdacc(c->H, c->T, c->kappa, c->zlist, c->spart, c->W, c->Nz, c->zchunk_fwd, invN, c->twfH,
-                              c->stream));
-    RIHVR_HIP(fft_rows_sumparts(c->W, c->spart, c->nparts, c->N, c->htmp, c->H, c->twiW, c->stream));
-    RIHVR_HIP(fft_cols_real(c->H, c->htmp, est, c->meas, c->resid, c->partial_holo, c->W, c->twiH, c->stream));
-    RIHVR_HIP(reduce_partials(c->partial_holo, c->W / 16, 1, c->scal + SL_RESID_A, c->stream));
+    RIHVR_TRY(timed(c, K_ROWS_FWD, [&] { return fft_rows_fwd(c->W, vol, c->phase, c->T, c->H, c->Nz, c->twfW, c->stream); }));
+    RIHVR_TRY(timed(c, K_COLS_FWDACC, [&] {
+        return fft_cols_fwdacc(c->H, c->T, c->kappa, c->zlist, c->spart, c->W, c->Nz, c->zchunk_fwd, invN, c->twfH,
+                               c->stream);
+    }));
+    RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] { return fft_rows_sumparts(c->W, c->spart, c->nparts, c->N, c->htmp, c->H, c->twiW, c->stream); }));
+    RIHVR_TRY(timed(c, K_COLS_REAL, [&] { return fft_cols_real(c->H, c->htmp, est, c->meas, c->resid, c->partial_holo, c->W, c->twiH, c->stream); }));
+    RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial_holo, c->W / 16, 1, c->scal + SL_RESID_A, c->stream); }));
     return RIHVR_OK;
 }
 
 // G = A^T resid  (SparseGradient::reconstruct + getPlane, sparse_gradient.cu:47-165)
 rihvr_status apply_adjoint(rihvr_ctx* c, const float2* r, float2* G) {
     const float invN = (float)(1.0 / (double)c->N);
-    RIHVR_HIP(fft_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream));
-    RIHVR_HIP(fft_cols_plain(c->H, -1, c->htmp, c->rhat, c->W, 1, c->twfH, c->stream));
-    RIHVR_HIP(fft_rows_adj(c->W, c->rhat, c->kappa, c->zlist, c->T, c->H, c->Nz, c->zchunk_adj, invN, c->twiW,
-                           c->stream));
-    RIHVR_HIP(fft_cols_adj(c->H, c->T, c->phase, G, c->W, c->Nz, c->twiH, c->stream));
+    RIHVR_TRY(timed(c, K_ROWS_PLAIN, [&] { return fft_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream); }));
+    RIHVR_TRY(timed(c, K_COLS_PLAIN, [&] { return fft_cols_plain(c->H, -1, c->htmp, c->rhat, c->W, 1, c->twfH, c->stream); }));
+    RIHVR_TRY(timed(c, K_ROWS_ADJ, [&] {
+        return fft_rows_adj(c->W, c->rhat, c->kappa, c->zlist, c->T, c->H, c->Nz, c->zchunk_adj, invN, c->twiW,
+                            c->stream);
+    }));
+    RIHVR_TRY(timed(c, K_COLS_ADJ, [&] { return fft_cols_adj(c->H, c->T, c->phase, G, c->W, c->Nz, c->twiH, c->stream); }));
     return RIHVR_OK;
 }
 
@@ -181,13 +255,13 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
     a.partial = c->partial;
     int rows;
     if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) {
-        RIHVR_HIP(prox_fl2d(a, c->stream));
+        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
         rows = prox_fl2d_num_waves(c->W, c->H, c->Nz, a.n_tv);
     } else {
-        RIHVR_HIP(prox_l1(a, c->stream));
+        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_l1(a, c->stream); }));
         rows = prox_l1_num_blocks(c->W, c->H, c->Nz);
     }
-    RIHVR_HIP(reduce_partials(c->partial, rows, PS_COUNT, c->scal + SL_PROX, c->stream));
+    RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial, rows, PS_COUNT, c->scal + SL_PROX, c->stream); }));
     return RIHVR_OK;
 }
 
@@ -205,6 +279,8 @@ void release(rihvr_ctx* c) {
                     c->zlist, c->phase, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
                     c->dflag};
     for (void* q : ptrs) (void)hipFree(q);
+    for (auto& p : c->tm.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (hipEvent_t e : c->tm.pool) (void)hipEventDestroy(e);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -285,12 +361,18 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     return RIHVR_OK;
 }
 
-// inverseReconstruct (:173-633), modes 5 and 7
-rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, const float* warm, int warm_loc,
-                              rihvr_stats* st) {
+// ---------------------------------------------------------------------------
+// inverseReconstruct (:173-633), modes 5 and 7, split at its seams:
+//   begin   = the set-up before the loop (:238-296)
+//   iterate = n passes of the loop body (:299-607)
+//   finish  = the returned momentum point and final objective (:483-489, :608-633)
+// ---------------------------------------------------------------------------
+rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const float* warm, int warm_loc,
+                        double* f_initial) {
     if (!holo) return RIHVR_E_INVALID_ARGUMENT;
-    const int mode = effective_mode(c);
-    const int n_it = c->p.num_inverse_iterations;
+    c->have_result = false;
+    c->ss = SolveState{};
+    c->ss.mode = effective_mode(c);
     RIHVR_TRY(copy_in(c, c->meas, holo, holo_loc, c->N));
     RIHVR_HIP(hipMemsetAsync(c->dflag, 0, sizeof(int), c->stream));
     RIHVR_HIP(check_finite(c->meas, c->N, c->dflag, c->stream));
@@ -299,7 +381,7 @@ rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, con
     RIHVR_HIP(hipStreamSynchronize(c->stream));
     if (hflag) return RIHVR_E_INVALID_DATA;
 
-    // start point x1 = x0 = y0 = warm start or zero (:258-259)
+    // start point x1 = x0 = y0 = warm start or zero (:258-259, sparse_volume.cu:44-48)
     c->iy = 0; c->iy0 = 1; c->iym = 2;
     float2* Y0 = c->Yb[c->iy0];
     float2* YM = c->Yb[c->iym];
@@ -309,53 +391,84 @@ rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, con
     RIHVR_HIP(hipMemsetAsync(c->hC, 0, c->N * sizeof(float2), c->stream));
     RIHVR_HIP(hipMemsetAsync(c->hP, 0, c->N * sizeof(float2), c->stream));
 
-    double t = 1.0 / (double)c->Nz;  // :247 (any forced step is overridden there too)
-    // initial residual and gradient (:273-286)
+    // initial residual, objective and gradient (:273-286)
     RIHVR_TRY(apply_forward(c, Y0, c->hE));
+    double l1 = 0, tv = 0;
+    if (warm) {
+        RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
+            return momentum_out(Y0, Y0, 0.0f, nullptr, c->W, c->H, c->Nz, c->partial, c->stream);
+        }));
+        RIHVR_TRY(timed(c, K_REDUCE, [&] {
+            return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
+                                   c->scal + SL_PROX, c->stream);
+        }));
+    }
     RIHVR_TRY(fetch_scalars(c));
-    double prev_summag = c->h_scal[SL_RESID_A];
-    double prev_f_resid = 0.5 * prev_summag;
+    if (warm) {
+        l1 = c->h_scal[SL_PROX + PS_L1X0];
+        tv = c->h_scal[SL_PROX + PS_TVX0];
+    }
+    c->ss.prev_summag = c->h_scal[SL_RESID_A];
+    c->ss.f_initial = objective(c, c->ss.mode, c->ss.prev_summag, l1, tv);
+    if (f_initial) *f_initial = c->ss.f_initial;
     RIHVR_TRY(apply_adjoint(c, c->resid, c->G));
+    c->ss.t = 1.0 / (double)c->Nz;  // :247 (a forced step is overridden there too)
+    c->ss.alpha1 = 1.0;
+    c->ss.s_mom = 0.0f;             // x0 = Y0 for the first prox
+    c->ss.active = true;
+    return RIHVR_OK;
+}
 
-    double alpha1 = 1.0;
-    float s_mom = 0.0f;  // x0 = Y0 for the first prox
-    int total_bt = 0;
-    double f_last = 0;
-    if (st) st->iterations = 0;
-    for (int it = 0; it < n_it; ++it) {
-        double alpha0 = alpha1;
+// objective of the current momentum point Y0 + s (Y0 - YM) with residual norm summag
+rihvr_status momentum_objective(rihvr_ctx* c, double summag, double* f) {
+    RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
+        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], c->ss.s_mom, nullptr, c->W, c->H, c->Nz, c->partial,
+                            c->stream);
+    }));
+    RIHVR_TRY(timed(c, K_REDUCE, [&] {
+        return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
+                               c->scal + SL_PROX, c->stream);
+    }));
+    RIHVR_TRY(fetch_scalars(c));
+    *f = objective(c, c->ss.mode, summag, c->h_scal[SL_PROX + PS_L1X0], c->h_scal[SL_PROX + PS_TVX0]);
+    return RIHVR_OK;
+}
+
+rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_by_finish) {
+    if (!c->ss.active) return RIHVR_E_INVALID_STATE;
+    if (n_it < 0) return RIHVR_E_INVALID_ARGUMENT;
+    SolveState& s = c->ss;
+    const int mode = s.mode;
+    for (int i = 0; i < n_it; ++i) {
+        double alpha0 = s.alpha1;
         int bt = 0;
         double ratio = 0, lim = 0;
         float2* Yn = c->Yb[c->iy];
-        Y0 = c->Yb[c->iy0];
-        YM = c->Yb[c->iym];
+        float2* Y0 = c->Yb[c->iy0];
+        float2* YM = c->Yb[c->iym];
         for (;;) {
-            RIHVR_TRY(apply_prox(c, mode, Y0, YM, s_mom, c->G, t, Yn));
+            RIHVR_TRY(apply_prox(c, mode, Y0, YM, s.s_mom, c->G, s.t, Yn));
             RIHVR_TRY(apply_forward(c, Yn, c->hE));
             RIHVR_TRY(fetch_scalars(c));
             const double* sc = c->h_scal;
-            if (bt == 0) {
-                if (it > 0) {
-                    // residual norm at the previous momentum point (:545-549)
-                    prev_summag = sc[SL_RESID_M];
-                    prev_f_resid = 0.5 * prev_summag;
-                    if (st && st->f_resid) st->f_resid[it - 1] = prev_f_resid;
-                }
-                // objective at the previous momentum point (the x0 of this prox)
-                double f = objective(c, mode, prev_summag, sc[SL_PROX + PS_L1X0], sc[SL_PROX + PS_TVX0]);
-                if (it == 0) {
-                    if (st) st->f_initial = f;
-                } else if (st && st->f) {
-                    st->f[it - 1] = f;
+            if (bt == 0 && s.iters > 0) {
+                // residual norm at the previous momentum point (:545-549), and the
+                // objective there from this prox's x0 reductions
+                s.prev_summag = sc[SL_RESID_M];
+                if (i > 0 && st) {
+                    if (st->f_resid) st->f_resid[i - 1] = 0.5 * s.prev_summag;
+                    if (st->f)
+                        st->f[i - 1] = objective(c, mode, s.prev_summag, sc[SL_PROX + PS_L1X0],
+                                                 sc[SL_PROX + PS_TVX0]);
                 }
             }
-            lim = sc[SL_PROX + PS_LS1] + sc[SL_PROX + PS_LS2] / (2 * t);  // :4381
-            double f_resid = 0.5 * sc[SL_RESID_A];
-            ratio = f_resid / (prev_f_resid + lim);  // :403
+            lim = sc[SL_PROX + PS_LS1] + sc[SL_PROX + PS_LS2] / (2 * s.t);  // :4381
+            const double f_resid = 0.5 * sc[SL_RESID_A];
+            ratio = f_resid / (0.5 * s.prev_summag + lim);  // :403
             if (((ratio > 1.0) || (ratio < 0)) && (bt < 2)) {  // :423
-                t *= 0.5;
+                s.t *= 0.5;
                 bt++;
-                total_bt++;
+                s.total_bt++;
                 continue;
             }
             break;
@@ -363,60 +476,93 @@ rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, con
         const double rs = c->h_scal[SL_PROX + PS_RESTART];
         const bool restart = rs > 0;  // :4481
         if (restart) alpha0 = 1;
-        alpha1 = (1.0 + std::sqrt(1.0 + 4.0 * alpha0 * alpha0)) / 2.0;  // :476
-        const double step = (alpha0 - 1) / alpha1;                        // :481
+        s.alpha1 = (1.0 + std::sqrt(1.0 + 4.0 * alpha0 * alpha0)) / 2.0;  // :476
+        const double step = (alpha0 - 1) / s.alpha1;                       // :481
         // hologram momentum by linearity (:494-498): prev <- cur, cur <- est, est <- cur + s (cur - prev)
-        std::swap(c->hP, c->hC);  // hP = old cur
-        std::swap(c->hC, c->hE);  // hC = A y_k, hE = scratch
-        RIHVR_HIP(holo_momentum(c->hC, c->hP, step, c->meas, c->hE, c->resid, c->N, c->partial_holo, c->stream));
-        RIHVR_HIP(reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->scal + SL_RESID_M, c->stream));
-        // the next gradient (:545-546)
-        RIHVR_TRY(apply_adjoint(c, c->resid, c->G));
+        std::swap(c->hP, c->hC);
+        std::swap(c->hC, c->hE);
+        RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
+            return holo_momentum(c->hC, c->hP, step, c->meas, c->hE, c->resid, c->N, c->partial_holo, c->stream);
+        }));
+        RIHVR_TRY(timed(c, K_REDUCE, [&] {
+            return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->scal + SL_RESID_M, c->stream);
+        }));
+        RIHVR_TRY(apply_adjoint(c, c->resid, c->G));  // the next gradient (:545-546)
         // rotate the prox-output ring: y_{k-2} <- y_{k-1} <- y_k
-        int old_m = c->iym;
+        const int old_m = c->iym;
         c->iym = c->iy0;
         c->iy0 = c->iy;
         c->iy = old_m;
-        s_mom = (float)step;
+        s.s_mom = (float)step;
+        s.iters++;
         if (st) {
-            st->iterations = it + 1;
-            if (st->ratio) st->ratio[it] = ratio;
-            if (st->lim) st->lim[it] = lim;
-            if (st->backtracks) st->backtracks[it] = bt;
-            if (st->restarts) st->restarts[it] = restart ? 1 : 0;
-            if (st->restart_sum) st->restart_sum[it] = rs;
+            if (st->ratio) st->ratio[i] = ratio;
+            if (st->lim) st->lim[i] = lim;
+            if (st->backtracks) st->backtracks[i] = bt;
+            if (st->restarts) st->restarts[i] = restart ? 1 : 0;
+            if (st->restart_sum) st->restart_sum[i] = rs;
         }
     }
-    // returned volume: x1 = y_K + s (y_K - y_{K-1})  (:483-489)
-    float2* outb = c->Yb[c->iy];
-    if (n_it > 0) {
-        RIHVR_HIP(momentum_out(c->Yb[c->iy0], c->Yb[c->iym], s_mom, outb, c->W, c->H, c->Nz, c->partial, c->stream));
-        RIHVR_HIP(reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
-                                  c->scal + SL_PROX, c->stream));
-    } else {
-        RIHVR_HIP(hipMemcpyAsync(outb, c->Yb[c->iy0], c->V * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
-        RIHVR_HIP(momentum_out(c->Yb[c->iy0], c->Yb[c->iy0], 0.0f, outb, c->W, c->H, c->Nz, c->partial, c->stream));
-        RIHVR_HIP(reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
-                                  c->scal + SL_PROX, c->stream));
+    if (st) {
+        st->iterations = n_it;
+        st->total_backtracks = s.total_bt;
+        st->stepsize = s.t;
+        st->f_initial = s.f_initial;
+        if (n_it > 0 && !last_f_by_finish && (st->f || st->f_resid)) {
+            RIHVR_TRY(fetch_scalars(c));
+            const double summag = c->h_scal[SL_RESID_M];
+            double f = 0;
+            RIHVR_TRY(momentum_objective(c, summag, &f));
+            if (st->f) st->f[n_it - 1] = f;
+            if (st->f_resid) st->f_resid[n_it - 1] = 0.5 * summag;
+        }
     }
+    return RIHVR_OK;
+}
+
+rihvr_status finish_impl(rihvr_ctx* c, rihvr_stats* st, int last_index) {
+    if (!c->ss.active) return RIHVR_E_INVALID_STATE;
+    SolveState& s = c->ss;
+    // returned volume: x1 = y_K + s (y_K - y_{K-1})  (:483-489); with no
+    // iteration s = 0 and the start point is returned
+    float2* outb = c->Yb[c->iy];
+    RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
+        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], s.s_mom, outb, c->W, c->H, c->Nz, c->partial,
+                            c->stream);
+    }));
+    RIHVR_TRY(timed(c, K_REDUCE, [&] {
+        return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT, c->scal + SL_PROX,
+                               c->stream);
+    }));
     RIHVR_TRY(fetch_scalars(c));
     const double* sc = c->h_scal;
-    if (n_it > 0) {
-        prev_summag = sc[SL_RESID_M];
-        if (st && st->f_resid) st->f_resid[n_it - 1] = 0.5 * prev_summag;
-    }
-    f_last = objective(c, mode, prev_summag, sc[SL_PROX + PS_L1X0], sc[SL_PROX + PS_TVX0]);
+    const double summag = s.iters > 0 ? sc[SL_RESID_M] : s.prev_summag;
+    const double f_last = objective(c, s.mode, summag, sc[SL_PROX + PS_L1X0], sc[SL_PROX + PS_TVX0]);
     c->out = outb;
     c->have_result = true;
+    s.active = false;
     if (st) {
-        if (n_it > 0 && st->f) st->f[n_it - 1] = f_last;
-        if (n_it == 0) st->f_initial = f_last;
-        st->total_backtracks = total_bt;
-        st->stepsize = t;
+        if (last_index >= 0) {
+            if (st->f) st->f[last_index] = f_last;
+            if (st->f_resid) st->f_resid[last_index] = 0.5 * summag;
+        }
+        st->total_backtracks = s.total_bt;
+        st->stepsize = s.t;
+        st->f_initial = s.f_initial;
         st->f_final = f_last;
         st->nnz = (int64_t)llround(sc[SL_PROX + PS_NNZX0]);
         st->sparsity = (double)st->nnz / (double)c->V;
     }
+    return RIHVR_OK;
+}
+
+rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, const float* warm, int warm_loc,
+                              rihvr_stats* st) {
+    const int n_it = c->p.num_inverse_iterations;
+    RIHVR_TRY(begin_impl(c, holo, holo_loc, warm, warm_loc, nullptr));
+    RIHVR_TRY(iterate_impl(c, n_it, st, true));
+    RIHVR_TRY(finish_impl(c, st, n_it - 1));
+    if (st) st->iterations = n_it;
     return RIHVR_OK;
 }
 
@@ -473,8 +619,50 @@ rihvr_status rihvr_destroy(rihvr_ctx* ctx) {
 rihvr_status rihvr_reconstruct(rihvr_ctx* ctx, const float* holo, int holo_loc, const float* warm, int warm_loc,
                                rihvr_stats* stats) {
     if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
-    ctx->have_result = false;
     return reconstruct_impl(ctx, holo, holo_loc, warm, warm_loc, stats);
+}
+
+rihvr_status rihvr_begin(rihvr_ctx* ctx, const float* holo, int holo_loc, const float* warm_start, int warm_loc,
+                         double* f_initial) {
+    if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
+    return begin_impl(ctx, holo, holo_loc, warm_start, warm_loc, f_initial);
+}
+
+rihvr_status rihvr_iterate(rihvr_ctx* ctx, int32_t n_iter, rihvr_stats* stats) {
+    if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
+    return iterate_impl(ctx, n_iter, stats, false);
+}
+
+rihvr_status rihvr_finish(rihvr_ctx* ctx, rihvr_stats* stats) {
+    if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
+    return finish_impl(ctx, stats, -1);
+}
+
+rihvr_status rihvr_set_timing(rihvr_ctx* ctx, int32_t enable) {
+    if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
+    RIHVR_HIP(hipStreamSynchronize(ctx->stream));
+    RIHVR_TRY(resolve_timing(ctx));
+    for (int k = 0; k < K_COUNT; ++k) {
+        ctx->tm.total_ms[k] = 0;
+        ctx->tm.launches[k] = 0;
+    }
+    ctx->tm.on = enable != 0;
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_get_timing(rihvr_ctx* ctx, rihvr_kernel_time* out, int32_t capacity, int32_t* count) {
+    if (!ctx || !count) return RIHVR_E_INVALID_ARGUMENT;
+    RIHVR_HIP(hipStreamSynchronize(ctx->stream));
+    RIHVR_TRY(resolve_timing(ctx));
+    *count = K_COUNT;
+    if (!out) return RIHVR_OK;
+    for (int k = 0; k < K_COUNT && k < capacity; ++k) {
+        std::memset(out[k].name, 0, sizeof(out[k].name));
+        std::strncpy(out[k].name, kKernelNames[k], sizeof(out[k].name) - 1);
+        out[k].launches = ctx->tm.launches[k];
+        out[k].total_ms = ctx->tm.total_ms[k];
+    }
+    return RIHVR_OK;
 }
 
 rihvr_status rihvr_get_volume(rihvr_ctx* ctx, float* out, int out_loc) {
@@ -508,6 +696,7 @@ rihvr_status rihvr_get_estimate(rihvr_ctx* ctx, float* estimate, float* residual
 rihvr_status rihvr_forward(rihvr_ctx* ctx, const float* volume, int vol_loc, float* holo_out, int holo_loc) {
     if (!ctx || !volume || !holo_out) return RIHVR_E_INVALID_ARGUMENT;
     ctx->have_result = false;
+    ctx->ss.active = false;
     float2* v = ctx->Yb[0];
     RIHVR_TRY(copy_in(ctx, v, volume, vol_loc, ctx->V));
     RIHVR_HIP(hipMemsetAsync(ctx->meas, 0, ctx->N * sizeof(float2), ctx->stream));
@@ -518,6 +707,7 @@ rihvr_status rihvr_forward(rihvr_ctx* ctx, const float* volume, int vol_loc, flo
 rihvr_status rihvr_adjoint(rihvr_ctx* ctx, const float* holo, int holo_loc, float* volume_out, int vol_loc) {
     if (!ctx || !holo || !volume_out) return RIHVR_E_INVALID_ARGUMENT;
     ctx->have_result = false;
+    ctx->ss.active = false;
     RIHVR_TRY(copy_in(ctx, ctx->resid, holo, holo_loc, ctx->N));
     RIHVR_TRY(apply_adjoint(ctx, ctx->resid, ctx->G));
     return copy_out(ctx, volume_out, ctx->G, vol_loc, ctx->V);
@@ -526,6 +716,7 @@ rihvr_status rihvr_adjoint(rihvr_ctx* ctx, const float* holo, int holo_loc, floa
 rihvr_status rihvr_prox(rihvr_ctx* ctx, const float* x0, const float* grad, double stepsize, float* out, int loc) {
     if (!ctx || !x0 || !grad || !out) return RIHVR_E_INVALID_ARGUMENT;
     ctx->have_result = false;
+    ctx->ss.active = false;
     RIHVR_TRY(copy_in(ctx, ctx->Yb[1], x0, loc, ctx->V));
     RIHVR_TRY(copy_in(ctx, ctx->G, grad, loc, ctx->V));
     RIHVR_TRY(apply_prox(ctx, effective_mode(ctx), ctx->Yb[1], ctx->Yb[1], 0.0f, ctx->G, stepsize, ctx->Yb[0]));

@@ -188,3 +188,34 @@ def test_error_codes(pkg, gpu_available):
         with pytest.raises(pkg.RihvrError) as e:
             rec.getPlane(0)
         assert e.value.status == 4
+
+
+def test_stepwise_equals_reconstruct(pkg, scenes, gpu_available):
+    """begin + iterate(a) + iterate(b) + finish == reconstruct(a + b), bit for bit."""
+    c = cfg(64, 64, 6, its=7)
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 5, 2.0, 4.0, 21, 0.01)
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        st = rec.inverseReconstruct(holo)
+        a = rec.getVolume()
+        f0 = rec.begin(holo)
+        s1 = rec.iterate(3, with_objective=True)
+        rec.set_timing(True)
+        s2 = rec.iterate(4, with_objective=True)
+        tm = rec.get_timing()
+        rec.set_timing(False)
+        fin = rec.finish()
+        b = rec.getVolume()
+    np.testing.assert_array_equal(a, b)
+    assert f0 == st["f_initial"]
+    assert fin["f_final"] == st["f_final"]
+    np.testing.assert_allclose(np.concatenate([s1["f"], s2["f"]]), st["f"], rtol=1e-12)
+    assert list(np.concatenate([s1["restarts"], s2["restarts"]])) == list(st["restarts"])
+    assert tm["prox"][0] >= 4 and tm["prox"][1] > 0
+    assert tm["fft_cols_fwdacc"][0] >= 4
+
+
+def test_iterate_without_begin_is_invalid_state(pkg, gpu_available):
+    with pkg.SparseCompressiveHolo(pkg.make_params(**cfg(32, 32, 2))) as rec:
+        with pytest.raises(pkg.RihvrError) as e:
+            rec.iterate(1)
+        assert e.value.status == 4
