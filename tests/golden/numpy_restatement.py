@@ -79,8 +79,14 @@ def soft(v, thr):
     return one(v.real) + 1j * one(v.imag)
 
 
+def f32(v):
+    """Parameters are float fields promoted to double (Q3: src/core/holo_ui.h:97-121)."""
+    return float(np.float32(v))
+
+
 def prox_fl2d(x0, g, t, reg, reg_tv, n_tv):
     """proximalFusedLasso2d (:3652-3732), vectorised over planes (axis 0)."""
+    reg, reg_tv = f32(reg), f32(reg_tv)
     xh = x0 - t * g
     gx = np.zeros_like(xh)
     gy = np.zeros_like(xh)
@@ -88,14 +94,15 @@ def prox_fl2d(x0, g, t, reg, reg_tv, n_tv):
     sc = 1.0 / (12.0 * t * reg_tv)
 
     def estimate():
-        div = (np.roll(gx, -1, axis=2) - gx) + (np.roll(gy, -1, axis=1) - gy)
+        # evaluation order of tv_denoised_estimate_2d_kernel: ((gx+ - gx) + gy+) - gy
+        div = ((np.roll(gx, -1, axis=2) - gx) + np.roll(gy, -1, axis=1)) - gy
         return xh - mu * div
 
     for _ in range(n_tv):
         y = estimate()
         ax = gx + sc * (np.roll(y, 1, axis=2) - y)
         ay = gy + sc * (np.roll(y, 1, axis=1) - y)
-        nrm = np.sqrt(np.abs(ax) ** 2 + np.abs(ay) ** 2)
+        nrm = np.sqrt(((ax.real * ax.real + ax.imag * ax.imag) + ay.real * ay.real) + ay.imag * ay.imag)
         nrm = np.maximum(nrm, 1.0)
         gx = ax / nrm
         gy = ay / nrm
@@ -107,7 +114,7 @@ def prox_fl2d(x0, g, t, reg, reg_tv, n_tv):
 
 def prox_l1(x0, g, t, reg):
     """proximalL1 (:3274-3392)."""
-    return soft(x0 - t * g, t * reg)
+    return soft(x0 - t * g, t * f32(reg))
 
 
 def tv2d(vol):
@@ -144,9 +151,9 @@ def solve(op: Operator, holo, iters, reg, reg_tv, n_tv, mode=7, warm=None):
         return prox_l1(x0_, G_, t_, reg)
 
     def objective(r, x):
-        f = 0.5 * np.sum(np.abs(r) ** 2) + reg * np.sum(np.abs(x))
+        f = 0.5 * np.sum(np.abs(r) ** 2) + f32(reg) * np.sum(np.abs(x))
         if mode == 7:
-            f += reg_tv * tv2d(x)
+            f += f32(reg_tv) * tv2d(x)
         return float(f)
 
     est = op.forward(x1)
