@@ -71,10 +71,125 @@ __device__ __forceinline__ int wrapi(int v, int n) {
     return v < 0 ? v + n : v;
 }
 
+// Lane shifts across the whole wave with DPP (no LDS traffic): lane i receives
+// lane i-1 (wave_shr:1) or lane i+1 (wave_shl:1); the edge lane receives 0 and
+// is halo.
+__device__ __forceinline__ float lane_from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float soft_fast(float v, float thr) {
+    // block_soft_threshold_kernel: v * y / (y + T), y = max(|v| - T, 0); y / (y + T)
+    // through the hardware reciprocal (the reference build divides with fast-math too)
+    float y = fmaxf(fabsf(v) - thr, 0.0f);
+    return v * (y * __builtin_amdgcn_rcpf(y + thr));
+}
+
+// One wave's state: the TV duals g_k (gx.re, gx.im, gy.re, gy.im) and y-hat_k of
+// the last two rows of every level, indexed [row parity][level]; a per-lane LDS
+// ring holds x-hat, x0, g and y_{k-1} of the last NTV + 3 rows.
+template <int NTV> struct ProxWave {
+    static constexpr int P = NTV + 3;
+    float4 gd[2][NTV + 1];
+    float2 yh[2][NTV + 1];
+    float acc_ls1 = 0.f, acc_ls2 = 0.f, acc_rs = 0.f, acc_l1 = 0.f, acc_tv = 0.f, acc_nnz = 0.f;
+
+    // Row tau of the stream; PAR = (tau - tau0) & 1.
+    template <int PAR>
+    __device__ __forceinline__ void step(const ProxDev& a, float2 (*ring)[P][64], int lane, int slot, int tau,
+                                         int r0, int r1, const size_t plane_off, int xcol, int ocol, bool out_lane) {
+        constexpr int Q = PAR ^ 1;
+        const int W = a.W, H = a.H;
+        // ---- row tau: momentum point x0 = y0 + s (y0 - ym), x-hat = x0 - t g ----
+        {
+            int row = tau % H;
+            row = row < 0 ? row + H : row;
+            const size_t ro = plane_off + (size_t)row * W;
+            const float2 y0 = a.Y0[ro + xcol], ym = a.YM[ro + xcol], g = a.G[ro + xcol];
+            const float2 x0 = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
+            ring[0][slot][lane] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
+            ring[1][slot][lane] = x0;
+            ring[2][slot][lane] = g;
+            ring[3][slot][lane] = y0;
+        }
+        // ---- TV levels: level k works on row tau - k ----
+#pragma unroll
+        for (int k = 1; k <= NTV; ++k) {
+            int sk = slot - k;
+            sk += sk < 0 ? P : 0;
+            const float2 xh = ring[0][sk][lane];
+            float2 y;
+            float4 go;  // g_{k-1} at row tau - k
+            if (k == 1) {
+                y = xh;
+                go = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                go = gd[Q][k - 1];
+                const float4 gb = gd[PAR][k - 1];  // row below
+                const float dvr = lane_from_right(go.x) - go.x + gb.z - go.z;
+                const float dvi = lane_from_right(go.y) - go.y + gb.w - go.w;
+                y = make_float2(fmaf(-a.mu, dvr, xh.x), fmaf(-a.mu, dvi, xh.y));
+            }
+            const float2 yu = yh[Q][k];
+            const float axr = fmaf(a.sc, lane_from_left(y.x) - y.x, go.x);
+            const float axi = fmaf(a.sc, lane_from_left(y.y) - y.y, go.y);
+            const float ayr = fmaf(a.sc, yu.x - y.x, go.z);
+            const float ayi = fmaf(a.sc, yu.y - y.y, go.w);
+            const float n2 = fmaf(axr, axr, fmaf(axi, axi, fmaf(ayr, ayr, ayi * ayi)));
+            const float inv = n2 > 1.0f ? __builtin_amdgcn_rsqf(n2) : 1.0f;
+            gd[PAR][k] = make_float4(axr * inv, axi * inv, ayr * inv, ayi * inv);
+            yh[PAR][k] = y;
+        }
+        // ---- final estimate, threshold and reductions: row o = tau - NTV - 1 ----
+        const int o = tau - NTV - 1;
+        int so = slot - NTV - 1;
+        so += so < 0 ? P : 0;
+        int su = so - 1;
+        su += su < 0 ? P : 0;
+        const float2 xh = ring[0][so][lane];
+        float2 yv;
+        if constexpr (NTV == 0) {
+            yv = xh;
+        } else {
+            const float4 gu = gd[Q][NTV];
+            const float4 gb = gd[PAR][NTV];
+            const float dvr = lane_from_right(gu.x) - gu.x + gb.z - gu.z;
+            const float dvi = lane_from_right(gu.y) - gu.y + gb.w - gu.w;
+            yv = make_float2(fmaf(-a.mu, dvr, xh.x), fmaf(-a.mu, dvi, xh.y));
+        }
+        if (a.do_soft) yv = make_float2(soft_fast(yv.x, a.thr), soft_fast(yv.y, a.thr));
+        const float2 x0o = ring[1][so][lane];
+        const float x0lr = lane_from_left(x0o.x);
+        const float x0li = lane_from_left(x0o.y);
+        if (o >= r0 && o < r1 && out_lane) {
+            a.Y[plane_off + (size_t)o * W + ocol] = yv;
+            const float2 x0u = ring[1][su][lane];
+            const float2 g = ring[2][so][lane];
+            const float2 y0 = ring[3][so][lane];
+            const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
+            acc_ls1 = fmaf(dr, g.x, fmaf(di, g.y, acc_ls1));
+            acc_ls2 = fmaf(dr, dr, fmaf(di, di, acc_ls2));
+            const float dre = (x0o.x - yv.x) * (yv.x - y0.x);
+            const float dim = (x0o.y - yv.y) * (yv.y - y0.y);
+            const int off = ((o * W) + ocol) & 1023;
+            acc_rs += off < 512 ? fmaf(2.0f, dim, dre) : dim;
+            const float m2 = fmaf(x0o.x, x0o.x, x0o.y * x0o.y);
+            acc_l1 += __builtin_sqrtf(m2);
+            acc_nnz += (x0o.x != 0.0f || x0o.y != 0.0f) ? 1.0f : 0.0f;
+            const float gxx = x0lr - x0o.x, gyx = x0u.x - x0o.x, gxy = x0li - x0o.y, gyy = x0u.y - x0o.y;
+            acc_tv += __builtin_sqrtf(fmaf(gxx, gxx, fmaf(gxy, gxy, fmaf(gyx, gyx, gyy * gyy))));
+        }
+    }
+};
+
 template <int NTV>
 __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
-    constexpr int P = NTV + 3;  // history depth: rows tau .. tau-NTV-2
-    __shared__ float2 ring[kProxWaves][4][P][64];  // fields: xhat, x0, g, y_{k-1}
+    using PW = ProxWave<NTV>;
+    constexpr int P = PW::P;
+    __shared__ float2 ring_s[kProxWaves][4][P][64];  // fields: x-hat, x0, g, y_{k-1}
     const int lane = threadIdx.x & 63;
     const int wl = threadIdx.x >> 6;
     const int w = blockIdx.x * kProxWaves + wl;
@@ -85,137 +200,50 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     const int rem = w - z * per_plane;
     const int chunk = rem / geo.nstrips;
     const int strip = rem - chunk * geo.nstrips;
-    const int W = a.W, H = a.H;
+    const int W = a.W;
     const int ocol = strip * geo.ow + (lane - NTV);  // output column of this lane
     const bool out_lane = (lane - NTV) >= 0 && (lane - NTV) < geo.ow && ocol < W;
     const int xcol = wrapi(strip * geo.ow - NTV + lane, W);
     const int r0 = chunk * geo.rc;
-    const int r1 = min(H, r0 + geo.rc);
-    const size_t plane_off = (size_t)z * H * W;
-    const float s = a.s_mom, t = a.t, mu = a.mu, sc = a.sc;
+    const int r1 = min(a.H, r0 + geo.rc);
+    const size_t plane_off = (size_t)z * a.H * W;
+    float2 (*ring)[P][64] = ring_s[wl];
 
-    float4 gcur[NTV + 1], gprev[NTV + 1];  // (gx.re, gx.im, gy.re, gy.im) per TV level
-    float2 yprev[NTV + 1];
+    PW st;
 #pragma unroll
-    for (int k = 0; k <= NTV; ++k) {
-        gcur[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-        gprev[k] = gcur[k];
-        yprev[k] = make_float2(0.f, 0.f);
-    }
-    double acc_ls1 = 0, acc_ls2 = 0, acc_rs = 0, acc_l1 = 0, acc_tv = 0, acc_nnz0 = 0, acc_nnzy = 0;
-
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int k = 0; k <= NTV; ++k) {
+            st.gd[p][k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            st.yh[p][k] = make_float2(0.f, 0.f);
+        }
     const int tau0 = r0 - (NTV > 0 ? NTV : 1);
     const int tau1 = r1 + NTV;  // inclusive
-    int slot = 0;               // ring slot of row tau
-    for (int tau = tau0; tau <= tau1; ++tau) {
-        // ---- load row tau: momentum point, x-hat ----
-        {
-            const size_t o = plane_off + (size_t)wrapi(tau, H) * W + xcol;
-            const float2 y0 = a.Y0[o], ym = a.YM[o], g = a.G[o];
-            float2 x0 = make_float2(fmaf(s, y0.x - ym.x, y0.x), fmaf(s, y0.y - ym.y, y0.y));
-            float2 xh = make_float2(fmaf(-t, g.x, x0.x), fmaf(-t, g.y, x0.y));
-            ring[wl][0][slot][lane] = xh;
-            ring[wl][1][slot][lane] = x0;
-            ring[wl][2][slot][lane] = g;
-            ring[wl][3][slot][lane] = y0;
-        }
-        // ---- TV levels: level k works on row tau - k ----
-#pragma unroll
-        for (int k = 1; k <= NTV; ++k) {
-            int sk = slot - k;
-            if (sk < 0) sk += P;
-            const float2 xh = ring[wl][0][sk][lane];
-            float2 yh;
-            float4 gold;  // g_{k-1} at this row
-            if (k == 1) {
-                yh = xh;
-                gold = make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                gold = gprev[k - 1];
-                const float gxr_p = __shfl_down(gold.x, 1, 64);
-                const float gxi_p = __shfl_down(gold.y, 1, 64);
-                const float4 gdn = gcur[k - 1];  // row below (tau-k+1)
-                const float dvr = gxr_p - gold.x + gdn.z - gold.z;
-                const float dvi = gxi_p - gold.y + gdn.w - gold.w;
-                yh = make_float2(fmaf(-mu, dvr, xh.x), fmaf(-mu, dvi, xh.y));
-            }
-            const float ylr = __shfl_up(yh.x, 1, 64);
-            const float yli = __shfl_up(yh.y, 1, 64);
-            const float2 yu = yprev[k];
-            float axr = fmaf(sc, ylr - yh.x, gold.x);
-            float axi = fmaf(sc, yli - yh.y, gold.y);
-            float ayr = fmaf(sc, yu.x - yh.x, gold.z);
-            float ayi = fmaf(sc, yu.y - yh.y, gold.w);
-            const float n2 = axr * axr + axi * axi + ayr * ayr + ayi * ayi;
-            const float inv = n2 > 1.0f ? __frsqrt_rn(n2) : 1.0f;
-            gprev[k] = gcur[k];
-            gcur[k] = make_float4(axr * inv, axi * inv, ayr * inv, ayi * inv);
-            yprev[k] = yh;
-        }
-        // ---- final estimate and threshold: row o = tau - NTV - 1 ----
-        const int o = tau - NTV - 1;
-        int so = slot - NTV - 1;
-        if (so < 0) so += P;
-        const float2 xh = ring[wl][0][so][lane];
-        float2 yv;
-        if constexpr (NTV == 0) {
-            yv = xh;
-        } else {
-            const float4 gu = gprev[NTV];
-            const float4 gd = gcur[NTV];
-            const float gxr_p = __shfl_down(gu.x, 1, 64);
-            const float gxi_p = __shfl_down(gu.y, 1, 64);
-            const float dvr = gxr_p - gu.x + gd.z - gu.z;
-            const float dvi = gxi_p - gu.y + gd.w - gu.w;
-            yv = make_float2(fmaf(-mu, dvr, xh.x), fmaf(-mu, dvi, xh.y));
-        }
-        if (a.do_soft) yv = make_float2(soft1(yv.x, a.thr), soft1(yv.y, a.thr));
-        // TV of x0 needs x0 at x-1 (shuffle: every lane participates)
-        const float2 x0o = ring[wl][1][so][lane];
-        const float x0lr = __shfl_up(x0o.x, 1, 64);
-        const float x0li = __shfl_up(x0o.y, 1, 64);
-        if (o >= r0 && out_lane) {
-            const size_t oo = plane_off + (size_t)o * W + ocol;
-            a.Y[oo] = yv;
-            int su = so - 1;
-            if (su < 0) su += P;
-            const float2 x0u = ring[wl][1][su][lane];
-            const float2 g = ring[wl][2][so][lane];
-            const float2 y0 = ring[wl][3][so][lane];
-            const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
-            acc_ls1 += (double)(dr * g.x) + (double)(di * g.y);
-            acc_ls2 += (double)dr * dr + (double)di * di;
-            const float dre = (x0o.x - yv.x) * (yv.x - y0.x);
-            const float dim = (x0o.y - yv.y) * (yv.y - y0.y);
-            const int off = (int)(((size_t)o * W + ocol) & 1023);
-            acc_rs += off < 512 ? (double)dre + 2.0 * (double)dim : (double)dim;
-            const bool nz0 = (x0o.x != 0.0f) || (x0o.y != 0.0f);
-            if (nz0) {
-                acc_l1 += (double)sqrtf(x0o.x * x0o.x + x0o.y * x0o.y);
-                acc_nnz0 += 1.0;
-            }
-            const float gxx = x0lr - x0o.x, gyx = x0u.x - x0o.x, gxy = x0li - x0o.y, gyy = x0u.y - x0o.y;
-            acc_tv += (double)sqrtf(gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy);
-            acc_nnzy += ((yv.x != 0.0f) || (yv.y != 0.0f)) ? 1.0 : 0.0;
-        }
+    int slot = 0;
+    int tau = tau0;
+    for (; tau + 1 <= tau1; tau += 2) {
+        st.template step<0>(a, ring, lane, slot, tau, r0, r1, plane_off, xcol, ocol, out_lane);
+        slot = slot + 1 == P ? 0 : slot + 1;
+        st.template step<1>(a, ring, lane, slot, tau + 1, r0, r1, plane_off, xcol, ocol, out_lane);
         slot = slot + 1 == P ? 0 : slot + 1;
     }
-    acc_ls1 = wave_sum(acc_ls1);
-    acc_ls2 = wave_sum(acc_ls2);
-    acc_rs = wave_sum(acc_rs);
-    acc_l1 = wave_sum(acc_l1);
-    acc_tv = wave_sum(acc_tv);
-    acc_nnz0 = wave_sum(acc_nnz0);
-    acc_nnzy = wave_sum(acc_nnzy);
+    if (tau <= tau1) st.template step<0>(a, ring, lane, slot, tau, r0, r1, plane_off, xcol, ocol, out_lane);
+
+    const double s_ls1 = wave_sum((double)st.acc_ls1);
+    const double s_ls2 = wave_sum((double)st.acc_ls2);
+    const double s_rs = wave_sum((double)st.acc_rs);
+    const double s_l1 = wave_sum((double)st.acc_l1);
+    const double s_tv = wave_sum((double)st.acc_tv);
+    const double s_nnz = wave_sum((double)st.acc_nnz);
     if (lane == 0) {
         double* p = a.partial + (size_t)w * PS_COUNT;
-        p[PS_LS1] = acc_ls1;
-        p[PS_LS2] = acc_ls2;
-        p[PS_RESTART] = acc_rs;
-        p[PS_L1X0] = acc_l1;
-        p[PS_TVX0] = acc_tv;
-        p[PS_NNZX0] = acc_nnz0;
-        p[PS_NNZY] = acc_nnzy;
+        p[PS_LS1] = s_ls1;
+        p[PS_LS2] = s_ls2;
+        p[PS_RESTART] = s_rs;
+        p[PS_L1X0] = s_l1;
+        p[PS_TVX0] = s_tv;
+        p[PS_NNZX0] = s_nnz;
+        p[PS_NNZY] = 0.0;
         p[7] = 0.0;
     }
 }

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -105,6 +106,7 @@ struct rihvr_ctx {
     double* h_scal = nullptr;
     int* dflag = nullptr;
     int zchunk_adj = 8, zchunk_fwd = 8, nparts = 1;
+    int plane_group = 1;  // planes per row+column FFT pass group
     bool have_result = false;
     bool force_step = false;
     double forced_step = 0;
@@ -211,11 +213,20 @@ rihvr_status fetch_scalars(rihvr_ctx* c) {
 // (backPropagate :1935-2077 + calcResidual :2676-2723 + calcSumMagnitude)
 rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
     const float invN = (float)(1.0 / (double)c->N);
-    RIHVR_TRY(timed(c, K_ROWS_FWD, [&] { return fft_rows_fwd(c->W, vol, c->phase, c->T, c->H, c->Nz, c->twfW, c->stream); }));
-    RIHVR_TRY(timed(c, K_COLS_FWDACC, [&] {
-        return fft_cols_fwdacc(c->H, c->T, c->kappa, c->zlist, c->spart, c->W, c->Nz, c->zchunk_fwd, invN, c->twfH,
-                               c->stream);
-    }));
+    // planes go through the row and column passes a group at a time, so the
+    // row-pass output of a group (group * W * H * 8 bytes) can stay in cache
+    for (int g0 = 0; g0 < c->Nz; g0 += c->plane_group) {
+        const int gz = std::min(c->plane_group, c->Nz - g0);
+        const float2* vg = vol + (size_t)g0 * c->N;
+        float2* spg = c->spart + (size_t)(g0 / c->zchunk_fwd) * c->N;
+        RIHVR_TRY(timed(c, K_ROWS_FWD, [&] {
+            return fft_rows_fwd(c->W, vg, c->phase + g0, c->T, c->H, gz, c->twfW, c->stream);
+        }));
+        RIHVR_TRY(timed(c, K_COLS_FWDACC, [&] {
+            return fft_cols_fwdacc(c->H, c->T, c->kappa, c->zlist + g0, spg, c->W, gz, c->zchunk_fwd, invN, c->twfH,
+                                   c->stream);
+        }));
+    }
     RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] { return fft_rows_sumparts(c->W, c->spart, c->nparts, c->N, c->htmp, c->H, c->twiW, c->stream); }));
     RIHVR_TRY(timed(c, K_COLS_REAL, [&] { return fft_cols_real(c->H, c->htmp, est, c->meas, c->resid, c->partial_holo, c->W, c->twiH, c->stream); }));
     RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial_holo, c->W / 16, 1, c->scal + SL_RESID_A, c->stream); }));
@@ -227,11 +238,16 @@ rihvr_status apply_adjoint(rihvr_ctx* c, const float2* r, float2* G) {
     const float invN = (float)(1.0 / (double)c->N);
     RIHVR_TRY(timed(c, K_ROWS_PLAIN, [&] { return fft_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream); }));
     RIHVR_TRY(timed(c, K_COLS_PLAIN, [&] { return fft_cols_plain(c->H, -1, c->htmp, c->rhat, c->W, 1, c->twfH, c->stream); }));
-    RIHVR_TRY(timed(c, K_ROWS_ADJ, [&] {
-        return fft_rows_adj(c->W, c->rhat, c->kappa, c->zlist, c->T, c->H, c->Nz, c->zchunk_adj, invN, c->twiW,
-                            c->stream);
-    }));
-    RIHVR_TRY(timed(c, K_COLS_ADJ, [&] { return fft_cols_adj(c->H, c->T, c->phase, G, c->W, c->Nz, c->twiH, c->stream); }));
+    for (int g0 = 0; g0 < c->Nz; g0 += c->plane_group) {
+        const int gz = std::min(c->plane_group, c->Nz - g0);
+        RIHVR_TRY(timed(c, K_ROWS_ADJ, [&] {
+            return fft_rows_adj(c->W, c->rhat, c->kappa, c->zlist + g0, c->T, c->H, gz, std::min(c->zchunk_adj, gz),
+                                invN, c->twiW, c->stream);
+        }));
+        RIHVR_TRY(timed(c, K_COLS_ADJ, [&] {
+            return fft_cols_adj(c->H, c->T, c->phase + g0, G + (size_t)g0 * c->N, c->W, gz, c->twiH, c->stream);
+        }));
+    }
     return RIHVR_OK;
 }
 
@@ -300,6 +316,15 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     c->zchunk_fwd = std::min(c->Nz, 8);
     c->nparts = (c->Nz + c->zchunk_fwd - 1) / c->zchunk_fwd;
     c->zchunk_adj = std::min(c->Nz, 8);
+    // plane group: all planes, or RIHVR_PLANE_GROUP (rounded to a multiple of the z chunk)
+    c->plane_group = c->Nz;
+    if (const char* e = std::getenv("RIHVR_PLANE_GROUP")) {
+        int g = std::atoi(e);
+        if (g > 0) {
+            g = ((g + c->zchunk_fwd - 1) / c->zchunk_fwd) * c->zchunk_fwd;
+            c->plane_group = std::min(c->Nz, g);
+        }
+    }
     for (auto& b : c->Yb) RIHVR_TRY(dalloc(&b, c->V));
     RIHVR_TRY(dalloc(&c->G, c->V));
     RIHVR_TRY(dalloc(&c->T, c->V));
