@@ -102,3 +102,33 @@ def test_scene_objects_refocus_under_the_adjoint(orc, scenes):
     inside = masks[3] > 0
     coherent = [abs(g[k][inside].mean()) for k in range(6)]   # ~1 (the opaque disk) only in focus
     assert int(np.argmax(coherent)) == 3 and coherent[3] > 0.9
+
+
+def test_cpp_mirror_compiles_and_throws_holoerror(tmp_path, pkg):
+    """include/rihvr.hpp builds against librihvr.so; invalid parameters throw the
+    reference's HoloError value before any device work."""
+    src = tmp_path / "mirror.cpp"
+    src.write_text(r'''
+#include "rihvr.hpp"
+#include <cstdio>
+int main() {
+    if (rihvr_abi_version() != RIHVR_ABI_VERSION) return 3;
+    rihvr_params p{};
+    p.width = 40; p.height = 32; p.num_planes = 2; p.wavelength = 0.633f; p.resolution = 2.0f;
+    p.mode = 7;
+    try {
+        rihvr::SparseCompressiveHolo recon(p);
+    } catch (rihvr::HoloError e) {
+        std::printf("%d\n", (int)e);
+        return e == rihvr::HOLO_ERROR_INVALID_ARGUMENT ? 0 : 4;
+    }
+    return 5;
+}
+''')
+    lib_dir = pkg.LIB_PATH.parent
+    exe = tmp_path / "mirror"
+    r = subprocess.run(["g++", "-std=c++17", "-I", str(ROOT / "include"), str(src), "-L", str(lib_dir), "-lrihvr",
+                        f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "7", (r.returncode, r.stdout, r.stderr)
