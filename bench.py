@@ -37,10 +37,10 @@ def algorithmic_bytes(name: str, W: int, H: int, Nz: int, nparts: int) -> float:
     return {
         # reads y_{k-1}, y_{k-2}, g; writes y_k
         "prox": 32.0 * V,
-        # reads y_k, writes the row-transformed planes
-        "fft_rows_fwd": 16.0 * V,
-        # reads the row-transformed planes, writes nparts partial spectra
-        "fft_cols_fwdacc": 8.0 * V + 8.0 * N * nparts,
+        # reads y_k, writes the column-transformed planes
+        "fft_cols_fwd": 16.0 * V,
+        # reads the column-transformed planes and kappa rows, writes nparts partial spectra
+        "fft_rows_fwdacc": 8.0 * V + 8.0 * N * nparts + 8.0 * N * nparts,
         # reads R-hat and kappa (16 B / pixel), writes the row-transformed planes
         "fft_rows_adj": 8.0 * V + 16.0 * N,
         # reads the row-transformed planes, writes g
@@ -122,7 +122,7 @@ def main():
 
     elapsed = t1 - t0
     ips = args.steps / elapsed
-    nparts = (Nz + 7) // 8
+    nparts = (Nz + int(os.environ.get("RIHVR_ZCHUNK", "16")) - 1) // int(os.environ.get("RIHVR_ZCHUNK", "16"))
     # dominant kernel: the largest share of GPU time in the timed region
     dom = max(timing.items(), key=lambda kv: kv[1][1])
     dom_name, (dom_launches, dom_ms) = dom
@@ -132,6 +132,12 @@ def main():
     it_bytes = iteration_bytes(W, H, Nz)
     it_gbps = it_bytes * ips / 1e9
     gpu_ms_total = sum(v[1] for v in timing.values())
+    # HBM traffic of the dominant kernel from the committed rocprofv3 counter passes
+    traffic = None
+    counters = os.path.join(ROOT, "profiles", "c2_hbm_counters.json")
+    if args.workload == "c2" and os.path.exists(counters):
+        kc = json.load(open(counters)).get("kernels", {}).get(dom_name)
+        traffic = kc["hbm_bytes"] if kc else None
 
     out = {
         "metric": METRIC,
@@ -159,7 +165,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/c2_hbm_counters.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)" if traffic else None,
             "avg_launch_ms": dom_avg_ms,
             "algorithmic_bytes_per_launch": dom_bytes,
         },

@@ -43,14 +43,31 @@ __device__ __forceinline__ void sincos_accurate(float a, float* s, float* c) {
     *c = cv;
 }
 
+// sin and cos of a float argument through the hardware units: the argument is
+// reduced to revolutions in double (exact to ~1e-12 rev for |a| < 2^20), rounded
+// to float in [-0.5, 0.5] and fed to v_sin_f32 / v_cos_f32 (which take
+// revolutions).  Absolute error ~1e-6; used where that is far below the
+// quantity's other error sources.
+__device__ __forceinline__ void sincos_hw(float a, float* s, float* c) {
+    const double kInvTwoPi = 0.15915494309189533577;
+    double rev = (double)a * kInvTwoPi;
+    rev -= rint(rev);
+    const float r = (float)rev;
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
+
 // e^{i * float(z * kappa)}: the reference's transfer-function factor H_z
 // (sparse_rs_phase_mult_kernel, sparse_compressive_holo.cu:1863-1888; and
 // sparse_rs_phase_mult_kernel2, sparse_gradient.cu:77-101): the double product
 // is rounded to float, then sin/cos are taken.
+#ifndef RIHVR_TRANSFER_SINCOS
+#define RIHVR_TRANSFER_SINCOS sincos_hw
+#endif
 __device__ __forceinline__ float2 transfer(double z, double kappa) {
     float arg = (float)(z * kappa);
     float s, c;
-    sincos_accurate(arg, &s, &c);
+    RIHVR_TRANSFER_SINCOS(arg, &s, &c);
     return make_float2(c, s);
 }
 

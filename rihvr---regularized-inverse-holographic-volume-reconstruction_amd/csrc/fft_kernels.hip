@@ -30,7 +30,7 @@ __device__ __forceinline__ void load_twiddles(float2* tw_s, const float2* __rest
 // ----------------------------------------------------------------------------
 // Row kernels
 // ----------------------------------------------------------------------------
-enum RowMode { ROW_PLAIN = 0, ROW_ADJ = 1, ROW_FWD = 2, ROW_SUMPART = 3 };
+enum RowMode { ROW_PLAIN = 0, ROW_ADJ = 1, ROW_FWDACC = 2, ROW_SUMPART = 3 };
 
 struct RowArgs {
     const float2* in;       // PLAIN: [nrows][L]; FWD: volume; SUMPART: [nparts][H][L]
@@ -43,8 +43,7 @@ struct RowArgs {
     const float* zlist;     // plane positions
     int nz, zchunk;
     float invN;
-    // FWD
-    const float2* phase;    // per plane (cos phi_z, sin phi_z)
+    // (FWDACC uses kappa, zlist, nz, zchunk, invN, part_stride)
     // SUMPART
     int nparts;
     long part_stride;
@@ -83,21 +82,49 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
                 return s;
             },
             [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
-    } else if constexpr (MODE == ROW_FWD) {
-        const int z = blockIdx.y;
-        const float2 ph = a.phase[z];
-        // e^{-i phi_z}: the A prologue (sparse_phase_scale_kernel with -z)
-        const float px = ph.x, py = -ph.y;
-        const size_t off = ((size_t)z * a.nrows + rrow) * L;
-        const float2* src_row = a.in + off;
-        float2* dst_row = a.out + off;
-        FftRun<L, SIGN, typename Plan::radices>::run(
-            tw_s, t,
-            [&](int, int, int j) {
-                float2 d = src_row[j];
-                return make_float2(d.x * px - d.y * py, d.y * px + d.x * py);
-            },
-            [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
+    } else if constexpr (MODE == ROW_FWDACC) {
+        // A, second pass: forward row FFT of the column-transformed planes of a
+        // chunk of planes; epilogue H_{-z} / (Nx Ny) summed over the chunk
+        // (sparse_rs_phase_mult_kernel + sparse_add_plane_kernel)
+        using LPs = LastPass<L>;
+        float2 acc[LPs::NB][LPs::R];
+        double kp[LPs::NB][LPs::R];
+        const double* krow_p = a.kappa + (size_t)rrow * L;
+#pragma unroll
+        for (int i = 0; i < LPs::NB; ++i) {
+            const int b = t + i * LPs::TPT;
+            const int bb = b < LPs::NBF ? b : LPs::NBF - 1;
+#pragma unroll
+            for (int r = 0; r < LPs::R; ++r) {
+                acc[i][r] = make_float2(0.0f, 0.0f);
+                kp[i][r] = krow_p[bb + r * LPs::NBF];
+            }
+        }
+        const int z0 = blockIdx.y * a.zchunk;
+        const int z1 = min(a.nz, z0 + a.zchunk);
+        for (int z = z0; z < z1; ++z) {
+            const double mz = -(double)a.zlist[z];
+            const float2* src_row = a.in + ((size_t)z * a.nrows + rrow) * L;
+            FftRun<L, SIGN, typename Plan::radices>::run(
+                tw_s, t, [&](int, int, int j) { return src_row[j]; },
+                [&](int i, int r, int, float2 v) {
+                    float2 h = transfer(mz, kp[i][r]);
+                    acc[i][r] = cadd(acc[i][r], cscale(cmul(h, v), a.invN));
+                },
+                lds);
+            __syncthreads();
+        }
+        if (valid) {
+            float2* dst_row = a.out + (size_t)blockIdx.y * a.part_stride + (size_t)rrow * L;
+#pragma unroll
+            for (int i = 0; i < LPs::NB; ++i) {
+                const int b = t + i * LPs::TPT;
+                if (b < LPs::NBF) {
+#pragma unroll
+                    for (int r = 0; r < LPs::R; ++r) dst_row[b + r * LPs::NBF] = acc[i][r];
+                }
+            }
+        }
     } else {  // ROW_ADJ: H_z * R_hat / N, then inverse FFT along x, for a chunk of planes
         constexpr int NBF0 = L / Plan::first;
         constexpr int NB0 = (NBF0 + C::TPT - 1) / C::TPT;
@@ -137,20 +164,15 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
 // (or of a chunk of planes, accumulating).  LDS layout [j][16] keeps the 16
 // columns of one row contiguous.
 // ----------------------------------------------------------------------------
-enum ColMode { COL_PLAIN = 0, COL_ADJ = 1, COL_FWDACC = 2, COL_REAL = 3 };
+enum ColMode { COL_PLAIN = 0, COL_ADJ = 1, COL_FWD = 2, COL_REAL = 3 };
 
 struct ColArgs {
     const float2* in;       // [planes][L][W]
     float2* out;            // PLAIN/ADJ: [planes][L][W]; FWDACC: [chunks][L][W]; REAL: est [L][W]
     int W;
     const float2* twg;
-    // ADJ
+    // ADJ, FWD
     const float2* phase;
-    // FWDACC
-    const double* kappa;
-    const float* zlist;
-    int nz, zchunk;
-    float invN;
     // REAL
     const float2* meas;
     float2* resid;
@@ -186,45 +208,20 @@ __global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
                 dst[(size_t)j * W] = v;
             },
             lds);
-    } else if constexpr (MODE == COL_FWDACC) {
-        using LP = LastPass<L>;
-        float2 acc[LP::NB][LP::R];
-        double kp[LP::NB][LP::R];
-#pragma unroll
-        for (int i = 0; i < LP::NB; ++i) {
-            const int b = t + i * LP::TPT;
-            const int bb = b < LP::NBF ? b : LP::NBF - 1;
-#pragma unroll
-            for (int r = 0; r < LP::R; ++r) {
-                acc[i][r] = make_float2(0.0f, 0.0f);
-                kp[i][r] = a.kappa[(size_t)(bb + r * LP::NBF) * W + col];
-            }
-        }
-        const int z0 = blockIdx.y * a.zchunk;
-        const int z1 = min(a.nz, z0 + a.zchunk);
-        for (int z = z0; z < z1; ++z) {
-            const double mz = -(double)a.zlist[z];
-            const float2* src = a.in + (size_t)z * L * W + col;
-            FftRun<L, SIGN, typename Plan::radices>::run(
-                tw_s, t, [&](int, int, int j) { return src[(size_t)j * W]; },
-                [&](int i, int r, int, float2 v) {
-                    // H_{-z} . FFT(...) / (Nx Ny), summed over planes
-                    // (sparse_rs_phase_mult_kernel + sparse_add_plane_kernel)
-                    float2 h = transfer(mz, kp[i][r]);
-                    acc[i][r] = cadd(acc[i][r], cscale(cmul(h, v), a.invN));
-                },
-                lds);
-            __syncthreads();
-        }
-        float2* dst = a.out + (size_t)blockIdx.y * L * W + col;
-#pragma unroll
-        for (int i = 0; i < LP::NB; ++i) {
-            const int b = t + i * LP::TPT;
-            if (b < LP::NBF) {
-#pragma unroll
-                for (int r = 0; r < LP::R; ++r) dst[(size_t)(b + r * LP::NBF) * W] = acc[i][r];
-            }
-        }
+    } else if constexpr (MODE == COL_FWD) {
+        // A, first pass: x_z e^{-i phi_z} (sparse_phase_scale_kernel with -z), forward column FFT
+        const int z = blockIdx.y;
+        const float2 ph = a.phase[z];
+        const float px = ph.x, py = -ph.y;
+        const float2* src = a.in + (size_t)z * L * W + col;
+        float2* dst = a.out + (size_t)z * L * W + col;
+        FftRun<L, SIGN, typename Plan::radices>::run(
+            tw_s, t,
+            [&](int, int, int j) {
+                float2 d = src[(size_t)j * W];
+                return make_float2(d.x * px - d.y * py, d.y * px + d.x * py);
+            },
+            [&](int, int, int j, float2 v) { dst[(size_t)j * W] = v; }, lds);
     } else {  // COL_REAL: est = Re(IFFT(...)), r = est - meas, partial sum |r|^2
         const float2* src = a.in + col;
         double s = 0.0;
@@ -310,12 +307,14 @@ hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_s
     }
 }
 
-hipError_t fft_rows_fwd(int L, const float2* vol, const float2* phase, float2* out, int H, int nz,
-                        const float2* tw_fwd, hipStream_t s) {
+hipError_t fft_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts,
+                          int H, int nz, int zchunk, float invN, const float2* tw_fwd, hipStream_t s) {
     RowArgs a{};
-    a.in = vol; a.out = out; a.nrows = H; a.twg = tw_fwd; a.phase = phase;
+    a.in = in; a.out = parts; a.nrows = H; a.twg = tw_fwd; a.kappa = kappa; a.zlist = zlist; a.nz = nz;
+    a.zchunk = zchunk; a.invN = invN; a.part_stride = (long)H * L;
+    const int gy = (nz + zchunk - 1) / zchunk;
     switch (L) {
-#define X(n) case n: return launch_rows_t<n, -1, ROW_FWD>(a, H, nz, s);
+#define X(n) case n: return launch_rows_t<n, -1, ROW_FWDACC>(a, H, gy, s);
         RIHVR_FOR_EACH_L(X)
 #undef X
         default: return hipErrorInvalidValue;
@@ -361,15 +360,12 @@ hipError_t fft_cols_adj(int L, const float2* in, const float2* phase, float2* ou
     }
 }
 
-hipError_t fft_cols_fwdacc(int L, const float2* in, const double* kappa, const float* zlist,
-                           float2* parts, int W, int nz, int zchunk, float invN, const float2* tw_fwd,
-                           hipStream_t s) {
+hipError_t fft_cols_fwd(int L, const float2* in, const float2* phase, float2* out, int W, int nz,
+                        const float2* tw_fwd, hipStream_t s) {
     ColArgs a{};
-    a.in = in; a.out = parts; a.W = W; a.twg = tw_fwd; a.kappa = kappa; a.zlist = zlist; a.nz = nz;
-    a.zchunk = zchunk; a.invN = invN;
-    const int gy = (nz + zchunk - 1) / zchunk;
+    a.in = in; a.out = out; a.W = W; a.twg = tw_fwd; a.phase = phase;
     switch (L) {
-#define X(n) case n: return launch_cols_t<n, -1, COL_FWDACC>(a, gy, s);
+#define X(n) case n: return launch_cols_t<n, -1, COL_FWD>(a, nz, s);
         RIHVR_FOR_EACH_L(X)
 #undef X
         default: return hipErrorInvalidValue;

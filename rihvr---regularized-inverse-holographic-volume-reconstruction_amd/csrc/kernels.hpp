@@ -14,9 +14,13 @@ hipError_t fft_rows_plain(int L, int sign, const float2* in, float2* out, int nr
 // Inverse row FFT of sum_p parts[p] (nparts partial spectra, part_stride elements apart).
 hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out,
                              int nrows, const float2* tw_inv, hipStream_t s);
-// A, first pass: forward row FFT of vol[z] * e^{-i phi_z} for nz planes of H rows.
-hipError_t fft_rows_fwd(int L, const float2* vol, const float2* phase, float2* out, int H, int nz,
+// A, first pass: forward column FFT (length L = H) of vol[z] * e^{-i phi_z}, nz planes of width W.
+hipError_t fft_cols_fwd(int L, const float2* vol, const float2* phase, float2* out, int W, int nz,
                         const float2* tw_fwd, hipStream_t s);
+// A, second pass: forward row FFT (length L = W), epilogue H_{-z} / N summed over zchunk
+// planes into parts[chunk] ([H][W] each).
+hipError_t fft_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts,
+                          int H, int nz, int zchunk, float invN, const float2* tw_fwd, hipStream_t s);
 // A^T, first pass: inverse row FFT of H_z * rhat / N, planes processed zchunk per workgroup.
 hipError_t fft_rows_adj(int L, const float2* rhat, const double* kappa, const float* zlist, float2* out,
                         int H, int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s);
@@ -26,11 +30,6 @@ hipError_t fft_cols_plain(int L, int sign, const float2* in, float2* out, int W,
 // A^T, second pass: inverse column FFT, epilogue * e^{+i phi_z}.
 hipError_t fft_cols_adj(int L, const float2* in, const float2* phase, float2* out, int W, int nz,
                         const float2* tw_inv, hipStream_t s);
-// A, second pass: forward column FFT, epilogue H_{-z} / N summed over zchunk planes
-// into parts[chunk].
-hipError_t fft_cols_fwdacc(int L, const float2* in, const double* kappa, const float* zlist,
-                           float2* parts, int W, int nz, int zchunk, float invN, const float2* tw_fwd,
-                           hipStream_t s);
 // Final inverse column FFT of one plane: est = (Re, 0), resid = est - meas,
 // partial[blockIdx.x] = sum |resid|^2.
 hipError_t fft_cols_real(int L, const float2* in, float2* est, const float2* meas, float2* resid,
@@ -60,10 +59,12 @@ struct ProxLaunch {
     double reg;         // L1 weight
     double reg_tv;      // TV weight
     int n_tv;
+    int row_chunk;      // output rows per wave of the mode-7 prox (<= 0: default)
     double* partial;    // [nwaves][PS_COUNT]
 };
 // Number of per-wave partial rows the mode-7 prox writes for this geometry.
-int prox_fl2d_num_waves(int W, int H, int nz, int n_tv);
+// row_chunk: output rows per wave (<= 0: default 128)
+int prox_fl2d_num_waves(int W, int H, int nz, int n_tv, int row_chunk);
 bool prox_fl2d_supported(int n_tv);
 hipError_t prox_fl2d(const ProxLaunch& a, hipStream_t s);
 

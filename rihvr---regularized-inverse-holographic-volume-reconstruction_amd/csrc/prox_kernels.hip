@@ -19,30 +19,33 @@
 // strip's output range are recomputed halo (n_tv on each side in x, and at the
 // ends of the row chunk in y).  No workgroup barriers; per-lane history rings
 // of x-hat, x0, g and y_{k-1} live in LDS.
+#include <utility>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace rihvr {
 
-constexpr int kRowChunk = 128;   // output rows per wave
+constexpr int kDefaultRowChunk = 128;  // output rows per wave
 constexpr int kProxWaves = 4;    // waves per workgroup
 constexpr int kMaxNtv = 10;
 
 struct ProxGeom {
     int ow, nstrips, rc, nchunks;
 };
-__host__ __device__ inline ProxGeom prox_geom(int W, int H, int n) {
+inline ProxGeom prox_geom(int W, int H, int n, int row_chunk) {
+    if (row_chunk <= 0) row_chunk = kDefaultRowChunk;
     ProxGeom g;
     g.ow = 64 - 2 * n;
     if (g.ow > W) g.ow = W;
     g.nstrips = (W + g.ow - 1) / g.ow;
-    g.rc = H < kRowChunk ? H : kRowChunk;
+    g.rc = H < row_chunk ? H : row_chunk;
     g.nchunks = (H + g.rc - 1) / g.rc;
     return g;
 }
 
-int prox_fl2d_num_waves(int W, int H, int nz, int n_tv) {
-    ProxGeom g = prox_geom(W, H, n_tv);
+int prox_fl2d_num_waves(int W, int H, int nz, int n_tv, int row_chunk) {
+    ProxGeom g = prox_geom(W, H, n_tv, row_chunk);
     return g.nstrips * g.nchunks * nz;
 }
 bool prox_fl2d_supported(int n_tv) { return n_tv >= 0 && n_tv <= kMaxNtv; }
@@ -90,17 +93,22 @@ __device__ __forceinline__ float soft_fast(float v, float thr) {
 
 // One wave's state: the TV duals g_k (gx.re, gx.im, gy.re, gy.im) and y-hat_k of
 // the last two rows of every level, indexed [row parity][level]; a per-lane LDS
-// ring holds x-hat, x0, g and y_{k-1} of the last NTV + 3 rows.
+// ring holds x-hat, x0, g and y_{k-1} of the last NTV + 3 rows.  The row loop is
+// unrolled by U (a multiple of the ring length and of 2), so every ring slot and
+// row parity is a compile-time constant.
 template <int NTV> struct ProxWave {
     static constexpr int P = NTV + 3;
+    static constexpr int U = (P % 2 == 0) ? P : 2 * P;
     float4 gd[2][NTV + 1];
     float2 yh[2][NTV + 1];
     float acc_ls1 = 0.f, acc_ls2 = 0.f, acc_rs = 0.f, acc_l1 = 0.f, acc_tv = 0.f, acc_nnz = 0.f;
 
-    // Row tau of the stream; PAR = (tau - tau0) & 1.
-    template <int PAR>
-    __device__ __forceinline__ void step(const ProxDev& a, float2 (*ring)[P][64], int lane, int slot, int tau,
-                                         int r0, int r1, const size_t plane_off, int xcol, int ocol, bool out_lane) {
+    // Step S of an unrolled block: row tau, ring slot S mod P, row parity S & 1.
+    template <int S>
+    __device__ __forceinline__ void step(const ProxDev& a, float2 (*ring)[P][64], int lane, int tau, int r0, int r1,
+                                         size_t plane_off, int xcol, int ocol, bool out_lane) {
+        constexpr int SLOT = S % P;
+        constexpr int PAR = S & 1;
         constexpr int Q = PAR ^ 1;
         const int W = a.W, H = a.H;
         // ---- row tau: momentum point x0 = y0 + s (y0 - ym), x-hat = x0 - t g ----
@@ -108,18 +116,20 @@ template <int NTV> struct ProxWave {
             int row = tau % H;
             row = row < 0 ? row + H : row;
             const size_t ro = plane_off + (size_t)row * W;
-            const float2 y0 = a.Y0[ro + xcol], ym = a.YM[ro + xcol], g = a.G[ro + xcol];
+            const float2* __restrict__ py0 = a.Y0 + ro;
+            const float2* __restrict__ pym = a.YM + ro;
+            const float2* __restrict__ pg = a.G + ro;
+            const float2 y0 = py0[xcol], ym = pym[xcol], g = pg[xcol];
             const float2 x0 = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
-            ring[0][slot][lane] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
-            ring[1][slot][lane] = x0;
-            ring[2][slot][lane] = g;
-            ring[3][slot][lane] = y0;
+            ring[0][SLOT][lane] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
+            ring[1][SLOT][lane] = x0;
+            ring[2][SLOT][lane] = g;
+            ring[3][SLOT][lane] = y0;
         }
         // ---- TV levels: level k works on row tau - k ----
 #pragma unroll
         for (int k = 1; k <= NTV; ++k) {
-            int sk = slot - k;
-            sk += sk < 0 ? P : 0;
+            const int sk = (SLOT - k + 2 * P) % P;
             const float2 xh = ring[0][sk][lane];
             float2 y;
             float4 go;  // g_{k-1} at row tau - k
@@ -139,17 +149,16 @@ template <int NTV> struct ProxWave {
             const float ayr = fmaf(a.sc, yu.x - y.x, go.z);
             const float ayi = fmaf(a.sc, yu.y - y.y, go.w);
             const float n2 = fmaf(axr, axr, fmaf(axi, axi, fmaf(ayr, ayr, ayi * ayi)));
-            const float inv = n2 > 1.0f ? __builtin_amdgcn_rsqf(n2) : 1.0f;
+            // projection: divide by max(|a|, 1)
+            const float inv = __builtin_amdgcn_rsqf(fmaxf(n2, 1.0f));
             gd[PAR][k] = make_float4(axr * inv, axi * inv, ayr * inv, ayi * inv);
             yh[PAR][k] = y;
         }
         // ---- final estimate, threshold and reductions: row o = tau - NTV - 1 ----
         const int o = tau - NTV - 1;
-        int so = slot - NTV - 1;
-        so += so < 0 ? P : 0;
-        int su = so - 1;
-        su += su < 0 ? P : 0;
-        const float2 xh = ring[0][so][lane];
+        constexpr int SO = (SLOT - NTV - 1 + 2 * P) % P;
+        constexpr int SU = (SO - 1 + P) % P;
+        const float2 xh = ring[0][SO][lane];
         float2 yv;
         if constexpr (NTV == 0) {
             yv = xh;
@@ -161,14 +170,15 @@ template <int NTV> struct ProxWave {
             yv = make_float2(fmaf(-a.mu, dvr, xh.x), fmaf(-a.mu, dvi, xh.y));
         }
         if (a.do_soft) yv = make_float2(soft_fast(yv.x, a.thr), soft_fast(yv.y, a.thr));
-        const float2 x0o = ring[1][so][lane];
+        const float2 x0o = ring[1][SO][lane];
         const float x0lr = lane_from_left(x0o.x);
         const float x0li = lane_from_left(x0o.y);
         if (o >= r0 && o < r1 && out_lane) {
-            a.Y[plane_off + (size_t)o * W + ocol] = yv;
-            const float2 x0u = ring[1][su][lane];
-            const float2 g = ring[2][so][lane];
-            const float2 y0 = ring[3][so][lane];
+            float2* __restrict__ py = a.Y + plane_off + (size_t)o * W;
+            py[ocol] = yv;
+            const float2 x0u = ring[1][SU][lane];
+            const float2 g = ring[2][SO][lane];
+            const float2 y0 = ring[3][SO][lane];
             const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
             acc_ls1 = fmaf(dr, g.x, fmaf(di, g.y, acc_ls1));
             acc_ls2 = fmaf(dr, dr, fmaf(di, di, acc_ls2));
@@ -183,15 +193,31 @@ template <int NTV> struct ProxWave {
             acc_tv += __builtin_sqrtf(fmaf(gxx, gxx, fmaf(gxy, gxy, fmaf(gyx, gyx, gyy * gyy))));
         }
     }
+
+    template <int... S>
+    __device__ __forceinline__ void block(std::integer_sequence<int, S...>, const ProxDev& a, float2 (*ring)[P][64],
+                                          int lane, int tau, int r0, int r1, size_t plane_off, int xcol, int ocol,
+                                          bool out_lane) {
+        (step<S>(a, ring, lane, tau + S, r0, r1, plane_off, xcol, ocol, out_lane), ...);
+    }
+    template <int... S>
+    __device__ __forceinline__ void tail(std::integer_sequence<int, S...>, int rem, const ProxDev& a,
+                                         float2 (*ring)[P][64], int lane, int tau, int r0, int r1, size_t plane_off,
+                                         int xcol, int ocol, bool out_lane) {
+        ((S < rem ? step<S>(a, ring, lane, tau + S, r0, r1, plane_off, xcol, ocol, out_lane) : void()), ...);
+    }
 };
 
 template <int NTV>
 __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     using PW = ProxWave<NTV>;
     constexpr int P = PW::P;
+    constexpr int U = PW::U;
     __shared__ float2 ring_s[kProxWaves][4][P][64];  // fields: x-hat, x0, g, y_{k-1}
     const int lane = threadIdx.x & 63;
-    const int wl = threadIdx.x >> 6;
+    // wave index, made wave-uniform for the compiler so all row/plane
+    // arithmetic below stays in scalar registers
+    const int wl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int w = blockIdx.x * kProxWaves + wl;
     if (w >= a.nwaves) return;  // whole wave exits; no barriers below
     const ProxGeom geo = a.geo;
@@ -219,15 +245,12 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
         }
     const int tau0 = r0 - (NTV > 0 ? NTV : 1);
     const int tau1 = r1 + NTV;  // inclusive
-    int slot = 0;
     int tau = tau0;
-    for (; tau + 1 <= tau1; tau += 2) {
-        st.template step<0>(a, ring, lane, slot, tau, r0, r1, plane_off, xcol, ocol, out_lane);
-        slot = slot + 1 == P ? 0 : slot + 1;
-        st.template step<1>(a, ring, lane, slot, tau + 1, r0, r1, plane_off, xcol, ocol, out_lane);
-        slot = slot + 1 == P ? 0 : slot + 1;
-    }
-    if (tau <= tau1) st.template step<0>(a, ring, lane, slot, tau, r0, r1, plane_off, xcol, ocol, out_lane);
+    for (; tau + U - 1 <= tau1; tau += U)
+        st.block(std::make_integer_sequence<int, U>{}, a, ring, lane, tau, r0, r1, plane_off, xcol, ocol, out_lane);
+    if (tau <= tau1)
+        st.tail(std::make_integer_sequence<int, U - 1>{}, tau1 - tau + 1, a, ring, lane, tau, r0, r1, plane_off, xcol,
+                ocol, out_lane);
 
     const double s_ls1 = wave_sum((double)st.acc_ls1);
     const double s_ls2 = wave_sum((double)st.acc_ls2);
@@ -258,7 +281,7 @@ static ProxDev make_dev(const ProxLaunch& a) {
     d.sc = a.reg_tv != 0.0 ? (float)(1 / (4 * 3 * a.t * a.reg_tv)) : 0.0f;
     d.thr = (float)(a.t * a.reg);
     d.do_soft = a.reg > 0.0;
-    d.geo = prox_geom(a.W, a.H, a.n_tv);
+    d.geo = prox_geom(a.W, a.H, a.n_tv, a.row_chunk);
     d.nwaves = d.geo.nstrips * d.geo.nchunks * a.nz;
     d.partial = a.partial;
     return d;

@@ -47,11 +47,11 @@ enum Slot { SL_PROX = 0, SL_RESID_A = 8, SL_RESID_M = 9, SL_COUNT = 16 };
 // Kernels timed with HIP events when timing is enabled (the reference's
 // DECLARE_TIMING / SAVE_TIMING phases, src/core/umnholo.h:79-124).
 enum KernelId {
-    K_ROWS_FWD = 0, K_COLS_FWDACC, K_ROWS_SUMPARTS, K_COLS_REAL, K_PROX, K_HOLO_MOM, K_ROWS_PLAIN,
+    K_COLS_FWD = 0, K_ROWS_FWDACC, K_ROWS_SUMPARTS, K_COLS_REAL, K_PROX, K_HOLO_MOM, K_ROWS_PLAIN,
     K_COLS_PLAIN, K_ROWS_ADJ, K_COLS_ADJ, K_REDUCE, K_MOM_OUT, K_COUNT
 };
 const char* const kKernelNames[K_COUNT] = {
-    "fft_rows_fwd", "fft_cols_fwdacc", "fft_rows_sumparts", "fft_cols_real", "prox", "holo_momentum",
+    "fft_cols_fwd", "fft_rows_fwdacc", "fft_rows_sumparts", "fft_cols_real", "prox", "holo_momentum",
     "fft_rows_plain", "fft_cols_plain", "fft_rows_adj", "fft_cols_adj", "reduce_partials", "momentum_out"};
 
 struct Timer {
@@ -107,6 +107,7 @@ struct rihvr_ctx {
     int* dflag = nullptr;
     int zchunk_adj = 8, zchunk_fwd = 8, nparts = 1;
     int plane_group = 1;  // planes per row+column FFT pass group
+    int prox_rows = 0;    // output rows per prox wave (0: default)
     bool have_result = false;
     bool force_step = false;
     double forced_step = 0;
@@ -219,11 +220,11 @@ rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
         const int gz = std::min(c->plane_group, c->Nz - g0);
         const float2* vg = vol + (size_t)g0 * c->N;
         float2* spg = c->spart + (size_t)(g0 / c->zchunk_fwd) * c->N;
-        RIHVR_TRY(timed(c, K_ROWS_FWD, [&] {
-            return fft_rows_fwd(c->W, vg, c->phase + g0, c->T, c->H, gz, c->twfW, c->stream);
+        RIHVR_TRY(timed(c, K_COLS_FWD, [&] {
+            return fft_cols_fwd(c->H, vg, c->phase + g0, c->T, c->W, gz, c->twfH, c->stream);
         }));
-        RIHVR_TRY(timed(c, K_COLS_FWDACC, [&] {
-            return fft_cols_fwdacc(c->H, c->T, c->kappa, c->zlist + g0, spg, c->W, gz, c->zchunk_fwd, invN, c->twfH,
+        RIHVR_TRY(timed(c, K_ROWS_FWDACC, [&] {
+            return fft_rows_fwdacc(c->W, c->T, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, invN, c->twfW,
                                    c->stream);
         }));
     }
@@ -268,11 +269,12 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
     a.reg = (double)c->p.regularization_param;
     a.reg_tv = (double)c->p.regularization_TV_param;
     a.n_tv = c->p.num_tv_iterations;
+    a.row_chunk = c->prox_rows;
     a.partial = c->partial;
     int rows;
     if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) {
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
-        rows = prox_fl2d_num_waves(c->W, c->H, c->Nz, a.n_tv);
+        rows = prox_fl2d_num_waves(c->W, c->H, c->Nz, a.n_tv, c->prox_rows);
     } else {
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_l1(a, c->stream); }));
         rows = prox_l1_num_blocks(c->W, c->H, c->Nz);
@@ -313,7 +315,10 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     c->V = c->N * c->Nz;
     RIHVR_HIP(hipGetDevice(&c->device));
     RIHVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->zchunk_fwd = std::min(c->Nz, 8);
+    int zc = 16;
+    if (const char* e = std::getenv("RIHVR_ZCHUNK")) zc = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("RIHVR_PROX_ROWS")) c->prox_rows = std::max(0, std::atoi(e));
+    c->zchunk_fwd = std::min(c->Nz, zc);
     c->nparts = (c->Nz + c->zchunk_fwd - 1) / c->zchunk_fwd;
     c->zchunk_adj = std::min(c->Nz, 8);
     // plane group: all planes, or RIHVR_PLANE_GROUP (rounded to a multiple of the z chunk)
@@ -338,7 +343,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_TRY(dalloc(&c->twiW, c->W));
     RIHVR_TRY(dalloc(&c->twfH, c->H));
     RIHVR_TRY(dalloc(&c->twiH, c->H));
-    long rows = std::max<long>(prox_fl2d_num_waves(c->W, c->H, c->Nz, std::min(params->num_tv_iterations, 10)),
+    long rows = std::max<long>(prox_fl2d_num_waves(c->W, c->H, c->Nz, std::min(params->num_tv_iterations, 10), c->prox_rows),
                                prox_l1_num_blocks(c->W, c->H, c->Nz));
     rows = std::max<long>(rows, momentum_out_num_blocks(c->W, c->H, c->Nz));
     c->partial_rows = rows;

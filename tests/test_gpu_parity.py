@@ -212,7 +212,7 @@ def test_stepwise_equals_reconstruct(pkg, scenes, gpu_available):
     np.testing.assert_allclose(np.concatenate([s1["f"], s2["f"]]), st["f"], rtol=1e-6)
     assert list(np.concatenate([s1["restarts"], s2["restarts"]])) == list(st["restarts"])
     assert tm["prox"][0] >= 4 and tm["prox"][1] > 0
-    assert tm["fft_cols_fwdacc"][0] >= 4
+    assert tm["fft_rows_fwdacc"][0] >= 4
 
 
 def test_iterate_without_begin_is_invalid_state(pkg, gpu_available):
