@@ -15,7 +15,9 @@ template <int L> struct RowCfg {
     static constexpr int TPT = FftPlan<L>::TPT;
     static constexpr int RPW = (256 / TPT) > 0 ? (256 / TPT) : 1;  // rows per workgroup
     static constexpr int THREADS = TPT * RPW;
-    static constexpr int LP = L + 1;  // padded LDS row pitch (float2)
+    // LDS row: element j at j + j/16 (one pad slot per 16 elements breaks the
+    // 16-element store stride of a radix-16 pass across the banks)
+    static constexpr int LP = L + L / 16 + 1;
 };
 template <int L> struct ColCfg {
     static constexpr int TPT = FftPlan<L>::TPT;
@@ -59,7 +61,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
     const int tr = threadIdx.x / C::TPT;
     const int t = threadIdx.x - tr * C::TPT;
     float2* mybuf = buf + tr * C::LP;
-    auto lds = [&](int j) -> float2& { return mybuf[j]; };
+    auto lds = [&](int j) -> float2& { return mybuf[j + (j >> 4)]; };
     const int row = blockIdx.x * C::RPW + tr;  // row within a plane (ADJ/FWD) or global row
     const bool valid = row < a.nrows;
     const int rrow = valid ? row : a.nrows - 1;
