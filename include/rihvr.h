@@ -103,6 +103,7 @@ typedef struct rihvr_stats {
     int32_t *backtracks;       /* backtracks taken in iteration it (<= 2, :423) */
     int32_t *restarts;         /* 1 when the restart test fired in iteration it (:471-475) */
     double  *restart_sum;      /* the restart reduction value (:4481) */
+    double  *sparsity_it;      /* nnz / (W*H*Nz) of the momentum point after iteration it (:563) */
 } rihvr_stats;
 
 typedef struct rihvr_ctx rihvr_ctx;
@@ -165,6 +166,9 @@ rihvr_status rihvr_set_timing(rihvr_ctx* ctx, int32_t enable);
 rihvr_status rihvr_get_timing(rihvr_ctx* ctx, rihvr_kernel_time* out, int32_t capacity,
                               int32_t* count);
 
+/* The parameter set the context was created with. */
+rihvr_status rihvr_get_params(const rihvr_ctx* ctx, rihvr_params* out);
+
 /* Copies the reconstructed volume ([Nz][H][W] complex64) out.  Replaces reading
  * x1 through SparseVolume::getPlane for every plane / saveSparse (:874-879). */
 rihvr_status rihvr_get_volume(rihvr_ctx* ctx, float* out, int out_loc);
@@ -178,6 +182,14 @@ rihvr_status rihvr_get_plane(rihvr_ctx* ctx, int32_t z, float* out, int out_loc)
  * (:247): the forced step only skips the Lipschitz estimate, whose result the
  * reference discards.  Kept for API parity. */
 rihvr_status rihvr_set_stepsize(rihvr_ctx* ctx, double step);
+
+/* Makes the current point of an active solve (the momentum point x1 after the
+ * last rihvr_iterate, and its estimated hologram and residual) readable through
+ * rihvr_get_volume / rihvr_get_plane / rihvr_get_estimate / rihvr_save_*, without
+ * ending the solve; the next rihvr_iterate or rihvr_finish supersedes it.  The
+ * reference's per-iteration outputs with output_steps set read x1 there
+ * (sparse_compressive_holo.cu:571-597).  RIHVR_E_INVALID_STATE without an active solve. */
+rihvr_status rihvr_snapshot(rihvr_ctx* ctx);
 
 /* Final estimated hologram and residual of the last solve ([H][W] complex64),
  * the data the reference writes to estimate_final.tif / residual_final.tif
@@ -202,6 +214,88 @@ rihvr_status rihvr_adjoint(rihvr_ctx* ctx, const float* holo, int holo_loc,
  * All volumes [Nz][H][W] complex64 at loc. */
 rihvr_status rihvr_prox(rihvr_ctx* ctx, const float* x0, const float* grad,
                         double stepsize, float* out, int loc);
+
+/* ------------------------------------------------------------------------
+ * Host-side input and output of the reconstruction CLI (no device work).
+ * ------------------------------------------------------------------------ */
+
+/* The parameter file of the reference CLI: lines "name id value"; lines that start
+ * with '%' or '#' are comments (src/core/holo_ui.cpp:171-486).  Defaults are
+ * those of HoloUi::HoloUi (holo_ui.cpp:10-62); mode is 7 (main_sparse_inverse.cu:138). */
+#define RIHVR_PATH_MAX 4096
+typedef struct rihvr_file_params {
+    rihvr_params solver;              /* width/height stay 0: they come from the image */
+    int32_t image_count;              /* 1001 */
+    int32_t start_image;              /* 1002 */
+    int32_t background_start;         /* 1003 */
+    int32_t background_end;           /* 1004 */
+    char    image_filename[RIHVR_PATH_MAX];       /* 2001, else 2002 + 2003; a printf format */
+    char    image_path[RIHVR_PATH_MAX];           /* 2002 */
+    char    image_formatted_name[RIHVR_PATH_MAX]; /* 2003 */
+    char    output_path[RIHVR_PATH_MAX];          /* 2004 */
+    char    background_filename[RIHVR_PATH_MAX];  /* 2005 (sets remove_background) */
+    int32_t remove_background;
+    float   center_x, center_y, center_z; /* 3006, 3007, 3008 */
+    int32_t roi_size;                 /* 3009 */
+    int32_t roi_x, roi_y, roi_w, roi_h; /* 3011..3014 */
+    int32_t output_planes;            /* 7001 */
+    int32_t output_steps;             /* 7005 */
+    int32_t unknown_id;               /* id of the line that stopped parsing, else 0 */
+} rihvr_file_params;
+
+/* Replaces HoloUi::readFile (holo_ui.cpp:171-486).  As in the reference, parsing
+ * stops at the first unknown id and the values read so far are kept: the call then
+ * returns RIHVR_E_INVALID_FILE with out->unknown_id set.  A missing file:
+ * RIHVR_E_BAD_FILENAME. */
+rihvr_status rihvr_read_param_file(const char* path, rihvr_file_params* out);
+
+/* Reads a grayscale image as raw values (Hologram::read, hologram.cu:139-199 via
+ * cv::imread): uncompressed baseline TIFF (8/16-bit gray, 8-bit RGB converted with
+ * BGR->gray weights, strips, either byte order) or binary PGM (P5, 8/16-bit).
+ * Call with pixels == NULL to get the size.  RIHVR_E_BAD_FILENAME if unreadable,
+ * RIHVR_E_UNKNOWN_TYPE for an unsupported encoding. */
+rihvr_status rihvr_read_image(const char* path, float* pixels, int32_t* width, int32_t* height);
+
+/* The input contract of main_sparse_inverse.cu:101-124: rescale to 0..1 (/255),
+ * then I/mean(I) - 1 (normalize_mean0, hologram.cu:332-358), or with a background
+ * (I - B) / max(sqrt(B), 0.1) (removeBackground, hologram.cu:1365-1432); crop
+ * (ROI rectangle, else centre/size; hologram.cu:768-880) and zero-pad by
+ * fp->solver.zero_padding (hologram.cu:882-945).  Output: [H'][W'] complex64 with
+ * zero imaginary part; call with out == NULL for the size.  The reference's
+ * zeroPad allocates rows/cols swapped (hologram.cu:905), so only square images
+ * pass it: a non-square crop returns RIHVR_E_INVALID_ARGUMENT. */
+rihvr_status rihvr_prepare_hologram(const rihvr_file_params* fp, const float* image,
+                                    const float* background, int32_t width, int32_t height,
+                                    float* out, int32_t* out_width, int32_t* out_height);
+
+/* 8-bit grayscale uncompressed TIFF writer (the reference writes with cv::imwrite). */
+rihvr_status rihvr_write_tiff8(const char* path, const uint8_t* pixels, int32_t width,
+                               int32_t height);
+
+/* Hologram::write (hologram.cu:1587-1652): the real part of a [H][W] complex64
+ * hologram as 8-bit; scale_to_minmax != 0 stretches min..max to 0..255 (enh.tif),
+ * else 255 * (v - lo) / (hi - lo) (estimate_final.tif, residual_final.tif). */
+rihvr_status rihvr_write_hologram(const char* path, const float* holo, int32_t width,
+                                  int32_t height, int32_t scale_to_minmax, double lo, double hi);
+
+/* Outputs of the last solve (SparseVolume::saveData, sparse_volume.cu:518-560):
+ * "x, y, z, value" rows with value = re^2 + im^2 (%e) for every non-zero voxel,
+ * in z, then y*W + x order (the reference's order is that of its atomic compaction). */
+rihvr_status rihvr_save_sparse(rihvr_ctx* ctx, const char* csv_path);
+
+/* SparseVolume::saveProjections (sparse_volume.cu:684-813), MAX method: magnitude
+ * normalised by its global min/max; xy (planes 1.. only, as the reference), xz and
+ * yz maximum projections.  File names as the reference: <output_path><prefix>_xy.tif
+ * (no separator is inserted), or with prefix_as_suffix <output_path>xy_<prefix>.tif;
+ * likewise _xz and _yz.  A file that cannot be written: RIHVR_E_UNKNOWN_ERROR (the
+ * reference's imwrite check). */
+rihvr_status rihvr_save_projections(rihvr_ctx* ctx, const char* output_path, const char* prefix,
+                                    int32_t prefix_as_suffix);
+
+/* SparseCompressiveHolo::savePlanes (sparse_compressive_holo.cu:1039-1194):
+ * <prefix>_%04d.tif (magnitude), _phase_%04d.tif, _real_%04d.tif, _imag_%04d.tif.
+ * A file that cannot be written: RIHVR_E_UNKNOWN_ERROR. */
+rihvr_status rihvr_save_planes(rihvr_ctx* ctx, const char* path_prefix);
 
 #ifdef __cplusplus
 }

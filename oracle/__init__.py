@@ -57,6 +57,7 @@ class Stats(ctypes.Structure):
         ("backtracks", _P(ctypes.c_int32)),
         ("restarts", _P(ctypes.c_int32)),
         ("restart_sum", _P(ctypes.c_double)),
+        ("sparsity_it", _P(ctypes.c_double)),
     ]
 
 
@@ -81,10 +82,12 @@ class StatsBuffers:
         self.backtracks = np.zeros(n, np.int32)
         self.restarts = np.zeros(n, np.int32)
         self.restart_sum = np.zeros(n, np.float64)
+        self.sparsity_it = np.zeros(n, np.float64)
         d = lambda a: a.ctypes.data_as(_P(ctypes.c_double))  # noqa: E731
         i = lambda a: a.ctypes.data_as(_P(ctypes.c_int32))  # noqa: E731
         self.stats = Stats(0, 0, 0.0, 0.0, 0.0, 0.0, 0, d(self.f), d(self.f_resid), d(self.ratio),
-                           d(self.lim), i(self.backtracks), i(self.restarts), d(self.restart_sum))
+                           d(self.lim), i(self.backtracks), i(self.restarts), d(self.restart_sum),
+                           d(self.sparsity_it))
 
     def as_dict(self) -> dict:
         s = self.stats

@@ -37,7 +37,11 @@ STATUS_NAMES = {
 EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_destroy",
             "rihvr_reconstruct", "rihvr_begin", "rihvr_iterate", "rihvr_finish", "rihvr_set_timing",
             "rihvr_get_timing", "rihvr_get_volume", "rihvr_get_plane", "rihvr_set_stepsize",
-            "rihvr_get_estimate", "rihvr_forward", "rihvr_adjoint", "rihvr_prox")
+            "rihvr_get_estimate", "rihvr_forward", "rihvr_adjoint", "rihvr_prox", "rihvr_get_params",
+            "rihvr_snapshot", "rihvr_read_param_file", "rihvr_read_image", "rihvr_prepare_hologram",
+            "rihvr_write_tiff8", "rihvr_write_hologram", "rihvr_save_sparse", "rihvr_save_projections",
+            "rihvr_save_planes")
+PATH_MAX = 4096
 
 
 class RihvrError(RuntimeError):
@@ -87,6 +91,7 @@ class Stats(ctypes.Structure):
         ("backtracks", _P(ctypes.c_int32)),
         ("restarts", _P(ctypes.c_int32)),
         ("restart_sum", _P(ctypes.c_double)),
+        ("sparsity_it", _P(ctypes.c_double)),
     ]
 
 
@@ -94,6 +99,35 @@ class KernelTime(ctypes.Structure):
     """``rihvr_kernel_time`` (include/rihvr.h)."""
 
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double)]
+
+
+class FileParams(ctypes.Structure):
+    """Mirror of ``rihvr_file_params`` (include/rihvr.h): the parameter file."""
+
+    _fields_ = [
+        ("solver", Params),
+        ("image_count", ctypes.c_int32),
+        ("start_image", ctypes.c_int32),
+        ("background_start", ctypes.c_int32),
+        ("background_end", ctypes.c_int32),
+        ("image_filename", ctypes.c_char * PATH_MAX),
+        ("image_path", ctypes.c_char * PATH_MAX),
+        ("image_formatted_name", ctypes.c_char * PATH_MAX),
+        ("output_path", ctypes.c_char * PATH_MAX),
+        ("background_filename", ctypes.c_char * PATH_MAX),
+        ("remove_background", ctypes.c_int32),
+        ("center_x", ctypes.c_float),
+        ("center_y", ctypes.c_float),
+        ("center_z", ctypes.c_float),
+        ("roi_size", ctypes.c_int32),
+        ("roi_x", ctypes.c_int32),
+        ("roi_y", ctypes.c_int32),
+        ("roi_w", ctypes.c_int32),
+        ("roi_h", ctypes.c_int32),
+        ("output_planes", ctypes.c_int32),
+        ("output_steps", ctypes.c_int32),
+        ("unknown_id", ctypes.c_int32),
+    ]
 
 
 def make_params(width, height, num_planes, start_plane, plane_stepsize, wavelength, resolution,
@@ -115,10 +149,12 @@ class StatsBuffers:
         self.backtracks = np.zeros(n, np.int32)
         self.restarts = np.zeros(n, np.int32)
         self.restart_sum = np.zeros(n, np.float64)
+        self.sparsity_it = np.zeros(n, np.float64)
         d = lambda a: a.ctypes.data_as(_P(ctypes.c_double))  # noqa: E731
         i = lambda a: a.ctypes.data_as(_P(ctypes.c_int32))  # noqa: E731
         self.stats = Stats(0, 0, 0.0, 0.0, 0.0, 0.0, 0, d(self.f), d(self.f_resid), d(self.ratio),
-                           d(self.lim), i(self.backtracks), i(self.restarts), d(self.restart_sum))
+                           d(self.lim), i(self.backtracks), i(self.restarts), d(self.restart_sum),
+                           d(self.sparsity_it))
 
     def as_dict(self) -> dict:
         s = self.stats
@@ -127,7 +163,8 @@ class StatsBuffers:
                     f_initial=s.f_initial, f_final=s.f_final, sparsity=s.sparsity, nnz=s.nnz,
                     f=self.f[:n].copy(), f_resid=self.f_resid[:n].copy(), ratio=self.ratio[:n].copy(),
                     lim=self.lim[:n].copy(), backtracks=self.backtracks[:n].copy(),
-                    restarts=self.restarts[:n].copy(), restart_sum=self.restart_sum[:n].copy())
+                    restarts=self.restarts[:n].copy(), restart_sum=self.restart_sum[:n].copy(),
+                    sparsity_it=self.sparsity_it[:n].copy())
 
 
 _lib = None
@@ -175,6 +212,29 @@ def lib():
         L.rihvr_adjoint.restype = st
         L.rihvr_prox.argtypes = [vp, vp, vp, ctypes.c_double, vp, ip]
         L.rihvr_prox.restype = st
+        L.rihvr_get_params.argtypes = [vp, _P(Params)]
+        L.rihvr_get_params.restype = st
+        L.rihvr_snapshot.argtypes = [vp]
+        L.rihvr_snapshot.restype = st
+        cp, i32p = ctypes.c_char_p, _P(ctypes.c_int32)
+        L.rihvr_read_param_file.argtypes = [cp, _P(FileParams)]
+        L.rihvr_read_param_file.restype = st
+        L.rihvr_read_image.argtypes = [cp, vp, i32p, i32p]
+        L.rihvr_read_image.restype = st
+        L.rihvr_prepare_hologram.argtypes = [_P(FileParams), vp, vp, ctypes.c_int32, ctypes.c_int32, vp, i32p,
+                                             i32p]
+        L.rihvr_prepare_hologram.restype = st
+        L.rihvr_write_tiff8.argtypes = [cp, vp, ctypes.c_int32, ctypes.c_int32]
+        L.rihvr_write_tiff8.restype = st
+        L.rihvr_write_hologram.argtypes = [cp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_double, ctypes.c_double]
+        L.rihvr_write_hologram.restype = st
+        L.rihvr_save_sparse.argtypes = [vp, cp]
+        L.rihvr_save_sparse.restype = st
+        L.rihvr_save_projections.argtypes = [vp, cp, cp, ctypes.c_int32]
+        L.rihvr_save_projections.restype = st
+        L.rihvr_save_planes.argtypes = [vp, cp]
+        L.rihvr_save_planes.restype = st
         _lib = L
     return _lib
 
@@ -182,6 +242,65 @@ def lib():
 def _check(rc: int, what: str):
     if rc != 0:
         raise RihvrError(rc, what)
+
+
+def _path(p) -> bytes:
+    return str(p).encode()
+
+
+# -- input and output of the reconstruction executable (host only) ----------
+def read_param_file(path):
+    """HoloUi::readFile (holo_ui.cpp:171-486).  Returns (FileParams, status):
+    status 0, RIHVR_E_INVALID_FILE (9) when parsing stopped at an unknown id
+    (FileParams.unknown_id; the values read before it are kept), or
+    RIHVR_E_BAD_FILENAME (2) when the file cannot be opened (defaults kept)."""
+    fp = FileParams()
+    rc = lib().rihvr_read_param_file(_path(path), ctypes.byref(fp))
+    return fp, int(rc)
+
+
+def read_image(path) -> np.ndarray:
+    """Raw grayscale values ([H][W] float32) of an uncompressed TIFF or PGM file."""
+    w, h = ctypes.c_int32(0), ctypes.c_int32(0)
+    _check(lib().rihvr_read_image(_path(path), None, ctypes.byref(w), ctypes.byref(h)), "rihvr_read_image")
+    out = np.empty((h.value, w.value), np.float32)
+    _check(lib().rihvr_read_image(_path(path), ctypes.c_void_p(out.ctypes.data), ctypes.byref(w),
+                                  ctypes.byref(h)), "rihvr_read_image")
+    return out
+
+
+def prepare_hologram(fp, image, background=None) -> np.ndarray:
+    """The executable's input contract (main_sparse_inverse.cu:101-124): [H'][W'] complex64."""
+    img = np.ascontiguousarray(image, np.float32)
+    bg = None if background is None else np.ascontiguousarray(background, np.float32)
+    if bg is not None and bg.shape != img.shape:
+        raise ValueError("background must have the image's shape")
+    h, w = img.shape
+    bp = None if bg is None else ctypes.c_void_p(bg.ctypes.data)
+    ow, oh = ctypes.c_int32(0), ctypes.c_int32(0)
+    _check(lib().rihvr_prepare_hologram(ctypes.byref(fp), ctypes.c_void_p(img.ctypes.data), bp, w, h, None,
+                                        ctypes.byref(ow), ctypes.byref(oh)), "rihvr_prepare_hologram")
+    out = np.empty((oh.value, ow.value), np.complex64)
+    _check(lib().rihvr_prepare_hologram(ctypes.byref(fp), ctypes.c_void_p(img.ctypes.data), bp, w, h,
+                                        ctypes.c_void_p(out.ctypes.data), ctypes.byref(ow), ctypes.byref(oh)),
+           "rihvr_prepare_hologram")
+    return out
+
+
+def write_tiff8(path, pixels):
+    px = np.ascontiguousarray(pixels, np.uint8)
+    h, w = px.shape
+    _check(lib().rihvr_write_tiff8(_path(path), ctypes.c_void_p(px.ctypes.data), w, h), "rihvr_write_tiff8")
+
+
+def write_hologram(path, holo, lo=None, hi=None):
+    """Hologram::write: min/max stretch of the real part, or 255 (v - lo) / (hi - lo)."""
+    a = np.ascontiguousarray(holo, np.complex64)
+    h, w = a.shape
+    minmax = lo is None
+    _check(lib().rihvr_write_hologram(_path(path), ctypes.c_void_p(a.ctypes.data), w, h, 1 if minmax else 0,
+                                      0.0 if minmax else float(lo), 0.0 if minmax else float(hi)),
+           "rihvr_write_hologram")
 
 
 def _is_torch(a) -> bool:
@@ -294,6 +413,26 @@ class SparseCompressiveHolo:
         p, loc, _k = _arg(out, self.shape_holo, writable=True)
         _check(lib().rihvr_get_plane(self._ctx, int(z), p, loc), "rihvr_get_plane")
         return out
+
+    def getParams(self) -> Params:
+        p = Params()
+        _check(lib().rihvr_get_params(self._ctx, ctypes.byref(p)), "rihvr_get_params")
+        return p
+
+    def snapshot(self):
+        """Makes the current momentum point of a stepwise solve readable (getVolume,
+        getPlane, getEstimate, save*) without ending the solve."""
+        _check(lib().rihvr_snapshot(self._ctx), "rihvr_snapshot")
+
+    def saveSparse(self, csv_path):
+        _check(lib().rihvr_save_sparse(self._ctx, _path(csv_path)), "rihvr_save_sparse")
+
+    def saveProjections(self, output_path, prefix, prefix_as_suffix=False):
+        _check(lib().rihvr_save_projections(self._ctx, _path(output_path), _path(prefix),
+                                            1 if prefix_as_suffix else 0), "rihvr_save_projections")
+
+    def savePlanes(self, prefix):
+        _check(lib().rihvr_save_planes(self._ctx, _path(prefix)), "rihvr_save_planes")
 
     def setStepsize(self, step: float):
         _check(lib().rihvr_set_stepsize(self._ctx, float(step)), "rihvr_set_stepsize")

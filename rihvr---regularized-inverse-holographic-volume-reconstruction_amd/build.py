@@ -1,8 +1,11 @@
-"""Builds librihvr.so (HIP, gfx950) in-tree: <package>/_build/librihvr.so.
+"""Builds librihvr.so (HIP, gfx950) and the sparse-inverse-recon executable in-tree.
 
-Every .hip source under csrc/ is compiled with hipcc for gfx950 only and the
+Every .hip source under csrc/ is compiled with hipcc for gfx950 only, the
+host-only .cpp sources (input/output) with the host C++ compiler, and the
 objects are linked into one shared library exporting the C ABI of
-include/rihvr.h.  Packed FP32 instructions are disabled for all device code.
+include/rihvr.h: <package>/_build/librihvr.so.  Packed FP32 instructions are
+disabled for all device code.  csrc/cli_main.cpp is linked against that
+library as <package>/_build/sparse-inverse-recon (runpath $ORIGIN).
 """
 from __future__ import annotations
 
@@ -18,6 +21,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 LIB = OUT / "librihvr.so"
+CLI = OUT / "sparse-inverse-recon"
+CLI_MAIN = "cli_main.cpp"
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -29,18 +34,28 @@ CXXFLAGS = [
 ]
 
 
+HOST_CXX = os.environ.get("CXX", "g++")
+HOST_CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROOT / 'include'}"]
+ROCM_LIB = pathlib.Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib"
+
+
 def sources():
-    return sorted(CSRC.glob("*.hip"))
+    hip = sorted(CSRC.glob("*.hip"))
+    host = sorted(p for p in CSRC.glob("*.cpp") if p.name != CLI_MAIN)
+    return hip + host
 
 
 def _compile(src: pathlib.Path, verbose: bool) -> pathlib.Path:
     obj = OUT / (src.stem + ".o")
-    cmd = [HIPCC, *CXXFLAGS, "-c", str(src), "-o", str(obj)]
+    if src.suffix == ".hip":
+        cmd = [HIPCC, *CXXFLAGS, "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [HOST_CXX, *HOST_CXXFLAGS, "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+        raise RuntimeError(f"compile failed for {src.name}:\n{r.stdout}\n{r.stderr}")
     if r.stderr.strip() and verbose:
         print(r.stderr, file=sys.stderr)
     return obj
@@ -62,6 +77,15 @@ def build(verbose: bool = False, jobs: int | None = None) -> pathlib.Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    tmp = CLI.with_name(CLI.name + ".tmp")
+    cmd = [HOST_CXX, *HOST_CXXFLAGS, str(CSRC / CLI_MAIN), "-o", str(tmp), f"-L{OUT}", "-lrihvr",
+           f"-L{ROCM_LIB}", f"-Wl,-rpath-link,{ROCM_LIB}", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of {CLI.name} failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CLI)
     return LIB
 
 

@@ -467,6 +467,7 @@ rihvr_status momentum_objective(rihvr_ctx* c, double summag, double* f) {
 rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_by_finish) {
     if (!c->ss.active) return RIHVR_E_INVALID_STATE;
     if (n_it < 0) return RIHVR_E_INVALID_ARGUMENT;
+    if (n_it > 0) c->have_result = false;  // a snapshot lives in the ring slot the prox writes
     SolveState& s = c->ss;
     const int mode = s.mode;
     for (int i = 0; i < n_it; ++i) {
@@ -486,6 +487,7 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
                 // objective there from this prox's x0 reductions
                 s.prev_summag = sc[SL_RESID_M];
                 if (i > 0 && st) {
+                    if (st->sparsity_it) st->sparsity_it[i - 1] = sc[SL_PROX + PS_NNZX0] / (double)c->V;
                     if (st->f_resid) st->f_resid[i - 1] = 0.5 * s.prev_summag;
                     if (st->f)
                         st->f[i - 1] = objective(c, mode, s.prev_summag, sc[SL_PROX + PS_L1X0],
@@ -538,13 +540,14 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
         st->total_backtracks = s.total_bt;
         st->stepsize = s.t;
         st->f_initial = s.f_initial;
-        if (n_it > 0 && !last_f_by_finish && (st->f || st->f_resid)) {
+        if (n_it > 0 && !last_f_by_finish && (st->f || st->f_resid || st->sparsity_it)) {
             RIHVR_TRY(fetch_scalars(c));
             const double summag = c->h_scal[SL_RESID_M];
             double f = 0;
             RIHVR_TRY(momentum_objective(c, summag, &f));
             if (st->f) st->f[n_it - 1] = f;
             if (st->f_resid) st->f_resid[n_it - 1] = 0.5 * summag;
+            if (st->sparsity_it) st->sparsity_it[n_it - 1] = c->h_scal[SL_PROX + PS_NNZX0] / (double)c->V;
         }
     }
     return RIHVR_OK;
@@ -575,6 +578,7 @@ rihvr_status finish_impl(rihvr_ctx* c, rihvr_stats* st, int last_index) {
         if (last_index >= 0) {
             if (st->f) st->f[last_index] = f_last;
             if (st->f_resid) st->f_resid[last_index] = 0.5 * summag;
+            if (st->sparsity_it) st->sparsity_it[last_index] = sc[SL_PROX + PS_NNZX0] / (double)c->V;
         }
         st->total_backtracks = s.total_bt;
         st->stepsize = s.t;
@@ -583,6 +587,20 @@ rihvr_status finish_impl(rihvr_ctx* c, rihvr_stats* st, int last_index) {
         st->nnz = (int64_t)llround(sc[SL_PROX + PS_NNZX0]);
         st->sparsity = (double)st->nnz / (double)c->V;
     }
+    return RIHVR_OK;
+}
+
+// x1 = y_K + s (y_K - y_{K-1}) into the free ring slot; the solve stays active
+rihvr_status snapshot_impl(rihvr_ctx* c) {
+    if (!c->ss.active) return RIHVR_E_INVALID_STATE;
+    float2* outb = c->Yb[c->iy];
+    RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
+        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], c->ss.s_mom, outb, c->W, c->H, c->Nz, c->partial,
+                            c->stream);
+    }));
+    RIHVR_HIP(hipStreamSynchronize(c->stream));
+    c->out = outb;
+    c->have_result = true;
     return RIHVR_OK;
 }
 
@@ -663,6 +681,11 @@ rihvr_status rihvr_iterate(rihvr_ctx* ctx, int32_t n_iter, rihvr_stats* stats) {
     return iterate_impl(ctx, n_iter, stats, false);
 }
 
+rihvr_status rihvr_snapshot(rihvr_ctx* ctx) {
+    if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
+    return snapshot_impl(ctx);
+}
+
 rihvr_status rihvr_finish(rihvr_ctx* ctx, rihvr_stats* stats) {
     if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
     return finish_impl(ctx, stats, -1);
@@ -692,6 +715,12 @@ rihvr_status rihvr_get_timing(rihvr_ctx* ctx, rihvr_kernel_time* out, int32_t ca
         out[k].launches = ctx->tm.launches[k];
         out[k].total_ms = ctx->tm.total_ms[k];
     }
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_get_params(const rihvr_ctx* ctx, rihvr_params* out) {
+    if (!ctx || !out) return RIHVR_E_INVALID_ARGUMENT;
+    *out = ctx->p;
     return RIHVR_OK;
 }
 

@@ -16,7 +16,7 @@ HEADER = (ROOT / "include" / "rihvr.h").read_text()
 
 
 def declared_functions():
-    return sorted(set(re.findall(r"\b(rihvr_[a-z_]+)\s*\(", HEADER)))
+    return sorted(set(re.findall(r"\b(rihvr_[a-z0-9_]+)\s*\(", HEADER)))
 
 
 def test_library_exports_every_declared_function(pkg):
