@@ -189,8 +189,7 @@ def test_cli_end_to_end_matches_oracle(pkg, orc, holo_image, tmp_path, gpu_avail
     assert rc == 0
     holo = pkg.prepare_hologram(fp, pkg.read_image(img_dir / "holo_0003.tif"))
     assert holo.shape == (64, 64)
-    p = pkg.make_params(**c)
-    ref, _, _, rst = orc.solve(p, holo)
+    ref, _, _, rst = orc.solve(orc.make_params(**c), holo)
     idx, val = _read_csv(out / "inverse_voxels.csv")
     z, y, x = np.nonzero(ref)
     got = np.zeros(ref.shape, np.float64)
