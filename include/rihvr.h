@@ -57,7 +57,7 @@ typedef enum rihvr_status {
  * src/core/sparse_compressive_holo.cu:178-188). */
 typedef enum rihvr_mode {
     RIHVR_MODE_FASTA_L1 = 5,              /* proximalL1 */
-    RIHVR_MODE_FASTA_FUSED_LASSO = 6,     /* 3D TV + L1 (not built; returns RIHVR_E_UNKNOWN_MODE) */
+    RIHVR_MODE_FASTA_FUSED_LASSO = 6,     /* proximalFusedLasso: L1/2, 3D z-periodic TV, L1/2 */
     RIHVR_MODE_FASTA_FUSED_LASSO_2D = 7   /* per-plane 2D TV + L1: the CLI default */
 } rihvr_mode;
 
@@ -115,8 +115,10 @@ const char* rihvr_status_string(rihvr_status s);
  * (src/core/sparse_compressive_holo.cu:58-135): validates the parameter set,
  * allocates the volumes on the current HIP device and builds the FFT schedules.
  * Errors: RIHVR_E_INVALID_ARGUMENT (W or H not a multiple of 16, Nz < 1,
- * non-positive wavelength/resolution, unsupported FFT length), RIHVR_E_UNKNOWN_MODE,
- * RIHVR_E_OUT_OF_MEMORY, RIHVR_E_DEVICE. */
+ * non-positive wavelength/resolution, unsupported FFT length, a mode other than
+ * 5, 6, 7 -- the reference's inverseReconstruct throws HOLO_ERROR_INVALID_ARGUMENT
+ * for those, :178-189), RIHVR_E_OUT_OF_MEMORY, RIHVR_E_DEVICE.  Any mode with a
+ * zero TV weight runs as mode 5 (:192-195). */
 rihvr_status rihvr_create(const rihvr_params* params, rihvr_ctx** out);
 
 /* Replaces SparseCompressiveHolo::destroy (:137-145).  NULL is accepted. */

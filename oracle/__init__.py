@@ -134,6 +134,8 @@ def lib():
         L.ro_l1_f32.restype = ctypes.c_double
         L.ro_tv2d_f32.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.ro_tv2d_f32.restype = ctypes.c_double
+        L.ro_tv3d_f32.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ro_tv3d_f32.restype = ctypes.c_double
         L.ro_restart_sum_f32.argtypes = [vp, vp, vp, ctypes.c_long, ctypes.c_int]
         L.ro_restart_sum_f32.restype = ctypes.c_double
         _lib = L
@@ -243,6 +245,12 @@ def tv2d_f32(vol):
     vol = _c(vol, np.complex64)
     Nz, H, W = vol.shape
     return float(lib().ro_tv2d_f32(_ptr(vol), Nz, H, W))
+
+
+def tv3d_f32(vol):
+    vol = _c(vol, np.complex64)
+    Nz, H, W = vol.shape
+    return float(lib().ro_tv3d_f32(_ptr(vol), Nz, H, W))
 
 
 def restart_sum_f32(x0, x1, y0):

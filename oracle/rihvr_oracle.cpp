@@ -305,6 +305,61 @@ double tv2d_norm(const C2<double>* v, int Nz, int H, int W) {
     return total;
 }
 
+// tv_norm_kernel (sparse_volume.cu:236-265) per voxel: periodic backward
+// differences in x and y, and against the previous plane (plane Nz-1 for plane 0).
+template <class T> inline T tv3_pixel(const C2<T>* d, const C2<T>* dp, int x, int y, int W, int H) {
+    int xm = (x > 0) ? x - 1 : W - 1;
+    int ym = (y > 0) ? y - 1 : H - 1;
+    size_t i = (size_t)y * W + x;
+    C2<T> c = d[i], a = d[(size_t)y * W + xm], b = d[(size_t)ym * W + x], q = dp[i];
+    T gxx = a.x - c.x, gyx = b.x - c.x, gzx = q.x - c.x;
+    T gxy = a.y - c.y, gyy = b.y - c.y, gzy = q.y - c.y;
+    T nrm = gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy + gzx * gzx + gzy * gzy;
+    return std::sqrt(nrm);
+}
+
+// calcTVNorm (sparse_volume.cu:303-363): per plane tv_summation_kernel<512>
+// tree, block partials added on the host into one double in z order.
+double tv3d_norm(const C2<float>* v, int Nz, int H, int W) {
+    const long N = (long)H * W;
+    long nb = (N + 1023) / 1024;
+    std::vector<std::vector<float>> part(Nz, std::vector<float>(nb));
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < Nz; ++z) {
+        const C2<float>* d = v + (size_t)z * N;
+        const C2<float>* dp = v + (size_t)(z > 0 ? z - 1 : Nz - 1) * N;
+        std::vector<float> pix(N);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) pix[(size_t)y * W + x] = tv3_pixel(d, dp, x, y, W, H);
+        float s[512];
+        for (long b = 0; b < nb; ++b) {
+            for (int t = 0; t < 512; ++t) {
+                long i = b * 1024 + t, i2 = i + 512;
+                float a = 0.0f;
+                a += i < N ? pix[i] : 0.0f;
+                a += i2 < N ? pix[i2] : 0.0f;
+                s[t] = a;
+            }
+            part[z][b] = tree512(s);
+        }
+    }
+    double total = 0;
+    for (int z = 0; z < Nz; ++z)
+        for (long b = 0; b < nb; ++b) total += part[z][b];
+    return total;
+}
+double tv3d_norm(const C2<double>* v, int Nz, int H, int W) {
+    const long N = (long)H * W;
+    double total = 0;
+    for (int z = 0; z < Nz; ++z) {
+        const C2<double>* d = v + (size_t)z * N;
+        const C2<double>* dp = v + (size_t)(z > 0 ? z - 1 : Nz - 1) * N;
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) total += tv3_pixel(d, dp, x, y, W, H);
+    }
+    return total;
+}
+
 // Per-plane block partials of sparse_linesearch_1/2_kernel (sparse_compressive_holo.cu:4226-4305)
 // and sparse_fistaRestart_kernel (:4393-4432), FP32 semantics.  The host sums
 // the partials of all planes in z order into one float (:4348-4352, :4464-4469).
@@ -588,6 +643,74 @@ template <class T> struct Solver {
         }
     }
 
+    // proximalFusedLasso, mode 6 (:3422-3475): block_soft_threshold_FASTA_kernel at
+    // regularization/2, totalVariationProjection (:3867-3888) over the whole volume with
+    // tv_denoised_estimate_kernel (:3890-3915) and tv_gradient_projection_kernel
+    // (:3964-3997) -- z-periodic, normalizer forced to 1 (:3991) -- then
+    // block_soft_threshold_kernel at regularization/2.
+    void prox_fl3d(const C2<T>* x0, const C2<T>* G, double t, C2<T>* out) const {
+        const long V = (long)Nz * N;
+        const double reg = (double)p.regularization_param;
+        const double regTV = (double)p.regularization_TV_param;
+        const double scale = 1 / (4 * 3 * t * regTV);
+        const double mu = t * regTV;
+        const T thr = (T)(t * (reg / 2.0));
+        std::vector<C2<T>> x1(V), y(V), gx(V, C2<T>{0, 0}), gy(V, C2<T>{0, 0}), gz(V, C2<T>{0, 0});
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < V; ++i) {
+            T inx = (T)((double)x0[i].x - t * (double)G[i].x);
+            T iny = (T)((double)x0[i].y - t * (double)G[i].y);
+            x1[i] = {soft1(inx, thr), soft1(iny, thr)};
+        }
+        auto estimate = [&]() {
+#pragma omp parallel for schedule(static)
+            for (int zi = 0; zi < Nz; ++zi) {
+                const size_t o = (size_t)zi * N, on = (size_t)(zi < Nz - 1 ? zi + 1 : 0) * N;
+                for (int yy = 0; yy < H; ++yy) {
+                    int yp = (yy < H - 1) ? yy + 1 : 0;
+                    for (int xx = 0; xx < W; ++xx) {
+                        int xp = (xx < W - 1) ? xx + 1 : 0;
+                        size_t idx = (size_t)yy * W + xx;
+                        size_t ixp = (size_t)yy * W + xp, iyp = (size_t)yp * W + xx;
+                        T dx = gx[o + ixp].x - gx[o + idx].x + gy[o + iyp].x - gy[o + idx].x + gz[on + idx].x -
+                               gz[o + idx].x;
+                        T dy = gx[o + ixp].y - gx[o + idx].y + gy[o + iyp].y - gy[o + idx].y + gz[on + idx].y -
+                               gz[o + idx].y;
+                        y[o + idx] = {(T)((double)x1[o + idx].x - mu * (double)dx),
+                                      (T)((double)x1[o + idx].y - mu * (double)dy)};
+                    }
+                }
+            }
+        };
+        for (int it = 0; it < p.num_tv_iterations; ++it) {
+            estimate();
+#pragma omp parallel for schedule(static)
+            for (int zi = 0; zi < Nz; ++zi) {
+                const size_t o = (size_t)zi * N, op = (size_t)(zi > 0 ? zi - 1 : Nz - 1) * N;
+                for (int yy = 0; yy < H; ++yy) {
+                    int ym = (yy > 0) ? yy - 1 : H - 1;
+                    for (int xx = 0; xx < W; ++xx) {
+                        int xm = (xx > 0) ? xx - 1 : W - 1;
+                        size_t idx = (size_t)yy * W + xx;
+                        size_t ixm = (size_t)yy * W + xm, iym = (size_t)ym * W + xx;
+                        const C2<T> c = y[o + idx];
+                        T a[6] = {y[o + ixm].x - c.x, y[o + iym].x - c.x, y[op + idx].x - c.x,
+                                  y[o + ixm].y - c.y, y[o + iym].y - c.y, y[op + idx].y - c.y};
+                        gx[o + idx] = {(T)((double)gx[o + idx].x + scale * (double)a[0]),
+                                       (T)((double)gx[o + idx].y + scale * (double)a[3])};
+                        gy[o + idx] = {(T)((double)gy[o + idx].x + scale * (double)a[1]),
+                                       (T)((double)gy[o + idx].y + scale * (double)a[4])};
+                        gz[o + idx] = {(T)((double)gz[o + idx].x + scale * (double)a[2]),
+                                       (T)((double)gz[o + idx].y + scale * (double)a[5])};
+                    }
+                }
+            }
+        }
+        estimate();
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < V; ++i) out[i] = {soft1(y[i].x, thr), soft1(y[i].y, thr)};
+    }
+
     // denoise (:2948-2995) dispatch, every plane
     int prox(int mode, const C2<T>* x0, const C2<T>* G, double t, C2<T>* out) const {
         if (mode == RIHVR_MODE_FASTA_L1) {
@@ -607,7 +730,11 @@ template <class T> struct Solver {
             }
             return RIHVR_OK;
         }
-        return RIHVR_E_UNKNOWN_MODE;
+        if (mode == RIHVR_MODE_FASTA_FUSED_LASSO) {
+            prox_fl3d(x0, G, t, out);
+            return RIHVR_OK;
+        }
+        return RIHVR_E_INVALID_ARGUMENT;
     }
 
     // calcObjectiveFunction (:2838-2899)
@@ -617,6 +744,8 @@ template <class T> struct Solver {
         double obj = 0.5 * resid_norm + (double)p.regularization_param * est_norm;
         if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D)
             obj = obj + (double)p.regularization_TV_param * tv2d_norm(x, Nz, H, W);
+        else if (mode == RIHVR_MODE_FASTA_FUSED_LASSO)
+            obj = obj + (double)p.regularization_TV_param * tv3d_norm(x, Nz, H, W);
         return obj;
     }
 
@@ -655,13 +784,14 @@ template <class T> struct Solver {
         }
     }
 
-    // inverseReconstruct (:173-633), modes 5 and 7
+    // inverseReconstruct (:173-633), modes 5, 6 and 7
     int solve(const C2<T>* meas, const C2<T>* warm, C2<T>* out, C2<T>* est_out,
               C2<T>* res_out, rihvr_stats* st) {
         int mode = p.mode;
-        if (mode != RIHVR_MODE_FASTA_L1 && mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D)
-            return RIHVR_E_UNKNOWN_MODE;
-        if (p.regularization_TV_param == 0.0f) mode = RIHVR_MODE_FASTA_L1;  // :192-195
+        if (mode != RIHVR_MODE_FASTA_L1 && mode != RIHVR_MODE_FASTA_FUSED_LASSO &&
+            mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D)
+            return RIHVR_E_INVALID_ARGUMENT;  // :178-189
+        if (p.regularization_TV_param == 0.0f) mode = RIHVR_MODE_FASTA_L1;  // :192-195, any mode
         for (long i = 0; i < N; ++i)
             if (!std::isfinite((double)meas[i].x) || !std::isfinite((double)meas[i].y))
                 return RIHVR_E_INVALID_DATA;
@@ -857,6 +987,9 @@ double ro_summag_f32(const float* d, long n) { return summag((const C2<float>*)d
 double ro_l1_f32(const float* v, long n) { return l1_norm((const C2<float>*)v, n); }
 double ro_tv2d_f32(const float* v, int Nz, int H, int W) {
     return tv2d_norm((const C2<float>*)v, Nz, H, W);
+}
+double ro_tv3d_f32(const float* v, int Nz, int H, int W) {
+    return tv3d_norm((const C2<float>*)v, Nz, H, W);
 }
 double ro_restart_sum_f32(const float* x0, const float* x1, const float* y0, long n_plane, int Nz) {
     float s = 0;

@@ -58,6 +58,8 @@ int ro_prox_f64(const rihvr_params* p, const double* x0, const double* grad,
 double ro_summag_f32(const float* data, long n);         /* sum re^2+im^2 */
 double ro_l1_f32(const float* vol, long n);              /* sum |v| (modulus) */
 double ro_tv2d_f32(const float* vol, int Nz, int H, int W);
+/* calcTVNorm (sparse_volume.cu:303-363): 3D, z-periodic */
+double ro_tv3d_f32(const float* vol, int Nz, int H, int W);
 double ro_restart_sum_f32(const float* x0, const float* x1, const float* y0, long n_plane, int Nz);
 
 /* Full solve.  warm may be NULL.  out: [Nz][H][W].  estimate/residual: [H][W]

@@ -71,11 +71,17 @@ hipError_t prox_fl2d(const ProxLaunch& a, hipStream_t s);
 int prox_l1_num_blocks(int W, int H, int nz);
 hipError_t prox_l1(const ProxLaunch& a, hipStream_t s);
 
+// Mode 6 (3D TV, z-periodic): three kernels, partials [nblocks][PS_COUNT];
+// scratch is a volume-sized buffer distinct from a.Y.
+int prox_fl3d_num_blocks(int W, int H, int nz);
+hipError_t prox_fl3d(const ProxLaunch& a, float2* scratch, hipStream_t s);
+
 // out = Y + s (Y - Y0) (the returned momentum point) with partials
-// [nblocks][PS_COUNT]: L1, TV and nnz of out in slots PS_L1X0, PS_TVX0, PS_NNZX0.
+// [nblocks][PS_COUNT]: L1, TV (2D per plane, or 3D z-periodic with tv3d) and nnz
+// of out in slots PS_L1X0, PS_TVX0, PS_NNZX0.
 int momentum_out_num_blocks(int W, int H, int nz);
 hipError_t momentum_out(const float2* Y, const float2* Y0, float s_mom, float2* out, int W, int H,
-                        int nz, double* partial, hipStream_t s);
+                        int nz, bool tv3d, double* partial, hipStream_t s);
 
 // Hologram momentum by linearity (sparse_compressive_holo.cu:494-498):
 // est = cur + s (cur - prev); resid = est - meas; partial[block] = sum |resid|^2.

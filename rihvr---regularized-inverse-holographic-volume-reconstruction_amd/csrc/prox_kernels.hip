@@ -367,13 +367,156 @@ hipError_t prox_l1(const ProxLaunch& a, hipStream_t s) {
 }
 
 // ----------------------------------------------------------------------------
+// Mode 6: proximalFusedLasso (:3422-3475).  Soft threshold at half the L1 weight
+// (block_soft_threshold_FASTA_kernel, :3438-3442), the 3D TV stage
+// (totalVariationProjection :3867-3888) and a second soft threshold at half the
+// weight (:3458-3463).  The TV stage's dual update has its normaliser forced to
+// 1 (tv_gradient_projection_kernel :3991), so it is linear; with
+// mu * scale = t lambda_TV / (12 t lambda_TV) = 1/12 and zero initial duals it
+// telescopes to y_{i+1} = y_i - M y_i / 12, M = div(grad) =
+// 6 y - (sum of the six periodic neighbours), i.e. y_{i+1} = y_i / 2 + nb / 12.
+// Applying that stencil n_tv times gives the reference's n_tv dual iterations in
+// exact arithmetic with one volume of state instead of three dual volumes.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kEwThreads) k_fl3d_pre(ProxDev a) {
+    __shared__ double red[kEwThreads / 64];
+    const int W = a.W, H = a.H;
+    const long N = (long)W * H;
+    const long n = N * a.nz;
+    double l1 = 0, tv = 0, nnz = 0;
+    const float s = a.s_mom;
+    auto mom = [&](long i) {
+        const float2 y0 = a.Y0[i], ym = a.YM[i];
+        return make_float2(fmaf(s, y0.x - ym.x, y0.x), fmaf(s, y0.y - ym.y, y0.y));
+    };
+    for (int it = 0; it < kEwItems; ++it) {
+        long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
+        if (i >= n) break;
+        const float2 x0 = mom(i), g = a.G[i];
+        const float2 xh = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
+        a.Y[i] = make_float2(soft1(xh.x, a.thr), soft1(xh.y, a.thr));
+        // objective terms of the momentum point x0: L1, nnz and the z-periodic 3D TV
+        // (calcL1NormComplex, calcTVNorm / tv_norm_kernel sparse_volume.cu:236-265)
+        if (x0.x != 0.0f || x0.y != 0.0f) {
+            l1 += (double)sqrtf(x0.x * x0.x + x0.y * x0.y);
+            nnz += 1.0;
+        }
+        const long z = i / N;
+        const long zo = z * N;
+        const int p = (int)(i - zo);
+        const int y = p / W, x = p - y * W;
+        const int xm = x > 0 ? x - 1 : W - 1, ym = y > 0 ? y - 1 : H - 1;
+        const long zpo = (z > 0 ? z - 1 : a.nz - 1) * N;
+        const float2 qa = mom(zo + (long)y * W + xm), qb = mom(zo + (long)ym * W + x), qc = mom(zpo + p);
+        const float gxx = qa.x - x0.x, gyx = qb.x - x0.x, gzx = qc.x - x0.x;
+        const float gxy = qa.y - x0.y, gyy = qb.y - x0.y, gzy = qc.y - x0.y;
+        tv += (double)sqrtf(gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy + gzx * gzx + gzy * gzy);
+    }
+    double* pp = a.partial + (size_t)blockIdx.x * PS_COUNT;
+    double r;
+    r = block_sum(l1, red);
+    if (threadIdx.x == 0) pp[PS_L1X0] = r;
+    r = block_sum(tv, red);
+    if (threadIdx.x == 0) pp[PS_TVX0] = r;
+    r = block_sum(nnz, red);
+    if (threadIdx.x == 0) pp[PS_NNZX0] = r;
+}
+
+// One application of y <- y / 2 + (sum of the six periodic neighbours) / 12.
+// Thread = one voxel; neighbours come through the cache.
+__global__ void __launch_bounds__(256) k_fl3d_sweep(const float2* __restrict__ in, float2* __restrict__ out, int W,
+                                                    int H, int nz) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= W || y >= H) return;
+    const long N = (long)W * H;
+    const long zo = (long)z * N;
+    const long zpo = (long)(z > 0 ? z - 1 : nz - 1) * N;
+    const long zno = (long)(z < nz - 1 ? z + 1 : 0) * N;
+    const int xm = x > 0 ? x - 1 : W - 1, xp = x < W - 1 ? x + 1 : 0;
+    const int ym = y > 0 ? y - 1 : H - 1, yp = y < H - 1 ? y + 1 : 0;
+    const long row = (long)y * W;
+    const float2 c = in[zo + row + x];
+    const float2 a0 = in[zo + row + xm], a1 = in[zo + row + xp];
+    const float2 b0 = in[zo + (long)ym * W + x], b1 = in[zo + (long)yp * W + x];
+    const float2 c0 = in[zpo + row + x], c1 = in[zno + row + x];
+    const float sx = ((a0.x + a1.x) + (b0.x + b1.x)) + (c0.x + c1.x);
+    const float sy = ((a0.y + a1.y) + (b0.y + b1.y)) + (c0.y + c1.y);
+    const float k12 = 1.0f / 12.0f;
+    out[zo + row + x] = make_float2(fmaf(sx, k12, 0.5f * c.x), fmaf(sy, k12, 0.5f * c.y));
+}
+
+// Second half threshold, with the line-search and restart reductions of the
+// iteration (x0 formed again from y_{k-1}, y_{k-2}).
+__global__ void __launch_bounds__(kEwThreads) k_fl3d_final(ProxDev a, const float2* ytv) {  // ytv may be a.Y
+    __shared__ double red[kEwThreads / 64];
+    const long N = (long)a.W * a.H;
+    const long n = N * a.nz;
+    double ls1 = 0, ls2 = 0, rs = 0, nnzy = 0;
+    const float s = a.s_mom;
+    for (int it = 0; it < kEwItems; ++it) {
+        long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
+        if (i >= n) break;
+        const float2 y0 = a.Y0[i], ym = a.YM[i], g = a.G[i], v = ytv[i];
+        const float2 x0 = make_float2(fmaf(s, y0.x - ym.x, y0.x), fmaf(s, y0.y - ym.y, y0.y));
+        const float2 yv = make_float2(soft1(v.x, a.thr), soft1(v.y, a.thr));
+        a.Y[i] = yv;
+        const float dr = yv.x - x0.x, di = yv.y - x0.y;
+        ls1 += (double)(dr * g.x) + (double)(di * g.y);
+        ls2 += (double)dr * dr + (double)di * di;
+        const float dre = (x0.x - yv.x) * (yv.x - y0.x);
+        const float dim = (x0.y - yv.y) * (yv.y - y0.y);
+        const int off = (int)((i % N) & 1023);
+        rs += off < 512 ? (double)dre + 2.0 * (double)dim : (double)dim;
+        nnzy += (yv.x != 0.0f || yv.y != 0.0f) ? 1.0 : 0.0;
+    }
+    double* pp = a.partial + (size_t)blockIdx.x * PS_COUNT;
+    double r;
+    r = block_sum(ls1, red);
+    if (threadIdx.x == 0) pp[PS_LS1] = r;
+    r = block_sum(ls2, red);
+    if (threadIdx.x == 0) pp[PS_LS2] = r;
+    r = block_sum(rs, red);
+    if (threadIdx.x == 0) pp[PS_RESTART] = r;
+    r = block_sum(nnzy, red);
+    if (threadIdx.x == 0) {
+        pp[PS_NNZY] = r;
+        pp[7] = 0.0;
+    }
+}
+
+int prox_fl3d_num_blocks(int W, int H, int nz) { return prox_l1_num_blocks(W, H, nz); }
+
+hipError_t prox_fl3d(const ProxLaunch& a, float2* scratch, hipStream_t s) {
+    if (!scratch || scratch == a.Y) return hipErrorInvalidValue;
+    ProxDev d = make_dev(a);
+    d.thr = (float)(a.t * (a.reg / 2.0));  // both thresholds at regularization / 2 (:3441, :3461)
+    const dim3 eg(prox_fl3d_num_blocks(a.W, a.H, a.nz));
+    hipLaunchKernelGGL(k_fl3d_pre, eg, dim3(kEwThreads), 0, s, d);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // ping-pong between the output volume and the scratch volume
+    float2* buf[2] = {a.Y, scratch};
+    int src = 0;
+    const dim3 sg((a.W + 63) / 64, (a.H + 3) / 4, a.nz);
+    for (int k = 0; k < a.n_tv; ++k) {
+        hipLaunchKernelGGL(k_fl3d_sweep, sg, dim3(256), 0, s, buf[src], buf[src ^ 1], a.W, a.H, a.nz);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        src ^= 1;
+    }
+    hipLaunchKernelGGL(k_fl3d_final, eg, dim3(kEwThreads), 0, s, d, buf[src]);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
 // Returned volume: x1 = y + s (y - y0) with its L1 / TV / nnz.
 // ----------------------------------------------------------------------------
 int momentum_out_num_blocks(int W, int H, int nz) { return prox_l1_num_blocks(W, H, nz); }
 
 __global__ void __launch_bounds__(kEwThreads)
 k_momentum_out(const float2* __restrict__ Y, const float2* __restrict__ Y0, float s, float2* out, int W, int H,
-               int nz, double* partial) {
+               int nz, int tv3d, double* partial) {
     __shared__ double red[kEwThreads / 64];
     const long N = (long)W * H;
     const long n = N * nz;
@@ -397,7 +540,13 @@ k_momentum_out(const float2* __restrict__ Y, const float2* __restrict__ Y0, floa
         const int xm = x > 0 ? x - 1 : W - 1, ym = y > 0 ? y - 1 : H - 1;
         const float2 a = mom(zo + (long)y * W + xm), b = mom(zo + (long)ym * W + x);
         const float gxx = a.x - v.x, gyx = b.x - v.x, gxy = a.y - v.y, gyy = b.y - v.y;
-        tv += (double)sqrtf(gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy);
+        float n2 = gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy;
+        if (tv3d) {  // calcTVNorm: against the previous plane, z-periodic (sparse_volume.cu:236-265)
+            const float2 c = mom((zo > 0 ? zo - N : (long)(nz - 1) * N) + p);
+            const float gzx = c.x - v.x, gzy = c.y - v.y;
+            n2 = n2 + gzx * gzx + gzy * gzy;
+        }
+        tv += (double)sqrtf(n2);
     }
     double* pp = partial + (size_t)blockIdx.x * PS_COUNT;
     double r;
@@ -413,9 +562,10 @@ k_momentum_out(const float2* __restrict__ Y, const float2* __restrict__ Y0, floa
 }
 
 hipError_t momentum_out(const float2* Y, const float2* Y0, float s_mom, float2* out, int W, int H, int nz,
-                        double* partial, hipStream_t s) {
+                        bool tv3d, double* partial, hipStream_t s) {
     dim3 grid(momentum_out_num_blocks(W, H, nz));
-    hipLaunchKernelGGL(k_momentum_out, grid, dim3(kEwThreads), 0, s, Y, Y0, s_mom, out, W, H, nz, partial);
+    hipLaunchKernelGGL(k_momentum_out, grid, dim3(kEwThreads), 0, s, Y, Y0, s_mom, out, W, H, nz, tv3d ? 1 : 0,
+                       partial);
     return hipGetLastError();
 }
 

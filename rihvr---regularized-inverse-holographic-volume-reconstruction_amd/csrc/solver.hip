@@ -124,9 +124,11 @@ rihvr_status validate(const rihvr_params* p) {
     if (p->width % 16 || p->height % 16) return RIHVR_E_INVALID_ARGUMENT;
     if (!(p->wavelength > 0) || !(p->resolution > 0)) return RIHVR_E_INVALID_ARGUMENT;
     if (p->num_inverse_iterations < 0 || p->num_tv_iterations < 0) return RIHVR_E_INVALID_ARGUMENT;
-    if (p->mode == RIHVR_MODE_FASTA_FUSED_LASSO) return RIHVR_E_UNKNOWN_MODE;
-    if (p->mode != RIHVR_MODE_FASTA_L1 && p->mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D)
-        return RIHVR_E_UNKNOWN_MODE;
+    // inverseReconstruct accepts modes 5, 6, 7 and throws HOLO_ERROR_INVALID_ARGUMENT
+    // for any other (sparse_compressive_holo.cu:178-189)
+    if (p->mode != RIHVR_MODE_FASTA_L1 && p->mode != RIHVR_MODE_FASTA_FUSED_LASSO &&
+        p->mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D)
+        return RIHVR_E_INVALID_ARGUMENT;
     if (!fft_length_supported(p->width) || !fft_length_supported(p->height)) return RIHVR_E_INVALID_ARGUMENT;
     if (p->mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && !prox_fl2d_supported(p->num_tv_iterations))
         return RIHVR_E_INVALID_ARGUMENT;
@@ -253,11 +255,12 @@ rihvr_status apply_adjoint(rihvr_ctx* c, const float2* r, float2* G) {
 }
 
 int effective_mode(const rihvr_ctx* c) {
-    // mode 7 becomes mode 5 when the TV weight is zero (:192-195)
-    if (c->p.mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && c->p.regularization_TV_param == 0.0f)
-        return RIHVR_MODE_FASTA_L1;
+    // any mode becomes mode 5 when the TV weight is zero (:192-195)
+    if (c->p.regularization_TV_param == 0.0f) return RIHVR_MODE_FASTA_L1;
     return c->p.mode;
 }
+
+bool tv3d(const rihvr_ctx* c) { return c->ss.mode == RIHVR_MODE_FASTA_FUSED_LASSO; }
 
 // y = prox(x0 - t g), x0 = Y0 + s (Y0 - YM); reductions into scal[SL_PROX..]
 rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* YM, float s_mom, const float2* G,
@@ -275,6 +278,10 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
     if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) {
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
         rows = prox_fl2d_num_waves(c->W, c->H, c->Nz, a.n_tv, c->prox_rows);
+    } else if (mode == RIHVR_MODE_FASTA_FUSED_LASSO) {
+        // the FFT scratch volume is free between A^T and A
+        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl3d(a, c->T, c->stream); }));
+        rows = prox_fl3d_num_blocks(c->W, c->H, c->Nz);
     } else {
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_l1(a, c->stream); }));
         rows = prox_l1_num_blocks(c->W, c->H, c->Nz);
@@ -286,7 +293,9 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
 double objective(const rihvr_ctx* c, int mode, double summag, double l1, double tv) {
     // calcObjectiveFunction (:2838-2899)
     double f = 0.5 * summag + (double)c->p.regularization_param * l1;
-    if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) f = f + (double)c->p.regularization_TV_param * tv;
+    // TV term: 2D per plane for mode 7 (calcTVNorm2d), 3D z-periodic for mode 6 (calcTVNorm)
+    if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D || mode == RIHVR_MODE_FASTA_FUSED_LASSO)
+        f = f + (double)c->p.regularization_TV_param * tv;
     return f;
 }
 
@@ -345,6 +354,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_TRY(dalloc(&c->twiH, c->H));
     long rows = std::max<long>(prox_fl2d_num_waves(c->W, c->H, c->Nz, std::min(params->num_tv_iterations, 10), c->prox_rows),
                                prox_l1_num_blocks(c->W, c->H, c->Nz));
+    rows = std::max<long>(rows, prox_fl3d_num_blocks(c->W, c->H, c->Nz));
     rows = std::max<long>(rows, momentum_out_num_blocks(c->W, c->H, c->Nz));
     c->partial_rows = rows;
     RIHVR_TRY(dalloc(&c->partial, (size_t)rows * PS_COUNT));
@@ -392,7 +402,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
 }
 
 // ---------------------------------------------------------------------------
-// inverseReconstruct (:173-633), modes 5 and 7, split at its seams:
+// inverseReconstruct (:173-633), modes 5, 6 and 7, split at its seams:
 //   begin   = the set-up before the loop (:238-296)
 //   iterate = n passes of the loop body (:299-607)
 //   finish  = the returned momentum point and final objective (:483-489, :608-633)
@@ -426,7 +436,7 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     double l1 = 0, tv = 0;
     if (warm) {
         RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
-            return momentum_out(Y0, Y0, 0.0f, nullptr, c->W, c->H, c->Nz, c->partial, c->stream);
+            return momentum_out(Y0, Y0, 0.0f, nullptr, c->W, c->H, c->Nz, tv3d(c), c->partial, c->stream);
         }));
         RIHVR_TRY(timed(c, K_REDUCE, [&] {
             return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
@@ -452,7 +462,7 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
 // objective of the current momentum point Y0 + s (Y0 - YM) with residual norm summag
 rihvr_status momentum_objective(rihvr_ctx* c, double summag, double* f) {
     RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
-        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], c->ss.s_mom, nullptr, c->W, c->H, c->Nz, c->partial,
+        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], c->ss.s_mom, nullptr, c->W, c->H, c->Nz, tv3d(c), c->partial,
                             c->stream);
     }));
     RIHVR_TRY(timed(c, K_REDUCE, [&] {
@@ -560,7 +570,7 @@ rihvr_status finish_impl(rihvr_ctx* c, rihvr_stats* st, int last_index) {
     // iteration s = 0 and the start point is returned
     float2* outb = c->Yb[c->iy];
     RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
-        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], s.s_mom, outb, c->W, c->H, c->Nz, c->partial,
+        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], s.s_mom, outb, c->W, c->H, c->Nz, tv3d(c), c->partial,
                             c->stream);
     }));
     RIHVR_TRY(timed(c, K_REDUCE, [&] {
@@ -595,7 +605,7 @@ rihvr_status snapshot_impl(rihvr_ctx* c) {
     if (!c->ss.active) return RIHVR_E_INVALID_STATE;
     float2* outb = c->Yb[c->iy];
     RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
-        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], c->ss.s_mom, outb, c->W, c->H, c->Nz, c->partial,
+        return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], c->ss.s_mom, outb, c->W, c->H, c->Nz, tv3d(c), c->partial,
                             c->stream);
     }));
     RIHVR_HIP(hipStreamSynchronize(c->stream));

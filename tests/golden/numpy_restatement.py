@@ -112,6 +112,40 @@ def prox_fl2d(x0, g, t, reg, reg_tv, n_tv):
     return y
 
 
+def prox_fl3d(x0, g, t, reg, reg_tv, n_tv):
+    """proximalFusedLasso, mode 6 (:3422-3475, :3867-4047): soft threshold at half the
+    weight, 3D dual iterations with periodic x, y and z and the normaliser forced to 1
+    (:3991), soft threshold at half the weight again.  Volume axes (z, y, x)."""
+    reg, reg_tv = f32(reg), f32(reg_tv)
+    thr = t * (reg / 2.0)
+    x1 = soft(x0 - t * g, thr)
+    gx = np.zeros_like(x1)
+    gy = np.zeros_like(x1)
+    gz = np.zeros_like(x1)
+    mu = t * reg_tv
+    sc = 1.0 / (12.0 * t * reg_tv)
+
+    def estimate():
+        # tv_denoised_estimate_kernel evaluation order: ((((gx+ - gx) + gy+) - gy) + gz_next) - gz
+        div = ((((np.roll(gx, -1, axis=2) - gx) + np.roll(gy, -1, axis=1)) - gy) + np.roll(gz, -1, axis=0)) - gz
+        return x1 - mu * div
+
+    for _ in range(n_tv):
+        y = estimate()
+        gx = gx + sc * (np.roll(y, 1, axis=2) - y)
+        gy = gy + sc * (np.roll(y, 1, axis=1) - y)
+        gz = gz + sc * (np.roll(y, 1, axis=0) - y)  # previous plane, plane Nz-1 before plane 0
+    return soft(estimate(), thr)
+
+
+def tv3d(vol):
+    """calcTVNorm (sparse_volume.cu:236-363): z-periodic backward differences."""
+    a = np.roll(vol, 1, axis=2) - vol
+    b = np.roll(vol, 1, axis=1) - vol
+    c = np.roll(vol, 1, axis=0) - vol
+    return float(np.sum(np.sqrt(np.abs(a) ** 2 + np.abs(b) ** 2 + np.abs(c) ** 2)))
+
+
 def prox_l1(x0, g, t, reg):
     """proximalL1 (:3274-3392)."""
     return soft(x0 - t * g, t * f32(reg))
@@ -148,12 +182,16 @@ def solve(op: Operator, holo, iters, reg, reg_tv, n_tv, mode=7, warm=None):
     def prox(x0_, G_, t_):
         if mode == 7:
             return prox_fl2d(x0_, G_, t_, reg, reg_tv, n_tv)
+        if mode == 6:
+            return prox_fl3d(x0_, G_, t_, reg, reg_tv, n_tv)
         return prox_l1(x0_, G_, t_, reg)
 
     def objective(r, x):
         f = 0.5 * np.sum(np.abs(r) ** 2) + f32(reg) * np.sum(np.abs(x))
         if mode == 7:
             f += f32(reg_tv) * tv2d(x)
+        elif mode == 6:
+            f += f32(reg_tv) * tv3d(x)
         return float(f)
 
     est = op.forward(x1)

@@ -157,6 +157,37 @@ def test_solve_mode5_and_warm_start(pkg, orc, scenes, gpu_available):
     assert list(st["restarts"]) == list(rst["restarts"])
 
 
+@pytest.mark.parametrize("ntv", [0, 1, 5])
+@pytest.mark.parametrize("W,H,Nz", [(32, 32, 3), (64, 48, 5), (256, 128, 4)])
+def test_prox_fl3d_matches_oracle(pkg, orc, gpu_available, ntv, W, H, Nz):
+    """Mode 6 (proximalFusedLasso, :3422-3475): half thresholds around the 3D,
+    z-periodic TV stage."""
+    c = cfg(W, H, Nz, reg=0.05, tv=0.02, ntv=ntv, mode=6)
+    rng = np.random.default_rng(ntv * 1000 + W + H + Nz)
+    x0 = rand_vol(rng, Nz, H, W, sparsity=0.2) * np.float32(0.5)
+    g = rand_vol(rng, Nz, H, W) * np.float32(2.0)
+    t = 1.0 / Nz
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        y = rec.prox(x0, g, t)
+    ref = orc.prox(orc.make_params(**c), x0, g, t, "f32")
+    assert rel(y, ref) < 2e-5, rel(y, ref)
+    diff = ((y.real != 0) | (y.imag != 0)) != ((ref.real != 0) | (ref.imag != 0))
+    assert diff.sum() <= max(2, int(1e-4 * y.size)), diff.sum()
+
+
+def test_solve_mode6_matches_oracle(pkg, orc, scenes, gpu_available):
+    c = cfg(64, 64, 8, its=8, reg=0.01, tv=0.01, ntv=3, mode=6)
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 6, 2.0, 4.0, 14, 0.01)
+    vol, st, ref, rst, (est, res, rest, rres) = _solve_both(pkg, orc, c, holo)
+    assert rel(vol, ref) < 1e-4, rel(vol, ref)
+    assert list(st["restarts"]) == list(rst["restarts"])
+    assert list(st["backtracks"]) == list(rst["backtracks"])
+    assert _support_mismatch(vol, ref, 0) <= max(2, int(1e-4 * vol.size))
+    # the objective carries the 3D TV norm (calcTVNorm)
+    np.testing.assert_allclose(st["f"], rst["f"], rtol=1e-4)
+    assert abs(st["f_final"] - rst["f_final"]) <= 1e-4 * abs(rst["f_final"])
+
+
 def test_device_pointers_match_host(pkg, scenes, gpu_available):
     import torch
     c = cfg(64, 64, 4, its=3)
@@ -177,8 +208,8 @@ def test_error_codes(pkg, gpu_available):
         pkg.SparseCompressiveHolo(pkg.make_params(**cfg(40, 32, 2)))
     assert e.value.status == 7
     with pytest.raises(pkg.RihvrError) as e:
-        pkg.SparseCompressiveHolo(pkg.make_params(**cfg(32, 32, 2, mode=6)))
-    assert e.value.status == 6
+        pkg.SparseCompressiveHolo(pkg.make_params(**cfg(32, 32, 2, mode=3)))
+    assert e.value.status == 7  # modes other than 5, 6, 7: HOLO_ERROR_INVALID_ARGUMENT (:178-189)
     with pkg.SparseCompressiveHolo(pkg.make_params(**cfg(32, 32, 2))) as rec:
         bad = np.zeros((32, 32), np.complex64)
         bad[3, 4] = np.nan

@@ -333,3 +333,112 @@ def test_planted_mistakes_are_caught():
         gx, gy = ax / n, ay / n
     y = R.soft(xh - mu * ((np.roll(gx, -1, 2) - gx) + (np.roll(gy, -1, 1) - gy)), 0.25 * 0.05)
     assert rel(y, good_p) > 1e-3
+    # (d) mode 6 with z not periodic (plane 0 has no previous plane) or with the
+    # isotropic projection left on: both fail the closed form of the TV stage
+    rng = np.random.default_rng(31)
+    Nz6, H6, W6 = 6, 16, 32
+    x6 = rng.standard_normal((Nz6, H6, W6)) + 1j * rng.standard_normal((Nz6, H6, W6)) + 0.5
+    closed = np.fft.ifftn(np.fft.fftn(x6) * (1 - _laplace_symbol(Nz6, H6, W6) / 12.0) ** 3)
+    for variant in ("open_z", "projected"):
+        mu6, sc6 = 0.1, 1 / (12 * 0.1)
+        gx, gy, gz = (np.zeros_like(x6) for _ in range(3))
+
+        def est():
+            gzn = np.roll(gz, -1, 0)
+            if variant == "open_z":
+                gzn[-1] = 0
+            return x6 - mu6 * ((np.roll(gx, -1, 2) - gx) + (np.roll(gy, -1, 1) - gy) + (gzn - gz))
+        for _ in range(3):
+            y = est()
+            yp = np.roll(y, 1, 0)
+            if variant == "open_z":
+                yp[0] = y[0]
+            ax = gx + sc6 * (np.roll(y, 1, 2) - y)
+            ay = gy + sc6 * (np.roll(y, 1, 1) - y)
+            az = gz + sc6 * (yp - y)
+            n = 1.0
+            if variant == "projected":
+                n = np.maximum(np.sqrt(np.abs(ax) ** 2 + np.abs(ay) ** 2 + np.abs(az) ** 2), 1)
+            gx, gy, gz = ax / n, ay / n, az / n
+        assert rel(est(), closed) > 1e-3, variant
+
+
+# ------------------------------------------------------------------ mode 6 (3D)
+def _laplace_symbol(Nz, H, W):
+    """Eigenvalues of div(grad) for the reference's stencils (backward difference
+    gradient, forward difference divergence, periodic): sum_d 4 sin^2(pi k_d / N_d)."""
+    kz = np.fft.fftfreq(Nz)[:, None, None]
+    ky = np.fft.fftfreq(H)[None, :, None]
+    kx = np.fft.fftfreq(W)[None, None, :]
+    return 4 * (np.sin(np.pi * kz) ** 2 + np.sin(np.pi * ky) ** 2 + np.sin(np.pi * kx) ** 2)
+
+
+@pytest.mark.parametrize("ntv", [1, 3, 5])
+def test_mode6_tv_stage_closed_form(orc, ntv):
+    """With the normaliser forced to 1 the dual iteration is linear and, as
+    mu * sc = 1/12, telescopes: y = (I - div grad / 12)^n x.  With no exact zeros
+    and zero L1 weight the soft thresholds are the identity."""
+    rng = np.random.default_rng(31)
+    Nz, H, W = 6, 16, 32
+    x0 = rng.standard_normal((Nz, H, W)) + 1j * rng.standard_normal((Nz, H, W)) + 0.5
+    g = rng.standard_normal((Nz, H, W)) + 1j * rng.standard_normal((Nz, H, W))
+    p = params(orc, W, H, Nz, reg=0.0, tv=0.07, ntv=ntv, mode=6)
+    t = 0.25
+    xh = x0 - t * g
+    exp = np.fft.ifftn(np.fft.fftn(xh) * (1 - _laplace_symbol(Nz, H, W) / 12.0) ** ntv)
+    assert rel(orc.prox(p, x0, g, t, "f64"), exp) < 1e-13
+    assert rel(orc.prox(p, x0, g, t, "f32"), exp) < 2e-6
+
+
+def test_mode6_invariants(orc):
+    rng = np.random.default_rng(32)
+    Nz, H, W = 4, 16, 16
+    x0 = rng.standard_normal((Nz, H, W)) + 1j * rng.standard_normal((Nz, H, W))
+    g = rng.standard_normal((Nz, H, W)) + 1j * rng.standard_normal((Nz, H, W))
+    # n_tv = 0: two half thresholds compose to one full threshold (mode 5's prox)
+    p6 = params(orc, W, H, Nz, reg=0.8, tv=0.05, ntv=0, mode=6)
+    p5 = params(orc, W, H, Nz, reg=0.8, tv=0.0, mode=5)
+    np.testing.assert_allclose(orc.prox(p6, x0, g, 0.5, "f64"), orc.prox(p5, x0, g, 0.5, "f64"), atol=1e-14)
+    # a constant volume passes the TV stage unchanged
+    c = np.full((Nz, H, W), 0.3 - 0.2j)
+    pc = params(orc, W, H, Nz, reg=0.0, tv=0.05, ntv=5, mode=6)
+    np.testing.assert_allclose(orc.prox(pc, c, np.zeros_like(c), 0.5, "f64"), c, atol=1e-15)
+    # z is periodic: rolling the input along z rolls the output
+    a = orc.prox(pc, np.roll(x0, 1, axis=0), np.roll(g, 1, axis=0), 0.5, "f64")
+    np.testing.assert_allclose(a, np.roll(orc.prox(pc, x0, g, 0.5, "f64"), 1, axis=0), atol=1e-14)
+
+
+def test_mode6_prox_and_solve_match_restatement(orc):
+    rng = np.random.default_rng(33)
+    Nz, H, W = 4, 16, 32
+    x0 = rng.standard_normal((Nz, H, W)) + 1j * rng.standard_normal((Nz, H, W))
+    g = rng.standard_normal((Nz, H, W)) + 1j * rng.standard_normal((Nz, H, W))
+    p = params(orc, W, H, Nz, reg=0.3, tv=0.02, ntv=5, mode=6)
+    exp = R.prox_fl3d(x0, g, 0.25, 0.3, 0.02, 5)
+    assert rel(orc.prox(p, x0, g, 0.25, "f64"), exp) < 1e-12
+    assert rel(orc.prox(p, x0, g, 0.25, "f32"), exp) < 2e-6
+    # whole solve in FP64 against the numpy solve
+    holo = GOLD["A_holo"]
+    Hh, Wh = holo.shape
+    ps = params(orc, Wh, Hh, 4, its=6, reg=0.01, tv=0.01, ntv=3, mode=6)
+    op = R.Operator(Wh, Hh, 4, 1000.0, 50.0, 0.633, 2.0)
+    ref, _, _, log = R.solve(op, holo, 6, 0.01, 0.01, 3, mode=6)
+    vol, _, _, st = orc.solve(ps, holo, None, "f64")
+    assert rel(vol, ref) < 1e-9
+    np.testing.assert_allclose(st["f"], log["f"], rtol=1e-9)
+    assert list(st["restarts"]) == list(log["restarts"])
+
+
+def test_tv3d_norm(orc):
+    rng = np.random.default_rng(34)
+    v = (rng.standard_normal((5, 16, 32)) + 1j * rng.standard_normal((5, 16, 32))).astype(np.complex64)
+    v[1] = 0  # sparse planes as the solver produces
+    assert orc.tv3d_f32(v) == pytest.approx(R.tv3d(v.astype(np.complex128)), rel=1e-6)
+    # the z term is what separates it from the 2D norm
+    assert orc.tv3d_f32(v) > orc.tv2d_f32(v) * 1.05
+
+
+def test_unsupported_mode_is_invalid_argument(orc):
+    p = params(orc, 16, 16, 2, mode=3)
+    with pytest.raises(RuntimeError, match="status 7"):
+        orc.solve(p, np.zeros((16, 16), np.complex64))
