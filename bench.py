@@ -8,6 +8,10 @@ occur are inside the step).  The hologram is generated synthetically (500 tracer
 region; W warm-up iterations run, then exactly K iterations are timed between
 device synchronisations.
 
+A second leg ("frames") measures s/frame to an objective tolerance on a short
+time sequence at the same optics, cold start against a warm start from the
+previous frame (rihvr_reconstruct_batch).
+
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -78,6 +82,49 @@ def cpu_baseline(cfg: dict, holo, iters_hi=2, iters_lo=1):
                         f"{iters_lo}-iteration solve, {times[iters_hi] + times[iters_lo]:.1f} s total)"))
 
 
+def frames_leg(pkg, scenes, cfg, n_frames: int, tol: float = 1e-3, seed: int = 20261203):
+    """s/frame to an objective tolerance on a time sequence at the workload's optics:
+    plain cold-start FISTA for num_inverse_iterations gives f_plain per frame; then
+    cold start and warm start (the previous frame's volume, kept on the device) both
+    run until f <= (1 + tol) f_plain or the same cap.  Frames are device-resident."""
+    import torch
+    optics = scenes.optics_of(cfg)
+    holos = scenes.sequence(optics, n_frames, n_tracers=500, n_rods=20, seed=seed, sigma=0.02)
+    hd = torch.from_numpy(holos).cuda()
+    cap = cfg["num_inverse_iterations"]
+    rec = pkg.SparseCompressiveHolo(pkg.make_params(**cfg))
+    try:
+        def run(**kw):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fs = rec.reconstructBatch(hd, **kw)
+            torch.cuda.synchronize()
+            return fs, time.perf_counter() - t0
+        plain, t_plain = run()
+        target = [(1.0 + tol) * f["f_final"] for f in plain]
+        cold, t_cold = run(f_target=target, max_iterations=cap)
+        warm, t_warm = run(f_target=target, max_iterations=cap, warm_from_previous=True)
+    finally:
+        rec.destroy()
+
+    def leg(fs, t):
+        return {"s_per_frame": t / n_frames, "iterations": [f["iterations"] for f in fs],
+                "reached": [f["reached"] for f in fs], "f_final": [f["f_final"] for f in fs]}
+    # frame 0 has no predecessor: the warm-start comparison is over frames 1..n-1
+    it_cold = sum(f["iterations"] for f in cold[1:])
+    it_warm = sum(f["iterations"] for f in warm[1:])
+    return {
+        "workload": (f"{n_frames} frames, {cfg['width']}x{cfg['height']}x{cfg['num_planes']} at the same optics, "
+                     "500 tracers + 20 rods advected by a seeded divergence-free 16-mode flow (rms 3 px/frame)"),
+        "tolerance": f"f <= (1 + {tol}) * f_plain({cap})",
+        "warm_start": "previous frame's returned volume (temporal); not a learned predictor",
+        "plain_fixed": dict(leg(plain, t_plain), iterations_each=cap),
+        "plain_to_tol": leg(cold, t_cold),
+        "warm_previous_to_tol": leg(warm, t_warm),
+        "iterations_ratio_cold_over_warm_frames_1_on": (it_cold / it_warm) if it_warm else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +132,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--frames", type=int, default=3,
+                    help="frames of the s/frame-to-tolerance leg (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -182,6 +231,8 @@ def main():
         "solver": {"backtracks_timed": int(np.sum(st["backtracks"])), "restarts_timed": int(np.sum(st["restarts"])),
                    "stepsize": st["stepsize"], "f_final": fin["f_final"], "sparsity": fin["sparsity"]},
     }
+    if args.frames > 0:
+        out["frames"] = frames_leg(pkg, scenes, cfg, args.frames)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cfg, holo)

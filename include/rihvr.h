@@ -136,6 +136,35 @@ rihvr_status rihvr_reconstruct(rihvr_ctx* ctx, const float* holo, int holo_loc,
                                const float* warm_start, int warm_loc,
                                rihvr_stats* stats);
 
+/* Per-frame result of rihvr_reconstruct_batch. */
+typedef struct rihvr_frame_stats {
+    int32_t iterations;        /* iterations run for this frame */
+    int32_t total_backtracks;
+    int32_t reached;           /* 1 if f_final <= the frame's target (or no target) */
+    int32_t pad_;
+    double  f_initial;         /* objective at the start point */
+    double  f_final;           /* objective at the returned volume */
+    int64_t nnz;               /* non-zero voxels of the returned volume */
+} rihvr_frame_stats;
+
+/* A sequence of holograms of this context's geometry (holos: [n_frames][H][W]
+ * complex64 at holo_loc), reconstructed one after the other on the device.  No
+ * counterpart call in the reference, whose executable reconstructs one image per
+ * run (main_sparse_inverse.cu:101-104); each frame is the reference's
+ * inverseReconstruct with these start and stop rules:
+ *   start: frame 0 from warm_start (NULL: zero, as the reference); frame k > 0 from
+ *          zero, or with warm_from_previous != 0 from frame k-1's returned volume
+ *          (kept on the device, no copy);
+ *   stop:  f_target NULL -- max_iterations iterations (<= 0: num_inverse_iterations);
+ *          else frame k stops after the first iteration whose objective at the
+ *          momentum point is <= f_target[k], or at max_iterations.
+ * volumes ([n_frames][Nz][H][W] at vol_loc) may be NULL; the last frame's result
+ * stays readable through rihvr_get_volume.  frame_stats ([n_frames]) may be NULL. */
+rihvr_status rihvr_reconstruct_batch(rihvr_ctx* ctx, const float* holos, int holo_loc, int32_t n_frames,
+                                     const float* warm_start, int warm_loc, int32_t warm_from_previous,
+                                     const double* f_target, int32_t max_iterations, float* volumes,
+                                     int vol_loc, rihvr_frame_stats* frame_stats);
+
 /* Stepwise form of rihvr_reconstruct, split where the reference's
  * inverseReconstruct splits (sparse_compressive_holo.cu:173-633):
  *   rihvr_begin   -- the set-up before the loop (:238-296): start point, initial

@@ -40,7 +40,7 @@ EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_d
             "rihvr_get_estimate", "rihvr_forward", "rihvr_adjoint", "rihvr_prox", "rihvr_get_params",
             "rihvr_snapshot", "rihvr_read_param_file", "rihvr_read_image", "rihvr_prepare_hologram",
             "rihvr_write_tiff8", "rihvr_write_hologram", "rihvr_save_sparse", "rihvr_save_projections",
-            "rihvr_save_planes")
+            "rihvr_save_planes", "rihvr_reconstruct_batch")
 PATH_MAX = 4096
 
 
@@ -99,6 +99,20 @@ class KernelTime(ctypes.Structure):
     """``rihvr_kernel_time`` (include/rihvr.h)."""
 
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double)]
+
+
+class FrameStats(ctypes.Structure):
+    """Mirror of ``rihvr_frame_stats`` (include/rihvr.h)."""
+
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("total_backtracks", ctypes.c_int32),
+        ("reached", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("f_initial", ctypes.c_double),
+        ("f_final", ctypes.c_double),
+        ("nnz", ctypes.c_int64),
+    ]
 
 
 class FileParams(ctypes.Structure):
@@ -235,6 +249,9 @@ def lib():
         L.rihvr_save_projections.restype = st
         L.rihvr_save_planes.argtypes = [vp, cp]
         L.rihvr_save_planes.restype = st
+        L.rihvr_reconstruct_batch.argtypes = [vp, vp, ip, ctypes.c_int32, vp, ip, ctypes.c_int32,
+                                              _P(ctypes.c_double), ctypes.c_int32, vp, ip, _P(FrameStats)]
+        L.rihvr_reconstruct_batch.restype = st
         _lib = L
     return _lib
 
@@ -368,6 +385,27 @@ class SparseCompressiveHolo:
                "rihvr_reconstruct")
         self.last_stats = sb.as_dict()
         return self.last_stats
+
+    def reconstructBatch(self, holos, warm_start=None, warm_from_previous=False, f_target=None,
+                         max_iterations=0, volumes=None) -> list:
+        """rihvr_reconstruct_batch: frames [n][H][W] one after the other; see the
+        header for the start and stop rules.  Returns one dict per frame."""
+        n = int(holos.shape[0])
+        hp, hl, _k1 = _arg(holos, (n,) + self.shape_holo)
+        wp, wl, _k2 = _arg(warm_start, self.shape_vol)
+        tg = None
+        if f_target is not None:
+            tg_arr = np.ascontiguousarray(f_target, np.float64)
+            if tg_arr.shape != (n,):
+                raise ValueError("f_target needs one value per frame")
+            tg = tg_arr.ctypes.data_as(_P(ctypes.c_double))
+        vp_, vl, _k3 = (None, HOST, None) if volumes is None else _arg(volumes, (n,) + self.shape_vol,
+                                                                       writable=True)
+        fs = (FrameStats * max(1, n))()
+        _check(lib().rihvr_reconstruct_batch(self._ctx, hp, hl, n, wp, wl, 1 if warm_from_previous else 0, tg,
+                                             int(max_iterations), vp_, vl, fs), "rihvr_reconstruct_batch")
+        return [dict(iterations=f.iterations, total_backtracks=f.total_backtracks, reached=bool(f.reached),
+                     f_initial=f.f_initial, f_final=f.f_final, nnz=f.nnz) for f in fs[:n]]
 
     # -- stepwise solve (rihvr_begin / rihvr_iterate / rihvr_finish) -------
     def begin(self, holo, warm_start=None) -> float:

@@ -91,8 +91,10 @@ struct rihvr_ctx {
     float2* T = nullptr;                          // FFT scratch volume
     float2* out = nullptr;                        // points at the returned volume
     // holograms
-    float2 *meas = nullptr, *hE = nullptr, *hC = nullptr, *hP = nullptr, *resid = nullptr, *rhat = nullptr,
-           *htmp = nullptr, *spart = nullptr;
+    // hE: estimate at the momentum point; hC/hP: A y_k, A y_{k-1}; hX: A of the prox
+    // output being tested (becomes hC once the iteration is accepted)
+    float2 *meas = nullptr, *hE = nullptr, *hC = nullptr, *hP = nullptr, *hX = nullptr, *resid = nullptr,
+           *rhat = nullptr, *htmp = nullptr, *spart = nullptr;
     // tables
     double* kappa = nullptr;
     float* zlist = nullptr;
@@ -302,7 +304,7 @@ double objective(const rihvr_ctx* c, int mode, double summag, double l1, double 
 void release(rihvr_ctx* c) {
     if (!c) return;
     for (auto& b : c->Yb) (void)hipFree(b);
-    void* ptrs[] = {c->G, c->T, c->meas, c->hE, c->hC, c->hP, c->resid, c->rhat, c->htmp, c->spart, c->kappa,
+    void* ptrs[] = {c->G, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->spart, c->kappa,
                     c->zlist, c->phase, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
                     c->dflag};
     for (void* q : ptrs) (void)hipFree(q);
@@ -342,7 +344,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     for (auto& b : c->Yb) RIHVR_TRY(dalloc(&b, c->V));
     RIHVR_TRY(dalloc(&c->G, c->V));
     RIHVR_TRY(dalloc(&c->T, c->V));
-    float2** holos[] = {&c->meas, &c->hE, &c->hC, &c->hP, &c->resid, &c->rhat, &c->htmp};
+    float2** holos[] = {&c->meas, &c->hE, &c->hC, &c->hP, &c->hX, &c->resid, &c->rhat, &c->htmp};
     for (float2** h : holos) RIHVR_TRY(dalloc(h, c->N));
     RIHVR_TRY(dalloc(&c->spart, (size_t)c->N * c->nparts));
     RIHVR_TRY(dalloc(&c->kappa, c->N));
@@ -421,12 +423,21 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     RIHVR_HIP(hipStreamSynchronize(c->stream));
     if (hflag) return RIHVR_E_INVALID_DATA;
 
-    // start point x1 = x0 = y0 = warm start or zero (:258-259, sparse_volume.cu:44-48)
-    c->iy = 0; c->iy0 = 1; c->iym = 2;
+    // start point x1 = x0 = y0 = warm start or zero (:258-259, sparse_volume.cu:44-48);
+    // a warm start that already is one of the ring volumes (the previous frame's
+    // result in a batch) becomes y_{k-1} in place
+    int in_ring = -1;
+    for (int k = 0; k < 3; ++k)
+        if (warm && warm_loc == RIHVR_DEVICE && warm == (const float*)c->Yb[k]) in_ring = k;
+    if (in_ring >= 0) {
+        c->iy0 = in_ring; c->iy = (in_ring + 1) % 3; c->iym = (in_ring + 2) % 3;
+    } else {
+        c->iy = 0; c->iy0 = 1; c->iym = 2;
+    }
     float2* Y0 = c->Yb[c->iy0];
     float2* YM = c->Yb[c->iym];
-    if (warm) RIHVR_TRY(copy_in(c, Y0, warm, warm_loc, c->V));
-    else RIHVR_HIP(hipMemsetAsync(Y0, 0, c->V * sizeof(float2), c->stream));
+    if (warm && in_ring < 0) RIHVR_TRY(copy_in(c, Y0, warm, warm_loc, c->V));
+    else if (!warm) RIHVR_HIP(hipMemsetAsync(Y0, 0, c->V * sizeof(float2), c->stream));
     RIHVR_HIP(hipMemcpyAsync(YM, Y0, c->V * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
     RIHVR_HIP(hipMemsetAsync(c->hC, 0, c->N * sizeof(float2), c->stream));
     RIHVR_HIP(hipMemsetAsync(c->hP, 0, c->N * sizeof(float2), c->stream));
@@ -474,9 +485,16 @@ rihvr_status momentum_objective(rihvr_ctx* c, double summag, double* f) {
     return RIHVR_OK;
 }
 
-rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_by_finish) {
+// f_stop > 0: stop once the objective at the momentum point of a completed iteration
+// is <= f_stop.  That objective comes with the next iteration's first prox (its x0
+// reductions), so the test runs there and a stop abandons that iteration: the ring
+// is not rotated and the estimate at the accepted point is still in hE.  *done
+// receives the number of iterations completed.
+rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_by_finish, double f_stop = 0.0,
+                          int* done = nullptr) {
     if (!c->ss.active) return RIHVR_E_INVALID_STATE;
     if (n_it < 0) return RIHVR_E_INVALID_ARGUMENT;
+    if (done) *done = n_it;
     if (n_it > 0) c->have_result = false;  // a snapshot lives in the ring slot the prox writes
     SolveState& s = c->ss;
     const int mode = s.mode;
@@ -489,19 +507,29 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
         float2* YM = c->Yb[c->iym];
         for (;;) {
             RIHVR_TRY(apply_prox(c, mode, Y0, YM, s.s_mom, c->G, s.t, Yn));
-            RIHVR_TRY(apply_forward(c, Yn, c->hE));
+            RIHVR_TRY(apply_forward(c, Yn, c->hX));
             RIHVR_TRY(fetch_scalars(c));
             const double* sc = c->h_scal;
             if (bt == 0 && s.iters > 0) {
                 // residual norm at the previous momentum point (:545-549), and the
                 // objective there from this prox's x0 reductions
                 s.prev_summag = sc[SL_RESID_M];
+                const double f_prev =
+                    objective(c, mode, s.prev_summag, sc[SL_PROX + PS_L1X0], sc[SL_PROX + PS_TVX0]);
                 if (i > 0 && st) {
                     if (st->sparsity_it) st->sparsity_it[i - 1] = sc[SL_PROX + PS_NNZX0] / (double)c->V;
                     if (st->f_resid) st->f_resid[i - 1] = 0.5 * s.prev_summag;
-                    if (st->f)
-                        st->f[i - 1] = objective(c, mode, s.prev_summag, sc[SL_PROX + PS_L1X0],
-                                                 sc[SL_PROX + PS_TVX0]);
+                    if (st->f) st->f[i - 1] = f_prev;
+                }
+                if (f_stop > 0 && f_prev <= f_stop) {
+                    // keep the accepted point: its residual est - meas into resid again
+                    // (the forward pass above overwrote it)
+                    RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
+                        return holo_residual(c->hE, c->meas, c->resid, c->N, c->partial_holo, c->stream);
+                    }));
+                    if (done) *done = i;
+                    n_it = i;
+                    break;
                 }
             }
             lim = sc[SL_PROX + PS_LS1] + sc[SL_PROX + PS_LS2] / (2 * s.t);  // :4381
@@ -515,14 +543,15 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
             }
             break;
         }
+        if (i == n_it) break;  // stopped at the objective target
         const double rs = c->h_scal[SL_PROX + PS_RESTART];
         const bool restart = rs > 0;  // :4481
         if (restart) alpha0 = 1;
         s.alpha1 = (1.0 + std::sqrt(1.0 + 4.0 * alpha0 * alpha0)) / 2.0;  // :476
         const double step = (alpha0 - 1) / s.alpha1;                       // :481
-        // hologram momentum by linearity (:494-498): prev <- cur, cur <- est, est <- cur + s (cur - prev)
+        // hologram momentum by linearity (:494-498): prev <- cur, cur <- A y_k, est <- cur + s (cur - prev)
         std::swap(c->hP, c->hC);
-        std::swap(c->hC, c->hE);
+        std::swap(c->hC, c->hX);
         RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
             return holo_momentum(c->hC, c->hP, step, c->meas, c->hE, c->resid, c->N, c->partial_holo, c->stream);
         }));
@@ -614,6 +643,56 @@ rihvr_status snapshot_impl(rihvr_ctx* c) {
     return RIHVR_OK;
 }
 
+// One frame run until the objective at the momentum point of a completed iteration
+// reaches f_target, or max_it iterations; f_target <= 0 runs max_it iterations.  The
+// objective test costs no extra pass (see iterate_impl); a start point already on
+// target runs no iteration.
+rihvr_status solve_frame(rihvr_ctx* c, const float* holo, int holo_loc, const float* warm, int warm_loc,
+                         double f_target, int max_it, rihvr_frame_stats* fs) {
+    double f0 = 0;
+    RIHVR_TRY(begin_impl(c, holo, holo_loc, warm, warm_loc, &f0));
+    int its = 0;
+    const double f = f0;
+    if (f_target > 0) {
+        if (!(f <= f_target)) RIHVR_TRY(iterate_impl(c, max_it, nullptr, true, f_target, &its));
+    } else {
+        RIHVR_TRY(iterate_impl(c, max_it, nullptr, true));
+        its = max_it;
+    }
+    rihvr_stats fin{};
+    RIHVR_TRY(finish_impl(c, &fin, -1));
+    if (fs) {
+        fs->iterations = its;
+        fs->total_backtracks = fin.total_backtracks;
+        fs->f_initial = f0;
+        fs->f_final = fin.f_final;
+        fs->nnz = fin.nnz;
+        // stopped by the target (its < max_it), or at the cap with the final objective on target
+        fs->reached = f_target > 0 ? ((its < max_it || fin.f_final <= f_target) ? 1 : 0) : 1;
+    }
+    return RIHVR_OK;
+}
+
+rihvr_status batch_impl(rihvr_ctx* c, const float* holos, int holo_loc, int n_frames, const float* warm0,
+                        int warm_loc, int warm_previous, const double* f_target, int32_t max_iterations,
+                        float* volumes, int vol_loc, rihvr_frame_stats* fs) {
+    if (!holos || n_frames < 0) return RIHVR_E_INVALID_ARGUMENT;
+    const int max_it = max_iterations > 0 ? max_iterations : c->p.num_inverse_iterations;
+    for (int k = 0; k < n_frames; ++k) {
+        const float* h = holos + 2 * (size_t)k * c->N;
+        const float* w = warm0;
+        int wl = warm_loc;
+        if (k > 0 && warm_previous) {
+            w = (const float*)c->out;  // the previous frame's result, still on the device
+            wl = RIHVR_DEVICE;
+        }
+        const double tgt = f_target ? f_target[k] : 0.0;
+        RIHVR_TRY(solve_frame(c, h, holo_loc, w, wl, tgt, max_it, fs ? fs + k : nullptr));
+        if (volumes) RIHVR_TRY(copy_out(c, volumes + 2 * (size_t)k * c->V, c->out, vol_loc, c->V));
+    }
+    return RIHVR_OK;
+}
+
 rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, const float* warm, int warm_loc,
                               rihvr_stats* st) {
     const int n_it = c->p.num_inverse_iterations;
@@ -678,6 +757,15 @@ rihvr_status rihvr_reconstruct(rihvr_ctx* ctx, const float* holo, int holo_loc, 
                                rihvr_stats* stats) {
     if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
     return reconstruct_impl(ctx, holo, holo_loc, warm, warm_loc, stats);
+}
+
+rihvr_status rihvr_reconstruct_batch(rihvr_ctx* ctx, const float* holos, int holo_loc, int32_t n_frames,
+                                     const float* warm_start, int warm_loc, int32_t warm_from_previous,
+                                     const double* f_target, int32_t max_iterations, float* volumes, int vol_loc,
+                                     rihvr_frame_stats* frame_stats) {
+    if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
+    return batch_impl(ctx, holos, holo_loc, n_frames, warm_start, warm_loc, warm_from_previous, f_target,
+                      max_iterations, volumes, vol_loc, frame_stats);
 }
 
 rihvr_status rihvr_begin(rihvr_ctx* ctx, const float* holo, int holo_loc, const float* warm_start, int warm_loc,

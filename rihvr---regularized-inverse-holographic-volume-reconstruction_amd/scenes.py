@@ -121,6 +121,72 @@ def tracers_rods_scene(optics: Optics, n_tracers: int, n_rods: int, seed: int, s
     return hologram_from_masks(masks, optics, sigma, rng), masks
 
 
+class FlowField:
+    """Seeded divergence-free in-plane velocity field from a stream function of
+    random Fourier modes on the periodic W x H domain, scaled to an rms speed in
+    pixels per frame."""
+
+    def __init__(self, W, H, rng, n_modes=16, rms=3.0, kmax=3):
+        self.W, self.H = W, H
+        ks = []
+        while len(ks) < n_modes:
+            kx, ky = rng.integers(-kmax, kmax + 1, size=2)
+            if kx or ky:
+                ks.append((kx, ky))
+        self.k = np.array(ks, np.float64) * (2 * np.pi / np.array([W, H], np.float64))
+        self.a = rng.normal(size=n_modes)
+        self.ph = rng.uniform(0, 2 * np.pi, size=n_modes)
+        gy, gx = np.mgrid[0:H:8, 0:W:8]
+        u, v = self._raw(gx.ravel().astype(np.float64), gy.ravel().astype(np.float64))
+        self.scale = rms / max(np.sqrt(np.mean(u * u + v * v)), 1e-12)
+
+    def _raw(self, x, y):
+        arg = np.outer(x, self.k[:, 0]) + np.outer(y, self.k[:, 1]) + self.ph
+        c = np.cos(arg) * self.a
+        # psi = sum a sin(k.r + ph):  u = dpsi/dy, v = -dpsi/dx
+        return c @ self.k[:, 1], -(c @ self.k[:, 0])
+
+    def __call__(self, x, y):
+        u, v = self._raw(np.asarray(x, np.float64), np.asarray(y, np.float64))
+        return u * self.scale, v * self.scale
+
+
+def sequence(optics: Optics, n_frames: int, n_tracers: int, n_rods: int, seed: int, sigma: float,
+             rms=3.0, return_tracks=False, tracer_diam=(1.0, 2.0), rod_len=(20.0, 80.0), rod_width=3.0):
+    """Time sequence of holograms: tracers and rods (fixed depth and orientation)
+    advected between frames by a FlowField.  Returns holograms [n][H][W] complex64
+    (and the tracer tracks [n][n_tracers][3] = x, y, plane with return_tracks)."""
+    rng = np.random.default_rng(seed)
+    H, W, Nz = optics.height, optics.width, optics.num_planes
+    flow = FlowField(W, H, rng, rms=rms)
+    tx, ty = rng.uniform(0, W, n_tracers), rng.uniform(0, H, n_tracers)
+    tz = rng.integers(0, Nz, n_tracers)
+    td = rng.uniform(*tracer_diam, n_tracers)
+    rx, ry, rz = rng.uniform(0, W, n_rods), rng.uniform(0, H, n_rods), rng.uniform(0, Nz, n_rods)
+    rl = rng.uniform(*rod_len, n_rods)
+    rv = rng.normal(size=(n_rods, 3))
+    rv /= np.linalg.norm(rv, axis=1, keepdims=True)
+    z_per_px = optics.resolution / optics.plane_stepsize
+    holos = np.zeros((n_frames, H, W), np.complex64)
+    tracks = np.zeros((n_frames, n_tracers, 3))
+    for f in range(n_frames):
+        masks = np.zeros((Nz, H, W), np.float32)
+        for i in range(n_tracers):
+            _draw_disc(masks[tz[i]], tx[i], ty[i], td[i] / 2)
+        for i in range(n_rods):
+            for s_ in np.arange(-rl[i] / 2, rl[i] / 2 + 1e-9, 0.5):
+                zi = int(np.floor(rz[i] + s_ * rv[i, 2] * z_per_px))
+                if 0 <= zi < Nz:
+                    _draw_disc(masks[zi], rx[i] + s_ * rv[i, 0], ry[i] + s_ * rv[i, 1], rod_width / 2)
+        holos[f] = hologram_from_masks(masks, optics, sigma, rng)
+        tracks[f] = np.stack([tx, ty, tz.astype(np.float64)], 1)
+        u, v = flow(tx, ty)
+        tx, ty = (tx + u) % W, (ty + v) % H
+        u, v = flow(rx, ry)
+        rx, ry = (rx + u) % W, (ry + v) % H
+    return (holos, tracks) if return_tracks else holos
+
+
 # Workloads (the parameter sets of the reference's CLI parameter file)
 C1 = dict(width=256, height=256, num_planes=32, start_plane=1000.0, plane_stepsize=50.0,
           wavelength=0.633, resolution=2.0, num_inverse_iterations=50, regularization_param=0.01,

@@ -57,6 +57,8 @@ def test_python_mirrors_match_header(pkg, orc):
     for py in (pkg.Stats, orc.Stats):
         assert [f[0] for f in py._fields_] == _struct_fields("rihvr_stats")
     assert [f[0] for f in pkg.KernelTime._fields_] == _struct_fields("rihvr_kernel_time")
+    assert [f[0] for f in pkg.FrameStats._fields_] == _struct_fields("rihvr_frame_stats")
+    assert ctypes.sizeof(pkg.FrameStats) == 40
     assert ctypes.sizeof(pkg.Params) == 13 * 4
 
 
@@ -87,6 +89,28 @@ def test_scene_generator_is_seeded_and_normalised(scenes):
     assert a.dtype == np.complex64 and np.all(a.imag == 0)
     assert abs(float(a.real.mean())) < 1e-5          # b = I / mean(I) - 1
     assert m.sum() > 0
+
+
+def test_sequence_is_seeded_and_moves_smoothly(scenes):
+    cfg = dict(width=64, height=64, num_planes=6, start_plane=1000.0, plane_stepsize=50.0, wavelength=0.633,
+               resolution=2.0)
+    o = scenes.optics_of(cfg)
+    holos, tracks = scenes.sequence(o, 4, n_tracers=12, n_rods=1, seed=41, sigma=0.01, return_tracks=True)
+    again = scenes.sequence(o, 4, n_tracers=12, n_rods=1, seed=41, sigma=0.01)
+    np.testing.assert_array_equal(holos, again)
+    assert holos.shape == (4, 64, 64) and holos.dtype == np.complex64
+    d = np.diff(tracks, axis=0)[..., :2]
+    d = (d + 32) % 64 - 32  # periodic wrap
+    step = np.sqrt((d ** 2).sum(-1))
+    assert 1.0 < float(np.sqrt((step ** 2).mean())) < 6.0  # about 3 px per frame
+    assert np.all(np.diff(tracks, axis=0)[..., 2] == 0)  # depth is kept
+    # the field is divergence-free: a finite-difference check on a grid
+    f = scenes.FlowField(64, 64, np.random.default_rng(5))
+    y, x = np.mgrid[0:64, 0:64].astype(np.float64)
+    h = 1e-4
+    du = (f(x.ravel() + h, y.ravel())[0] - f(x.ravel() - h, y.ravel())[0]) / (2 * h)
+    dv = (f(x.ravel(), y.ravel() + h)[1] - f(x.ravel(), y.ravel() - h)[1]) / (2 * h)
+    assert np.abs(du + dv).max() < 1e-6 * max(np.abs(du).max(), 1.0)
 
 
 def test_scene_objects_refocus_under_the_adjoint(orc, scenes):
