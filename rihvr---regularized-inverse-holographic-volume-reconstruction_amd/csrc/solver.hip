@@ -74,6 +74,7 @@ struct SolveState {
     double f_initial = 0;
     int total_bt = 0;
     int iters = 0;
+    bool g_ready = false;     // G = A^T r of the current residual (else only R-hat is)
 };
 
 }  // namespace
@@ -83,7 +84,10 @@ struct rihvr_ctx {
     int W = 0, H = 0, Nz = 0;
     long N = 0, V = 0;
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // every launch and copy, in order
+    hipStream_t aux = nullptr;     // A^T of the next plane group while prox and A run on `stream`
+    std::vector<hipEvent_t> ev_adj;  // per plane group: its G planes are written (on aux)
+    hipEvent_t ev_rhat = nullptr;    // R-hat is written (on stream)
     // volumes
     float2* Yb[3] = {nullptr, nullptr, nullptr};  // ring of prox outputs
     int iy = 0, iy0 = 1, iym = 2;                 // y_k, y_{k-1}, y_{k-2}
@@ -108,7 +112,8 @@ struct rihvr_ctx {
     double* h_scal = nullptr;
     int* dflag = nullptr;
     int zchunk_adj = 8, zchunk_fwd = 8, nparts = 1;
-    int plane_group = 1;  // planes per row+column FFT pass group
+    int pipe_planes = 1;  // planes per pipeline group (a multiple of the z chunks)
+    int n_groups = 1;
     int prox_rows = 0;    // output rows per prox wave (0: default)
     bool have_result = false;
     bool force_step = false;
@@ -172,8 +177,9 @@ rihvr_status copy_out(rihvr_ctx* c, float* dst, const float2* src, int loc, size
     return RIHVR_OK;
 }
 
-// Launches through `launch` (returns hipError_t), bracketed by events when timing is on.
-template <class F> rihvr_status timed(rihvr_ctx* c, int id, F&& launch) {
+// Launches through `launch` (returns hipError_t) on stream `st`, bracketed by events
+// on that stream when timing is on.
+template <class F> rihvr_status timed_on(rihvr_ctx* c, int id, hipStream_t st, F&& launch) {
     if (!c->tm.on) {
         RIHVR_HIP(launch());
         return RIHVR_OK;
@@ -187,11 +193,14 @@ template <class F> rihvr_status timed(rihvr_ctx* c, int id, F&& launch) {
             c->tm.pool.pop_back();
         }
     }
-    RIHVR_HIP(hipEventRecord(ev[0], c->stream));
+    RIHVR_HIP(hipEventRecord(ev[0], st));
     RIHVR_HIP(launch());
-    RIHVR_HIP(hipEventRecord(ev[1], c->stream));
+    RIHVR_HIP(hipEventRecord(ev[1], st));
     c->tm.pending.push_back({id, ev[0], ev[1]});
     return RIHVR_OK;
+}
+template <class F> rihvr_status timed(rihvr_ctx* c, int id, F&& launch) {
+    return timed_on(c, id, c->stream, std::forward<F>(launch));
 }
 
 // Accumulates the durations of completed event pairs (call after a stream sync).
@@ -214,45 +223,67 @@ rihvr_status fetch_scalars(rihvr_ctx* c) {
     return resolve_timing(c);
 }
 
-// est = A vol, resid = est - meas, scal[SL_RESID_A] = sum |resid|^2
-// (backPropagate :1935-2077 + calcResidual :2676-2723 + calcSumMagnitude)
-rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
+// A for planes [g0, g0 + gz) on stream st: the column pass of vol * e^{-i phi} into the
+// scratch planes T[g0..], then the row pass with the H_{-z}/N z-sum into the partial
+// spectra of those planes' z chunks (backPropagate :1935-2077)
+rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipStream_t st) {
     const float invN = (float)(1.0 / (double)c->N);
-    // planes go through the row and column passes a group at a time, so the
-    // row-pass output of a group (group * W * H * 8 bytes) can stay in cache
-    for (int g0 = 0; g0 < c->Nz; g0 += c->plane_group) {
-        const int gz = std::min(c->plane_group, c->Nz - g0);
-        const float2* vg = vol + (size_t)g0 * c->N;
-        float2* spg = c->spart + (size_t)(g0 / c->zchunk_fwd) * c->N;
-        RIHVR_TRY(timed(c, K_COLS_FWD, [&] {
-            return fft_cols_fwd(c->H, vg, c->phase + g0, c->T, c->W, gz, c->twfH, c->stream);
-        }));
-        RIHVR_TRY(timed(c, K_ROWS_FWDACC, [&] {
-            return fft_rows_fwdacc(c->W, c->T, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, invN, c->twfW,
-                                   c->stream);
-        }));
-    }
+    float2* Tg = c->T + (size_t)g0 * c->N;
+    float2* spg = c->spart + (size_t)(g0 / c->zchunk_fwd) * c->N;
+    RIHVR_TRY(timed_on(c, K_COLS_FWD, st, [&] {
+        return fft_cols_fwd(c->H, vol + (size_t)g0 * c->N, c->phase + g0, Tg, c->W, gz, c->twfH, st);
+    }));
+    RIHVR_TRY(timed_on(c, K_ROWS_FWDACC, st, [&] {
+        return fft_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, invN, c->twfW, st);
+    }));
+    return RIHVR_OK;
+}
+
+// The end of A: sum of the partial spectra, inverse row and column FFT, real part;
+// est, resid = est - meas and scal[SL_RESID_A] = sum |resid|^2 (calcResidual :2676-2723
+// + calcSumMagnitude)
+rihvr_status forward_finish(rihvr_ctx* c, float2* est) {
     RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] { return fft_rows_sumparts(c->W, c->spart, c->nparts, c->N, c->htmp, c->H, c->twiW, c->stream); }));
     RIHVR_TRY(timed(c, K_COLS_REAL, [&] { return fft_cols_real(c->H, c->htmp, est, c->meas, c->resid, c->partial_holo, c->W, c->twiH, c->stream); }));
     RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial_holo, c->W / 16, 1, c->scal + SL_RESID_A, c->stream); }));
     return RIHVR_OK;
 }
 
-// G = A^T resid  (SparseGradient::reconstruct + getPlane, sparse_gradient.cu:47-165)
-rihvr_status apply_adjoint(rihvr_ctx* c, const float2* r, float2* G) {
-    const float invN = (float)(1.0 / (double)c->N);
+rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
+    for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
+        RIHVR_TRY(forward_group(c, vol, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
+    return forward_finish(c, est);
+}
+
+// R-hat = FFT(r) (SparseGradient::reconstruct, sparse_gradient.cu:47-75); ev_rhat marks it
+rihvr_status spectrum(rihvr_ctx* c, const float2* r) {
     RIHVR_TRY(timed(c, K_ROWS_PLAIN, [&] { return fft_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream); }));
     RIHVR_TRY(timed(c, K_COLS_PLAIN, [&] { return fft_cols_plain(c->H, -1, c->htmp, c->rhat, c->W, 1, c->twfH, c->stream); }));
-    for (int g0 = 0; g0 < c->Nz; g0 += c->plane_group) {
-        const int gz = std::min(c->plane_group, c->Nz - g0);
-        RIHVR_TRY(timed(c, K_ROWS_ADJ, [&] {
-            return fft_rows_adj(c->W, c->rhat, c->kappa, c->zlist + g0, c->T, c->H, gz, std::min(c->zchunk_adj, gz),
-                                invN, c->twiW, c->stream);
-        }));
-        RIHVR_TRY(timed(c, K_COLS_ADJ, [&] {
-            return fft_cols_adj(c->H, c->T, c->phase + g0, G + (size_t)g0 * c->N, c->W, gz, c->twiH, c->stream);
-        }));
-    }
+    RIHVR_HIP(hipEventRecord(c->ev_rhat, c->stream));
+    return RIHVR_OK;
+}
+
+// A^T from R-hat for planes [g0, g0 + gz) on stream st (SparseGradient::getPlane,
+// sparse_gradient.cu:125-165): row pass of H_z R-hat / N into T[g0..], column pass
+// times e^{+i phi_z} into G[g0..]
+rihvr_status adjoint_group(rihvr_ctx* c, float2* G, int g0, int gz, hipStream_t st) {
+    const float invN = (float)(1.0 / (double)c->N);
+    float2* Tg = c->T + (size_t)g0 * c->N;
+    RIHVR_TRY(timed_on(c, K_ROWS_ADJ, st, [&] {
+        return fft_rows_adj(c->W, c->rhat, c->kappa, c->zlist + g0, Tg, c->H, gz, std::min(c->zchunk_adj, gz), invN,
+                            c->twiW, st);
+    }));
+    RIHVR_TRY(timed_on(c, K_COLS_ADJ, st, [&] {
+        return fft_cols_adj(c->H, Tg, c->phase + g0, G + (size_t)g0 * c->N, c->W, gz, c->twiH, st);
+    }));
+    return RIHVR_OK;
+}
+
+// G = A^T r on the main stream
+rihvr_status apply_adjoint(rihvr_ctx* c, const float2* r, float2* G) {
+    RIHVR_TRY(spectrum(c, r));
+    for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
+        RIHVR_TRY(adjoint_group(c, G, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
     return RIHVR_OK;
 }
 
@@ -292,6 +323,41 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
     return RIHVR_OK;
 }
 
+// First pass of an iteration whose gradient is not formed yet (mode 7): A^T of plane
+// group g runs on the auxiliary stream while the prox and A of earlier groups run on
+// the main stream; the main stream waits for group g's G planes before its prox.
+// Same kernels, same arithmetic and the same partial-row layout as the one-stream
+// sequence A^T (all planes), prox (all planes), A (all planes).
+rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, float s_mom, double t, float2* Yn) {
+    RIHVR_HIP(hipStreamWaitEvent(c->aux, c->ev_rhat, 0));
+    for (int g = 0, g0 = 0; g < c->n_groups; ++g, g0 += c->pipe_planes) {
+        RIHVR_TRY(adjoint_group(c, c->G, g0, std::min(c->pipe_planes, c->Nz - g0), c->aux));
+        RIHVR_HIP(hipEventRecord(c->ev_adj[g], c->aux));
+    }
+    ProxLaunch a;
+    a.W = c->W; a.H = c->H;
+    a.s_mom = s_mom; a.t = t;
+    a.reg = (double)c->p.regularization_param;
+    a.reg_tv = (double)c->p.regularization_TV_param;
+    a.n_tv = c->p.num_tv_iterations;
+    a.row_chunk = c->prox_rows;
+    const int per_plane = prox_fl2d_num_waves(c->W, c->H, 1, a.n_tv, c->prox_rows);
+    for (int g = 0, g0 = 0; g < c->n_groups; ++g, g0 += c->pipe_planes) {
+        const int gz = std::min(c->pipe_planes, c->Nz - g0);
+        const size_t off = (size_t)g0 * c->N;
+        RIHVR_HIP(hipStreamWaitEvent(c->stream, c->ev_adj[g], 0));
+        a.Y0 = Y0 + off; a.YM = YM + off; a.G = c->G + off; a.Y = Yn + off;
+        a.nz = gz;
+        a.partial = c->partial + (size_t)g0 * per_plane * PS_COUNT;
+        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
+        RIHVR_TRY(forward_group(c, Yn, g0, gz, c->stream));
+    }
+    RIHVR_TRY(timed(c, K_REDUCE, [&] {
+        return reduce_partials(c->partial, per_plane * c->Nz, PS_COUNT, c->scal + SL_PROX, c->stream);
+    }));
+    return forward_finish(c, c->hX);
+}
+
 double objective(const rihvr_ctx* c, int mode, double summag, double l1, double tv) {
     // calcObjectiveFunction (:2838-2899)
     double f = 0.5 * summag + (double)c->p.regularization_param * l1;
@@ -311,6 +377,9 @@ void release(rihvr_ctx* c) {
     for (auto& p : c->tm.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : c->tm.pool) (void)hipEventDestroy(e);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
+    for (hipEvent_t e : c->ev_adj) (void)hipEventDestroy(e);
+    if (c->ev_rhat) (void)hipEventDestroy(c->ev_rhat);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -326,21 +395,27 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     c->V = c->N * c->Nz;
     RIHVR_HIP(hipGetDevice(&c->device));
     RIHVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    RIHVR_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+    RIHVR_HIP(hipEventCreateWithFlags(&c->ev_rhat, hipEventDisableTiming));
     int zc = 16;
     if (const char* e = std::getenv("RIHVR_ZCHUNK")) zc = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("RIHVR_PROX_ROWS")) c->prox_rows = std::max(0, std::atoi(e));
     c->zchunk_fwd = std::min(c->Nz, zc);
     c->nparts = (c->Nz + c->zchunk_fwd - 1) / c->zchunk_fwd;
     c->zchunk_adj = std::min(c->Nz, 8);
-    // plane group: all planes, or RIHVR_PLANE_GROUP (rounded to a multiple of the z chunk)
-    c->plane_group = c->Nz;
-    if (const char* e = std::getenv("RIHVR_PLANE_GROUP")) {
-        int g = std::atoi(e);
-        if (g > 0) {
-            g = ((g + c->zchunk_fwd - 1) / c->zchunk_fwd) * c->zchunk_fwd;
-            c->plane_group = std::min(c->Nz, g);
-        }
-    }
+    // pipeline groups: one group (everything on the main stream) unless RIHVR_PIPE_PLANES
+    // asks for groups of that many planes (rounded up to a multiple of 16).  Measured at
+    // 1024^2 x 128 on MI355X the two-stream overlap loses (groups of 32: 287 iter/s,
+    // 64: 308, one stream: 316): the prox and the column FFTs each fill the CUs' LDS, so
+    // they time-share CUs instead of co-issuing
+    int pp = c->Nz;
+    if (const char* e = std::getenv("RIHVR_PIPE_PLANES")) pp = std::atoi(e) > 0 ? std::atoi(e) : c->Nz;
+    pp = ((pp + 15) / 16) * 16;
+    c->pipe_planes = std::min(c->Nz, std::max(16, pp));
+    if (c->pipe_planes % c->zchunk_fwd || c->pipe_planes % c->zchunk_adj) c->pipe_planes = c->Nz;
+    c->n_groups = (c->Nz + c->pipe_planes - 1) / c->pipe_planes;
+    c->ev_adj.assign(c->n_groups, nullptr);
+    for (hipEvent_t& e : c->ev_adj) RIHVR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& b : c->Yb) RIHVR_TRY(dalloc(&b, c->V));
     RIHVR_TRY(dalloc(&c->G, c->V));
     RIHVR_TRY(dalloc(&c->T, c->V));
@@ -462,7 +537,7 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     c->ss.prev_summag = c->h_scal[SL_RESID_A];
     c->ss.f_initial = objective(c, c->ss.mode, c->ss.prev_summag, l1, tv);
     if (f_initial) *f_initial = c->ss.f_initial;
-    RIHVR_TRY(apply_adjoint(c, c->resid, c->G));
+    RIHVR_TRY(spectrum(c, c->resid));  // the gradient's planes are formed with the first prox
     c->ss.t = 1.0 / (double)c->Nz;  // :247 (a forced step is overridden there too)
     c->ss.alpha1 = 1.0;
     c->ss.s_mom = 0.0f;             // x0 = Y0 for the first prox
@@ -506,8 +581,16 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
         float2* Y0 = c->Yb[c->iy0];
         float2* YM = c->Yb[c->iym];
         for (;;) {
-            RIHVR_TRY(apply_prox(c, mode, Y0, YM, s.s_mom, c->G, s.t, Yn));
-            RIHVR_TRY(apply_forward(c, Yn, c->hX));
+            if (!s.g_ready && mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && c->n_groups > 1) {
+                RIHVR_TRY(pipelined_pass(c, Y0, YM, s.s_mom, s.t, Yn));
+            } else {
+                if (!s.g_ready)
+                    for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
+                        RIHVR_TRY(adjoint_group(c, c->G, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
+                RIHVR_TRY(apply_prox(c, mode, Y0, YM, s.s_mom, c->G, s.t, Yn));
+                RIHVR_TRY(apply_forward(c, Yn, c->hX));
+            }
+            s.g_ready = true;
             RIHVR_TRY(fetch_scalars(c));
             const double* sc = c->h_scal;
             if (bt == 0 && s.iters > 0) {
@@ -558,7 +641,9 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
         RIHVR_TRY(timed(c, K_REDUCE, [&] {
             return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->scal + SL_RESID_M, c->stream);
         }));
-        RIHVR_TRY(apply_adjoint(c, c->resid, c->G));  // the next gradient (:545-546)
+        // the next gradient (:545-546): R-hat now, its planes with the next prox
+        RIHVR_TRY(spectrum(c, c->resid));
+        s.g_ready = false;
         // rotate the prox-output ring: y_{k-2} <- y_{k-1} <- y_k
         const int old_m = c->iym;
         c->iym = c->iy0;
@@ -748,6 +833,7 @@ rihvr_status rihvr_create(const rihvr_params* params, rihvr_ctx** out) {
 rihvr_status rihvr_destroy(rihvr_ctx* ctx) {
     if (!ctx) return RIHVR_OK;
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
     release(ctx);
     delete ctx;
     return RIHVR_OK;
