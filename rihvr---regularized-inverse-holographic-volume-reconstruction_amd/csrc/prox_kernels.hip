@@ -17,8 +17,8 @@
 // every stencil neighbour in y is a register of the same lane (the row above)
 // and every neighbour in x is a lane shuffle.  Rows and columns outside the
 // strip's output range are recomputed halo (n_tv on each side in x, and at the
-// ends of the row chunk in y).  No workgroup barriers; per-lane history rings
-// of x-hat, x0, g and y_{k-1} live in LDS.
+// ends of the row chunk in y).  No workgroup barriers and no LDS: the per-lane
+// history of x-hat, x0, g and y_{k-1} lives in registers.
 #include <utility>
 
 #include "device_common.hpp"
@@ -92,45 +92,53 @@ __device__ __forceinline__ float soft_fast(float v, float thr) {
 }
 
 // One wave's state: the TV duals g_k (gx.re, gx.im, gy.re, gy.im) and y-hat_k of
-// the last two rows of every level, indexed [row parity][level]; a per-lane LDS
-// ring holds x-hat, x0, g and y_{k-1} of the last NTV + 3 rows.  The row loop is
-// unrolled by U (a multiple of the ring length and of 2), so every ring slot and
-// row parity is a compile-time constant.
+// the last two rows of every level, indexed [row parity][level]; a per-lane
+// history ring holds x-hat, x0, g and y_{k-1} of the last NTV + 3 rows.  The row
+// loop is unrolled by U (a multiple of the ring length and of 2), so every ring
+// slot and row parity is a compile-time constant and the ring is registers.
 template <int NTV> struct ProxWave {
     static constexpr int P = NTV + 3;
     static constexpr int U = (P % 2 == 0) ? P : 2 * P;
     float4 gd[2][NTV + 1];
     float2 yh[2][NTV + 1];
+    // rows tau and tau + 1 of y_{k-1}, y_{k-2} and g, loaded two steps ahead
+    float2 pf[2][3];
+    // x-hat, x0, g and y_{k-1} of the last P rows, indexed by compile-time slots: registers
+    float2 hist[4][P];
     float acc_ls1 = 0.f, acc_ls2 = 0.f, acc_rs = 0.f, acc_l1 = 0.f, acc_tv = 0.f, acc_nnz = 0.f;
+
+    __device__ __forceinline__ void load_row(const ProxDev& a, int slot, int tau, size_t plane_off, int xcol) {
+        int row = tau % a.H;
+        row = row < 0 ? row + a.H : row;
+        const size_t ro = plane_off + (size_t)row * a.W + xcol;
+        pf[slot][0] = a.Y0[ro];
+        pf[slot][1] = a.YM[ro];
+        pf[slot][2] = a.G[ro];
+    }
 
     // Step S of an unrolled block: row tau, ring slot S mod P, row parity S & 1.
     template <int S>
-    __device__ __forceinline__ void step(const ProxDev& a, float2 (*ring)[P][64], int lane, int tau, int r0, int r1,
-                                         size_t plane_off, int xcol, int ocol, bool out_lane) {
+    __device__ __forceinline__ void step(const ProxDev& a, int tau, int r0, int r1,
+                                         size_t plane_off, int xcol, int ocol, bool out_lane, int tau_end) {
         constexpr int SLOT = S % P;
         constexpr int PAR = S & 1;
         constexpr int Q = PAR ^ 1;
-        const int W = a.W, H = a.H;
+        const int W = a.W;
         // ---- row tau: momentum point x0 = y0 + s (y0 - ym), x-hat = x0 - t g ----
         {
-            int row = tau % H;
-            row = row < 0 ? row + H : row;
-            const size_t ro = plane_off + (size_t)row * W;
-            const float2* __restrict__ py0 = a.Y0 + ro;
-            const float2* __restrict__ pym = a.YM + ro;
-            const float2* __restrict__ pg = a.G + ro;
-            const float2 y0 = py0[xcol], ym = pym[xcol], g = pg[xcol];
+            const float2 y0 = pf[PAR][0], ym = pf[PAR][1], g = pf[PAR][2];
             const float2 x0 = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
-            ring[0][SLOT][lane] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
-            ring[1][SLOT][lane] = x0;
-            ring[2][SLOT][lane] = g;
-            ring[3][SLOT][lane] = y0;
+            hist[0][SLOT] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
+            hist[1][SLOT] = x0;
+            hist[2][SLOT] = g;
+            hist[3][SLOT] = y0;
+            if (tau + 2 <= tau_end) load_row(a, PAR, tau + 2, plane_off, xcol);
         }
         // ---- TV levels: level k works on row tau - k ----
 #pragma unroll
         for (int k = 1; k <= NTV; ++k) {
             const int sk = (SLOT - k + 2 * P) % P;
-            const float2 xh = ring[0][sk][lane];
+            const float2 xh = hist[0][sk];
             float2 y;
             float4 go;  // g_{k-1} at row tau - k
             if (k == 1) {
@@ -158,7 +166,7 @@ template <int NTV> struct ProxWave {
         const int o = tau - NTV - 1;
         constexpr int SO = (SLOT - NTV - 1 + 2 * P) % P;
         constexpr int SU = (SO - 1 + P) % P;
-        const float2 xh = ring[0][SO][lane];
+        const float2 xh = hist[0][SO];
         float2 yv;
         if constexpr (NTV == 0) {
             yv = xh;
@@ -170,15 +178,15 @@ template <int NTV> struct ProxWave {
             yv = make_float2(fmaf(-a.mu, dvr, xh.x), fmaf(-a.mu, dvi, xh.y));
         }
         if (a.do_soft) yv = make_float2(soft_fast(yv.x, a.thr), soft_fast(yv.y, a.thr));
-        const float2 x0o = ring[1][SO][lane];
+        const float2 x0o = hist[1][SO];
         const float x0lr = lane_from_left(x0o.x);
         const float x0li = lane_from_left(x0o.y);
         if (o >= r0 && o < r1 && out_lane) {
             float2* __restrict__ py = a.Y + plane_off + (size_t)o * W;
             py[ocol] = yv;
-            const float2 x0u = ring[1][SU][lane];
-            const float2 g = ring[2][SO][lane];
-            const float2 y0 = ring[3][SO][lane];
+            const float2 x0u = hist[1][SU];
+            const float2 g = hist[2][SO];
+            const float2 y0 = hist[3][SO];
             const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
             acc_ls1 = fmaf(dr, g.x, fmaf(di, g.y, acc_ls1));
             acc_ls2 = fmaf(dr, dr, fmaf(di, di, acc_ls2));
@@ -186,25 +194,27 @@ template <int NTV> struct ProxWave {
             const float dim = (x0o.y - yv.y) * (yv.y - y0.y);
             const int off = ((o * W) + ocol) & 1023;
             acc_rs += off < 512 ? fmaf(2.0f, dim, dre) : dim;
+            // objective terms of x0: the hardware square root (about 1 ulp) feeds sums
+            // accumulated per wave and reported in double
             const float m2 = fmaf(x0o.x, x0o.x, x0o.y * x0o.y);
-            acc_l1 += __builtin_sqrtf(m2);
+            acc_l1 += __builtin_amdgcn_sqrtf(m2);
             acc_nnz += (x0o.x != 0.0f || x0o.y != 0.0f) ? 1.0f : 0.0f;
             const float gxx = x0lr - x0o.x, gyx = x0u.x - x0o.x, gxy = x0li - x0o.y, gyy = x0u.y - x0o.y;
-            acc_tv += __builtin_sqrtf(fmaf(gxx, gxx, fmaf(gxy, gxy, fmaf(gyx, gyx, gyy * gyy))));
+            acc_tv += __builtin_amdgcn_sqrtf(fmaf(gxx, gxx, fmaf(gxy, gxy, fmaf(gyx, gyx, gyy * gyy))));
         }
     }
 
     template <int... S>
-    __device__ __forceinline__ void block(std::integer_sequence<int, S...>, const ProxDev& a, float2 (*ring)[P][64],
-                                          int lane, int tau, int r0, int r1, size_t plane_off, int xcol, int ocol,
-                                          bool out_lane) {
-        (step<S>(a, ring, lane, tau + S, r0, r1, plane_off, xcol, ocol, out_lane), ...);
+    __device__ __forceinline__ void block(std::integer_sequence<int, S...>, const ProxDev& a,
+                                          int tau, int r0, int r1, size_t plane_off, int xcol, int ocol,
+                                          bool out_lane, int tau_end) {
+        (step<S>(a, tau + S, r0, r1, plane_off, xcol, ocol, out_lane, tau_end), ...);
     }
     template <int... S>
     __device__ __forceinline__ void tail(std::integer_sequence<int, S...>, int rem, const ProxDev& a,
-                                         float2 (*ring)[P][64], int lane, int tau, int r0, int r1, size_t plane_off,
-                                         int xcol, int ocol, bool out_lane) {
-        ((S < rem ? step<S>(a, ring, lane, tau + S, r0, r1, plane_off, xcol, ocol, out_lane) : void()), ...);
+                                         int tau, int r0, int r1, size_t plane_off,
+                                         int xcol, int ocol, bool out_lane, int tau_end) {
+        ((S < rem ? step<S>(a, tau + S, r0, r1, plane_off, xcol, ocol, out_lane, tau_end) : void()), ...);
     }
 };
 
@@ -213,7 +223,6 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     using PW = ProxWave<NTV>;
     constexpr int P = PW::P;
     constexpr int U = PW::U;
-    __shared__ float2 ring_s[kProxWaves][4][P][64];  // fields: x-hat, x0, g, y_{k-1}
     const int lane = threadIdx.x & 63;
     // wave index, made wave-uniform for the compiler so all row/plane
     // arithmetic below stays in scalar registers
@@ -233,7 +242,6 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     const int r0 = chunk * geo.rc;
     const int r1 = min(a.H, r0 + geo.rc);
     const size_t plane_off = (size_t)z * a.H * W;
-    float2 (*ring)[P][64] = ring_s[wl];
 
     PW st;
 #pragma unroll
@@ -245,12 +253,16 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
         }
     const int tau0 = r0 - (NTV > 0 ? NTV : 1);
     const int tau1 = r1 + NTV;  // inclusive
+    // the first two rows; every step then loads the row two ahead (U is even)
+    st.load_row(a, 0, tau0, plane_off, xcol);
+    st.load_row(a, 1, tau0 + 1, plane_off, xcol);
     int tau = tau0;
     for (; tau + U - 1 <= tau1; tau += U)
-        st.block(std::make_integer_sequence<int, U>{}, a, ring, lane, tau, r0, r1, plane_off, xcol, ocol, out_lane);
+        st.block(std::make_integer_sequence<int, U>{}, a, tau, r0, r1, plane_off, xcol, ocol, out_lane,
+                 tau1);
     if (tau <= tau1)
-        st.tail(std::make_integer_sequence<int, U - 1>{}, tau1 - tau + 1, a, ring, lane, tau, r0, r1, plane_off, xcol,
-                ocol, out_lane);
+        st.tail(std::make_integer_sequence<int, U - 1>{}, tau1 - tau + 1, a, tau, r0, r1, plane_off, xcol,
+                ocol, out_lane, tau1);
 
     const double s_ls1 = wave_sum((double)st.acc_ls1);
     const double s_ls2 = wave_sum((double)st.acc_ls2);
