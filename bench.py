@@ -171,7 +171,8 @@ def main():
 
     elapsed = t1 - t0
     ips = args.steps / elapsed
-    nparts = (Nz + int(os.environ.get("RIHVR_ZCHUNK", "16")) - 1) // int(os.environ.get("RIHVR_ZCHUNK", "16"))
+    zchunk = min(Nz, int(os.environ.get("RIHVR_ZCHUNK", "32")))  # planes per partial spectrum (library default 32)
+    nparts = (Nz + zchunk - 1) // zchunk
     # dominant kernel: the largest share of GPU time in the timed region
     dom = max(timing.items(), key=lambda kv: kv[1][1])
     dom_name, (dom_launches, dom_ms) = dom

@@ -92,8 +92,10 @@ hipError_t holo_momentum(const float2* cur, const float2* prev, double s_mom, co
 hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, long n, double* partial,
                          hipStream_t s);
 
-// sums rows of partial [nrows][ncols] into out[ncols] (fixed order: deterministic)
-hipError_t reduce_partials(const double* partial, int nrows, int ncols, double* out, hipStream_t s);
+// sums rows of partial [nrows][ncols] into out[ncols] (fixed order: deterministic);
+// work holds kReduceBlocks * ncols doubles
+constexpr int kReduceBlocks = 64;
+hipError_t reduce_partials(const double* partial, int nrows, int ncols, double* work, double* out, hipStream_t s);
 
 // flags[0] = 1 if any element of a [n] complex array is not finite
 hipError_t check_finite(const float2* a, long n, int* flag, hipStream_t s);

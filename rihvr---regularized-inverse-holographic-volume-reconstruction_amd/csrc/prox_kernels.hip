@@ -638,18 +638,41 @@ hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, l
     return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ partial, int nrows, int ncols,
-                                                         double* out) {
-    __shared__ double red[4];
-    const int q = blockIdx.x;
-    double acc = 0;
-    for (int r = threadIdx.x; r < nrows; r += 256) acc += partial[(size_t)r * ncols + q];
-    double v = block_sum(acc, red);
-    if (threadIdx.x == 0) out[q] = v;
+// Two fixed-order passes (deterministic): kReduceBlocks workgroups each sum a
+// contiguous slice of rows of every column (row-contiguous, coalesced loads) into
+// work[block][ncols]; one workgroup then sums those rows in block order.
+constexpr int kReduceThreads = 256;
+__global__ void __launch_bounds__(kReduceThreads) k_reduce_rows(const double* __restrict__ partial, int nrows,
+                                                                int ncols, int rows_per_block, double* out) {
+    __shared__ double red[kReduceThreads / 64];
+    double acc[PS_COUNT];
+#pragma unroll
+    for (int q = 0; q < PS_COUNT; ++q) acc[q] = 0.0;
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = min(nrows, r0 + rows_per_block);
+    for (int r = r0 + threadIdx.x; r < r1; r += kReduceThreads) {
+        const double* row = partial + (size_t)r * ncols;
+#pragma unroll
+        for (int q = 0; q < PS_COUNT; ++q)
+            if (q < ncols) acc[q] += row[q];
+    }
+#pragma unroll
+    for (int q = 0; q < PS_COUNT; ++q) {
+        if (q >= ncols) break;
+        const double v = block_sum(acc[q], red);
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * ncols + q] = v;
+    }
 }
 
-hipError_t reduce_partials(const double* partial, int nrows, int ncols, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(ncols), dim3(256), 0, s, partial, nrows, ncols, out);
+hipError_t reduce_partials(const double* partial, int nrows, int ncols, double* work, double* out, hipStream_t s) {
+    if (ncols < 1 || ncols > PS_COUNT || nrows < 0) return hipErrorInvalidValue;
+    const int rpb = (nrows + kReduceBlocks - 1) / kReduceBlocks;
+    const int nb = rpb > 0 ? (nrows + rpb - 1) / rpb : 1;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(nb), dim3(kReduceThreads), 0, s, partial, nrows, ncols, rpb > 0 ? rpb : 1,
+                       work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kReduceThreads), 0, s, (const double*)work, nb, ncols, nb, out);
     return hipGetLastError();
 }
 

@@ -109,6 +109,7 @@ struct rihvr_ctx {
     double* partial_holo = nullptr;
     long partial_rows = 0, holo_rows = 0;
     double* scal = nullptr;
+    double* red_work = nullptr;  // first-pass sums of reduce_partials
     double* h_scal = nullptr;
     int* dflag = nullptr;
     int zchunk_adj = 8, zchunk_fwd = 8, nparts = 1;
@@ -245,7 +246,7 @@ rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipS
 rihvr_status forward_finish(rihvr_ctx* c, float2* est) {
     RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] { return fft_rows_sumparts(c->W, c->spart, c->nparts, c->N, c->htmp, c->H, c->twiW, c->stream); }));
     RIHVR_TRY(timed(c, K_COLS_REAL, [&] { return fft_cols_real(c->H, c->htmp, est, c->meas, c->resid, c->partial_holo, c->W, c->twiH, c->stream); }));
-    RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial_holo, c->W / 16, 1, c->scal + SL_RESID_A, c->stream); }));
+    RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial_holo, c->W / 16, 1, c->red_work, c->scal + SL_RESID_A, c->stream); }));
     return RIHVR_OK;
 }
 
@@ -319,7 +320,7 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_l1(a, c->stream); }));
         rows = prox_l1_num_blocks(c->W, c->H, c->Nz);
     }
-    RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial, rows, PS_COUNT, c->scal + SL_PROX, c->stream); }));
+    RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial, rows, PS_COUNT, c->red_work, c->scal + SL_PROX, c->stream); }));
     return RIHVR_OK;
 }
 
@@ -353,7 +354,7 @@ rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, fl
         RIHVR_TRY(forward_group(c, Yn, g0, gz, c->stream));
     }
     RIHVR_TRY(timed(c, K_REDUCE, [&] {
-        return reduce_partials(c->partial, per_plane * c->Nz, PS_COUNT, c->scal + SL_PROX, c->stream);
+        return reduce_partials(c->partial, per_plane * c->Nz, PS_COUNT, c->red_work, c->scal + SL_PROX, c->stream);
     }));
     return forward_finish(c, c->hX);
 }
@@ -372,7 +373,7 @@ void release(rihvr_ctx* c) {
     for (auto& b : c->Yb) (void)hipFree(b);
     void* ptrs[] = {c->G, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->spart, c->kappa,
                     c->zlist, c->phase, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
-                    c->dflag};
+                    c->red_work, c->dflag};
     for (void* q : ptrs) (void)hipFree(q);
     for (auto& p : c->tm.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : c->tm.pool) (void)hipEventDestroy(e);
@@ -397,12 +398,14 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     RIHVR_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
     RIHVR_HIP(hipEventCreateWithFlags(&c->ev_rhat, hipEventDisableTiming));
-    int zc = 16;
+    int zc = 32;
     if (const char* e = std::getenv("RIHVR_ZCHUNK")) zc = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("RIHVR_PROX_ROWS")) c->prox_rows = std::max(0, std::atoi(e));
     c->zchunk_fwd = std::min(c->Nz, zc);
     c->nparts = (c->Nz + c->zchunk_fwd - 1) / c->zchunk_fwd;
-    c->zchunk_adj = std::min(c->Nz, 8);
+    int za = 16;
+    if (const char* e = std::getenv("RIHVR_ZCHUNK_ADJ")) za = std::max(1, std::atoi(e));
+    c->zchunk_adj = std::min(c->Nz, za);
     // pipeline groups: one group (everything on the main stream) unless RIHVR_PIPE_PLANES
     // asks for groups of that many planes (rounded up to a multiple of 16).  Measured at
     // 1024^2 x 128 on MI355X the two-stream overlap loses (groups of 32: 287 iter/s,
@@ -438,6 +441,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     c->holo_rows = std::max<long>(holo_num_blocks(c->N), c->W / 16);
     RIHVR_TRY(dalloc(&c->partial_holo, c->holo_rows));
     RIHVR_TRY(dalloc(&c->scal, SL_COUNT));
+    RIHVR_TRY(dalloc(&c->red_work, (size_t)kReduceBlocks * PS_COUNT));
     RIHVR_TRY(dalloc(&c->dflag, 4));
     RIHVR_HIP(hipHostMalloc((void**)&c->h_scal, SL_COUNT * sizeof(double), hipHostMallocDefault));
 
@@ -526,7 +530,7 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
         }));
         RIHVR_TRY(timed(c, K_REDUCE, [&] {
             return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
-                                   c->scal + SL_PROX, c->stream);
+                                   c->red_work, c->scal + SL_PROX, c->stream);
         }));
     }
     RIHVR_TRY(fetch_scalars(c));
@@ -553,7 +557,7 @@ rihvr_status momentum_objective(rihvr_ctx* c, double summag, double* f) {
     }));
     RIHVR_TRY(timed(c, K_REDUCE, [&] {
         return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
-                               c->scal + SL_PROX, c->stream);
+                               c->red_work, c->scal + SL_PROX, c->stream);
     }));
     RIHVR_TRY(fetch_scalars(c));
     *f = objective(c, c->ss.mode, summag, c->h_scal[SL_PROX + PS_L1X0], c->h_scal[SL_PROX + PS_TVX0]);
@@ -639,7 +643,7 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
             return holo_momentum(c->hC, c->hP, step, c->meas, c->hE, c->resid, c->N, c->partial_holo, c->stream);
         }));
         RIHVR_TRY(timed(c, K_REDUCE, [&] {
-            return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->scal + SL_RESID_M, c->stream);
+            return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work, c->scal + SL_RESID_M, c->stream);
         }));
         // the next gradient (:545-546): R-hat now, its planes with the next prox
         RIHVR_TRY(spectrum(c, c->resid));
@@ -688,8 +692,8 @@ rihvr_status finish_impl(rihvr_ctx* c, rihvr_stats* st, int last_index) {
                             c->stream);
     }));
     RIHVR_TRY(timed(c, K_REDUCE, [&] {
-        return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT, c->scal + SL_PROX,
-                               c->stream);
+        return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT, c->red_work,
+                               c->scal + SL_PROX, c->stream);
     }));
     RIHVR_TRY(fetch_scalars(c));
     const double* sc = c->h_scal;
