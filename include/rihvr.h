@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define RIHVR_ABI_VERSION 1
+#define RIHVR_ABI_VERSION 2
 
 /* Status codes.  1..12 are the reference's umnholo::HoloError values
  * (src/core/umnholo.h:132-146), thrown by the reference where this ABI returns. */
