@@ -50,6 +50,15 @@ int prox_fl2d_num_waves(int W, int H, int nz, int n_tv, int row_chunk) {
 }
 bool prox_fl2d_supported(int n_tv) { return n_tv >= 0 && n_tv <= kMaxNtv; }
 
+// Bijection of [0, nblocks): block b, which the hardware places on XCD b % 8, gets the
+// logical index of the (b / 8)-th block of a contiguous run assigned to that XCD.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_local_block(int b, int nblocks) {
+    const int q = nblocks / kXcds, r = nblocks % kXcds;
+    const int x = b % kXcds, k = b / kXcds;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 struct ProxDev {
     const float2* Y0;
     const float2* YM;
@@ -227,7 +236,10 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     // wave index, made wave-uniform for the compiler so all row/plane
     // arithmetic below stays in scalar registers
     const int wl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int w = blockIdx.x * kProxWaves + wl;
+    // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD b mod 8),
+    // each with its own L2.  Renumber them so that consecutive strips of a row chunk --
+    // which re-read each other's halo columns -- run on the same XCD.
+    const int w = xcd_local_block(blockIdx.x, gridDim.x) * kProxWaves + wl;
     if (w >= a.nwaves) return;  // whole wave exits; no barriers below
     const ProxGeom geo = a.geo;
     const int per_plane = geo.nstrips * geo.nchunks;
