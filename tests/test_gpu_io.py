@@ -238,3 +238,31 @@ def test_cli_output_steps_and_subfolder(pkg, holo_image, tmp_path, gpu_available
         assert (out / f"estimate_iter{it:03d}.tif").exists() and (out / f"residual_iter{it:03d}.tif").exists()
     sub = out / "0003"
     assert len(list(sub.glob("inverse_*.tif"))) == 4 * c["num_planes"]
+
+
+def test_cli_background_removal_matches_pipeline(pkg, orc, holo_image, tmp_path, gpu_available):
+    """The remove_background branch (2005 line): (I - B) / max(sqrt(B), 0.1)."""
+    img_dir, img = holo_image
+    rng = np.random.default_rng(5)
+    bg = np.clip(img.astype(np.float64) * 0.0 + 120 + rng.normal(0, 3, img.shape), 0, 255).astype(np.uint8)
+    (img_dir / "bg.tif").write_bytes(tiff_bytes(bg))
+    c = cfg(48, 48, 4, its=5)
+    out = tmp_path / "o"
+    out.mkdir()
+    pf = tmp_path / "params.txt"
+    _param_file(pf, img_dir, out, c, pad=0, output_planes=False)
+    with open(pf, "a") as f:
+        f.write(f"background_image      2005 {img_dir}/bg.tif\n")
+    r = subprocess.run([str(_exe(pkg)), "-F", str(pf)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Removing background" in r.stdout
+    fp, rc = pkg.read_param_file(pf)
+    assert rc == 0 and fp.remove_background == 1
+    holo = pkg.prepare_hologram(fp, pkg.read_image(img_dir / "holo_0003.tif"), pkg.read_image(img_dir / "bg.tif"))
+    ref, _, _, _ = orc.solve(orc.make_params(**c), holo)
+    idx, val = _read_csv(out / "inverse_voxels.csv")
+    got = np.zeros(ref.shape, np.float64)
+    got[idx[:, 2], idx[:, 1], idx[:, 0]] = val
+    inten = np.abs(ref.astype(np.complex128)) ** 2
+    assert int(((got != 0) != (inten != 0)).sum()) <= 2
+    assert np.linalg.norm(got - inten) <= 2e-4 * max(np.linalg.norm(inten), 1e-30)

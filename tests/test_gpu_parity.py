@@ -209,6 +209,21 @@ def test_pipelined_iteration_matches_one_stream_and_oracle(pkg, orc, scenes, gpu
     assert list(st["restarts"]) == list(rst["restarts"])
 
 
+@pytest.mark.parametrize("W,H,Nz,mode", [(448, 448, 6, 7), (640, 576, 4, 7), (832, 832, 2, 7), (576, 576, 4, 6),
+                                          (256, 256, 16, 5)])
+def test_solve_other_sizes_match_oracle(pkg, orc, scenes, gpu_available, W, H, Nz, mode):
+    """Zero-padded sizes of the reference's CLI (448 = 384 + 2*32, 576, 832 = 768 + 64,
+    non-square 640x576) through radix-7/9/13/5 FFT plans, and modes 6 and 5."""
+    tv = 0.0 if mode == 5 else 0.01
+    c = cfg(W, H, Nz, its=6, reg=0.01, tv=tv, ntv=3, mode=mode)
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 8, 2.0, 5.0, 16 + W + Nz, 0.01)
+    vol, st, ref, rst, (est, res, rest, rres) = _solve_both(pkg, orc, c, holo)
+    assert rel(vol, ref) < 1e-4, rel(vol, ref)
+    assert list(st["restarts"]) == list(rst["restarts"])
+    assert _support_mismatch(vol, ref, 0) <= max(2, int(1e-4 * vol.size))
+    assert rel(est, rest) < 1e-4
+
+
 def test_device_pointers_match_host(pkg, scenes, gpu_available):
     import torch
     c = cfg(64, 64, 4, its=3)
