@@ -125,12 +125,37 @@ def frames_leg(pkg, scenes, cfg, n_frames: int, tol: float = 1e-3, seed: int = 2
     }
 
 
+def background_bench(pkg, steps: int, warmup: int, n_frames: int = 200, size: int = 1024):
+    """make-background's mean background over n_frames device-resident 1024^2 frames
+    (rihvr_background_mean): frames/s, and the two passes' compulsory bytes (read every
+    frame twice, write the background) per call.  The call's own device allocations and
+    the copy of the background to the host are inside the timed region."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(20261204)
+    frames = torch.randint(0, 256, (n_frames, size, size), generator=g, device="cuda").float()
+    for _ in range(warmup):
+        pkg.background_mean(frames)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pkg.background_mean(frames)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    nbytes = 2.0 * n_frames * size * size * 4 + size * size * 4
+    return {"metric": f"background frames/s (mean + statistics of {n_frames} frames {size}^2, FP32)",
+            "value": n_frames / dt, "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (uniform grey values)",
+            "config": {"workload": f"make-background mean path, {n_frames} frames of {size}x{size}"},
+            "achieved_GBps_wall": nbytes / dt / 1e9, "algorithmic_bytes_per_call": nbytes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "background"], default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--frames", type=int, default=3,
                     help="frames of the s/frame-to-tolerance leg (0 skips it)")
@@ -146,6 +171,9 @@ def main():
         sys.exit(1)
 
     pkg = entry.load_package()
+    if args.workload == "background":
+        print(json.dumps(background_bench(pkg, max(1, min(args.steps, 20)), min(args.warmup, 3))), flush=True)
+        return
     scenes = entry._load_submodule("scenes")
     cfg = dict(scenes.C2 if args.workload == "c2" else scenes.C1, mode=7)
     holo = scenes.c2_hologram() if args.workload == "c2" else scenes.c1_hologram()

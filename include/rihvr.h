@@ -271,6 +271,7 @@ typedef struct rihvr_file_params {
     int32_t roi_x, roi_y, roi_w, roi_h; /* 3011..3014 */
     int32_t output_planes;            /* 7001 */
     int32_t output_steps;             /* 7005 */
+    int32_t use_median_background;    /* 7006 (make-background) */
     int32_t unknown_id;               /* id of the line that stopped parsing, else 0 */
 } rihvr_file_params;
 
@@ -327,6 +328,33 @@ rihvr_status rihvr_save_projections(rihvr_ctx* ctx, const char* output_path, con
  * <prefix>_%04d.tif (magnitude), _phase_%04d.tif, _real_%04d.tif, _imag_%04d.tif.
  * A file that cannot be written: RIHVR_E_UNKNOWN_ERROR. */
 rihvr_status rihvr_save_planes(rihvr_ctx* ctx, const char* path_prefix);
+
+/* 8-bit grayscale PNG writer (stored deflate blocks; the reference writes its
+ * background images with cv::imwrite to .png). */
+rihvr_status rihvr_write_png8(const char* path, const uint8_t* pixels, int32_t width, int32_t height);
+
+/* ---- Background estimation (the reference's make-background executable,
+ * src/core/main_create_background.cu, and HoloSequence::useBackgroundSubtraction,
+ * holo_sequence.cu:309-437).  Frames are raw grey values, [n][H][W] float32, at
+ * frames_loc (RIHVR_HOST or RIHVR_DEVICE); the work runs on the current HIP device. */
+typedef struct rihvr_background_stats {
+    double alpha;    /* mean of the frames' means of (I - B) - 5 * mean of their std devs */
+    double beta;     /* 255 / (2 * 5 * mean of the std devs) */
+    double minimum;  /* the reference's bg_min_grey, always 0 */
+    double average;  /* mean grey value of the background B */
+} rihvr_background_stats;
+
+/* B = (sum of the frames, added in frame order in float) * float(1/n); then per frame the
+ * population mean and standard deviation of I - B (cv::meanStdDev) and the summary
+ * statistics above.  n >= 1, else RIHVR_E_MISSING_DATA.  background: [H][W] float32 on
+ * the host; stats may be NULL. */
+rihvr_status rihvr_background_mean(const float* frames, int32_t n_frames, int32_t width, int32_t height,
+                                   int frames_loc, float* background, rihvr_background_stats* stats);
+
+/* Per-pixel median of five frames (median5plane_kernel, main_create_background.cu:13-94);
+ * out: [H][W] float32 on the host. */
+rihvr_status rihvr_background_median5(const float* frames5, int32_t width, int32_t height, int frames_loc,
+                                      float* out);
 
 #ifdef __cplusplus
 }

@@ -40,7 +40,8 @@ EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_d
             "rihvr_get_estimate", "rihvr_forward", "rihvr_adjoint", "rihvr_prox", "rihvr_get_params",
             "rihvr_snapshot", "rihvr_read_param_file", "rihvr_read_image", "rihvr_prepare_hologram",
             "rihvr_write_tiff8", "rihvr_write_hologram", "rihvr_save_sparse", "rihvr_save_projections",
-            "rihvr_save_planes", "rihvr_reconstruct_batch")
+            "rihvr_save_planes", "rihvr_reconstruct_batch", "rihvr_write_png8", "rihvr_background_mean",
+            "rihvr_background_median5")
 PATH_MAX = 4096
 
 
@@ -115,6 +116,13 @@ class FrameStats(ctypes.Structure):
     ]
 
 
+class BackgroundStats(ctypes.Structure):
+    """Mirror of ``rihvr_background_stats`` (include/rihvr.h)."""
+
+    _fields_ = [("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("minimum", ctypes.c_double),
+                ("average", ctypes.c_double)]
+
+
 class FileParams(ctypes.Structure):
     """Mirror of ``rihvr_file_params`` (include/rihvr.h): the parameter file."""
 
@@ -140,6 +148,7 @@ class FileParams(ctypes.Structure):
         ("roi_h", ctypes.c_int32),
         ("output_planes", ctypes.c_int32),
         ("output_steps", ctypes.c_int32),
+        ("use_median_background", ctypes.c_int32),
         ("unknown_id", ctypes.c_int32),
     ]
 
@@ -252,6 +261,13 @@ def lib():
         L.rihvr_reconstruct_batch.argtypes = [vp, vp, ip, ctypes.c_int32, vp, ip, ctypes.c_int32,
                                               _P(ctypes.c_double), ctypes.c_int32, vp, ip, _P(FrameStats)]
         L.rihvr_reconstruct_batch.restype = st
+        L.rihvr_write_png8.argtypes = [cp, vp, ctypes.c_int32, ctypes.c_int32]
+        L.rihvr_write_png8.restype = st
+        L.rihvr_background_mean.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ip, vp,
+                                            _P(BackgroundStats)]
+        L.rihvr_background_mean.restype = st
+        L.rihvr_background_median5.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ip, vp]
+        L.rihvr_background_median5.restype = st
         _lib = L
     return _lib
 
@@ -308,6 +324,45 @@ def write_tiff8(path, pixels):
     px = np.ascontiguousarray(pixels, np.uint8)
     h, w = px.shape
     _check(lib().rihvr_write_tiff8(_path(path), ctypes.c_void_p(px.ctypes.data), w, h), "rihvr_write_tiff8")
+
+
+def write_png8(path, pixels):
+    px = np.ascontiguousarray(pixels, np.uint8)
+    h, w = px.shape
+    _check(lib().rihvr_write_png8(_path(path), ctypes.c_void_p(px.ctypes.data), w, h), "rihvr_write_png8")
+
+
+def _frames_arg(frames):
+    """(pointer, location, keepalive, shape) for float32 frames: numpy (host) or a CUDA tensor."""
+    if _is_torch(frames):
+        import torch
+        if frames.dtype != torch.float32 or not frames.is_contiguous() or not frames.is_cuda:
+            raise ValueError("device frames must be contiguous float32 CUDA tensors")
+        return ctypes.c_void_p(frames.data_ptr()), DEVICE, frames, tuple(frames.shape)
+    f = np.ascontiguousarray(frames, np.float32)
+    return ctypes.c_void_p(f.ctypes.data), HOST, f, f.shape
+
+
+def background_mean(frames):
+    """rihvr_background_mean: frames [n][H][W] raw grey values (numpy, or a float32 CUDA
+    tensor) -> (background [H][W] float32, stats dict: alpha, beta, minimum, average)."""
+    fp, loc, _keep, (n, h, w) = _frames_arg(frames)
+    bg = np.empty((h, w), np.float32)
+    st = BackgroundStats()
+    _check(lib().rihvr_background_mean(fp, n, w, h, loc, ctypes.c_void_p(bg.ctypes.data), ctypes.byref(st)),
+           "rihvr_background_mean")
+    return bg, dict(alpha=st.alpha, beta=st.beta, minimum=st.minimum, average=st.average)
+
+
+def background_median5(frames5):
+    fp, loc, _keep, shape = _frames_arg(frames5)
+    if shape[0] != 5:
+        raise ValueError("five frames")
+    _, h, w = shape
+    out = np.empty((h, w), np.float32)
+    _check(lib().rihvr_background_median5(fp, w, h, loc, ctypes.c_void_p(out.ctypes.data)),
+           "rihvr_background_median5")
+    return out
 
 
 def write_hologram(path, holo, lo=None, hi=None):

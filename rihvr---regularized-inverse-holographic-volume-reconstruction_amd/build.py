@@ -4,8 +4,8 @@ Every .hip source under csrc/ is compiled with hipcc for gfx950 only, the
 host-only .cpp sources (input/output) with the host C++ compiler, and the
 objects are linked into one shared library exporting the C ABI of
 include/rihvr.h: <package>/_build/librihvr.so.  Packed FP32 instructions are
-disabled for all device code.  csrc/cli_main.cpp is linked against that
-library as <package>/_build/sparse-inverse-recon (runpath $ORIGIN).
+disabled for all device code.  The executables' main sources (EXECUTABLES) are
+linked against that library under <package>/_build/ (runpath $ORIGIN).
 """
 from __future__ import annotations
 
@@ -21,8 +21,9 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 LIB = OUT / "librihvr.so"
-CLI = OUT / "sparse-inverse-recon"
-CLI_MAIN = "cli_main.cpp"
+# executables linked against the library: main source -> file name under _build/
+EXECUTABLES = {"cli_main.cpp": "sparse-inverse-recon", "make_background_main.cpp": "make-background"}
+CLI = OUT / EXECUTABLES["cli_main.cpp"]
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -41,7 +42,7 @@ ROCM_LIB = pathlib.Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib"
 
 def sources():
     hip = sorted(CSRC.glob("*.hip"))
-    host = sorted(p for p in CSRC.glob("*.cpp") if p.name != CLI_MAIN)
+    host = sorted(p for p in CSRC.glob("*.cpp") if p.name not in EXECUTABLES)
     return hip + host
 
 
@@ -77,15 +78,17 @@ def build(verbose: bool = False, jobs: int | None = None) -> pathlib.Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    tmp = CLI.with_name(CLI.name + ".tmp")
-    cmd = [HOST_CXX, *HOST_CXXFLAGS, str(CSRC / CLI_MAIN), "-o", str(tmp), f"-L{OUT}", "-lrihvr",
-           f"-L{ROCM_LIB}", f"-Wl,-rpath-link,{ROCM_LIB}", "-Wl,-rpath,$ORIGIN"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"link of {CLI.name} failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, CLI)
+    for main, name in EXECUTABLES.items():
+        exe = OUT / name
+        tmp = exe.with_name(name + ".tmp")
+        cmd = [HOST_CXX, *HOST_CXXFLAGS, str(CSRC / main), "-o", str(tmp), f"-L{OUT}", "-lrihvr",
+               f"-L{ROCM_LIB}", f"-Wl,-rpath-link,{ROCM_LIB}", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link of {name} failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, exe)
     return LIB
 
 

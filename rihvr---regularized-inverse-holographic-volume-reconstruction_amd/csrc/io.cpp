@@ -225,6 +225,85 @@ rihvr_status write_tiff(const char* path, const uint8_t* px, int32_t W, int32_t 
     return ok ? RIHVR_OK : RIHVR_E_BAD_FILENAME;
 }
 
+// ---------------------------------------------------------------------------
+// PNG (ISO/IEC 15948): signature, IHDR, one IDAT holding a zlib stream of stored
+// (uncompressed) deflate blocks (RFC 1950/1951), IEND; CRC-32 over each chunk's type
+// and data, Adler-32 over the filtered scanlines (filter byte 0 per row).
+// ---------------------------------------------------------------------------
+uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n) {
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; ++k) v = (v & 1) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
+            table[i] = v;
+        }
+        init = true;
+    }
+    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xff] ^ (c >> 8);
+    return c;
+}
+
+void put_be32(std::vector<uint8_t>& b, uint32_t v) {
+    for (int i = 3; i >= 0; --i) b.push_back((uint8_t)(v >> (8 * i)));
+}
+
+void png_chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_t>& data) {
+    put_be32(out, (uint32_t)data.size());
+    const size_t at = out.size();
+    out.insert(out.end(), type, type + 4);
+    out.insert(out.end(), data.begin(), data.end());
+    const uint32_t crc = crc32_update(0xFFFFFFFFu, out.data() + at, 4 + data.size()) ^ 0xFFFFFFFFu;
+    put_be32(out, crc);
+}
+
+rihvr_status write_png(const char* path, const uint8_t* px, int32_t W, int32_t H) {
+    if (!path || !px || W <= 0 || H <= 0) return RIHVR_E_INVALID_ARGUMENT;
+    std::vector<uint8_t> raw;
+    raw.reserve((size_t)(W + 1) * H);
+    for (int y = 0; y < H; ++y) {
+        raw.push_back(0);  // filter: none
+        raw.insert(raw.end(), px + (size_t)y * W, px + (size_t)(y + 1) * W);
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};  // zlib: deflate, 32 KiB window, no dictionary
+    for (size_t at = 0; at < raw.size() || at == 0;) {
+        const size_t len = std::min<size_t>(65535, raw.size() - at);
+        const bool last = at + len >= raw.size();
+        z.push_back(last ? 1 : 0);  // BFINAL, BTYPE = 00 (stored)
+        z.push_back((uint8_t)(len & 0xff));
+        z.push_back((uint8_t)(len >> 8));
+        z.push_back((uint8_t)(~len & 0xff));
+        z.push_back((uint8_t)((~len >> 8) & 0xff));
+        z.insert(z.end(), raw.begin() + at, raw.begin() + at + len);
+        at += len;
+        if (last) break;
+    }
+    uint32_t a = 1, b = 0;  // Adler-32
+    for (uint8_t v : raw) {
+        a = (a + v) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    put_be32(z, (b << 16) | a);
+    std::vector<uint8_t> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, (uint32_t)W);
+    put_be32(ihdr, (uint32_t)H);
+    ihdr.push_back(8);  // bit depth
+    ihdr.push_back(0);  // colour type: greyscale
+    ihdr.push_back(0);  // compression
+    ihdr.push_back(0);  // filter method
+    ihdr.push_back(0);  // no interlace
+    png_chunk(out, "IHDR", ihdr);
+    png_chunk(out, "IDAT", z);
+    png_chunk(out, "IEND", {});
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return RIHVR_E_BAD_FILENAME;
+    bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? RIHVR_OK : RIHVR_E_BAD_FILENAME;
+}
+
 struct Volume {
     rihvr_params p{};
     std::vector<float> v;  // interleaved complex64
@@ -334,7 +413,10 @@ rihvr_status rihvr_read_param_file(const char* path, rihvr_file_params* out) {
             case NUM_TV_ITS: p.num_tv_iterations = (int32_t)value; break;
             case FLAG_OUTPUT_PLANES: out->output_planes = flag_value(value, char_value); break;
             case FLAG_OUTPUT_STEPS: out->output_steps = flag_value(value, char_value); break;
-            case FLAG_STANDARD_RECON: case FLAG_INVERSE_RECON: case FLAG_DECONV_RECON: case FLAG_MEDIAN_BACKGROUND:
+            case FLAG_MEDIAN_BACKGROUND:
+                out->use_median_background = flag_value(value, char_value);
+                break;
+            case FLAG_STANDARD_RECON: case FLAG_INVERSE_RECON: case FLAG_DECONV_RECON:
                 break;
             default:
                 // holo_ui.cpp:480-484: stop at the first unknown id and return, keeping
@@ -431,6 +513,10 @@ rihvr_status rihvr_prepare_hologram(const rihvr_file_params* fp, const float* im
 
 rihvr_status rihvr_write_tiff8(const char* path, const uint8_t* pixels, int32_t width, int32_t height) {
     return write_tiff(path, pixels, width, height);
+}
+
+rihvr_status rihvr_write_png8(const char* path, const uint8_t* pixels, int32_t width, int32_t height) {
+    return write_png(path, pixels, width, height);
 }
 
 rihvr_status rihvr_write_hologram(const char* path, const float* holo, int32_t width, int32_t height,

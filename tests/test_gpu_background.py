@@ -1,0 +1,92 @@
+"""GPU tests of the background estimation (rihvr_background_mean / _median5) and of
+the make-background executable against the CPU restatement in oracle/background.py."""
+from __future__ import annotations
+
+import subprocess
+
+import numpy as np
+import pytest
+
+from imgfiles import tiff_bytes
+from test_background_cpu import read_png
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def bgo():
+    import oracle.background as b
+    return b
+
+
+def _frames(n, h=48, w=64, seed=3):
+    rng = np.random.default_rng(seed)
+    base = rng.integers(60, 200, size=(h, w)).astype(np.float64)
+    return np.clip(base[None] + rng.normal(0, 12, size=(n, h, w)), 0, 255).round().astype(np.uint8)
+
+
+def test_background_mean_matches_oracle(pkg, bgo, gpu_available):
+    f = _frames(7).astype(np.float32)
+    bg, st = pkg.background_mean(f)
+    rbg, rst = bgo.mean_background(f)
+    np.testing.assert_array_equal(bg, rbg)  # same float summation order
+    for k in ("alpha", "beta", "minimum", "average"):
+        assert st[k] == pytest.approx(rst[k], rel=1e-12, abs=1e-12), k
+    with pytest.raises(pkg.RihvrError) as e:
+        pkg.background_mean(np.zeros((0, 4, 4), np.float32))
+    assert e.value.status == 3
+
+
+def test_background_median5_matches_oracle(pkg, bgo, gpu_available):
+    f = _frames(5, seed=4).astype(np.float32)
+    np.testing.assert_array_equal(pkg.background_median5(f), bgo.median5(f))
+
+
+def _write_frames(d, frames):
+    for i, fr in enumerate(frames):
+        (d / f"f_{i:04d}.tif").write_bytes(tiff_bytes(fr))
+
+
+def _exe(pkg):
+    exe = pkg.PKG_DIR / "_build" / "make-background"
+    assert exe.exists(), "make-background is not built"
+    return exe
+
+
+def test_make_background_mean_end_to_end(pkg, bgo, tmp_path, gpu_available):
+    frames = _frames(12, seed=5)
+    _write_frames(tmp_path, frames)
+    out = tmp_path / "out"
+    out.mkdir()
+    p = tmp_path / "bg.txt"
+    # the layout the reference GUI writes (src/QTGUI/bgcompute.cpp:26-34)
+    p.write_text(f"num_images            1001 12\nstart_image           1002 0\nbg_start_image        1003 2\n"
+                 f"bg_end_image          1004 11\ninput_pathn           2002 {tmp_path}/\n"
+                 f"input_filename_format 2003 f_%04d.tif\noutput_pathn          2004 {out}\n"
+                 f"use_median_background 7006 0\n")
+    r = subprocess.run([str(_exe(pkg)), "-F", str(p)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    ids, step = bgo.frame_ids(0, 12, 2, 11)
+    assert f"Used {len(ids)} images for background calculation" in r.stdout
+    rbg, rst = bgo.mean_background(frames[ids].astype(np.float32))
+    np.testing.assert_array_equal(read_png(out / "background.png"), bgo.to_u8(rbg))
+    lines = (out / "background_stats.txt").read_text().splitlines()
+    assert lines == ["%f" % rst[k] for k in ("alpha", "beta", "minimum", "average")]
+
+
+def test_make_background_median_end_to_end(pkg, bgo, tmp_path, gpu_available):
+    frames = _frames(9, seed=6)
+    _write_frames(tmp_path, frames)
+    out = tmp_path / "out"
+    out.mkdir()
+    p = tmp_path / "bg.txt"
+    p.write_text(f"start_image 1002 0\nbg_start_image 1003 0\nbg_end_image 1004 9\n"
+                 f"input_pathn 2002 {tmp_path}/\ninput_filename_format 2003 f_%04d.tif\n"
+                 f"output_pathn 2004 {out}\nuse_median_background 7006 1\n")
+    r = subprocess.run([str(_exe(pkg)), "-F", str(p)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    sched = bgo.median_schedule(0, 9)
+    for oid, stack in sched:
+        exp = bgo.to_u8(bgo.median5(frames[stack].astype(np.float32)))
+        np.testing.assert_array_equal(read_png(out / f"background_{oid:04d}.png"), exp)
+    assert sorted(q.name for q in out.glob("background_*.png")) == [f"background_{o:04d}.png" for o, _ in sched]
