@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define RIHVR_ABI_VERSION 2
+#define RIHVR_ABI_VERSION 3
 
 /* Status codes.  1..12 are the reference's umnholo::HoloError values
  * (src/core/umnholo.h:132-146), thrown by the reference where this ABI returns. */
@@ -272,6 +272,9 @@ typedef struct rihvr_file_params {
     int32_t output_planes;            /* 7001 */
     int32_t output_steps;             /* 7005 */
     int32_t use_median_background;    /* 7006 (make-background) */
+    int32_t segment_min_area;         /* 5009 (segmentation), default 2 */
+    double  segment_min_intensity;    /* 5010 (segmentation), default 1 */
+    int32_t segment_close_size;       /* 5011 (segmentation), default 3 */
     int32_t unknown_id;               /* id of the line that stopped parsing, else 0 */
 } rihvr_file_params;
 
@@ -355,6 +358,108 @@ rihvr_status rihvr_background_mean(const float* frames, int32_t n_frames, int32_
  * out: [H][W] float32 on the host. */
 rihvr_status rihvr_background_median5(const float* frames5, int32_t width, int32_t height, int frames_loc,
                                       float* out);
+
+/* ---- Segmentation and object centroids (the reference's segmentation executable,
+ * src/core/main_segmentation.cu:83-125): SparseSegmentation (sparse_segmentation.cu)
+ * binarize -> close -> labelConnectedComponents, then ObjectCloud (object_cloud.cu)
+ * with Blob3d statistics (blob3d.cu) and blobSizeFilter.  A handle holds one volume of
+ * W x H x Nz voxels (W*H*Nz < 2^32 - 1) in dense device buffers; the dense passes run
+ * on the current HIP device on the handle's own stream; the object statistics, whose
+ * float sums the reference adds in voxel order, are one pass over the compacted
+ * object voxels on the host.  The call order is load -> binarize -> [close] -> label ->
+ * extract -> [size_filter]; a call out of that order returns RIHVR_E_INVALID_STATE. */
+typedef struct rihvr_segmentation rihvr_segmentation;
+
+typedef struct rihvr_object {
+    float   centroid[3];           /* x, y, z: integer sums / n in float (Blob3d::getCentroid, blob3d.cu:198-226) */
+    float   weighted_centroid[3];  /* float sums of w*x in voxel order / float sum of w (:228-265) */
+    float   max_value;             /* largest voxel value (getMaxValue, :320-324) */
+    float   mean_value;            /* double sum cast to float / float n (getMeanValue, :326-330) */
+    int64_t num_voxels;
+    int64_t label;                 /* smallest global voxel index z*W*H + y*W + x of the object */
+    int32_t bounds[6];             /* xmin, xmax, ymin, ymax, zmin, zmax (findBounds, :79-105) */
+} rihvr_object;
+
+/* SparseVolume::loadData (sparse_volume.cu:562-682): "x, y, z, value" rows of a sparse
+ * volume file (the first 14 characters must be that header, else RIHVR_E_INVALID_FILE).
+ * As the reference, plane z takes the next count[z] rows of the file, count[z] being the
+ * number of rows whose z is z; index = y*W + x, and the reference's bounds test x > W,
+ * y > H, z > depth is RIHVR_E_INVALID_DATA (x == W wraps to the next row).  Rows the
+ * reference would store out of range (negative, index >= W*H, z >= Nz) and malformed rows:
+ * RIHVR_E_INVALID_DATA / RIHVR_E_INVALID_FILE.  Call with plane == NULL to get *count;
+ * then plane, index (int32) and intensity (float, the file's value) hold *count entries.
+ * A missing file: RIHVR_E_BAD_FILENAME. */
+rihvr_status rihvr_read_sparse_voxels(const char* path, int32_t width, int32_t height, int32_t num_planes,
+                                      int64_t* count, int32_t* plane, int32_t* index, float* intensity);
+
+/* SparseSegmentation's state; create allocates the dense buffers (about 13 bytes per
+ * voxel).  RIHVR_E_INVALID_ARGUMENT for a non-positive size or W*H*Nz >= 2^32 - 1. */
+rihvr_status rihvr_segmentation_create(int32_t width, int32_t height, int32_t num_planes,
+                                       rihvr_segmentation** out);
+void rihvr_segmentation_destroy(rihvr_segmentation* seg);
+
+/* SparseSegmentation(SparseVolume) after loadData: the voxel list (plane, in-plane index,
+ * intensity) at loc; the volume's value is sqrt(intensity) as loadData stores it.  A voxel
+ * listed twice keeps its larger intensity (NaN ranking lowest).  An entry outside the
+ * volume: RIHVR_E_INVALID_DATA. */
+rihvr_status rihvr_segmentation_load(rihvr_segmentation* seg, int64_t count, const int32_t* plane,
+                                     const int32_t* index, const float* intensity, int loc);
+
+/* The same from a dense [Nz][H][W] complex64 volume at loc (for example a solver's
+ * rihvr_get_volume output): intensity = re^2 + im^2 in float for every non-zero voxel, as
+ * rihvr_save_sparse writes it but without the file's 7-digit rounding. */
+rihvr_status rihvr_segmentation_load_dense(rihvr_segmentation* seg, const float* volume, int loc);
+
+/* SparseSegmentation::binarize (sparse_segmentation.cu:63-111): max intensity over the
+ * loaded voxels, thr = min_intensity * max / 256 (both returned; either pointer may be
+ * NULL); voxels with intensity > thr form the binary volume and keep their intensity as
+ * value, the others get value 0. */
+rihvr_status rihvr_segmentation_binarize(rihvr_segmentation* seg, double min_intensity, double* max_intensity,
+                                         double* threshold);
+
+/* SparseSegmentation::close (:145-315): dilate then erode with the ball
+ * sx^2 + sy^2 + sz^2 <= close_size^2; the erosion keeps a voxel only where the whole ball
+ * lies inside the volume and inside the dilation.  close_size <= 0 leaves the volume as it
+ * is; close_size > 254: RIHVR_E_INVALID_ARGUMENT. */
+rihvr_status rihvr_segmentation_close(rihvr_segmentation* seg, int32_t close_size);
+
+/* labelConnectedComponents (:411-532): 26-neighbour components where the in-plane
+ * neighbours are the flat indices i + dy*W + dx inside 0..W*H-1 (so rows wrap at the x
+ * borders, as the reference's offsets do); every voxel's label is the smallest global
+ * index of its component. */
+rihvr_status rihvr_segmentation_label(rihvr_segmentation* seg);
+
+/* SparseSegmentation::extractObjects (:731-739): ObjectCloud(SparseVolume) and
+ * includeIntensity (object_cloud.cu:44-58, 127-289): objects in order of first appearance
+ * in the z-major raster, each voxel's value its binarized intensity.  No cap on the
+ * number of objects (the reference's 20*1024 cap, object_cloud.h:22, overruns its
+ * arrays). */
+rihvr_status rihvr_segmentation_extract(rihvr_segmentation* seg, int64_t* num_objects);
+
+/* ObjectCloud::blobSizeFilter (object_cloud.cu:313-328): objects with fewer than min_area
+ * voxels are replaced by the last object, which is tested next. */
+rihvr_status rihvr_segmentation_size_filter(rihvr_segmentation* seg, int32_t min_area, int64_t* num_removed);
+
+/* The current objects: up to capacity into out (may be NULL); *num_objects = their count. */
+rihvr_status rihvr_segmentation_objects(rihvr_segmentation* seg, rihvr_object* out, int64_t capacity,
+                                        int64_t* num_objects);
+
+/* ObjectCloud::writeCentroids (object_cloud.cu:408-470): weighted != 0 writes the weighted
+ * centroids, verbose != 0 the long format (both centroids, count, max, mean, bounds).
+ * A file that cannot be opened: RIHVR_E_BAD_FILENAME. */
+rihvr_status rihvr_segmentation_write_centroids(rihvr_segmentation* seg, const char* path, int32_t weighted,
+                                                int32_t verbose);
+
+/* Copies of the dense state ([Nz][H][W], to loc): the current binary volume (after
+ * binarize or close), the voxel values (after binarize) and the labels (after label;
+ * 0xFFFFFFFF off the objects). */
+rihvr_status rihvr_segmentation_mask(rihvr_segmentation* seg, uint8_t* out, int loc);
+rihvr_status rihvr_segmentation_values(rihvr_segmentation* seg, float* out, int loc);
+rihvr_status rihvr_segmentation_labels(rihvr_segmentation* seg, uint32_t* out, int loc);
+
+/* Device time (ms, HIP events on the handle's stream) of the last load, binarize, close,
+ * label and extract calls, in that order: ms[5]. */
+rihvr_status rihvr_segmentation_timing(rihvr_segmentation* seg, double* ms);
 
 #ifdef __cplusplus
 }

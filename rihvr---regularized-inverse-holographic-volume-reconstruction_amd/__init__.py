@@ -41,7 +41,12 @@ EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_d
             "rihvr_snapshot", "rihvr_read_param_file", "rihvr_read_image", "rihvr_prepare_hologram",
             "rihvr_write_tiff8", "rihvr_write_hologram", "rihvr_save_sparse", "rihvr_save_projections",
             "rihvr_save_planes", "rihvr_reconstruct_batch", "rihvr_write_png8", "rihvr_background_mean",
-            "rihvr_background_median5")
+            "rihvr_background_median5", "rihvr_read_sparse_voxels", "rihvr_segmentation_create",
+            "rihvr_segmentation_destroy", "rihvr_segmentation_load", "rihvr_segmentation_load_dense",
+            "rihvr_segmentation_binarize", "rihvr_segmentation_close", "rihvr_segmentation_label",
+            "rihvr_segmentation_extract", "rihvr_segmentation_size_filter", "rihvr_segmentation_objects",
+            "rihvr_segmentation_write_centroids", "rihvr_segmentation_mask", "rihvr_segmentation_values",
+            "rihvr_segmentation_labels", "rihvr_segmentation_timing")
 PATH_MAX = 4096
 
 
@@ -149,8 +154,30 @@ class FileParams(ctypes.Structure):
         ("output_planes", ctypes.c_int32),
         ("output_steps", ctypes.c_int32),
         ("use_median_background", ctypes.c_int32),
+        ("segment_min_area", ctypes.c_int32),
+        ("segment_min_intensity", ctypes.c_double),
+        ("segment_close_size", ctypes.c_int32),
         ("unknown_id", ctypes.c_int32),
     ]
+
+
+class Object(ctypes.Structure):
+    """Mirror of ``rihvr_object`` (include/rihvr.h): one segmented object."""
+
+    _fields_ = [
+        ("centroid", ctypes.c_float * 3),
+        ("weighted_centroid", ctypes.c_float * 3),
+        ("max_value", ctypes.c_float),
+        ("mean_value", ctypes.c_float),
+        ("num_voxels", ctypes.c_int64),
+        ("label", ctypes.c_int64),
+        ("bounds", ctypes.c_int32 * 6),
+    ]
+
+    def as_dict(self) -> dict:
+        return dict(centroid=list(self.centroid), weighted_centroid=list(self.weighted_centroid),
+                    max_value=self.max_value, mean_value=self.mean_value, num_voxels=self.num_voxels,
+                    label=self.label, bounds=list(self.bounds))
 
 
 def make_params(width, height, num_planes, start_plane, plane_stepsize, wavelength, resolution,
@@ -268,6 +295,37 @@ def lib():
         L.rihvr_background_mean.restype = st
         L.rihvr_background_median5.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ip, vp]
         L.rihvr_background_median5.restype = st
+        i64, i64p = ctypes.c_int64, _P(ctypes.c_int64)
+        L.rihvr_read_sparse_voxels.argtypes = [cp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i64p, vp, vp,
+                                               vp]
+        L.rihvr_read_sparse_voxels.restype = st
+        L.rihvr_segmentation_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(vp)]
+        L.rihvr_segmentation_create.restype = st
+        L.rihvr_segmentation_destroy.argtypes = [vp]
+        L.rihvr_segmentation_destroy.restype = None
+        L.rihvr_segmentation_load.argtypes = [vp, i64, vp, vp, vp, ip]
+        L.rihvr_segmentation_load.restype = st
+        L.rihvr_segmentation_load_dense.argtypes = [vp, vp, ip]
+        L.rihvr_segmentation_load_dense.restype = st
+        L.rihvr_segmentation_binarize.argtypes = [vp, ctypes.c_double, _P(ctypes.c_double), _P(ctypes.c_double)]
+        L.rihvr_segmentation_binarize.restype = st
+        L.rihvr_segmentation_close.argtypes = [vp, ctypes.c_int32]
+        L.rihvr_segmentation_close.restype = st
+        L.rihvr_segmentation_label.argtypes = [vp]
+        L.rihvr_segmentation_label.restype = st
+        L.rihvr_segmentation_extract.argtypes = [vp, i64p]
+        L.rihvr_segmentation_extract.restype = st
+        L.rihvr_segmentation_size_filter.argtypes = [vp, ctypes.c_int32, i64p]
+        L.rihvr_segmentation_size_filter.restype = st
+        L.rihvr_segmentation_objects.argtypes = [vp, vp, i64, i64p]
+        L.rihvr_segmentation_objects.restype = st
+        L.rihvr_segmentation_write_centroids.argtypes = [vp, cp, ctypes.c_int32, ctypes.c_int32]
+        L.rihvr_segmentation_write_centroids.restype = st
+        for name in ("rihvr_segmentation_mask", "rihvr_segmentation_values", "rihvr_segmentation_labels"):
+            getattr(L, name).argtypes = [vp, vp, ip]
+            getattr(L, name).restype = st
+        L.rihvr_segmentation_timing.argtypes = [vp, _P(ctypes.c_double)]
+        L.rihvr_segmentation_timing.restype = st
         _lib = L
     return _lib
 
@@ -563,3 +621,147 @@ class SparseCompressiveHolo:
             raise ValueError("out must be in the same memory as x0")
         _check(lib().rihvr_prox(self._ctx, xp, gp, float(stepsize), op, ol), "rihvr_prox")
         return out
+
+
+# -- segmentation and centroids (the segmentation executable) ---------------
+def read_sparse_voxels(path, width, height, num_planes):
+    """SparseVolume::loadData (sparse_volume.cu:562-682) of an "x, y, z, value" file:
+    -> (plane int32[n], index int32[n] = y*W + x, intensity float32[n])."""
+    n = ctypes.c_int64(0)
+    _check(lib().rihvr_read_sparse_voxels(_path(path), int(width), int(height), int(num_planes), ctypes.byref(n),
+                                          None, None, None), "rihvr_read_sparse_voxels")
+    plane = np.empty(n.value, np.int32)
+    index = np.empty(n.value, np.int32)
+    inten = np.empty(n.value, np.float32)
+    _check(lib().rihvr_read_sparse_voxels(_path(path), int(width), int(height), int(num_planes), ctypes.byref(n),
+                                          ctypes.c_void_p(plane.ctypes.data), ctypes.c_void_p(index.ctypes.data),
+                                          ctypes.c_void_p(inten.ctypes.data)), "rihvr_read_sparse_voxels")
+    return plane, index, inten
+
+
+def _list_arg(a, dtype):
+    """(pointer, location, keepalive) of a 1-D voxel-list array: numpy or a CUDA tensor."""
+    if _is_torch(a):
+        if not a.is_cuda or not a.is_contiguous() or a.element_size() != 4:
+            raise ValueError("device voxel lists must be contiguous 4-byte CUDA tensors")
+        return ctypes.c_void_p(a.data_ptr()), DEVICE, a
+    arr = np.ascontiguousarray(a, dtype)
+    return ctypes.c_void_p(arr.ctypes.data), HOST, arr
+
+
+class SparseSegmentation:
+    """Mirror of umnholo::SparseSegmentation (src/core/sparse_segmentation.h) over the C ABI:
+    construct with the volume size and its voxels (read_sparse_voxels' lists, or a dense
+    complex64 [Nz][H][W] volume), then binarize(min_intensity), close(close_size),
+    labelConnectedComponents(), extractObjects() -> ObjectCloud."""
+
+    def __init__(self, width, height, num_planes, voxels=None, dense=None):
+        self.shape = (int(num_planes), int(height), int(width))
+        self._h = ctypes.c_void_p()
+        _check(lib().rihvr_segmentation_create(int(width), int(height), int(num_planes), ctypes.byref(self._h)),
+               "rihvr_segmentation_create")
+        self.max_intensity = None
+        self.threshold = None
+        if voxels is not None:
+            self.load(*voxels)
+        elif dense is not None:
+            self.loadDense(dense)
+
+    def destroy(self):
+        if self._h:
+            lib().rihvr_segmentation_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def load(self, plane, index, intensity):
+        pp, pl, _k1 = _list_arg(plane, np.int32)
+        ip_, il, _k2 = _list_arg(index, np.int32)
+        vp, vl, _k3 = _list_arg(intensity, np.float32)
+        n = len(_k1)
+        if not (pl == il == vl) or len(_k2) != n or len(_k3) != n:
+            raise ValueError("plane, index and intensity must be equal-length lists in the same memory")
+        _check(lib().rihvr_segmentation_load(self._h, n, pp, ip_, vp, pl), "rihvr_segmentation_load")
+
+    def loadDense(self, volume):
+        p, loc, _k = _arg(volume, self.shape)
+        _check(lib().rihvr_segmentation_load_dense(self._h, p, loc), "rihvr_segmentation_load_dense")
+
+    def binarize(self, min_intensity=1.0):
+        mx, thr = ctypes.c_double(), ctypes.c_double()
+        _check(lib().rihvr_segmentation_binarize(self._h, float(min_intensity), ctypes.byref(mx), ctypes.byref(thr)),
+               "rihvr_segmentation_binarize")
+        self.max_intensity, self.threshold = mx.value, thr.value
+        return self.max_intensity, self.threshold
+
+    def close(self, close_size=3):
+        _check(lib().rihvr_segmentation_close(self._h, int(close_size)), "rihvr_segmentation_close")
+
+    def labelConnectedComponents(self):
+        _check(lib().rihvr_segmentation_label(self._h), "rihvr_segmentation_label")
+
+    def extractObjects(self) -> "ObjectCloud":
+        n = ctypes.c_int64()
+        _check(lib().rihvr_segmentation_extract(self._h, ctypes.byref(n)), "rihvr_segmentation_extract")
+        return ObjectCloud(self)
+
+    def _copy(self, fn, dtype, what):
+        out = np.empty(self.shape, dtype)
+        _check(fn(self._h, ctypes.c_void_p(out.ctypes.data), HOST), what)
+        return out
+
+    def mask(self) -> np.ndarray:
+        return self._copy(lib().rihvr_segmentation_mask, np.uint8, "rihvr_segmentation_mask")
+
+    def values(self) -> np.ndarray:
+        return self._copy(lib().rihvr_segmentation_values, np.float32, "rihvr_segmentation_values")
+
+    def labels(self) -> np.ndarray:
+        return self._copy(lib().rihvr_segmentation_labels, np.uint32, "rihvr_segmentation_labels")
+
+    def timing(self) -> dict:
+        ms = (ctypes.c_double * 5)()
+        _check(lib().rihvr_segmentation_timing(self._h, ms), "rihvr_segmentation_timing")
+        return dict(zip(("load", "binarize", "close", "label", "extract"), list(ms)))
+
+
+class ObjectCloud:
+    """Mirror of umnholo::ObjectCloud (src/core/object_cloud.h) for the objects of a
+    SparseSegmentation: getNumObjects(), objects(), blobSizeFilter(min_size),
+    writeCentroids(filename, use_weighted_centroid, verbose)."""
+
+    def __init__(self, seg: SparseSegmentation):
+        self._seg = seg
+
+    def getNumObjects(self) -> int:
+        n = ctypes.c_int64()
+        _check(lib().rihvr_segmentation_objects(self._seg._h, None, 0, ctypes.byref(n)), "rihvr_segmentation_objects")
+        return n.value
+
+    def objects(self) -> list:
+        n = self.getNumObjects()
+        arr = (Object * max(n, 1))()
+        got = ctypes.c_int64()
+        _check(lib().rihvr_segmentation_objects(self._seg._h, arr, n, ctypes.byref(got)),
+               "rihvr_segmentation_objects")
+        return [arr[i].as_dict() for i in range(n)]
+
+    def blobSizeFilter(self, min_size) -> int:
+        removed = ctypes.c_int64()
+        _check(lib().rihvr_segmentation_size_filter(self._seg._h, int(min_size), ctypes.byref(removed)),
+               "rihvr_segmentation_size_filter")
+        return removed.value
+
+    def writeCentroids(self, filename, use_weighted_centroid=False, verbose=False):
+        _check(lib().rihvr_segmentation_write_centroids(self._seg._h, _path(filename), 1 if use_weighted_centroid else 0,
+                                                        1 if verbose else 0), "rihvr_segmentation_write_centroids")

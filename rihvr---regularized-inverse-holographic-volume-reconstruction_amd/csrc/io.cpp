@@ -2,6 +2,7 @@
 // file, image decoding, the hologram input contract and the result writers.
 // Built on the public C ABI only; no device code.
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -360,6 +361,9 @@ rihvr_status rihvr_read_param_file(const char* path, rihvr_file_params* out) {
     out->center_x = out->center_y = out->center_z = -1;
     out->roi_size = -1;
     out->background_start = out->background_end = -1;
+    out->segment_min_area = 2;
+    out->segment_min_intensity = 1;
+    out->segment_close_size = 3;
     FILE* fid = std::fopen(path, "r");
     if (!fid) return RIHVR_E_BAD_FILENAME;
     char buff[RIHVR_PATH_MAX] = {0};
@@ -404,9 +408,11 @@ rihvr_status rihvr_read_param_file(const char* path, rihvr_file_params* out) {
             case ROI_RECT_H: out->roi_h = (int32_t)value; break;
             case BETA: case IIPE_ITERATIONS: case PARTICLE_DIAMETER: case SIM_CHAMBER_RADIUS:
             case SIM_CONCENTRATION: case SIM_LENGTH: case SIM_RESOLUTION: case SIM_IMAGE_SIZE:
-            case SIM_ROTATION_STEP: case SIM_OVERSAMPLE: case SEGMENT_MIN_AREA: case SEGMENT_MIN_INTENSITY:
-            case SEGMENT_CLOSE_SIZE: case INVERSE_MAX_SVD:
-                break;  // parameters of other executables: accepted, not used on this path
+            case SIM_ROTATION_STEP: case SIM_OVERSAMPLE: case INVERSE_MAX_SVD:
+                break;  // parameters of other executables: accepted, not used here
+            case SEGMENT_MIN_AREA: out->segment_min_area = (int32_t)value; break;
+            case SEGMENT_MIN_INTENSITY: out->segment_min_intensity = value; break;
+            case SEGMENT_CLOSE_SIZE: out->segment_close_size = (int32_t)value; break;
             case NUM_INVERSE_ITS: p.num_inverse_iterations = (int32_t)value; break;
             case INVERSE_REG_TAU: p.regularization_param = value; break;
             case INVERSE_REG_TV: p.regularization_TV_param = value; break;
@@ -538,6 +544,84 @@ rihvr_status rihvr_write_hologram(const char* path, const float* holo, int32_t w
         for (long i = 0; i < N; ++i) px[i] = to_u8((float)(255 * (holo[2 * i] - lo) / (hi - lo)));
     }
     return write_tiff(path, px.data(), width, height);
+}
+
+rihvr_status rihvr_read_sparse_voxels(const char* path, int32_t width, int32_t height, int32_t num_planes,
+                                      int64_t* count, int32_t* plane, int32_t* index, float* intensity) {
+    if (!path || !count || width <= 0 || height <= 0 || num_planes <= 0) return RIHVR_E_INVALID_ARGUMENT;
+    if (plane && (!index || !intensity)) return RIHVR_E_INVALID_ARGUMENT;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return RIHVR_E_BAD_FILENAME;
+    std::string text;
+    char chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) text.append(chunk, got);
+    std::fclose(f);
+    // the header check reads 14 characters (fgets(read_str, 15), sparse_volume.cu:577-597)
+    static const char kHeader[] = "x, y, z, value";
+    if (text.compare(0, 14, kHeader) != 0) return RIHVR_E_INVALID_FILE;
+    size_t pos = 14;
+    while (pos < text.size() && text[pos] != '\n') {
+        if (!std::isspace((unsigned char)text[pos])) return RIHVR_E_INVALID_FILE;
+        ++pos;
+    }
+    struct Row {
+        int32_t x, y, z;
+        float v;
+    };
+    std::vector<Row> rows;
+    const char* s = text.c_str();
+    const char* end = s + text.size();
+    const char* q = s + pos;
+    // rows "%d, %d, %d, %e\n" (:607-609): literal commas, any white space around
+    auto skip_ws = [&]() {
+        while (q < end && std::isspace((unsigned char)*q)) ++q;
+    };
+    auto read_int = [&](int32_t* out) -> bool {
+        skip_ws();
+        char* e = nullptr;
+        const long v = std::strtol(q, &e, 10);
+        if (e == q || v < INT32_MIN || v > INT32_MAX) return false;
+        *out = (int32_t)v;
+        q = e;
+        return true;
+    };
+    auto comma = [&]() -> bool {
+        if (q >= end || *q != ',') return false;
+        ++q;
+        return true;
+    };
+    const long W = width, H = height, D = num_planes;
+    for (;;) {
+        skip_ws();
+        if (q >= end) break;
+        Row r;
+        if (!read_int(&r.x) || !comma() || !read_int(&r.y) || !comma() || !read_int(&r.z) || !comma())
+            return RIHVR_E_INVALID_FILE;
+        skip_ws();
+        char* e = nullptr;
+        r.v = std::strtof(q, &e);
+        if (e == q) return RIHVR_E_INVALID_FILE;
+        q = e;
+        // the reference's test (:613-623), then what it would store out of range
+        if (r.x > W || r.y > H || r.z > D) return RIHVR_E_INVALID_DATA;
+        if (r.x < 0 || r.y < 0 || r.z < 0 || r.z >= D || (long)r.y * W + r.x >= W * H) return RIHVR_E_INVALID_DATA;
+        rows.push_back(r);
+    }
+    *count = (int64_t)rows.size();
+    if (!plane) return RIHVR_OK;
+    // plane z takes the next count[z] rows of the file (:629-654)
+    std::vector<int64_t> per(num_planes, 0);
+    for (const Row& r : rows) ++per[r.z];
+    size_t k = 0;
+    for (int32_t z = 0; z < num_planes; ++z)
+        for (int64_t j = 0; j < per[z]; ++j, ++k) {
+            const Row& r = rows[k];
+            plane[k] = z;
+            index[k] = (int32_t)(r.y * W + r.x);
+            intensity[k] = r.v;
+        }
+    return RIHVR_OK;
 }
 
 rihvr_status rihvr_save_sparse(rihvr_ctx* ctx, const char* csv_path) {

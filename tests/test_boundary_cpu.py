@@ -24,7 +24,7 @@ def test_library_exports_every_declared_function(pkg):
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, missing
     assert set(declared_functions()) == set(pkg.EXPORTED)
-    assert L.rihvr_abi_version() == 2
+    assert L.rihvr_abi_version() == 3
 
 
 def test_status_strings_cover_holoerror_codes(pkg):
