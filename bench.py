@@ -150,12 +150,106 @@ def background_bench(pkg, steps: int, warmup: int, n_frames: int = 200, size: in
             "achieved_GBps_wall": nbytes / dt / 1e9, "algorithmic_bytes_per_call": nbytes}
 
 
+def segmentation_bench(pkg, scenes, steps: int, warmup: int, iterations: int = 100, cpu: bool = True):
+    """The segmentation executable's chain on the 1024^2 x 128 reconstruction of the
+    seeded tracers-and-rods hologram (iterations of plain FISTA+TV, mode 7), read from
+    the device volume: load, binarize (min intensity 1), close (radius 3), label,
+    extract, size filter (2) per step, the host pass over the object voxels included.
+    Per-stage device times come from HIP events on the segmentation stream."""
+    import torch
+    cfg = dict(scenes.C2, mode=7)
+    W, H, Nz = cfg["width"], cfg["height"], cfg["num_planes"]
+    N = W * H * Nz
+    p = pkg.make_params(**dict(cfg, num_inverse_iterations=iterations))
+    vol = torch.empty((Nz, H, W), dtype=torch.complex64, device="cuda")
+    with pkg.SparseCompressiveHolo(p) as rec:
+        rec.inverseReconstruct(torch.from_numpy(scenes.c2_hologram()).cuda())
+        rec.getVolume(out=vol)
+    torch.cuda.synchronize()
+    nnz = int(torch.count_nonzero(vol).item())
+    seg = pkg.SparseSegmentation(W, H, Nz)
+
+    def chain():
+        seg.loadDense(vol)
+        seg.binarize(1.0)
+        seg.close(3)
+        seg.labelConnectedComponents()
+        cloud = seg.extractObjects()
+        n_obj = cloud.getNumObjects()
+        removed = cloud.blobSizeFilter(2)
+        return n_obj, removed
+
+    for _ in range(warmup):
+        chain()
+    torch.cuda.synchronize()
+    stage = dict.fromkeys(("load", "binarize", "close", "label", "extract"), 0.0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        n_obj, removed = chain()
+        for k, v in seg.timing().items():
+            stage[k] += v / steps
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    fg = int(seg.mask().sum())
+    seg.destroy()
+    # compulsory HBM bytes per stage (N voxels, bit volume N/8 bytes): load reads the
+    # complex volume and writes the u32 keys; binarize reads the keys, writes the values
+    # and the bit volume; close reads and writes the bit volume twice; label reads the bit
+    # volume three times (labels are touched only at object voxels); extract reads it
+    # three times (counts, roots, scatter)
+    stage_bytes = {"load": 12.0 * N, "binarize": 8.0 * N + N / 8, "close": 4 * N / 8, "label": 3 * N / 8,
+                   "extract": 3 * N / 8}
+    dom = max(stage, key=stage.get)
+    achieved = stage_bytes[dom] / (stage[dom] * 1e-3) / 1e9
+    out = {"metric": "segmentation volumes/s (1024^2x128 reconstruction -> centroids)",
+           "value": 1.0 / dt, "unit": "volumes/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32/u8/u32", "data": f"synthetic (C2 tracers+rods hologram, {iterations} FISTA+TV iterations)",
+           "config": {"workload": f"segmentation chain of the {W}x{H}x{Nz} reconstruction: min intensity 1, "
+                                  f"close radius 3, min area 2", "nonzero_voxels": nnz, "object_voxels": fg,
+                      "objects": n_obj, "objects_removed_by_size": removed},
+           "stages_ms": stage, "gpu_ms": sum(stage.values()),
+           "roofline": {"bound": "hbm", "kernel": f"stage:{dom}", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                        "avg_launch_ms": stage[dom], "algorithmic_bytes_per_launch": stage_bytes[dom]},
+           "chain_GBps": sum(stage_bytes.values()) / (sum(stage.values()) * 1e-3) / 1e9}
+    if cpu:
+        out["cpu_baseline"] = segmentation_cpu_baseline(vol)
+    return out
+
+
+def torch_argmax_abs(vol):
+    return vol.abs().reshape(-1).argmax().item()
+
+
+def segmentation_cpu_baseline(vol, box=(32, 128, 128)):
+    """The oracle's restatement of the chain (numpy, one process) on a box of the
+    same reconstruction; reported per voxel and scaled to volumes/s of the full size."""
+    import oracle.segmentation as so
+    Nz, H, W = box
+    fullN = vol.numel()
+    # the box around the brightest voxel, inside the volume
+    c = np.unravel_index(int(torch_argmax_abs(vol)), tuple(vol.shape))
+    z0, y0, x0 = (int(min(max(c[k] - n // 2, 0), vol.shape[k] - n)) for k, n in enumerate(box))
+    sub = vol[z0:z0 + Nz, y0:y0 + H, x0:x0 + W].cpu().numpy()
+    z, y, x = np.nonzero(sub)
+    v = sub[z, y, x]
+    inten = (v.real * v.real + v.imag * v.imag).astype(np.float32)
+    t0 = time.perf_counter()
+    so.segment(z.astype(np.int32), (y * W + x).astype(np.int32), inten, W, H, Nz, 1.0, 3, 2)
+    dt = time.perf_counter() - t0
+    per_voxel = dt / (Nz * H * W)
+    return {"value": 1.0 / (per_voxel * fullN), "unit": "volumes/s", "cores": 1, "kind": "port",
+            "sample": f"oracle chain on the {W}x{H}x{Nz} box around the brightest voxel of the same reconstruction "
+                      f"({len(z)} non-zero voxels, {dt:.1f} s), scaled per voxel"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["c2", "c1", "background"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "background", "segmentation"], default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--frames", type=int, default=3,
                     help="frames of the s/frame-to-tolerance leg (0 skips it)")
@@ -175,6 +269,10 @@ def main():
         print(json.dumps(background_bench(pkg, max(1, min(args.steps, 20)), min(args.warmup, 3))), flush=True)
         return
     scenes = entry._load_submodule("scenes")
+    if args.workload == "segmentation":
+        print(json.dumps(segmentation_bench(pkg, scenes, max(1, min(args.steps, 20)), min(args.warmup, 3),
+                                            cpu=not args.no_cpu)), flush=True)
+        return
     cfg = dict(scenes.C2 if args.workload == "c2" else scenes.C1, mode=7)
     holo = scenes.c2_hologram() if args.workload == "c2" else scenes.c1_hologram()
     W, H, Nz = cfg["width"], cfg["height"], cfg["num_planes"]

@@ -392,8 +392,8 @@ typedef struct rihvr_object {
 rihvr_status rihvr_read_sparse_voxels(const char* path, int32_t width, int32_t height, int32_t num_planes,
                                       int64_t* count, int32_t* plane, int32_t* index, float* intensity);
 
-/* SparseSegmentation's state; create allocates the dense buffers (about 13 bytes per
- * voxel).  RIHVR_E_INVALID_ARGUMENT for a non-positive size or W*H*Nz >= 2^32 - 1. */
+/* SparseSegmentation's state; create allocates the dense buffers (about 12.3 bytes per
+ * voxel: u32 values, u32 labels, u32 work, two bit-packed binary volumes).  RIHVR_E_INVALID_ARGUMENT for a non-positive size or W*H*Nz >= 2^32 - 1. */
 rihvr_status rihvr_segmentation_create(int32_t width, int32_t height, int32_t num_planes,
                                        rihvr_segmentation** out);
 void rihvr_segmentation_destroy(rihvr_segmentation* seg);
@@ -420,7 +420,8 @@ rihvr_status rihvr_segmentation_binarize(rihvr_segmentation* seg, double min_int
 /* SparseSegmentation::close (:145-315): dilate then erode with the ball
  * sx^2 + sy^2 + sz^2 <= close_size^2; the erosion keeps a voxel only where the whole ball
  * lies inside the volume and inside the dilation.  close_size <= 0 leaves the volume as it
- * is; close_size > 254: RIHVR_E_INVALID_ARGUMENT. */
+ * is; close_size > 63: RIHVR_E_INVALID_ARGUMENT (the ball's x reach is kept within one
+ * 64-voxel word of the packed binary volume). */
 rihvr_status rihvr_segmentation_close(rihvr_segmentation* seg, int32_t close_size);
 
 /* labelConnectedComponents (:411-532): 26-neighbour components where the in-plane

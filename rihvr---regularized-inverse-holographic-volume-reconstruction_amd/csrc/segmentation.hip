@@ -5,18 +5,19 @@
 //
 // The reference walks sparse per-plane lists on the host (bubble-sorted indices, a
 // binary search per neighbour, one kernel launch per voxel for includeIntensity).
-// Here every pass over the volume is a dense, HBM-streaming kernel over
-// [Nz][H][W] buffers:
+// Here every pass is a dense, HBM-streaming kernel over [Nz][H][W] buffers, with the
+// binary volume bit-packed (64 voxels per word, one wave ballot per word):
 //   load      scatter of the voxel list into a u32 key volume (atomicMax keeps the
 //             larger intensity of a repeated voxel) + max intensity
-//   binarize  key -> binary u8 volume and float values (in place of the keys)
-//   close     ball dilation then erosion, each as a row pass (distance along x to the
-//             nearest set / unset voxel, capped at r+1) and a (y, z) pass over the
-//             ball's rows: voxel t is dilated iff some row (dy, dz) of the ball has a
-//             set voxel within its half-width w(dy, dz) = floor(sqrt(r^2-dy^2-dz^2))
+//   binarize  key -> value (float, in place of the key) and the bit volume
+//   close     ball dilation then erosion on 64-voxel words: the OR over the ball's rows
+//             (dy, dz) of each row's x-dilation by w(dy, dz) = floor(sqrt(r^2-dy^2-dz^2))
+//             (shift-or within the word and its two neighbours); the erosion is the
+//             complement of the dilation of the complement with the outside set
 //   label     union-find with atomicMin links to the smaller root (every root is the
-//             smallest index of its tree), then a flattening pass: each voxel ends
-//             with the smallest global index of its component
+//             smallest index of its tree), then a flattening pass; labels are touched
+//             only at object voxels, so each voxel ends with the smallest global index
+//             of its component
 //   extract   ordered stream compaction of the object voxels (raster order), object
 //             numbers = rank of the root among the roots (first appearance order)
 // The object statistics need the reference's float sums in voxel order; they are one
@@ -35,9 +36,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kItems = 16;                        // voxels per thread in the compaction
-constexpr long kCompactSpan = (long)kThreads * kItems;  // voxels per compaction block
-constexpr int kMaxClose = 254;                    // distances are stored in a byte
+constexpr int kMaxClose = 63;                     // x-dilations reach one word to each side
 
 // key of a binarized intensity: 0 = no voxel, 1 = NaN, else float bits + 2 (v >= 0)
 __device__ __forceinline__ uint32_t intensity_key(float v) { return v != v ? 1u : __float_as_uint(v) + 2u; }
@@ -101,52 +100,6 @@ __global__ void __launch_bounds__(kThreads) k_seg_dense(const float2* __restrict
     if ((threadIdx.x & 63) == 0 && m) atomicMax(maxbits, m);
 }
 
-// binary = intensity > thr (float compared in double, binarize_kernel :39-50); the key
-// becomes the value of the masked volume (mask_kernel :52-61): intensity or 0
-__global__ void __launch_bounds__(kThreads) k_seg_binarize(uint32_t* __restrict__ key_value_bits,
-                                                           uint8_t* __restrict__ mask, long N, double thr) {
-    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) {
-        const float v = key_value(key_value_bits[i]);
-        const bool b = (double)v > thr;
-        mask[i] = b ? 1 : 0;
-        key_value_bits[i] = b ? __float_as_uint(v) : 0u;
-    }
-}
-
-// Row pass of the closing: dst = distance along x to the nearest "set" voxel, capped at
-// cap.  Dilation: set = src != 0, outside the row is not set.  Erosion: set = src == 0
-// or outside the row (a ball that leaves the volume fails the erosion).  One block per
-// 256-voxel segment of a row, the segment and a halo of cap voxels staged in LDS.
-__global__ void __launch_bounds__(kThreads) k_seg_rowdist(const uint8_t* __restrict__ src,
-                                                          uint8_t* __restrict__ dst, int W, long rows, int cap,
-                                                          int erode) {
-    __shared__ uint8_t s[kThreads + 2 * (kMaxClose + 1)];
-    const long segs = (W + kThreads - 1) / kThreads;
-    for (long b = blockIdx.x; b < rows * segs; b += gridDim.x) {
-        const long row = b / segs;
-        const int x0 = (int)(b - row * segs) * kThreads;
-        const uint8_t* r = src + row * (long)W;
-        __syncthreads();
-        for (int j = threadIdx.x; j < kThreads + 2 * cap; j += kThreads) {
-            const int x = x0 - cap + j;
-            const bool inside = x >= 0 && x < W;
-            s[j] = erode ? (!inside || r[inside ? x : 0] == 0) : (inside && r[inside ? x : 0] != 0);
-        }
-        __syncthreads();
-        const int x = x0 + threadIdx.x;
-        if (x < W) {
-            int d = cap;
-            const int c = cap + threadIdx.x;
-            for (int k = 0; k < cap; ++k)
-                if (s[c - k] | s[c + k]) {
-                    d = k;
-                    break;
-                }
-            dst[row * (long)W + x] = (uint8_t)d;
-        }
-    }
-}
-
 __device__ __forceinline__ int isqrt_floor(int n) {
     int r = (int)sqrtf((float)n);
     while (r * r > n) --r;
@@ -154,57 +107,181 @@ __device__ __forceinline__ int isqrt_floor(int n) {
     return r;
 }
 
-// (y, z) pass of the closing over the ball's rows (dy, dz), half-width w = floor(sqrt(
-// r^2 - dy^2 - dz^2)).  Dilation: some in-volume row with dist <= w.  Erosion: every row
-// inside the volume with dist > w (dist = distance to the nearest unset voxel or border).
-__global__ void __launch_bounds__(kThreads) k_seg_ball(const uint8_t* __restrict__ dist,
-                                                       uint8_t* __restrict__ out, int W, int H, int Nz, int r,
-                                                       int erode) {
-    const long N = (long)W * H * Nz;
-    const long plane = (long)W * H;
-    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) {
-        const int z = (int)(i / plane);
-        const long rem = i - (long)z * plane;
-        const int y = (int)(rem / W);
-        const int x = (int)(rem - (long)y * W);
-        bool res = erode != 0;
-        for (int dz = -r; dz <= r; ++dz) {
+// ---------------------------------------------------------------------------------
+// Binary volumes are bit-packed: row r = z*H + y holds R = ceil(W/64) 64-bit words,
+// bit j of word k is voxel x = 64k + j; bits past x = W - 1 are zero.  A wave of 64
+// lanes produces one word with a ballot.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t valid_bits(int wi, int R, int W) {
+    const int tail = W & 63;
+    return (wi < R - 1 || tail == 0) ? ~0ull : ((1ull << tail) - 1ull);
+}
+
+// binary = intensity > thr (float compared in double, binarize_kernel :39-50); the key
+// becomes the value of the masked volume (mask_kernel :52-61): intensity or 0.  One wave
+// per row at a time, kUnroll 64-bit words per step (their loads issued together).
+constexpr int kUnroll = 8;
+__global__ void __launch_bounds__(kThreads) k_seg_binarize(uint32_t* __restrict__ key_value_bits,
+                                                           uint64_t* __restrict__ bits, int W, int R, int rows,
+                                                           double thr) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (kThreads / 64);
+    for (int row = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); row < rows; row += waves) {
+        uint32_t* kr = key_value_bits + (size_t)row * W;
+        uint64_t* br = bits + (size_t)row * R;
+        for (int w0 = 0; w0 < R; w0 += kUnroll) {
+            uint32_t k[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const int x = (w0 + u) * 64 + lane;
+                k[u] = (w0 + u < R && x < W) ? kr[x] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const float v = key_value(k[u]);
+                const int x = (w0 + u) * 64 + lane;
+                const bool b = x < W && (double)v > thr;
+                if (k[u]) kr[x] = b ? __float_as_uint(v) : 0u;  // key 0 is already value 0
+                const uint64_t word = __ballot(b);
+                if (lane == 0 && w0 + u < R) br[w0 + u] = word;
+            }
+        }
+    }
+}
+
+// x-dilation of the middle word of (prev, cur, next) by w <= 63 voxels
+__device__ __forceinline__ uint64_t xdilate(uint64_t prev, uint64_t cur, uint64_t next, int w) {
+    uint64_t r = cur;
+    for (int k = 1; k <= w; ++k) r |= (cur >> k) | (next << (64 - k)) | (cur << k) | (prev >> (64 - k));
+    return r;
+}
+
+// One morphology step of the closing on 64-voxel words (one thread per word): the OR over
+// the ball's rows (dy, dz) of the row's x-dilation by w(dy, dz) = floor(sqrt(r^2 - dy^2 -
+// dz^2)).  Dilation (complement = 0): rows and words outside the volume are empty.
+// Erosion (complement = 1) = NOT dilation of (NOT src with everything outside set): a
+// voxel survives only if its whole ball lies inside the volume and inside src.
+__global__ void __launch_bounds__(kThreads) k_seg_morph(const uint64_t* __restrict__ src,
+                                                        uint64_t* __restrict__ dst, int W, int R, int H, int Nz,
+                                                        int r, int complement) {
+    const uint32_t nwords = (uint32_t)R * H * Nz;
+    const uint64_t flip = complement ? ~0ull : 0ull;  // src words are read as src ^ flip
+    for (uint32_t w = blockIdx.x * kThreads + threadIdx.x; w < nwords; w += gridDim.x * kThreads) {
+        const uint32_t row = w / (uint32_t)R;
+        const int wi = (int)(w - row * (uint32_t)R);
+        const int z = (int)(row / (uint32_t)H);
+        const int y = (int)(row - (uint32_t)z * H);
+        uint64_t acc = 0;
+        for (int dz = -r; dz <= r && acc != ~0ull; ++dz) {
             const int zz = z + dz;
             const int rz = r * r - dz * dz;
             const int wy = isqrt_floor(rz);
             if (zz < 0 || zz >= Nz) {
-                if (erode) {
-                    res = false;
-                    break;
-                }
+                acc |= flip;  // a whole plane of the ball outside the volume
                 continue;
             }
-            bool stop = false;
             for (int dy = -wy; dy <= wy; ++dy) {
                 const int yy = y + dy;
                 if (yy < 0 || yy >= H) {
-                    if (erode) {
-                        res = false;
-                        stop = true;
-                        break;
-                    }
+                    acc |= flip;
                     continue;
                 }
-                const int w = isqrt_floor(rz - dy * dy);
-                const int d = dist[((long)zz * H + yy) * W + x];
-                if (d <= w) {  // dilation: a set voxel in reach; erosion: an unset one
-                    res = !erode;
-                    stop = true;
-                    break;
-                }
+                const uint64_t* rowp = src + ((size_t)zz * H + yy) * R;
+                const uint64_t cur = rowp[wi] ^ flip;
+                const uint64_t prev = wi > 0 ? rowp[wi - 1] ^ flip : flip;
+                const uint64_t next = wi + 1 < R ? rowp[wi + 1] ^ flip : flip;
+                if ((prev | cur | next) != 0ull) acc |= xdilate(prev, cur, next, isqrt_floor(rz - dy * dy));
             }
-            if (stop) break;
         }
-        out[i] = res ? 1 : 0;
+        dst[w] = (acc ^ flip) & valid_bits(wi, R, W);
+    }
+}
+
+__device__ __forceinline__ bool test_bit(const uint64_t* bits, int R, long row, int x) {
+    return (bits[row * R + (x >> 6)] >> (x & 63)) & 1ull;
+}
+
+// --- ordered compaction -----------------------------------------------------
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int k = 0; k < kThreads / 64; ++k) {
+        const uint32_t c = lds[k];
+        before += k < wave ? c : 0;
+        all += c;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// object voxels per block of 256 words (thread t: word b*256 + t)
+__global__ void __launch_bounds__(kThreads) k_list_count(const uint64_t* __restrict__ bits, uint32_t nwords,
+                                                         uint32_t* __restrict__ counts) {
+    __shared__ uint32_t lds[kThreads / 64];
+    const uint32_t w = blockIdx.x * kThreads + threadIdx.x;
+    const uint64_t m = w < nwords ? bits[w] : 0ull;
+    uint32_t all;
+    block_exclusive_scan((uint32_t)__popcll(m), lds, &all);
+    if (threadIdx.x == 0) counts[blockIdx.x] = all;
+}
+
+// exclusive scan of n block counts in one workgroup of 1024 threads; *total = the sum
+__global__ void __launch_bounds__(1024) k_scan(const uint32_t* __restrict__ counts, uint32_t n,
+                                               uint32_t* __restrict__ offsets, uint32_t* __restrict__ total) {
+    __shared__ uint32_t sa[1024];
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t b0 = threadIdx.x * per;
+    const uint32_t b1 = min(n, b0 + per);
+    uint32_t a = 0;
+    for (uint32_t j = b0; j < b1; ++j) a += counts[j];
+    sa[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t xa = threadIdx.x >= (unsigned)off ? sa[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sa[threadIdx.x] += xa;
+        __syncthreads();
+    }
+    uint32_t ra = sa[threadIdx.x] - a;
+    for (uint32_t j = b0; j < b1; ++j) {
+        offsets[j] = ra;
+        ra += counts[j];
+    }
+    if (threadIdx.x == 1023) *total = sa[1023];
+}
+
+// the object voxels' global indices in raster order (thread order, then bit order)
+__global__ void __launch_bounds__(kThreads) k_list_scatter(const uint64_t* __restrict__ bits, int W, int R,
+                                                           uint32_t nwords, const uint32_t* __restrict__ offsets,
+                                                           uint32_t* __restrict__ list) {
+    __shared__ uint32_t lds[kThreads / 64];
+    const uint32_t w = blockIdx.x * kThreads + threadIdx.x;
+    uint64_t m = w < nwords ? bits[w] : 0ull;
+    uint32_t all;
+    uint32_t pos = offsets[blockIdx.x] + block_exclusive_scan((uint32_t)__popcll(m), lds, &all);
+    if (m) {
+        const uint32_t row = w / (uint32_t)R;
+        const uint32_t base = row * (uint32_t)W + (w - row * (uint32_t)R) * 64u;
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1ull;
+            list[pos++] = base + (uint32_t)j;
+        }
     }
 }
 
 // --- connected components ---------------------------------------------------
+// One thread per object voxel of the compacted list; labels are read and written only at
+// object voxels.
 __device__ __forceinline__ uint32_t load_label(const uint32_t* lab, uint32_t i) {
     return __hip_atomic_load(lab + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -233,175 +310,108 @@ __device__ void unite(uint32_t* lab, uint32_t a, uint32_t b) {
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_ccl_init(const uint8_t* __restrict__ mask, uint32_t* __restrict__ lab,
-                                                       long N) {
-    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads)
-        lab[i] = mask[i] ? (uint32_t)i : kNone;
+__global__ void __launch_bounds__(kThreads) k_ccl_init(const uint32_t* __restrict__ list, uint32_t n,
+                                                       uint32_t* __restrict__ lab) {
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) lab[list[i]] = list[i];
 }
 
 // Each object voxel unites with its 13 earlier neighbours (the other 13 unite with it):
-// in-plane flat offsets dy*W + dx kept inside 0..P-1 (labelConnectedComponents'
-// offsets, :452-480), on the previous plane all nine, on the same plane the four below 0.
-__global__ void __launch_bounds__(kThreads) k_ccl_merge(const uint8_t* __restrict__ mask, uint32_t* lab, int W,
-                                                        long P, long N) {
-    const long off[4] = {-(long)W - 1, -(long)W, -(long)W + 1, -1};
-    for (long v = (long)blockIdx.x * kThreads + threadIdx.x; v < N; v += (long)gridDim.x * kThreads) {
-        if (!mask[v]) continue;
-        const long z = v / P;
-        const long i = v - z * P;
-        for (int k = 0; k < 4; ++k) {
-            const long j = i + off[k];
-            if (j >= 0 && mask[v + off[k]]) unite(lab, (uint32_t)v, (uint32_t)(v + off[k]));
+// in-plane flat offsets dy*W + dx kept inside 0..W*H-1 (labelConnectedComponents'
+// offsets, :452-480), so x = -1 is x = W-1 of the row above and x = W is x = 0 of the
+// row below; on the previous plane all nine, on the same plane the four below 0.
+// Sixteen lanes per voxel, lane k < 13 taking neighbour k (dz = -1: k = 0..8 for
+// (dy, dx) in raster order; dz = 0: k = 9..12 for (-1,-1), (-1,0), (-1,1), (0,-1)).
+__global__ void __launch_bounds__(kThreads) k_ccl_merge(const uint32_t* __restrict__ list, uint32_t n,
+                                                        const uint64_t* __restrict__ bits, uint32_t* lab, int W,
+                                                        int R, int H) {
+    const long total = (long)n * 16;
+    for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < total; t += (long)gridDim.x * kThreads) {
+        const uint32_t k = (uint32_t)(t & 15);
+        if (k >= 13u) continue;
+        const uint32_t v = list[t >> 4];
+        const uint32_t row = v / (uint32_t)W;
+        const int x = (int)(v - row * (uint32_t)W);
+        const int z = (int)(row / (uint32_t)H);
+        const int y = (int)(row - (uint32_t)z * H);
+        const int dz = k < 9u ? -1 : 0;
+        const int q = k < 9u ? (int)k : (int)k - 9;
+        const int dy = q / 3 - 1, dx = q % 3 - 1;
+        if (z + dz < 0) continue;
+        int yy = y + dy, xx = x + dx;
+        if (xx < 0) {
+            xx = W - 1;
+            --yy;
+        } else if (xx >= W) {
+            xx = 0;
+            ++yy;
         }
-        if (z > 0) {
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const long o = (long)dy * W + dx;
-                    const long j = i + o;
-                    if (j >= 0 && j < P && mask[v - P + o]) unite(lab, (uint32_t)v, (uint32_t)(v - P + o));
-                }
-        }
+        if (yy < 0 || yy >= H) continue;  // flat index outside the plane
+        const long nrow = (long)(z + dz) * H + yy;
+        if (test_bit(bits, R, nrow, xx)) unite(lab, v, (uint32_t)(nrow * W + xx));
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_ccl_flatten(uint32_t* lab, long N) {
-    for (long v = (long)blockIdx.x * kThreads + threadIdx.x; v < N; v += (long)gridDim.x * kThreads) {
+__global__ void __launch_bounds__(kThreads) k_ccl_flatten(const uint32_t* __restrict__ list, uint32_t n,
+                                                          uint32_t* lab) {
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        const uint32_t v = list[i];
         const uint32_t l = lab[v];
-        if (l != kNone && l != (uint32_t)v) lab[v] = find_root(lab, l);
+        if (l != v) lab[v] = find_root(lab, l);
     }
 }
 
-// --- ordered compaction -----------------------------------------------------
-// A block covers kCompactSpan consecutive voxels; item k of thread t is voxel
-// base + k*256 + t, so (k, t) order is raster order.
-__device__ __forceinline__ uint32_t block_prefix(bool f, uint32_t* wave_counts, uint32_t* total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t bal = __ballot(f);
-    const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) wave_counts[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-    for (int w = 0; w < kThreads / 64; ++w) {
-        const uint32_t c = wave_counts[w];
-        before += w < wave ? c : 0;
-        all += c;
-    }
-    __syncthreads();
-    *total = all;
-    return before + below;
+// --- objects ----------------------------------------------------------------
+// roots (label == own index) per block of 256 list entries
+__global__ void __launch_bounds__(kThreads) k_root_count(const uint32_t* __restrict__ list, uint32_t n,
+                                                         const uint32_t* __restrict__ lab,
+                                                         uint32_t* __restrict__ counts) {
+    __shared__ uint32_t lds[kThreads / 64];
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    const bool f = i < n && lab[list[i]] == list[i];
+    uint32_t all;
+    block_exclusive_scan(f ? 1u : 0u, lds, &all);
+    if (threadIdx.x == 0) counts[blockIdx.x] = all;
 }
 
-__global__ void __launch_bounds__(kThreads) k_cmp_count(const uint32_t* __restrict__ lab, long N,
-                                                        uint32_t* __restrict__ counts) {
-    __shared__ uint32_t wc[kThreads / 64];
-    const long base = (long)blockIdx.x * kCompactSpan;
-    uint32_t nobj = 0, nroot = 0;
-    for (int k = 0; k < kItems; ++k) {
-        const long v = base + (long)k * kThreads + threadIdx.x;
-        const uint32_t l = v < N ? lab[v] : kNone;
-        nobj += l != kNone;
-        nroot += l == (uint32_t)v;
-    }
-    uint32_t t = nobj;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) t += (uint32_t)__shfl_xor((int)t, off, 64);
-    uint32_t u = nroot;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) u += (uint32_t)__shfl_xor((int)u, off, 64);
-    __shared__ uint32_t wr[kThreads / 64];
-    if ((threadIdx.x & 63) == 0) {
-        wc[threadIdx.x >> 6] = t;
-        wr[threadIdx.x >> 6] = u;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t a = 0, b = 0;
-        for (int w = 0; w < kThreads / 64; ++w) {
-            a += wc[w];
-            b += wr[w];
-        }
-        counts[2 * blockIdx.x] = a;
-        counts[2 * blockIdx.x + 1] = b;
-    }
-}
-
-// exclusive scan of the (objects, roots) block counts in one workgroup of 1024 threads;
-// totals[0..1] = the sums
-__global__ void __launch_bounds__(1024) k_cmp_scan(const uint32_t* __restrict__ counts, long nblocks,
-                                                   uint32_t* __restrict__ offsets, uint32_t* __restrict__ totals) {
-    __shared__ uint32_t sa[1024], sb[1024];
-    const long per = (nblocks + 1023) / 1024;
-    const long b0 = (long)threadIdx.x * per;
-    const long b1 = min(nblocks, b0 + per);
-    uint32_t a = 0, b = 0;
-    for (long j = b0; j < b1; ++j) {
-        a += counts[2 * j];
-        b += counts[2 * j + 1];
-    }
-    sa[threadIdx.x] = a;
-    sb[threadIdx.x] = b;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-        const uint32_t xa = threadIdx.x >= (unsigned)off ? sa[threadIdx.x - off] : 0u;
-        const uint32_t xb = threadIdx.x >= (unsigned)off ? sb[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sa[threadIdx.x] += xa;
-        sb[threadIdx.x] += xb;
-        __syncthreads();
-    }
-    uint32_t ra = sa[threadIdx.x] - a, rb = sb[threadIdx.x] - b;
-    for (long j = b0; j < b1; ++j) {
-        offsets[2 * j] = ra;
-        offsets[2 * j + 1] = rb;
-        ra += counts[2 * j];
-        rb += counts[2 * j + 1];
-    }
-    if (threadIdx.x == 1023) {
-        totals[0] = sa[1023];
-        totals[1] = sb[1023];
-    }
-}
-
-// rank of every root among the roots in raster order (its object number)
-__global__ void __launch_bounds__(kThreads) k_cmp_roots(const uint32_t* __restrict__ lab, long N,
+// rank of every root among the roots in list (raster) order: its object number
+__global__ void __launch_bounds__(kThreads) k_root_rank(const uint32_t* __restrict__ list, uint32_t n,
+                                                        const uint32_t* __restrict__ lab,
                                                         const uint32_t* __restrict__ offsets,
                                                         uint32_t* __restrict__ rank) {
-    __shared__ uint32_t wc[kThreads / 64];
-    const long base = (long)blockIdx.x * kCompactSpan;
-    uint32_t run = offsets[2 * blockIdx.x + 1];
-    for (int k = 0; k < kItems; ++k) {
-        const long v = base + (long)k * kThreads + threadIdx.x;
-        const bool f = v < N && lab[v] == (uint32_t)v;
-        uint32_t all;
-        const uint32_t pos = block_prefix(f, wc, &all);
-        if (f) rank[v] = run + pos;
-        run += all;
+    __shared__ uint32_t lds[kThreads / 64];
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t v = i < n ? list[i] : 0u;
+    const bool f = i < n && lab[v] == v;
+    uint32_t all;
+    const uint32_t pos = offsets[blockIdx.x] + block_exclusive_scan(f ? 1u : 0u, lds, &all);
+    if (f) rank[v] = pos;
+}
+
+// object number and value of every list entry
+__global__ void __launch_bounds__(kThreads) k_object_of(const uint32_t* __restrict__ list, uint32_t n,
+                                                        const uint32_t* __restrict__ lab,
+                                                        const uint32_t* __restrict__ rank,
+                                                        const float* __restrict__ value,
+                                                        uint32_t* __restrict__ out_object,
+                                                        float* __restrict__ out_value) {
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        const uint32_t v = list[i];
+        out_object[i] = rank[lab[v]];
+        out_value[i] = value[v];
     }
 }
 
-// object voxels in raster order: global index, object number, value
-__global__ void __launch_bounds__(kThreads) k_cmp_scatter(const uint32_t* __restrict__ lab,
-                                                          const float* __restrict__ value, long N,
-                                                          const uint32_t* __restrict__ offsets,
-                                                          const uint32_t* __restrict__ rank,
-                                                          uint32_t* __restrict__ out_index,
-                                                          uint32_t* __restrict__ out_object,
-                                                          float* __restrict__ out_value) {
-    __shared__ uint32_t wc[kThreads / 64];
-    const long base = (long)blockIdx.x * kCompactSpan;
-    uint32_t run = offsets[2 * blockIdx.x];
-    for (int k = 0; k < kItems; ++k) {
-        const long v = base + (long)k * kThreads + threadIdx.x;
-        const uint32_t l = v < N ? lab[v] : kNone;
-        const bool f = l != kNone;
-        uint32_t all;
-        const uint32_t pos = block_prefix(f, wc, &all);
-        if (f) {
-            out_index[run + pos] = (uint32_t)v;
-            out_object[run + pos] = rank[l];
-            out_value[run + pos] = value[v];
-        }
-        run += all;
+// dense copies of the state for the caller: the bit volume as bytes, the labels with
+// 0xFFFFFFFF off the objects
+__global__ void __launch_bounds__(kThreads) k_unpack(const uint64_t* __restrict__ bits,
+                                                     const uint32_t* __restrict__ lab, int W, int R, long N,
+                                                     uint8_t* __restrict__ mask_out, uint32_t* __restrict__ lab_out) {
+    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) {
+        const long row = i / W;
+        const int x = (int)(i - row * W);
+        const bool b = test_bit(bits, R, row, x);
+        if (mask_out) mask_out[i] = b ? 1 : 0;
+        if (lab_out) lab_out[i] = b ? lab[i] : kNone;
     }
 }
 
@@ -430,16 +440,20 @@ struct rihvr_segmentation {
     long P = 0, N = 0;
     int device = 0;
     hipStream_t stream = nullptr;
+    int R = 0;                   // 64-bit words per row of a bit volume
+    long nwords = 0;
     uint32_t* key = nullptr;     // keys, then values (float bits)
-    uint8_t* mask = nullptr;     // binary volume
-    uint8_t* work = nullptr;     // 4N bytes: two byte volumes for the closing, or the root ranks
-    uint32_t* lab = nullptr;     // labels
-    uint32_t* counts = nullptr;  // 2 per compaction block
+    uint64_t* bits = nullptr;    // binary volume, bit-packed rows
+    uint64_t* bits2 = nullptr;   // the dilation during the closing
+    uint32_t* work = nullptr;    // N words: root ranks, or dense copies for the caller
+    uint32_t* lab = nullptr;     // labels (object voxels only)
+    uint32_t* counts = nullptr;  // per scan block
     uint32_t* offsets = nullptr;
-    uint32_t* small = nullptr;   // [0] max bits, [1] bad entries, [2..3] totals
-    long nblocks = 0;
-    void* list = nullptr;        // staging of a host voxel list or volume
+    uint32_t* small = nullptr;   // [0] max bits, [1] bad entries, [2] object voxels, [3] objects
+    long nblocks = 0;            // capacity of counts / offsets
+    void* list = nullptr;        // staging of a host voxel list or volume; then the object voxels
     size_t list_bytes = 0;
+    uint32_t n_list = 0;         // object voxels in the compacted list (after label)
     hipEvent_t ev[T_COUNT][2] = {};
     bool timed[T_COUNT] = {};
     Stage stage = EMPTY;
@@ -592,7 +606,9 @@ rihvr_status rihvr_segmentation_create(int32_t width, int32_t height, int32_t nu
     s->Nz = num_planes;
     s->P = (long)width * height;
     s->N = N;
-    s->nblocks = (N + kCompactSpan - 1) / kCompactSpan;
+    s->R = (width + 63) / 64;
+    s->nwords = (long)s->R * height * num_planes;
+    s->nblocks = std::max(s->nwords, N) / kThreads + 1;
     rihvr_status st = RIHVR_OK;
     auto fail = [&](hipError_t e) {
         std::fprintf(stderr, "rihvr: HIP error %s in rihvr_segmentation_create\n", hipGetErrorString(e));
@@ -601,11 +617,12 @@ rihvr_status rihvr_segmentation_create(int32_t width, int32_t height, int32_t nu
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void**)&s->key, (size_t)N * 4);
-    if (e == hipSuccess) e = hipMalloc((void**)&s->mask, (size_t)N);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->bits, (size_t)s->nwords * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->bits2, (size_t)s->nwords * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&s->work, (size_t)N * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->lab, (size_t)N * 4);
-    if (e == hipSuccess) e = hipMalloc((void**)&s->counts, (size_t)s->nblocks * 8);
-    if (e == hipSuccess) e = hipMalloc((void**)&s->offsets, (size_t)s->nblocks * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->counts, (size_t)s->nblocks * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->offsets, (size_t)s->nblocks * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->small, 16 * sizeof(uint32_t));
     for (int t = 0; t < T_COUNT && e == hipSuccess; ++t)
         for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&s->ev[t][k]);
@@ -622,7 +639,7 @@ void rihvr_segmentation_destroy(rihvr_segmentation* s) {
     if (!s) return;
     DeviceGuard g(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (void* p : {(void*)s->key, (void*)s->mask, (void*)s->work, (void*)s->lab, (void*)s->counts,
+    for (void* p : {(void*)s->key, (void*)s->bits, (void*)s->bits2, (void*)s->work, (void*)s->lab, (void*)s->counts,
                     (void*)s->offsets, (void*)s->small, s->list})
         if (p) (void)hipFree(p);
     for (int t = 0; t < T_COUNT; ++t)
@@ -704,8 +721,9 @@ rihvr_status rihvr_segmentation_binarize(rihvr_segmentation* s, double min_inten
     const double thr = min_intensity * mx / 256.0;  // sparse_segmentation.cu:69-71
     rihvr_status st = begin_timer(s, T_BINARIZE);
     if (st) return st;
-    hipLaunchKernelGGL(k_seg_binarize, dim3(grid_for(s->N)), dim3(kThreads), 0, s->stream, s->key, s->mask, s->N,
-                       thr);
+    const int rows = s->H * s->Nz;
+    hipLaunchKernelGGL(k_seg_binarize, dim3((unsigned)std::min(rows / 4 + 1, 64 * 1024)), dim3(kThreads), 0,
+                       s->stream, s->key, s->bits, s->W, s->R, rows, thr);
     SEG_HIP(hipGetLastError());
     st = end_timer(s, T_BINARIZE);
     if (st) return st;
@@ -723,20 +741,12 @@ rihvr_status rihvr_segmentation_close(rihvr_segmentation* s, int32_t close_size)
     if (close_size > kMaxClose) return RIHVR_E_INVALID_ARGUMENT;
     if (close_size <= 0) return RIHVR_OK;  // simpleMorph returns at once (:225)
     DeviceGuard g(s->device);
-    const long rows = (long)s->H * s->Nz;
-    const long segs = (s->W + kThreads - 1) / kThreads;
-    const unsigned row_grid = (unsigned)std::min(rows * segs, 1024L * 1024L);
-    uint8_t* dist = s->work;
-    uint8_t* dil = s->work + s->N;
-    const int cap = close_size + 1;
     rihvr_status st = begin_timer(s, T_CLOSE);
     if (st) return st;
-    hipLaunchKernelGGL(k_seg_rowdist, dim3(row_grid), dim3(kThreads), 0, s->stream, s->mask, dist, s->W, rows, cap,
-                       0);
-    hipLaunchKernelGGL(k_seg_ball, dim3(grid_for(s->N)), dim3(kThreads), 0, s->stream, dist, dil, s->W, s->H, s->Nz,
-                       close_size, 0);
-    hipLaunchKernelGGL(k_seg_rowdist, dim3(row_grid), dim3(kThreads), 0, s->stream, dil, dist, s->W, rows, cap, 1);
-    hipLaunchKernelGGL(k_seg_ball, dim3(grid_for(s->N)), dim3(kThreads), 0, s->stream, dist, s->mask, s->W, s->H,
+    const unsigned grid = grid_for(s->nwords);
+    hipLaunchKernelGGL(k_seg_morph, dim3(grid), dim3(kThreads), 0, s->stream, s->bits, s->bits2, s->W, s->R, s->H,
+                       s->Nz, close_size, 0);
+    hipLaunchKernelGGL(k_seg_morph, dim3(grid), dim3(kThreads), 0, s->stream, s->bits2, s->bits, s->W, s->R, s->H,
                        s->Nz, close_size, 1);
     SEG_HIP(hipGetLastError());
     st = end_timer(s, T_CLOSE);
@@ -751,14 +761,32 @@ rihvr_status rihvr_segmentation_label(rihvr_segmentation* s) {
     DeviceGuard g(s->device);
     rihvr_status st = begin_timer(s, T_LABEL);
     if (st) return st;
-    const unsigned grid = grid_for(s->N);
-    hipLaunchKernelGGL(k_ccl_init, dim3(grid), dim3(kThreads), 0, s->stream, s->mask, s->lab, s->N);
-    hipLaunchKernelGGL(k_ccl_merge, dim3(grid), dim3(kThreads), 0, s->stream, s->mask, s->lab, s->W, s->P, s->N);
-    hipLaunchKernelGGL(k_ccl_flatten, dim3(grid), dim3(kThreads), 0, s->stream, s->lab, s->N);
+    // the object voxels in raster order
+    const uint32_t nw = (uint32_t)s->nwords;
+    const uint32_t nbw = (nw + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_list_count, dim3(nbw), dim3(kThreads), 0, s->stream, s->bits, nw, s->counts);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s->stream, s->counts, nbw, s->offsets, s->small + 2);
     SEG_HIP(hipGetLastError());
+    uint32_t n = 0;
+    SEG_HIP(hipMemcpyAsync(&n, s->small + 2, sizeof(n), hipMemcpyDeviceToHost, s->stream));
+    SEG_HIP(hipStreamSynchronize(s->stream));
+    if (n > 0) {
+        st = stage_buffer(s, (size_t)n * 12);  // list, object numbers, values
+        if (st) return st;
+        uint32_t* list = (uint32_t*)s->list;
+        hipLaunchKernelGGL(k_list_scatter, dim3(nbw), dim3(kThreads), 0, s->stream, s->bits, s->W, s->R, nw,
+                           s->offsets, list);
+        const unsigned grid = grid_for(n);
+        hipLaunchKernelGGL(k_ccl_init, dim3(grid), dim3(kThreads), 0, s->stream, list, n, s->lab);
+        hipLaunchKernelGGL(k_ccl_merge, dim3(grid_for((long)n * 16)), dim3(kThreads), 0, s->stream, list, n, s->bits,
+                           s->lab, s->W, s->R, s->H);
+        hipLaunchKernelGGL(k_ccl_flatten, dim3(grid), dim3(kThreads), 0, s->stream, list, n, s->lab);
+        SEG_HIP(hipGetLastError());
+    }
     st = end_timer(s, T_LABEL);
     if (st) return st;
     SEG_HIP(hipStreamSynchronize(s->stream));
+    s->n_list = n;
     s->stage = LABELED;
     return RIHVR_OK;
 }
@@ -769,26 +797,22 @@ rihvr_status rihvr_segmentation_extract(rihvr_segmentation* s, int64_t* num_obje
     DeviceGuard g(s->device);
     rihvr_status st = begin_timer(s, T_EXTRACT);
     if (st) return st;
-    const unsigned nb = (unsigned)s->nblocks;
-    hipLaunchKernelGGL(k_cmp_count, dim3(nb), dim3(kThreads), 0, s->stream, s->lab, s->N, s->counts);
-    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(1024), 0, s->stream, s->counts, s->nblocks, s->offsets,
-                       s->small + 2);
-    uint32_t* rank = (uint32_t*)s->work;
-    hipLaunchKernelGGL(k_cmp_roots, dim3(nb), dim3(kThreads), 0, s->stream, s->lab, s->N, s->offsets, rank);
-    SEG_HIP(hipGetLastError());
-    uint32_t totals[2] = {0, 0};
-    SEG_HIP(hipMemcpyAsync(totals, s->small + 2, sizeof(totals), hipMemcpyDeviceToHost, s->stream));
-    SEG_HIP(hipStreamSynchronize(s->stream));
-    const size_t nvox = totals[0];
-    if (nvox > 0) {
-        st = stage_buffer(s, nvox * 12);
-        if (st) return st;
-        uint32_t* oi = (uint32_t*)s->list;
-        uint32_t* oo = oi + nvox;
+    const uint32_t n = s->n_list;
+    const size_t nvox = n;
+    uint32_t n_roots = 0;
+    if (n > 0) {
+        const uint32_t* list = (const uint32_t*)s->list;
+        uint32_t* oo = (uint32_t*)s->list + nvox;
         float* ov = (float*)(oo + nvox);
-        hipLaunchKernelGGL(k_cmp_scatter, dim3(nb), dim3(kThreads), 0, s->stream, s->lab, (const float*)s->key,
-                           s->N, s->offsets, rank, oi, oo, ov);
+        const uint32_t nb = (n + kThreads - 1) / kThreads;
+        uint32_t* rank = s->work;
+        hipLaunchKernelGGL(k_root_count, dim3(nb), dim3(kThreads), 0, s->stream, list, n, s->lab, s->counts);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s->stream, s->counts, nb, s->offsets, s->small + 3);
+        hipLaunchKernelGGL(k_root_rank, dim3(nb), dim3(kThreads), 0, s->stream, list, n, s->lab, s->offsets, rank);
+        hipLaunchKernelGGL(k_object_of, dim3(grid_for(n)), dim3(kThreads), 0, s->stream, list, n, s->lab, rank,
+                           (const float*)s->key, oo, ov);
         SEG_HIP(hipGetLastError());
+        SEG_HIP(hipMemcpyAsync(&n_roots, s->small + 3, sizeof(n_roots), hipMemcpyDeviceToHost, s->stream));
     }
     st = end_timer(s, T_EXTRACT);
     if (st) return st;
@@ -803,7 +827,7 @@ rihvr_status rihvr_segmentation_extract(rihvr_segmentation* s, int64_t* num_obje
     SEG_HIP(hipStreamSynchronize(s->stream));
     st = accumulate_objects(s, gi, ob, va, s->objects);
     if (st) return st;
-    if (s->objects.size() != totals[1]) return RIHVR_E_UNKNOWN_ERROR;
+    if (s->objects.size() != n_roots) return RIHVR_E_UNKNOWN_ERROR;
     s->stage = EXTRACTED;
     if (num_objects) *num_objects = (int64_t)s->objects.size();
     return RIHVR_OK;
@@ -869,7 +893,10 @@ rihvr_status rihvr_segmentation_mask(rihvr_segmentation* s, uint8_t* out, int lo
     if (!s || !out) return RIHVR_E_INVALID_ARGUMENT;
     if (s->stage < BINARIZED) return RIHVR_E_INVALID_STATE;
     DeviceGuard g(s->device);
-    return copy_out(s, out, s->mask, (size_t)s->N, loc);
+    hipLaunchKernelGGL(k_unpack, dim3(grid_for(s->N)), dim3(kThreads), 0, s->stream, s->bits, s->lab, s->W, s->R,
+                       s->N, (uint8_t*)s->work, (uint32_t*)nullptr);
+    SEG_HIP(hipGetLastError());
+    return copy_out(s, out, s->work, (size_t)s->N, loc);
 }
 
 rihvr_status rihvr_segmentation_values(rihvr_segmentation* s, float* out, int loc) {
@@ -883,7 +910,10 @@ rihvr_status rihvr_segmentation_labels(rihvr_segmentation* s, uint32_t* out, int
     if (!s || !out) return RIHVR_E_INVALID_ARGUMENT;
     if (s->stage < LABELED) return RIHVR_E_INVALID_STATE;
     DeviceGuard g(s->device);
-    return copy_out(s, out, s->lab, (size_t)s->N * 4, loc);
+    hipLaunchKernelGGL(k_unpack, dim3(grid_for(s->N)), dim3(kThreads), 0, s->stream, s->bits, s->lab, s->W, s->R,
+                       s->N, (uint8_t*)nullptr, s->work);
+    SEG_HIP(hipGetLastError());
+    return copy_out(s, out, s->work, (size_t)s->N * 4, loc);
 }
 
 rihvr_status rihvr_segmentation_timing(rihvr_segmentation* s, double* ms) {

@@ -217,7 +217,7 @@ def test_errors(pkg, gpu_available):
         assert e.value.status == 4
         seg.binarize(1.0)
         with pytest.raises(pkg.RihvrError) as e:
-            seg.close(255)
+            seg.close(64)
         assert e.value.status == 7
         with pytest.raises(pkg.RihvrError) as e:
             seg.binarize(1.0)
