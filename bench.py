@@ -150,6 +150,33 @@ def background_bench(pkg, steps: int, warmup: int, n_frames: int = 200, size: in
             "achieved_GBps_wall": nbytes / dt / 1e9, "algorithmic_bytes_per_call": nbytes}
 
 
+def preprocess_bench(pkg, steps: int, warmup: int, n_frames: int = 100, size: int = 1024):
+    """The input contract on the device (rihvr_prepare_holograms) for n_frames 8-bit
+    device-resident frames of size^2 into complex64 holograms on the device: frames/s.
+    Compulsory bytes: the frames twice (mean pass, write pass) + the holograms."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(20261205)
+    frames = torch.randint(0, 256, (n_frames, size, size), generator=g, device="cuda").to(torch.uint8)
+    out = torch.empty((n_frames, size, size), dtype=torch.complex64, device="cuda")
+    fp = pkg.FileParams()
+    fp.roi_size = -1
+    for _ in range(warmup):
+        pkg.prepare_holograms(fp, frames, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pkg.prepare_holograms(fp, frames, out=out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    nbytes = n_frames * size * size * (2.0 * 1 + 8)
+    return {"metric": f"preprocessing frames/s ({n_frames} 8-bit frames {size}^2 -> complex64 holograms)",
+            "value": n_frames / dt, "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u8->f32", "data": "synthetic (uniform grey values)",
+            "config": {"workload": f"rescale /255, I/mean - 1, no crop, no padding; {n_frames} frames of {size}x{size}"},
+            "achieved_GBps_wall": nbytes / dt / 1e9, "algorithmic_bytes_per_call": nbytes}
+
+
 def segmentation_bench(pkg, scenes, steps: int, warmup: int, iterations: int = 100, cpu: bool = True):
     """The segmentation executable's chain on the 1024^2 x 128 reconstruction of the
     seeded tracers-and-rods hologram (iterations of plain FISTA+TV, mode 7), read from
@@ -249,7 +276,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["c2", "c1", "background", "segmentation"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "background", "segmentation", "preprocess"], default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--frames", type=int, default=3,
                     help="frames of the s/frame-to-tolerance leg (0 skips it)")
@@ -267,6 +294,9 @@ def main():
     pkg = entry.load_package()
     if args.workload == "background":
         print(json.dumps(background_bench(pkg, max(1, min(args.steps, 20)), min(args.warmup, 3))), flush=True)
+        return
+    if args.workload == "preprocess":
+        print(json.dumps(preprocess_bench(pkg, max(1, min(args.steps, 20)), min(args.warmup, 3))), flush=True)
         return
     scenes = entry._load_submodule("scenes")
     if args.workload == "segmentation":

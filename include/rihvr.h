@@ -303,6 +303,19 @@ rihvr_status rihvr_prepare_hologram(const rihvr_file_params* fp, const float* im
                                     const float* background, int32_t width, int32_t height,
                                     float* out, int32_t* out_width, int32_t* out_height);
 
+/* The same input contract on the device for a batch of frames: frames [n][height][width]
+ * of raw grey values (pixel_type RIHVR_PIXEL_U8, _U16 or _F32) at in_loc, background
+ * [height][width] float32 raw grey values at in_loc when fp->remove_background; out:
+ * [n][H'][W'] complex64 at out_loc (call with out == NULL for the size).  Each frame equals
+ * rihvr_prepare_hologram's output for it; the errors are that function's, and
+ * RIHVR_E_UNKNOWN_TYPE for another pixel_type.  Runs on the current HIP device. */
+#define RIHVR_PIXEL_U8 1
+#define RIHVR_PIXEL_U16 2
+#define RIHVR_PIXEL_F32 4
+rihvr_status rihvr_prepare_holograms(const rihvr_file_params* fp, const void* frames, int32_t pixel_type,
+                                     int32_t n_frames, const float* background, int32_t width, int32_t height,
+                                     int in_loc, float* out, int out_loc, int32_t* out_width, int32_t* out_height);
+
 /* 8-bit grayscale uncompressed TIFF writer (the reference writes with cv::imwrite). */
 rihvr_status rihvr_write_tiff8(const char* path, const uint8_t* pixels, int32_t width,
                                int32_t height);

@@ -46,7 +46,7 @@ EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_d
             "rihvr_segmentation_binarize", "rihvr_segmentation_close", "rihvr_segmentation_label",
             "rihvr_segmentation_extract", "rihvr_segmentation_size_filter", "rihvr_segmentation_objects",
             "rihvr_segmentation_write_centroids", "rihvr_segmentation_mask", "rihvr_segmentation_values",
-            "rihvr_segmentation_labels", "rihvr_segmentation_timing")
+            "rihvr_segmentation_labels", "rihvr_segmentation_timing", "rihvr_prepare_holograms")
 PATH_MAX = 4096
 
 
@@ -326,6 +326,9 @@ def lib():
             getattr(L, name).restype = st
         L.rihvr_segmentation_timing.argtypes = [vp, _P(ctypes.c_double)]
         L.rihvr_segmentation_timing.restype = st
+        L.rihvr_prepare_holograms.argtypes = [_P(FileParams), vp, ctypes.c_int32, ctypes.c_int32, vp, ctypes.c_int32,
+                                              ctypes.c_int32, ip, vp, ip, i32p, i32p]
+        L.rihvr_prepare_holograms.restype = st
         _lib = L
     return _lib
 
@@ -375,6 +378,47 @@ def prepare_hologram(fp, image, background=None) -> np.ndarray:
     _check(lib().rihvr_prepare_hologram(ctypes.byref(fp), ctypes.c_void_p(img.ctypes.data), bp, w, h,
                                         ctypes.c_void_p(out.ctypes.data), ctypes.byref(ow), ctypes.byref(oh)),
            "rihvr_prepare_hologram")
+    return out
+
+
+PIXEL_TYPES = {"uint8": 1, "uint16": 2, "float32": 4}
+
+
+def prepare_holograms(fp, frames, background=None, out=None):
+    """rihvr_prepare_holograms: the input contract on the device for frames [n][H][W] of
+    raw grey values (numpy or CUDA tensor; uint8, uint16 or float32) -> complex64
+    holograms [n][H'][W'] (numpy, or written into ``out``, a CUDA tensor)."""
+    if _is_torch(frames):
+        if not frames.is_cuda or not frames.is_contiguous():
+            raise ValueError("device frames must be contiguous CUDA tensors")
+        dtype = str(frames.dtype).replace("torch.", "")
+        fptr, floc, keep = ctypes.c_void_p(frames.data_ptr()), DEVICE, frames
+    else:
+        keep = np.ascontiguousarray(frames)
+        dtype = str(keep.dtype)
+        fptr, floc = ctypes.c_void_p(keep.ctypes.data), HOST
+    if dtype not in PIXEL_TYPES:
+        raise ValueError(f"pixel type {dtype}")
+    n, h, w = keep.shape
+    bkeep = None
+    bptr = None
+    if background is not None:
+        if floc == DEVICE:
+            bkeep = background
+            bptr = ctypes.c_void_p(background.data_ptr())
+        else:
+            bkeep = np.ascontiguousarray(background, np.float32)
+            bptr = ctypes.c_void_p(bkeep.ctypes.data)
+    ow, oh = ctypes.c_int32(), ctypes.c_int32()
+    L = lib()
+    _check(L.rihvr_prepare_holograms(ctypes.byref(fp), fptr, PIXEL_TYPES[dtype], n, bptr, w, h, floc, None, HOST,
+                                     ctypes.byref(ow), ctypes.byref(oh)), "rihvr_prepare_holograms")
+    shape = (n, oh.value, ow.value)
+    if out is None:
+        out = np.empty(shape, np.complex64)
+    op, oloc, _k = _arg(out, shape, writable=True)
+    _check(L.rihvr_prepare_holograms(ctypes.byref(fp), fptr, PIXEL_TYPES[dtype], n, bptr, w, h, floc, op, oloc,
+                                     ctypes.byref(ow), ctypes.byref(oh)), "rihvr_prepare_holograms")
     return out
 
 

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/rihvr.h"
+#include "io_internal.hpp"
 
 namespace {
 
@@ -340,6 +341,50 @@ void global_mag_range(const Volume& vol, double* lo, double* hi) {
 
 }  // namespace
 
+namespace rihvr {
+
+rihvr_status crop_geometry(const rihvr_file_params* fp, int width, int height, CropGeometry* g) {
+    // crop (hologram.cu:768-880)
+    int rx, ry, rw, rh;
+    if (fp->roi_w * fp->roi_h > 0) {
+        rx = fp->roi_x; ry = fp->roi_y; rw = fp->roi_w; rh = fp->roi_h;
+    } else {
+        const int cx = (int)fp->center_x, cy = (int)fp->center_y, size = fp->roi_size;
+        int sx = cx - size / 2, sy = cy - size / 2;
+        rw = size; rh = size;
+        if (sx < 0) sx = 0;
+        if (sy < 0) sy = 0;
+        if (sx + size > width) sx = width - size;
+        if (sy + size > height) sy = height - size;
+        if (width < size) { sx = 0; rw = width; }
+        if (height < size) { sy = 0; rh = height; }
+        if (size < 1) { sx = 0; sy = 0; rw = width; rh = height; }
+        rx = sx; ry = sy;
+    }
+    // crop(Rect): zero borders where the ROI leaves the image (the right border is
+    // computed from the ROI height, hologram.cu:848-852)
+    int top = 0, bottom = 0, left = 0, right = 0, bw = width, bh = height;
+    if (ry < 0) { top = -ry; ry = 0; bh += top; }
+    if (ry + rh > bh) { bottom = ry + rh - bh; bh += bottom; }
+    if (rx < 0) { left = -rx; rx = 0; bw += left; }
+    if (rx + rw > bw) { right = rx + rh - bw; bw += right; }
+    if (rw <= 0 || rh <= 0 || rx + rw > bw || ry + rh > bh) return RIHVR_E_INVALID_ARGUMENT;
+    // zeroPad (hologram.cu:882-945)
+    const int pad = fp->solver.zero_padding;
+    if (pad < 0) return RIHVR_E_INVALID_ARGUMENT;
+    if (rw != rh) return RIHVR_E_INVALID_ARGUMENT;  // the reference's padded Mat has rows and cols swapped
+    g->src_x0 = rx - left;
+    g->src_y0 = ry - top;
+    g->rw = rw;
+    g->rh = rh;
+    g->pad = pad;
+    g->ow = rw + 2 * pad;
+    g->oh = rh + 2 * pad;
+    return RIHVR_OK;
+}
+
+}  // namespace rihvr
+
 extern "C" {
 
 rihvr_status rihvr_read_param_file(const char* path, rihvr_file_params* out) {
@@ -473,43 +518,17 @@ rihvr_status rihvr_prepare_hologram(const rihvr_file_params* fp, const float* im
         const float inv_mean = (float)(1.0 / (sum / (double)N));
         for (long i = 0; i < N; ++i) h[i] = h[i] * inv_mean - 1.0f;
     }
-    // crop (hologram.cu:768-880)
-    int rx, ry, rw, rh;
-    if (fp->roi_w * fp->roi_h > 0) {
-        rx = fp->roi_x; ry = fp->roi_y; rw = fp->roi_w; rh = fp->roi_h;
-    } else {
-        const int cx = (int)fp->center_x, cy = (int)fp->center_y, size = fp->roi_size;
-        int sx = cx - size / 2, sy = cy - size / 2;
-        rw = size; rh = size;
-        if (sx < 0) sx = 0;
-        if (sy < 0) sy = 0;
-        if (sx + size > width) sx = width - size;
-        if (sy + size > height) sy = height - size;
-        if (width < size) { sx = 0; rw = width; }
-        if (height < size) { sy = 0; rh = height; }
-        if (size < 1) { sx = 0; sy = 0; rw = width; rh = height; }
-        rx = sx; ry = sy;
-    }
-    // crop(Rect): zero borders where the ROI leaves the image (the right border is
-    // computed from the ROI height, hologram.cu:848-852)
-    int top = 0, bottom = 0, left = 0, right = 0, bw = width, bh = height;
-    if (ry < 0) { top = -ry; ry = 0; bh += top; }
-    if (ry + rh > bh) { bottom = ry + rh - bh; bh += bottom; }
-    if (rx < 0) { left = -rx; rx = 0; bw += left; }
-    if (rx + rw > bw) { right = rx + rh - bw; bw += right; }
-    if (rw <= 0 || rh <= 0 || rx + rw > bw || ry + rh > bh) return RIHVR_E_INVALID_ARGUMENT;
-    // zeroPad (hologram.cu:882-945)
-    const int pad = fp->solver.zero_padding;
-    if (pad < 0) return RIHVR_E_INVALID_ARGUMENT;
-    if (rw != rh) return RIHVR_E_INVALID_ARGUMENT;  // the reference's padded Mat has rows and cols swapped
-    const int ow = rw + 2 * pad, oh = rh + 2 * pad;
+    rihvr::CropGeometry g;
+    const rihvr_status gs = rihvr::crop_geometry(fp, width, height, &g);
+    if (gs) return gs;
+    const int ow = g.ow, oh = g.oh, pad = g.pad, rw = g.rw, rh = g.rh;
     *out_width = ow;
     *out_height = oh;
     if (!out) return RIHVR_OK;
     std::fill(out, out + 2 * (size_t)ow * oh, 0.0f);
     for (int y = 0; y < rh; ++y)
         for (int x = 0; x < rw; ++x) {
-            const int sy = ry + y - top, sx = rx + x - left;  // coordinates in the unbordered image
+            const int sy = g.src_y0 + y, sx = g.src_x0 + x;  // coordinates in the unbordered image
             float v = 0.0f;
             if (sy >= 0 && sy < height && sx >= 0 && sx < width) v = h[(size_t)sy * width + sx];
             out[2 * ((size_t)(y + pad) * ow + (x + pad))] = v;

@@ -1,0 +1,187 @@
+// preprocess.hip -- the reconstruction's input contract (main_sparse_inverse.cu:101-124)
+// on the device for a batch of frames: raw grey values (8-bit, 16-bit or float) ->
+// rescale /255 (hologram.cu:401-439) -> I/mean - 1 (normalize_mean0, :332-358) or
+// (I - B)/max(sqrt(B), 0.1) (removeBackground, :1365-1432) -> crop (:768-880) -> zero pad
+// (:882-945), written as complex64 holograms ready for the solver.  The arithmetic is
+// rihvr_prepare_hologram's (float products and quotients, the mean summed in double),
+// so each frame equals the host function's output.  HBM-bound: one pass over the frames
+// for the means, one pass writing the holograms.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/rihvr.h"
+#include "device_common.hpp"
+#include "io_internal.hpp"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;  // pixels per thread in the mean pass
+
+template <typename T>
+__device__ __forceinline__ float grey(const T* p, long i) {
+    return (float)p[i];
+}
+
+// per frame (blockIdx.y) and block: double sum of the rescaled pixels
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_pre_sum(const T* __restrict__ frames, long N, float inv255,
+                                                      double* __restrict__ partial) {
+    __shared__ double red[kThreads / 64];
+    const T* f = frames + (size_t)blockIdx.y * N;
+    double s = 0.0;
+    for (int it = 0; it < kItems; ++it) {
+        const long i = ((long)blockIdx.x * kItems + it) * kThreads + threadIdx.x;
+        if (i < N) s += (double)(grey(f, i) * inv255);
+    }
+    s = rihvr::block_sum(s, red);
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+// inv_mean[f] = float(1 / (sum / N)), the partial sums added in order
+__global__ void k_pre_mean(const double* __restrict__ partial, int blocks, int n_frames, long N,
+                           float* __restrict__ inv_mean) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_frames) return;
+    double s = 0.0;
+    for (int b = 0; b < blocks; ++b) s += partial[(size_t)f * blocks + b];
+    inv_mean[f] = (float)(1.0 / (s / (double)N));
+}
+
+struct Geometry {
+    int W, H;            // image
+    int src_x0, src_y0;  // source of the crop's first pixel
+    int rw, rh, pad, ow, oh;
+};
+
+// one thread per output pixel of every frame
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_pre_write(const T* __restrict__ frames,
+                                                        const float* __restrict__ background, Geometry g,
+                                                        int n_frames, float inv255,
+                                                        const float* __restrict__ inv_mean,
+                                                        float2* __restrict__ out) {
+#pragma clang fp contract(off)  // the host function's separate float roundings
+    const long per = (long)g.ow * g.oh;
+    const long total = per * n_frames;
+    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < total; i += (long)gridDim.x * kThreads) {
+        const int f = (int)(i / per);
+        const long r = i - (long)f * per;
+        const int y = (int)(r / g.ow);
+        const int x = (int)(r - (long)y * g.ow);
+        const int cx = x - g.pad, cy = y - g.pad;
+        float v = 0.0f;
+        if (cx >= 0 && cx < g.rw && cy >= 0 && cy < g.rh) {
+            const int sx = g.src_x0 + cx, sy = g.src_y0 + cy;
+            if (sx >= 0 && sx < g.W && sy >= 0 && sy < g.H) {
+                const long si = (long)sy * g.W + sx;
+                const float h = grey(frames + (size_t)f * g.W * g.H, si) * inv255;
+                if (background) {
+                    const float b = background[si] * inv255;
+                    const float sq = sqrtf(b);
+                    const float sb = sq < 0.1f ? 0.1f : sq;  // std::max(sqrt(b), 0.1f)
+                    v = (h - b) / sb;
+                } else {
+                    v = h * inv_mean[f] - 1.0f;
+                }
+            }
+        }
+        out[i] = make_float2(v, 0.0f);
+    }
+}
+
+#define PRE_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            std::fprintf(stderr, "rihvr: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                                \
+            return e_ == hipErrorOutOfMemory ? RIHVR_E_OUT_OF_MEMORY : RIHVR_E_DEVICE;              \
+        }                                                                                          \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+template <typename T>
+rihvr_status run(const T* frames, const float* bg, const Geometry& g, int n, float* out_dev) {
+    const float inv255 = (float)(1.0 / 255.0);  // cv::Mat / 255 scales by the float reciprocal
+    const long N = (long)g.W * g.H;
+    DevBuf hm, hp;
+    PRE_HIP(hipMalloc(&hm.p, (size_t)n * sizeof(float)));
+    if (!bg) {
+        const int blocks = (int)((N + (long)kThreads * kItems - 1) / ((long)kThreads * kItems));
+        PRE_HIP(hipMalloc(&hp.p, (size_t)n * blocks * sizeof(double)));
+        hipLaunchKernelGGL(k_pre_sum<T>, dim3(blocks, n), dim3(kThreads), 0, 0, frames, N, inv255, (double*)hp.p);
+        hipLaunchKernelGGL(k_pre_mean, dim3((n + 63) / 64), dim3(64), 0, 0, (const double*)hp.p, blocks, n, N,
+                           (float*)hm.p);
+        PRE_HIP(hipGetLastError());
+    }
+    const long total = (long)g.ow * g.oh * n;
+    const unsigned grid = (unsigned)std::max(1L, std::min((total + kThreads - 1) / kThreads, 256L * 1024L));
+    hipLaunchKernelGGL(k_pre_write<T>, dim3(grid), dim3(kThreads), 0, 0, frames, bg, g, n, inv255,
+                       (const float*)hm.p, (float2*)out_dev);
+    PRE_HIP(hipGetLastError());
+    PRE_HIP(hipDeviceSynchronize());
+    return RIHVR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rihvr_status rihvr_prepare_holograms(const rihvr_file_params* fp, const void* frames, int32_t pixel_type,
+                                     int32_t n_frames, const float* background, int32_t width, int32_t height,
+                                     int in_loc, float* out, int out_loc, int32_t* out_width, int32_t* out_height) {
+    if (!fp || !frames || width <= 0 || height <= 0 || n_frames < 1 || !out_width || !out_height)
+        return RIHVR_E_INVALID_ARGUMENT;
+    if (pixel_type != RIHVR_PIXEL_U8 && pixel_type != RIHVR_PIXEL_U16 && pixel_type != RIHVR_PIXEL_F32)
+        return RIHVR_E_UNKNOWN_TYPE;
+    if (fp->remove_background && !background) return RIHVR_E_MISSING_DATA;
+    rihvr::CropGeometry cg;
+    const rihvr_status gs = rihvr::crop_geometry(fp, width, height, &cg);
+    if (gs) return gs;
+    *out_width = cg.ow;
+    *out_height = cg.oh;
+    if (!out) return RIHVR_OK;
+    const Geometry g{width, height, cg.src_x0, cg.src_y0, cg.rw, cg.rh, cg.pad, cg.ow, cg.oh};
+    const size_t in_bytes = (size_t)n_frames * width * height * (size_t)pixel_type;
+    const size_t out_bytes = (size_t)n_frames * cg.ow * cg.oh * 8;
+    DevBuf hin, hbg, hout;
+    const void* din = frames;
+    const float* dbg = fp->remove_background ? background : nullptr;
+    if (in_loc != RIHVR_DEVICE) {
+        PRE_HIP(hipMalloc(&hin.p, in_bytes));
+        PRE_HIP(hipMemcpy(hin.p, frames, in_bytes, hipMemcpyHostToDevice));
+        din = hin.p;
+        if (dbg) {
+            PRE_HIP(hipMalloc(&hbg.p, (size_t)width * height * sizeof(float)));
+            PRE_HIP(hipMemcpy(hbg.p, dbg, (size_t)width * height * sizeof(float), hipMemcpyHostToDevice));
+            dbg = (const float*)hbg.p;
+        }
+    }
+    float* dout = out;
+    if (out_loc != RIHVR_DEVICE) {
+        PRE_HIP(hipMalloc(&hout.p, out_bytes));
+        dout = (float*)hout.p;
+    }
+    rihvr_status st;
+    if (pixel_type == RIHVR_PIXEL_U8)
+        st = run((const uint8_t*)din, dbg, g, n_frames, dout);
+    else if (pixel_type == RIHVR_PIXEL_U16)
+        st = run((const uint16_t*)din, dbg, g, n_frames, dout);
+    else
+        st = run((const float*)din, dbg, g, n_frames, dout);
+    if (st) return st;
+    if (out_loc != RIHVR_DEVICE) PRE_HIP(hipMemcpy(out, dout, out_bytes, hipMemcpyDeviceToHost));
+    return RIHVR_OK;
+}
+
+}  // extern "C"

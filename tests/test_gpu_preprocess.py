@@ -1,0 +1,97 @@
+"""GPU tests of rihvr_prepare_holograms (the input contract on the device for a batch of
+frames) against rihvr_prepare_hologram, the host function the CPU tests pin against a
+numpy restatement: every frame bit for bit, for 8-bit, 16-bit and float frames, mean
+normalisation and background removal, the centre/size and rectangle crops (inside the
+image and past its borders), zero padding, host and device buffers; and the errors."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _fp(pkg, pad=0, center=(-1, -1), size=-1, roi=(0, 0, 0, 0), remove_bg=False):
+    fp = pkg.FileParams()
+    fp.solver.zero_padding = pad
+    fp.center_x, fp.center_y = center
+    fp.roi_size = size
+    fp.roi_x, fp.roi_y, fp.roi_w, fp.roi_h = roi
+    fp.remove_background = 1 if remove_bg else 0
+    return fp
+
+
+def frames_of(dtype, n, h, w, seed):
+    rng = np.random.default_rng(seed)
+    if dtype == np.float32:
+        return (rng.random((n, h, w)) * 300).astype(np.float32)
+    hi = 256 if dtype == np.uint8 else 65536
+    return rng.integers(0, hi, size=(n, h, w)).astype(dtype)
+
+
+CASES = [
+    dict(),                                        # whole square image, no padding
+    dict(pad=16),
+    dict(center=(20, 30), size=32, pad=4),
+    dict(center=(2, 3), size=24),
+    dict(roi=(-5, -3, 40, 40), pad=8),             # past the top-left corner: zero fill
+    dict(roi=(50, 40, 32, 32)),                    # past the bottom-right corner
+]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("remove_bg", [False, True])
+def test_device_equals_host(pkg, gpu_available, dtype, case, remove_bg):
+    kw = CASES[case]
+    fp = _fp(pkg, remove_bg=remove_bg, **kw)
+    fr = frames_of(dtype, 3, 64, 64, seed=case)
+    bg = (np.random.default_rng(99).random((64, 64)) * 200).astype(np.float32) if remove_bg else None
+    if bg is not None:
+        bg[0, :5] = 0.0  # sqrt(B) below 0.1 takes the floor
+    got = pkg.prepare_holograms(fp, fr, bg)
+    for k in range(3):
+        exp = pkg.prepare_hologram(fp, fr[k].astype(np.float32), bg)
+        np.testing.assert_array_equal(got[k].view(np.uint32), exp.view(np.uint32))
+
+
+def test_device_buffers_and_batch_solve(pkg, gpu_available):
+    import torch
+    fp = _fp(pkg, pad=32)
+    fr = frames_of(np.uint8, 4, 64, 64, seed=7)
+    dev = torch.from_numpy(fr).cuda()
+    out = torch.empty((4, 128, 128), dtype=torch.complex64, device="cuda")
+    pkg.prepare_holograms(fp, dev, out=out)
+    torch.cuda.synchronize()
+    host = pkg.prepare_holograms(fp, fr)
+    np.testing.assert_array_equal(out.cpu().numpy(), host)
+    # the prepared batch feeds the solver directly from the device
+    p = pkg.make_params(128, 128, 4, 1000.0, 50.0, 0.633, 2.0, 3, 0.01, 0.01, 2)
+    with pkg.SparseCompressiveHolo(p) as rec:
+        v_dev = np.zeros((4, 4, 128, 128), np.complex64)
+        rec.reconstructBatch(out, volumes=v_dev)
+        v_host = np.zeros_like(v_dev)
+        rec.reconstructBatch(host, volumes=v_host)
+    np.testing.assert_array_equal(v_dev, v_host)
+
+
+def test_background_on_device(pkg, gpu_available):
+    import torch
+    fp = _fp(pkg, remove_bg=True, size=48, center=(30, 30))
+    fr = frames_of(np.uint16, 2, 64, 64, seed=3)
+    bg = (np.random.default_rng(4).random((64, 64)) * 5000).astype(np.float32)
+    got = pkg.prepare_holograms(fp, torch.from_numpy(fr).cuda(), torch.from_numpy(bg).cuda())
+    exp = pkg.prepare_holograms(fp, fr, bg)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_errors(pkg, gpu_available):
+    fr = frames_of(np.uint8, 1, 32, 32, seed=1)
+    with pytest.raises(pkg.RihvrError) as e:
+        pkg.prepare_holograms(_fp(pkg, remove_bg=True), fr)
+    assert e.value.status == 3                       # background missing
+    with pytest.raises(pkg.RihvrError) as e:
+        pkg.prepare_holograms(_fp(pkg, roi=(0, 0, 16, 8)), fr)
+    assert e.value.status == 7                       # non-square crop
+    with pytest.raises(ValueError):
+        pkg.prepare_holograms(_fp(pkg), fr.astype(np.int64))
