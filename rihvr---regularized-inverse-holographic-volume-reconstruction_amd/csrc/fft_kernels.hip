@@ -49,6 +49,8 @@ struct RowArgs {
     // SUMPART
     int nparts;
     long part_stride;
+    // FWDACC: 16-column strips present in the input planes ([nz][L/16]; 0 = read as zeros)
+    const uint8_t* smap;
 };
 
 template <int L, int SIGN, int MODE>
@@ -104,11 +106,22 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
         }
         const int z0 = blockIdx.y * a.zchunk;
         const int z1 = min(a.nz, z0 + a.zchunk);
+        constexpr int NS = L / 16;  // strips per row (<= 64)
+        const int lane = threadIdx.x & 63;
         for (int z = z0; z < z1; ++z) {
+            // the plane's strips as a wave-uniform mask (every wave loads the same bytes); a
+            // plane without any strip adds zeros to the accumulators (never -0: they start
+            // at +0), so it is skipped; absent strips are read as zeros
+            const uint8_t* sm = a.smap + (size_t)z * NS;
+            const uint64_t smask = __ballot(lane < NS && sm[lane < NS ? lane : 0] != 0);
+            if (!smask) continue;  // the same for every wave: no barrier is skipped by part
             const double mz = -(double)a.zlist[z];
             const float2* src_row = a.in + ((size_t)z * a.nrows + rrow) * L;
             FftRun<L, SIGN, typename Plan::radices>::run(
-                tw_s, t, [&](int, int, int j) { return src_row[j]; },
+                tw_s, t,
+                [&](int, int, int j) {
+                    return ((smask >> (j >> 4)) & 1ull) ? src_row[j] : make_float2(0.0f, 0.0f);
+                },
                 [&](int i, int r, int, float2 v) {
                     float2 h = transfer(mz, kp[i][r]);
                     acc[i][r] = cadd(acc[i][r], cscale(cmul(h, v), a.invN));
@@ -179,6 +192,9 @@ struct ColArgs {
     const float2* meas;
     float2* resid;
     double* partial;        // one double per workgroup: sum |r|^2
+    // FWD: line occupancy of the input ([nz][L] words, may be NULL) and the strip map written
+    const uint64_t* occ;
+    uint8_t* smap;
 };
 
 template <int L, int SIGN, int MODE>
@@ -211,8 +227,28 @@ __global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
             },
             lds);
     } else if constexpr (MODE == COL_FWD) {
-        // A, first pass: x_z e^{-i phi_z} (sparse_phase_scale_kernel with -z), forward column FFT
+        // A, first pass: x_z e^{-i phi_z} (sparse_phase_scale_kernel with -z), forward column FFT.
+        // The workgroup's 16 columns are line blockIdx.x of every row: rows whose line is
+        // empty are not read, and a strip empty in every row transforms to zeros, which are
+        // not written (the strip map tells the row pass).
         const int z = blockIdx.y;
+        const int line = blockIdx.x;
+        const uint64_t* occ = a.occ ? a.occ + (size_t)z * L : nullptr;
+        // the strip's row bits in LDS (from the map, or all set without one)
+        __shared__ uint32_t rows_s[(L + 31) / 32];
+        for (int k = threadIdx.x; k < (L + 31) / 32; k += C::THREADS) rows_s[k] = occ ? 0u : ~0u;
+        __syncthreads();
+        bool mine = false;
+        if (occ) {
+            for (int j = threadIdx.x; j < L; j += C::THREADS)
+                if ((occ[j] >> line) & 1ull) {
+                    atomicOr(&rows_s[j >> 5], 1u << (j & 31));
+                    mine = true;
+                }
+        }
+        const bool any = occ ? __syncthreads_or(mine) : true;
+        if (threadIdx.x == 0) a.smap[(size_t)z * (W / kColStrip) + line] = any ? 1 : 0;
+        if (!any) return;  // uniform: the whole workgroup leaves
         const float2 ph = a.phase[z];
         const float px = ph.x, py = -ph.y;
         const float2* src = a.in + (size_t)z * L * W + col;
@@ -220,6 +256,7 @@ __global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
         FftRun<L, SIGN, typename Plan::radices>::run(
             tw_s, t,
             [&](int, int, int j) {
+                if (!((rows_s[j >> 5] >> (j & 31)) & 1u)) return make_float2(0.0f, 0.0f);
                 float2 d = src[(size_t)j * W];
                 return make_float2(d.x * px - d.y * py, d.y * px + d.x * py);
             },
@@ -310,10 +347,11 @@ hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_s
 }
 
 hipError_t fft_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts,
-                          int H, int nz, int zchunk, float invN, const float2* tw_fwd, hipStream_t s) {
+                          int H, int nz, int zchunk, float invN, const float2* tw_fwd, const uint8_t* smap,
+                          hipStream_t s) {
     RowArgs a{};
     a.in = in; a.out = parts; a.nrows = H; a.twg = tw_fwd; a.kappa = kappa; a.zlist = zlist; a.nz = nz;
-    a.zchunk = zchunk; a.invN = invN; a.part_stride = (long)H * L;
+    a.zchunk = zchunk; a.invN = invN; a.part_stride = (long)H * L; a.smap = smap;
     const int gy = (nz + zchunk - 1) / zchunk;
     switch (L) {
 #define X(n) case n: return launch_rows_t<n, -1, ROW_FWDACC>(a, H, gy, s);
@@ -363,9 +401,9 @@ hipError_t fft_cols_adj(int L, const float2* in, const float2* phase, float2* ou
 }
 
 hipError_t fft_cols_fwd(int L, const float2* in, const float2* phase, float2* out, int W, int nz,
-                        const float2* tw_fwd, hipStream_t s) {
+                        const float2* tw_fwd, const uint64_t* occ, uint8_t* smap, hipStream_t s) {
     ColArgs a{};
-    a.in = in; a.out = out; a.W = W; a.twg = tw_fwd; a.phase = phase;
+    a.in = in; a.out = out; a.W = W; a.twg = tw_fwd; a.phase = phase; a.occ = occ; a.smap = smap;
     switch (L) {
 #define X(n) case n: return launch_cols_t<n, -1, COL_FWD>(a, nz, s);
         RIHVR_FOR_EACH_L(X)

@@ -1,8 +1,14 @@
 // kernels.hpp -- host-side launchers of the gfx950 kernels (one stream each).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace rihvr {
+
+// Line occupancy of a prox-output volume: one 64-bit word per (plane, row), bit l set when
+// the row's voxels 16l .. 16l+15 (one 128-byte line; W <= 1024) hold a non-zero value.  A
+// volume with a valid map holds exact zeros in every line whose bit is clear, so readers
+// may skip those lines and writers may skip zero lines that were already zero.
 
 // ---- FFT (fft_kernels.hip) -------------------------------------------------
 bool fft_length_supported(int L);
@@ -15,12 +21,17 @@ hipError_t fft_rows_plain(int L, int sign, const float2* in, float2* out, int nr
 hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out,
                              int nrows, const float2* tw_inv, hipStream_t s);
 // A, first pass: forward column FFT (length L = H) of vol[z] * e^{-i phi_z}, nz planes of width W.
+// occ (may be NULL): vol's line occupancy, [nz][L] words; lines with a clear bit are not
+// read, and a 16-column strip with no set bit in any row is neither transformed nor
+// written.  smap[z][W/16] (always written) = 1 for the strips written to out.
 hipError_t fft_cols_fwd(int L, const float2* vol, const float2* phase, float2* out, int W, int nz,
-                        const float2* tw_fwd, hipStream_t s);
+                        const float2* tw_fwd, const uint64_t* occ, uint8_t* smap, hipStream_t s);
 // A, second pass: forward row FFT (length L = W), epilogue H_{-z} / N summed over zchunk
-// planes into parts[chunk] ([H][W] each).
+// planes into parts[chunk] ([H][W] each); strips with smap 0 are read as zeros, planes
+// without any strip are skipped.
 hipError_t fft_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts,
-                          int H, int nz, int zchunk, float invN, const float2* tw_fwd, hipStream_t s);
+                          int H, int nz, int zchunk, float invN, const float2* tw_fwd, const uint8_t* smap,
+                          hipStream_t s);
 // A^T, first pass: inverse row FFT of H_z * rhat / N, planes processed zchunk per workgroup.
 hipError_t fft_rows_adj(int L, const float2* rhat, const double* kappa, const float* zlist, float2* out,
                         int H, int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s);
@@ -61,6 +72,14 @@ struct ProxLaunch {
     int n_tv;
     int row_chunk;      // output rows per wave of the mode-7 prox (<= 0: default)
     double* partial;    // [nwaves][PS_COUNT]
+    // mode 7 only, all optional ([nz][H] words, see the line occupancy above): the maps of
+    // Y0 and YM (their clear lines are not read), of Y's current content (its zero lines
+    // with a clear bit are not rewritten; NULL: write every line) and the map of the new
+    // Y (OR-ed in; must start cleared).  The output's zeros are +0.
+    const uint64_t* occ_y0 = nullptr;
+    const uint64_t* occ_ym = nullptr;
+    const uint64_t* occ_old = nullptr;
+    uint64_t* occ_new = nullptr;
 };
 // Number of per-wave partial rows the mode-7 prox writes for this geometry.
 // row_chunk: output rows per wave (<= 0: default 128)

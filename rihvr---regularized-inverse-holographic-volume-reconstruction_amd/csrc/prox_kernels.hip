@@ -70,6 +70,11 @@ struct ProxDev {
     ProxGeom geo;
     int nwaves;
     double* partial;
+    // line occupancy ([nz][H] words; NULL: none) of Y0, YM, Y's current content and new Y
+    const uint64_t* occ_y0;
+    const uint64_t* occ_ym;
+    const uint64_t* occ_old;
+    uint64_t* occ_new;
 };
 
 __device__ __forceinline__ float soft1(float v, float thr) {
@@ -114,21 +119,47 @@ template <int NTV> struct ProxWave {
     float2 pf[2][3];
     // x-hat, x0, g and y_{k-1} of the last P rows, indexed by compile-time slots: registers
     float2 hist[4][P];
+    // occupancy words (wave-uniform, kept in scalar registers), [row parity]: of y_{k-1} and
+    // y_{k-2} for the rows loaded next, and of the output's current content for the rows
+    // written next
+    uint64_t ow0[2] = {~0ull, ~0ull}, owm[2] = {~0ull, ~0ull}, oold[2] = {~0ull, ~0ull};
     float acc_ls1 = 0.f, acc_ls2 = 0.f, acc_rs = 0.f, acc_l1 = 0.f, acc_tv = 0.f, acc_nnz = 0.f;
 
+    static __device__ __forceinline__ int wrap_row(int tau, int H) {
+        const int row = tau % H;
+        return row < 0 ? row + H : row;
+    }
+    // A wave-uniform word of a map this kernel does not write: a scalar load through the
+    // constant address space (its own counter, so it never waits on the vector loads in
+    // flight for the rows ahead).
+    static __device__ __forceinline__ uint64_t uniform_word(const uint64_t* map, int i) {
+        if (!map) return ~0ull;
+        const __attribute__((address_space(4))) uint64_t* cmap =
+            (const __attribute__((address_space(4))) uint64_t*)map;
+        return cmap[i];
+    }
+    // occupancy words of y_{k-1} and y_{k-2} for row tau, one step before its loads
+    __device__ __forceinline__ void load_words(const ProxDev& a, int slot, int tau, int zrow) {
+        const int row = wrap_row(tau, a.H);
+        ow0[slot] = uniform_word(a.occ_y0, zrow + row);
+        owm[slot] = uniform_word(a.occ_ym, zrow + row);
+    }
+    // lines of y_{k-1} and y_{k-2} that are empty in their maps are zeros: not read
     __device__ __forceinline__ void load_row(const ProxDev& a, int slot, int tau, size_t plane_off, int xcol) {
-        int row = tau % a.H;
-        row = row < 0 ? row + a.H : row;
+        const int row = wrap_row(tau, a.H);
         const size_t ro = plane_off + (size_t)row * a.W + xcol;
-        pf[slot][0] = a.Y0[ro];
-        pf[slot][1] = a.YM[ro];
+        const int line = xcol >> 4;
+        const float2 zero = make_float2(0.0f, 0.0f);
+        pf[slot][0] = ((ow0[slot] >> line) & 1ull) ? a.Y0[ro] : zero;
+        pf[slot][1] = ((owm[slot] >> line) & 1ull) ? a.YM[ro] : zero;
         pf[slot][2] = a.G[ro];
     }
 
     // Step S of an unrolled block: row tau, ring slot S mod P, row parity S & 1.
     template <int S>
     __device__ __forceinline__ void step(const ProxDev& a, int tau, int r0, int r1,
-                                         size_t plane_off, int xcol, int ocol, bool out_lane, int tau_end) {
+                                         size_t plane_off, int xcol, int ocol, bool out_lane, int tau_end,
+                                         int zrow, int strip) {
         constexpr int SLOT = S % P;
         constexpr int PAR = S & 1;
         constexpr int Q = PAR ^ 1;
@@ -141,7 +172,10 @@ template <int NTV> struct ProxWave {
             hist[1][SLOT] = x0;
             hist[2][SLOT] = g;
             hist[3][SLOT] = y0;
+            // the output's current map for the row written in the next step
+            if (a.occ_old) oold[Q] = uniform_word(a.occ_old, zrow + wrap_row(tau - NTV, a.H));
             if (tau + 2 <= tau_end) load_row(a, PAR, tau + 2, plane_off, xcol);
+            if (tau + 3 <= tau_end) load_words(a, Q, tau + 3, zrow);
         }
         // ---- TV levels: level k works on row tau - k ----
 #pragma unroll
@@ -187,15 +221,45 @@ template <int NTV> struct ProxWave {
             yv = make_float2(fmaf(-a.mu, dvr, xh.x), fmaf(-a.mu, dvi, xh.y));
         }
         if (a.do_soft) yv = make_float2(soft_fast(yv.x, a.thr), soft_fast(yv.y, a.thr));
+        // zeros are stored as +0 (the soft threshold's v * 0 is -0 for negative v), so a
+        // line of zeros is a line of +0 whether it is written or skipped
+        yv = make_float2(yv.x + 0.0f, yv.y + 0.0f);
         const float2 x0o = hist[1][SO];
         const float x0lr = lane_from_left(x0o.x);
         const float x0li = lane_from_left(x0o.y);
-        if (o >= r0 && o < r1 && out_lane) {
-            float2* __restrict__ py = a.Y + plane_off + (size_t)o * W;
-            py[ocol] = yv;
-            const float2 x0u = hist[1][SU];
+        if (o >= r0 && o < r1 && a.occ_new) {  // row o: the new map and the skipped writes
+            const bool nz = out_lane && (yv.x != 0.0f || yv.y != 0.0f);
+            const uint64_t m = __ballot(nz);
+            if (m) {
+                // lanes NTV .. NTV + ow - 1 hold columns strip*ow + (lane - NTV): the lines
+                // touched by non-zero lanes, then one atomic OR per row and wave
+                const int c0 = strip * a.geo.ow - NTV;  // column of lane 0
+                uint64_t bits = 0;
+                const int l_last = min(c0 + NTV + a.geo.ow - 1, W - 1) >> 4;
+                for (int l = (c0 + NTV) >> 4; l <= l_last; ++l) {
+                    const int lo = max(16 * l - c0, NTV), hi = min(16 * l + 15 - c0, NTV + a.geo.ow - 1);
+                    if (lo > hi || lo > 63) continue;
+                    const uint64_t lanes = (hi >= 63 ? ~0ull : ((1ull << (hi + 1)) - 1ull)) & ~((1ull << lo) - 1ull);
+                    if (m & lanes) bits |= 1ull << l;
+                }
+                if ((threadIdx.x & 63) == 0) atomicOr((unsigned long long*)&a.occ_new[zrow + o], bits);
+            }
+            const bool keep = nz || ((oold[PAR] >> (ocol >> 4)) & 1ull);
+            if (out_lane && keep) a.Y[plane_off + (size_t)o * W + ocol] = yv;
+        }
+        if (o >= r0 && o < r1 && out_lane && !a.occ_new) a.Y[plane_off + (size_t)o * W + ocol] = yv;
+        // Every reduction term of row o is a product with (y - x0), (x0 - y), x0 or one of
+        // the x0 differences, so it is zero (adding it changes no sum, which starts at +0)
+        // when y, x0, y_{k-1} and the row above's x0 are zero in every lane of the wave,
+        // halo lanes included (they are the left neighbours).  Late in a solve almost every
+        // row is.
+        const float2 x0u = hist[1][SU];
+        const float2 y0 = hist[3][SO];
+        const uint32_t bitsor = (__float_as_uint(x0o.x) | __float_as_uint(x0o.y) | __float_as_uint(x0u.x) |
+                                 __float_as_uint(x0u.y) | __float_as_uint(yv.x) | __float_as_uint(yv.y) |
+                                 __float_as_uint(y0.x) | __float_as_uint(y0.y)) << 1;  // sign bits dropped
+        if (o >= r0 && o < r1 && __ballot(bitsor != 0u) && out_lane) {
             const float2 g = hist[2][SO];
-            const float2 y0 = hist[3][SO];
             const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
             acc_ls1 = fmaf(dr, g.x, fmaf(di, g.y, acc_ls1));
             acc_ls2 = fmaf(dr, dr, fmaf(di, di, acc_ls2));
@@ -216,14 +280,15 @@ template <int NTV> struct ProxWave {
     template <int... S>
     __device__ __forceinline__ void block(std::integer_sequence<int, S...>, const ProxDev& a,
                                           int tau, int r0, int r1, size_t plane_off, int xcol, int ocol,
-                                          bool out_lane, int tau_end) {
-        (step<S>(a, tau + S, r0, r1, plane_off, xcol, ocol, out_lane, tau_end), ...);
+                                          bool out_lane, int tau_end, int zrow, int strip) {
+        (step<S>(a, tau + S, r0, r1, plane_off, xcol, ocol, out_lane, tau_end, zrow, strip), ...);
     }
     template <int... S>
     __device__ __forceinline__ void tail(std::integer_sequence<int, S...>, int rem, const ProxDev& a,
                                          int tau, int r0, int r1, size_t plane_off,
-                                         int xcol, int ocol, bool out_lane, int tau_end) {
-        ((S < rem ? step<S>(a, tau + S, r0, r1, plane_off, xcol, ocol, out_lane, tau_end) : void()), ...);
+                                         int xcol, int ocol, bool out_lane, int tau_end, int zrow, int strip) {
+        ((S < rem ? step<S>(a, tau + S, r0, r1, plane_off, xcol, ocol, out_lane, tau_end, zrow, strip) : void()),
+         ...);
     }
 };
 
@@ -254,6 +319,7 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     const int r0 = chunk * geo.rc;
     const int r1 = min(a.H, r0 + geo.rc);
     const size_t plane_off = (size_t)z * a.H * W;
+    const int zrow = z * a.H;  // first row of plane z in the occupancy maps
 
     PW st;
 #pragma unroll
@@ -266,15 +332,19 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     const int tau0 = r0 - (NTV > 0 ? NTV : 1);
     const int tau1 = r1 + NTV;  // inclusive
     // the first two rows; every step then loads the row two ahead (U is even)
+    st.load_words(a, 0, tau0, zrow);
+    st.load_words(a, 1, tau0 + 1, zrow);
     st.load_row(a, 0, tau0, plane_off, xcol);
     st.load_row(a, 1, tau0 + 1, plane_off, xcol);
+    st.load_words(a, 0, tau0 + 2, zrow);
+    st.oold[0] = PW::uniform_word(a.occ_old, zrow + PW::wrap_row(tau0 - NTV - 1, a.H));
     int tau = tau0;
     for (; tau + U - 1 <= tau1; tau += U)
         st.block(std::make_integer_sequence<int, U>{}, a, tau, r0, r1, plane_off, xcol, ocol, out_lane,
-                 tau1);
+                 tau1, zrow, strip);
     if (tau <= tau1)
         st.tail(std::make_integer_sequence<int, U - 1>{}, tau1 - tau + 1, a, tau, r0, r1, plane_off, xcol,
-                ocol, out_lane, tau1);
+                ocol, out_lane, tau1, zrow, strip);
 
     const double s_ls1 = wave_sum((double)st.acc_ls1);
     const double s_ls2 = wave_sum((double)st.acc_ls2);
@@ -308,6 +378,10 @@ static ProxDev make_dev(const ProxLaunch& a) {
     d.geo = prox_geom(a.W, a.H, a.n_tv, a.row_chunk);
     d.nwaves = d.geo.nstrips * d.geo.nchunks * a.nz;
     d.partial = a.partial;
+    d.occ_y0 = a.occ_y0;
+    d.occ_ym = a.occ_ym;
+    d.occ_old = a.occ_old;
+    d.occ_new = a.occ_new;
     return d;
 }
 

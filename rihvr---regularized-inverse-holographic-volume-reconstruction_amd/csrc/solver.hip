@@ -91,6 +91,12 @@ struct rihvr_ctx {
     // volumes
     float2* Yb[3] = {nullptr, nullptr, nullptr};  // ring of prox outputs
     int iy = 0, iy0 = 1, iym = 2;                 // y_k, y_{k-1}, y_{k-2}
+    // line occupancy of each ring volume ([Nz][H] words, kernels.hpp); a volume written by
+    // anything but the mode-7 prox has no valid map (every line is then read)
+    uint64_t* occ[3] = {nullptr, nullptr, nullptr};
+    bool occ_valid[3] = {false, false, false};
+    uint64_t* occ_tmp = nullptr;                  // the map being built by a prox
+    uint8_t* smap = nullptr;                      // [Nz][W/16] strips of A's column pass
     float2* G = nullptr;                          // A^T r
     float2* T = nullptr;                          // FFT scratch volume
     float2* out = nullptr;                        // points at the returned volume
@@ -224,18 +230,39 @@ rihvr_status fetch_scalars(rihvr_ctx* c) {
     return resolve_timing(c);
 }
 
+// ring index of a volume pointer, or -1
+int ring_index(const rihvr_ctx* c, const float2* v) {
+    for (int k = 0; k < 3; ++k)
+        if (c->Yb[k] == v) return k;
+    return -1;
+}
+// the valid line-occupancy map of a ring volume, else NULL
+const uint64_t* occ_of(const rihvr_ctx* c, const float2* v) {
+    const int k = ring_index(c, v);
+    return (k >= 0 && c->occ_valid[k]) ? c->occ[k] : nullptr;
+}
+void invalidate_occ(rihvr_ctx* c, const float2* v) {
+    const int k = ring_index(c, v);
+    if (k >= 0) c->occ_valid[k] = false;
+}
+
 // A for planes [g0, g0 + gz) on stream st: the column pass of vol * e^{-i phi} into the
 // scratch planes T[g0..], then the row pass with the H_{-z}/N z-sum into the partial
 // spectra of those planes' z chunks (backPropagate :1935-2077)
-rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipStream_t st) {
+// occ: vol's line occupancy map (NULL: read every line); the strips the column pass
+// writes are recorded in smap for the row pass
+rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipStream_t st, const uint64_t* occ) {
     const float invN = (float)(1.0 / (double)c->N);
     float2* Tg = c->T + (size_t)g0 * c->N;
     float2* spg = c->spart + (size_t)(g0 / c->zchunk_fwd) * c->N;
+    uint8_t* smg = c->smap + (size_t)g0 * (c->W / 16);
+    const uint64_t* og = occ ? occ + (size_t)g0 * c->H : nullptr;
     RIHVR_TRY(timed_on(c, K_COLS_FWD, st, [&] {
-        return fft_cols_fwd(c->H, vol + (size_t)g0 * c->N, c->phase + g0, Tg, c->W, gz, c->twfH, st);
+        return fft_cols_fwd(c->H, vol + (size_t)g0 * c->N, c->phase + g0, Tg, c->W, gz, c->twfH, og, smg, st);
     }));
     RIHVR_TRY(timed_on(c, K_ROWS_FWDACC, st, [&] {
-        return fft_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, invN, c->twfW, st);
+        return fft_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, invN, c->twfW, smg,
+                               st);
     }));
     return RIHVR_OK;
 }
@@ -251,8 +278,9 @@ rihvr_status forward_finish(rihvr_ctx* c, float2* est) {
 }
 
 rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
+    const uint64_t* occ = occ_of(c, vol);
     for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
-        RIHVR_TRY(forward_group(c, vol, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
+        RIHVR_TRY(forward_group(c, vol, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream, occ));
     return forward_finish(c, est);
 }
 
@@ -309,9 +337,22 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
     a.row_chunk = c->prox_rows;
     a.partial = c->partial;
     int rows;
+    const int ky = ring_index(c, Y);
     if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) {
+        // the new map is built in occ_tmp, then becomes Y's
+        a.occ_y0 = occ_of(c, Y0);
+        a.occ_ym = occ_of(c, YM);
+        if (ky >= 0) {
+            a.occ_old = c->occ_valid[ky] ? c->occ[ky] : nullptr;
+            a.occ_new = c->occ_tmp;
+            RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
+        }
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
         rows = prox_fl2d_num_waves(c->W, c->H, c->Nz, a.n_tv, c->prox_rows);
+        if (ky >= 0) {
+            std::swap(c->occ[ky], c->occ_tmp);
+            c->occ_valid[ky] = true;
+        }
     } else if (mode == RIHVR_MODE_FASTA_FUSED_LASSO) {
         // the FFT scratch volume is free between A^T and A
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl3d(a, c->T, c->stream); }));
@@ -320,6 +361,7 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_l1(a, c->stream); }));
         rows = prox_l1_num_blocks(c->W, c->H, c->Nz);
     }
+    if (mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D && ky >= 0) c->occ_valid[ky] = false;
     RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial, rows, PS_COUNT, c->red_work, c->scal + SL_PROX, c->stream); }));
     return RIHVR_OK;
 }
@@ -343,15 +385,29 @@ rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, fl
     a.n_tv = c->p.num_tv_iterations;
     a.row_chunk = c->prox_rows;
     const int per_plane = prox_fl2d_num_waves(c->W, c->H, 1, a.n_tv, c->prox_rows);
+    const uint64_t* o0 = occ_of(c, Y0);
+    const uint64_t* om = occ_of(c, YM);
+    const int ky = ring_index(c, Yn);
+    const uint64_t* oold = (ky >= 0 && c->occ_valid[ky]) ? c->occ[ky] : nullptr;
+    if (ky >= 0) RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
     for (int g = 0, g0 = 0; g < c->n_groups; ++g, g0 += c->pipe_planes) {
         const int gz = std::min(c->pipe_planes, c->Nz - g0);
         const size_t off = (size_t)g0 * c->N;
+        const size_t roff = (size_t)g0 * c->H;  // first map word of the group
         RIHVR_HIP(hipStreamWaitEvent(c->stream, c->ev_adj[g], 0));
         a.Y0 = Y0 + off; a.YM = YM + off; a.G = c->G + off; a.Y = Yn + off;
         a.nz = gz;
         a.partial = c->partial + (size_t)g0 * per_plane * PS_COUNT;
+        a.occ_y0 = o0 ? o0 + roff : nullptr;
+        a.occ_ym = om ? om + roff : nullptr;
+        a.occ_old = oold ? oold + roff : nullptr;
+        a.occ_new = ky >= 0 ? c->occ_tmp + roff : nullptr;
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
-        RIHVR_TRY(forward_group(c, Yn, g0, gz, c->stream));
+        RIHVR_TRY(forward_group(c, Yn, g0, gz, c->stream, ky >= 0 ? c->occ_tmp : nullptr));
+    }
+    if (ky >= 0) {
+        std::swap(c->occ[ky], c->occ_tmp);
+        c->occ_valid[ky] = true;
     }
     RIHVR_TRY(timed(c, K_REDUCE, [&] {
         return reduce_partials(c->partial, per_plane * c->Nz, PS_COUNT, c->red_work, c->scal + SL_PROX, c->stream);
@@ -371,6 +427,9 @@ double objective(const rihvr_ctx* c, int mode, double summag, double l1, double 
 void release(rihvr_ctx* c) {
     if (!c) return;
     for (auto& b : c->Yb) (void)hipFree(b);
+    for (auto& m : c->occ) (void)hipFree(m);
+    (void)hipFree(c->occ_tmp);
+    (void)hipFree(c->smap);
     void* ptrs[] = {c->G, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->spart, c->kappa,
                     c->zlist, c->phase, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
                     c->red_work, c->dflag};
@@ -420,6 +479,9 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     c->ev_adj.assign(c->n_groups, nullptr);
     for (hipEvent_t& e : c->ev_adj) RIHVR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& b : c->Yb) RIHVR_TRY(dalloc(&b, c->V));
+    for (auto& m : c->occ) RIHVR_TRY(dalloc(&m, (size_t)c->Nz * c->H));
+    RIHVR_TRY(dalloc(&c->occ_tmp, (size_t)c->Nz * c->H));
+    RIHVR_TRY(dalloc(&c->smap, (size_t)c->Nz * (c->W / 16)));
     RIHVR_TRY(dalloc(&c->G, c->V));
     RIHVR_TRY(dalloc(&c->T, c->V));
     float2** holos[] = {&c->meas, &c->hE, &c->hC, &c->hP, &c->hX, &c->resid, &c->rhat, &c->htmp};
@@ -518,6 +580,11 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     if (warm && in_ring < 0) RIHVR_TRY(copy_in(c, Y0, warm, warm_loc, c->V));
     else if (!warm) RIHVR_HIP(hipMemsetAsync(Y0, 0, c->V * sizeof(float2), c->stream));
     RIHVR_HIP(hipMemcpyAsync(YM, Y0, c->V * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
+    // a zero start has the empty map; a warm start none (every line is read once)
+    for (int k : {c->iy0, c->iym}) {
+        c->occ_valid[k] = !warm;
+        if (!warm) RIHVR_HIP(hipMemsetAsync(c->occ[k], 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
+    }
     RIHVR_HIP(hipMemsetAsync(c->hC, 0, c->N * sizeof(float2), c->stream));
     RIHVR_HIP(hipMemsetAsync(c->hP, 0, c->N * sizeof(float2), c->stream));
 
@@ -687,6 +754,7 @@ rihvr_status finish_impl(rihvr_ctx* c, rihvr_stats* st, int last_index) {
     // returned volume: x1 = y_K + s (y_K - y_{K-1})  (:483-489); with no
     // iteration s = 0 and the start point is returned
     float2* outb = c->Yb[c->iy];
+    c->occ_valid[c->iy] = false;
     RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
         return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], s.s_mom, outb, c->W, c->H, c->Nz, tv3d(c), c->partial,
                             c->stream);
@@ -722,6 +790,7 @@ rihvr_status finish_impl(rihvr_ctx* c, rihvr_stats* st, int last_index) {
 rihvr_status snapshot_impl(rihvr_ctx* c) {
     if (!c->ss.active) return RIHVR_E_INVALID_STATE;
     float2* outb = c->Yb[c->iy];
+    c->occ_valid[c->iy] = false;
     RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
         return momentum_out(c->Yb[c->iy0], c->Yb[c->iym], c->ss.s_mom, outb, c->W, c->H, c->Nz, tv3d(c), c->partial,
                             c->stream);
@@ -945,6 +1014,7 @@ rihvr_status rihvr_forward(rihvr_ctx* ctx, const float* volume, int vol_loc, flo
     ctx->have_result = false;
     ctx->ss.active = false;
     float2* v = ctx->Yb[0];
+    ctx->occ_valid[0] = false;
     RIHVR_TRY(copy_in(ctx, v, volume, vol_loc, ctx->V));
     RIHVR_HIP(hipMemsetAsync(ctx->meas, 0, ctx->N * sizeof(float2), ctx->stream));
     RIHVR_TRY(apply_forward(ctx, v, ctx->hE));
@@ -964,6 +1034,7 @@ rihvr_status rihvr_prox(rihvr_ctx* ctx, const float* x0, const float* grad, doub
     if (!ctx || !x0 || !grad || !out) return RIHVR_E_INVALID_ARGUMENT;
     ctx->have_result = false;
     ctx->ss.active = false;
+    ctx->occ_valid[1] = false;
     RIHVR_TRY(copy_in(ctx, ctx->Yb[1], x0, loc, ctx->V));
     RIHVR_TRY(copy_in(ctx, ctx->G, grad, loc, ctx->V));
     RIHVR_TRY(apply_prox(ctx, effective_mode(ctx), ctx->Yb[1], ctx->Yb[1], 0.0f, ctx->G, stepsize, ctx->Yb[0]));
