@@ -35,14 +35,16 @@ HBM_PEAK_GBPS = 8000.0          # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3        # MI355X FP32 vector (vendor figure)
 
 
-def algorithmic_bytes(name: str, W: int, H: int, Nz: int, nparts: int) -> float:
-    """Compulsory HBM bytes of one launch of each solver kernel (complex64 = 8 B)."""
+def algorithmic_bytes(name: str, W: int, H: int, Nz: int, nparts: int, f_line: float = 1.0) -> float:
+    """Compulsory HBM bytes of one launch of each solver kernel (complex64 = 8 B).  f_line:
+    the fraction of 16-voxel lines of the volume that hold a non-zero (the solver reads and
+    writes only those lines of y; see the line occupancy maps)."""
     N, V = W * H, W * H * Nz
     return {
-        # reads y_{k-1}, y_{k-2}, g; writes y_k
-        "prox": 32.0 * V,
-        # reads y_k, writes the column-transformed planes
-        "fft_cols_fwd": 16.0 * V,
+        # reads g (dense) and the occupied lines of y_{k-1}, y_{k-2}; writes those of y_k
+        "prox": 8.0 * V + 3 * 8.0 * V * f_line,
+        # reads the occupied lines of y_k, writes the column-transformed planes (upper bound)
+        "fft_cols_fwd": 8.0 * V * f_line + 8.0 * V,
         # reads the column-transformed planes and kappa rows, writes nparts partial spectra
         "fft_rows_fwdacc": 8.0 * V + 8.0 * N * nparts + 8.0 * N * nparts,
         # reads R-hat and kappa (16 B / pixel), writes the row-transformed planes
@@ -50,6 +52,21 @@ def algorithmic_bytes(name: str, W: int, H: int, Nz: int, nparts: int) -> float:
         # reads the row-transformed planes, writes g
         "fft_cols_adj": 16.0 * V,
     }.get(name, float("nan"))
+
+
+def prox_flops_per_voxel(n_tv: int) -> int:
+    """FP32 operations per voxel of the mode-7 prox (fma = 2): momentum point and x-hat 10,
+    first TV level 25 (forward differences 6, the y-gradient 6, the 4-component norm 7,
+    max and rsqrt 2, the projection 4), each further level 35 (+ the divergence 6 and the
+    estimate 4), final estimate 10, soft threshold 12, zero canonicalisation 2.  The line
+    search, restart and objective sums (about 36 more) are counted out: they run only on
+    rows holding a non-zero."""
+    return 10 + 25 + 35 * (n_tv - 1) + 10 + 12 + 2 if n_tv > 0 else 10 + 12 + 2
+
+
+# the kernel's limiting unit: the prox's TV pipeline is issue-bound on the vector ALUs
+# (measured VALU issue in profiles/c2_hbm_counters.json), the FFT passes on HBM
+KERNEL_BOUND = {"prox": "valu"}
 
 
 def iteration_bytes(W, H, Nz):
@@ -311,39 +328,62 @@ def main():
     holo_dev = torch.from_numpy(holo).cuda()
     torch.cuda.synchronize()
     rec.begin(holo_dev)
+    # warm-up with every kernel timed: the dominant kernel of the solve
+    rec.set_timing(True)
     if args.warmup > 0:
         rec.iterate(args.warmup)
-    torch.cuda.synchronize()
-    rec.set_timing(True)
+    warm_timing = rec.get_timing()
+    dom_name = max(warm_timing.items(), key=lambda kv: kv[1][1])[0] if args.warmup > 0 else "prox"
+    # the timed region: HIP events around the dominant kernel's launches only (two per
+    # launch, on the solver's stream), so the rate is not slowed by event records
+    rec.set_timing(True, only=dom_name)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st = rec.iterate(args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     timing = rec.get_timing()
+    # after the timed region: every kernel timed over further iterations, for the breakdown
+    rec.set_timing(True)
+    n_after = max(1, min(20, args.steps))
+    rec.iterate(n_after)
+    breakdown = rec.get_timing()
     rec.set_timing(False)
     fin = rec.finish()
+    # fraction of 16-voxel lines holding a non-zero in the returned volume
+    vol = torch.empty((Nz, H, W), dtype=torch.complex64, device="cuda")
+    rec.getVolume(out=vol)
+    f_line = float((vol != 0).reshape(Nz, H, W // 16, 16).any(-1).float().mean().item())
+    del vol
     rec.destroy()
 
     elapsed = t1 - t0
     ips = args.steps / elapsed
     zchunk = min(Nz, int(os.environ.get("RIHVR_ZCHUNK", "32")))  # planes per partial spectrum (library default 32)
     nparts = (Nz + zchunk - 1) // zchunk
-    # dominant kernel: the largest share of GPU time in the timed region
-    dom = max(timing.items(), key=lambda kv: kv[1][1])
-    dom_name, (dom_launches, dom_ms) = dom
+    dom_launches, dom_ms = timing[dom_name]
     dom_avg_ms = dom_ms / max(1, dom_launches)
-    dom_bytes = algorithmic_bytes(dom_name, W, H, Nz, nparts)
-    achieved = dom_bytes / (dom_avg_ms * 1e-3) / 1e9
+    dom_bytes = algorithmic_bytes(dom_name, W, H, Nz, nparts, f_line)
+    bound = KERNEL_BOUND.get(dom_name, "hbm")
+    if bound == "valu":
+        dom_work = prox_flops_per_voxel(cfg["num_tv_iterations"]) * float(W * H * Nz)
+        achieved = dom_work / (dom_avg_ms * 1e-3) / 1e12
+        peak, unit = FP32_PEAK_TFLOPS, "TFLOP/s"
+    else:
+        dom_work = dom_bytes
+        achieved = dom_bytes / (dom_avg_ms * 1e-3) / 1e9
+        peak, unit = HBM_PEAK_GBPS, "GB/s"
     it_bytes = iteration_bytes(W, H, Nz)
     it_gbps = it_bytes * ips / 1e9
-    gpu_ms_total = sum(v[1] for v in timing.values())
+    gpu_ms_after = sum(v[1] for v in breakdown.values())
     # HBM traffic of the dominant kernel from the committed rocprofv3 counter passes
     traffic = None
+    valu_issue = None
     counters = os.path.join(ROOT, "profiles", "c2_hbm_counters.json")
     if args.workload == "c2" and os.path.exists(counters):
         kc = json.load(open(counters)).get("kernels", {}).get(dom_name)
         traffic = kc["hbm_bytes"] if kc else None
+        valu_issue = kc.get("valu_issue_frac") if kc else None
 
     out = {
         "metric": METRIC,
@@ -365,16 +405,24 @@ def main():
             "width": W, "height": H, "num_planes": Nz,
         },
         "roofline": {
-            "bound": "hbm",
+            "bound": bound,
             "kernel": dom_name,
             "achieved": achieved,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
+            "peak": peak,
+            "unit": unit,
+            "frac": achieved / peak,
             "traffic": traffic,
-            "traffic_source": "profiles/c2_hbm_counters.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)" if traffic else None,
+            "traffic_source": "profiles/c2_hbm_counters.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, timed-region iterations)" if traffic else None,
             "avg_launch_ms": dom_avg_ms,
+            "algorithmic_work_per_launch": dom_work,
             "algorithmic_bytes_per_launch": dom_bytes,
+            "hbm_achieved_GBps": dom_bytes / (dom_avg_ms * 1e-3) / 1e9,
+            "valu_issue_frac": valu_issue,
+            "note": ("FP32 operations counted per voxel (prox_flops_per_voxel) against the published FP32 vector "
+                     "peak, which assumes packed dual-issue FP32; this build compiles without packed FP32, so "
+                     "its issue ceiling is half of it; valu_issue_frac is the measured fraction of that ceiling"
+                     if bound == "valu" else None),
+            "line_fraction": f_line,
         },
         "iteration_roofline": {
             "bytes_per_iteration": it_bytes,
@@ -383,9 +431,11 @@ def main():
             "fp32_tflops": iteration_flops(W, H, Nz) * ips / 1e12,
             "fp32_frac": iteration_flops(W, H, Nz) * ips / 1e12 / FP32_PEAK_TFLOPS,
         },
-        "kernels_ms_per_iteration": {k: v[1] / args.steps for k, v in sorted(timing.items(), key=lambda kv: -kv[1][1])},
-        "gpu_busy_frac": gpu_ms_total / (1e3 * elapsed),
-        "solver": {"backtracks_timed": int(np.sum(st["backtracks"])), "restarts_timed": int(np.sum(st["restarts"])),
+        "kernels_ms_per_iteration": {k: v[1] / n_after for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][1])},
+        "kernels_breakdown_source": f"every kernel timed over {n_after} iterations after the timed region",
+        "gpu_ms_per_iteration_after": gpu_ms_after / n_after,
+        "solver": {"iterations_total": args.warmup + args.steps + n_after,
+                   "backtracks_timed": int(np.sum(st["backtracks"])), "restarts_timed": int(np.sum(st["restarts"])),
                    "stepsize": st["stepsize"], "f_final": fin["f_final"], "sparsity": fin["sparsity"]},
     }
     if args.frames > 0:

@@ -185,7 +185,9 @@ rihvr_status rihvr_finish(rihvr_ctx* ctx, rihvr_stats* stats);
 /* Per-kernel GPU timing with HIP events on the context's stream: the
  * counterpart of the reference's DECLARE_TIMING / SAVE_TIMING phase timers
  * (src/core/umnholo.h:79-124, compiled out by default there).
- * rihvr_set_timing(ctx, 1) resets the counters and starts recording; 0 stops.
+ * rihvr_set_timing(ctx, 1) resets the counters and starts recording every kernel;
+ * enable = 2 + k records only kernel k (the k-th record of rihvr_get_timing), so that
+ * one kernel is timed with two events per launch; 0 stops.
  * rihvr_get_timing writes up to `capacity` records and sets *count to the number
  * of timed kernels. */
 typedef struct rihvr_kernel_time {

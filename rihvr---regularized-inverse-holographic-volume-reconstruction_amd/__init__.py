@@ -587,8 +587,12 @@ class SparseCompressiveHolo:
         return dict(f_final=d["f_final"], nnz=d["nnz"], sparsity=d["sparsity"], stepsize=d["stepsize"],
                     total_backtracks=d["total_backtracks"], f_initial=d["f_initial"])
 
-    def set_timing(self, enable: bool):
-        _check(lib().rihvr_set_timing(self._ctx, 1 if enable else 0), "rihvr_set_timing")
+    def set_timing(self, enable: bool, only: str | None = None):
+        """Per-kernel event timing; with ``only`` (a name of get_timing) that kernel alone."""
+        code = 1 if enable else 0
+        if enable and only is not None:
+            code = 2 + list(self.get_timing().keys()).index(only)
+        _check(lib().rihvr_set_timing(self._ctx, code), "rihvr_set_timing")
 
     def get_timing(self) -> dict:
         n = ctypes.c_int32(0)

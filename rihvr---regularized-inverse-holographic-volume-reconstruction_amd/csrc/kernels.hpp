@@ -104,9 +104,11 @@ hipError_t momentum_out(const float2* Y, const float2* Y0, float s_mom, float2* 
 
 // Hologram momentum by linearity (sparse_compressive_holo.cu:494-498):
 // est = cur + s (cur - prev); resid = est - meas; partial[block] = sum |resid|^2.
+// restart_sum (device, may be NULL): when > 0 the step is 0 (a restarted iteration).
 int holo_num_blocks(long n);
 hipError_t holo_momentum(const float2* cur, const float2* prev, double s_mom, const float2* meas,
-                         float2* est, float2* resid, long n, double* partial, hipStream_t s);
+                         float2* est, float2* resid, long n, double* partial, hipStream_t s,
+                         const double* restart_sum = nullptr);
 // resid = est - meas with partial sum |resid|^2
 hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, long n, double* partial,
                          hipStream_t s);

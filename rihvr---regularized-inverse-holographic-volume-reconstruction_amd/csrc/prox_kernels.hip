@@ -89,13 +89,13 @@ __device__ __forceinline__ int wrapi(int v, int n) {
 }
 
 // Lane shifts across the whole wave with DPP (no LDS traffic): lane i receives
-// lane i-1 (wave_shr:1) or lane i+1 (wave_shl:1); the edge lane receives 0 and
-// is halo.
+// lane i-1 (wave_shr:1) or lane i+1 (wave_shl:1); the edge lane receives 0 (bound
+// control, so the shift folds into the consuming instruction) and is halo.
 __device__ __forceinline__ float lane_from_left(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float lane_from_right(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ float soft_fast(float v, float thr) {
@@ -674,8 +674,12 @@ int holo_num_blocks(long n) { return (int)((n + (long)kEwThreads * kEwItems - 1)
 
 __global__ void __launch_bounds__(kEwThreads)
 k_holo_momentum(const float2* __restrict__ cur, const float2* __restrict__ prev, double s_mom,
-                const float2* __restrict__ meas, float2* est, float2* resid, long n, double* partial) {
+                const float2* __restrict__ meas, float2* est, float2* resid, long n, double* partial,
+                const double* __restrict__ restart_sum) {
     __shared__ double red[kEwThreads / 64];
+    // with restart_sum: the step is 0 when the iteration restarts (restart_sum > 0), as the
+    // host's alpha update gives, so the step can be chosen before the host has the sum
+    if (restart_sum && *restart_sum > 0) s_mom = 0.0;
     double acc = 0;
     for (int it = 0; it < kEwItems; ++it) {
         long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
@@ -694,9 +698,9 @@ k_holo_momentum(const float2* __restrict__ cur, const float2* __restrict__ prev,
 }
 
 hipError_t holo_momentum(const float2* cur, const float2* prev, double s_mom, const float2* meas, float2* est,
-                         float2* resid, long n, double* partial, hipStream_t s) {
+                         float2* resid, long n, double* partial, hipStream_t s, const double* restart_sum) {
     hipLaunchKernelGGL(k_holo_momentum, dim3(holo_num_blocks(n)), dim3(kEwThreads), 0, s, cur, prev, s_mom,
-                       meas, est, resid, n, partial);
+                       meas, est, resid, n, partial, restart_sum);
     return hipGetLastError();
 }
 

@@ -56,6 +56,7 @@ const char* const kKernelNames[K_COUNT] = {
 
 struct Timer {
     bool on = false;
+    int only = -1;  // >= 0: record this kernel id only
     struct Pending { int id; hipEvent_t a, b; };
     std::vector<hipEvent_t> pool;
     std::vector<Pending> pending;
@@ -88,6 +89,7 @@ struct rihvr_ctx {
     hipStream_t aux = nullptr;     // A^T of the next plane group while prox and A run on `stream`
     std::vector<hipEvent_t> ev_adj;  // per plane group: its G planes are written (on aux)
     hipEvent_t ev_rhat = nullptr;    // R-hat is written (on stream)
+    hipEvent_t ev_scal = nullptr;    // the scalars of the last prox + A are in h_scal
     // volumes
     float2* Yb[3] = {nullptr, nullptr, nullptr};  // ring of prox outputs
     int iy = 0, iy0 = 1, iym = 2;                 // y_k, y_{k-1}, y_{k-2}
@@ -98,6 +100,7 @@ struct rihvr_ctx {
     uint64_t* occ_tmp = nullptr;                  // the map being built by a prox
     uint8_t* smap = nullptr;                      // [Nz][W/16] strips of A's column pass
     float2* G = nullptr;                          // A^T r
+    float2* G_alt = nullptr;                      // the next iteration's A^T r, formed ahead
     float2* T = nullptr;                          // FFT scratch volume
     float2* out = nullptr;                        // points at the returned volume
     // holograms
@@ -121,6 +124,7 @@ struct rihvr_ctx {
     int zchunk_adj = 8, zchunk_fwd = 8, nparts = 1;
     int pipe_planes = 1;  // planes per pipeline group (a multiple of the z chunks)
     int n_groups = 1;
+    bool groups_one_stream = false;  // run A^T, prox and A group by group on the main stream
     int prox_rows = 0;    // output rows per prox wave (0: default)
     bool have_result = false;
     bool force_step = false;
@@ -187,7 +191,7 @@ rihvr_status copy_out(rihvr_ctx* c, float* dst, const float2* src, int loc, size
 // Launches through `launch` (returns hipError_t) on stream `st`, bracketed by events
 // on that stream when timing is on.
 template <class F> rihvr_status timed_on(rihvr_ctx* c, int id, hipStream_t st, F&& launch) {
-    if (!c->tm.on) {
+    if (!c->tm.on || (c->tm.only >= 0 && id != c->tm.only)) {
         RIHVR_HIP(launch());
         return RIHVR_OK;
     }
@@ -211,8 +215,15 @@ template <class F> rihvr_status timed(rihvr_ctx* c, int id, F&& launch) {
 }
 
 // Accumulates the durations of completed event pairs (call after a stream sync).
+// Pairs whose end event has not completed yet (work queued after a partial wait) stay
+// pending; events complete in stream order.
 rihvr_status resolve_timing(rihvr_ctx* c) {
-    for (auto& p : c->tm.pending) {
+    size_t k = 0;
+    for (; k < c->tm.pending.size(); ++k) {
+        auto& p = c->tm.pending[k];
+        const hipError_t q = hipEventQuery(p.b);
+        if (q == hipErrorNotReady) break;
+        RIHVR_HIP(q);
         float ms = 0;
         RIHVR_HIP(hipEventElapsedTime(&ms, p.a, p.b));
         c->tm.total_ms[p.id] += ms;
@@ -220,13 +231,24 @@ rihvr_status resolve_timing(rihvr_ctx* c) {
         c->tm.pool.push_back(p.a);
         c->tm.pool.push_back(p.b);
     }
-    c->tm.pending.clear();
+    c->tm.pending.erase(c->tm.pending.begin(), c->tm.pending.begin() + k);
     return RIHVR_OK;
 }
 
 rihvr_status fetch_scalars(rihvr_ctx* c) {
     RIHVR_HIP(hipMemcpyAsync(c->h_scal, c->scal, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     RIHVR_HIP(hipStreamSynchronize(c->stream));
+    return resolve_timing(c);
+}
+
+// The same copy, waited for by an event so that work queued after it keeps the device busy.
+rihvr_status fetch_scalars_async(rihvr_ctx* c) {
+    RIHVR_HIP(hipMemcpyAsync(c->h_scal, c->scal, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RIHVR_HIP(hipEventRecord(c->ev_scal, c->stream));
+    return RIHVR_OK;
+}
+rihvr_status wait_scalars(rihvr_ctx* c) {
+    RIHVR_HIP(hipEventSynchronize(c->ev_scal));
     return resolve_timing(c);
 }
 
@@ -372,10 +394,12 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
 // Same kernels, same arithmetic and the same partial-row layout as the one-stream
 // sequence A^T (all planes), prox (all planes), A (all planes).
 rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, float s_mom, double t, float2* Yn) {
-    RIHVR_HIP(hipStreamWaitEvent(c->aux, c->ev_rhat, 0));
-    for (int g = 0, g0 = 0; g < c->n_groups; ++g, g0 += c->pipe_planes) {
-        RIHVR_TRY(adjoint_group(c, c->G, g0, std::min(c->pipe_planes, c->Nz - g0), c->aux));
-        RIHVR_HIP(hipEventRecord(c->ev_adj[g], c->aux));
+    if (!c->groups_one_stream) {
+        RIHVR_HIP(hipStreamWaitEvent(c->aux, c->ev_rhat, 0));
+        for (int g = 0, g0 = 0; g < c->n_groups; ++g, g0 += c->pipe_planes) {
+            RIHVR_TRY(adjoint_group(c, c->G, g0, std::min(c->pipe_planes, c->Nz - g0), c->aux));
+            RIHVR_HIP(hipEventRecord(c->ev_adj[g], c->aux));
+        }
     }
     ProxLaunch a;
     a.W = c->W; a.H = c->H;
@@ -394,7 +418,10 @@ rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, fl
         const int gz = std::min(c->pipe_planes, c->Nz - g0);
         const size_t off = (size_t)g0 * c->N;
         const size_t roff = (size_t)g0 * c->H;  // first map word of the group
-        RIHVR_HIP(hipStreamWaitEvent(c->stream, c->ev_adj[g], 0));
+        if (c->groups_one_stream)
+            RIHVR_TRY(adjoint_group(c, c->G, g0, gz, c->stream));
+        else
+            RIHVR_HIP(hipStreamWaitEvent(c->stream, c->ev_adj[g], 0));
         a.Y0 = Y0 + off; a.YM = YM + off; a.G = c->G + off; a.Y = Yn + off;
         a.nz = gz;
         a.partial = c->partial + (size_t)g0 * per_plane * PS_COUNT;
@@ -430,7 +457,7 @@ void release(rihvr_ctx* c) {
     for (auto& m : c->occ) (void)hipFree(m);
     (void)hipFree(c->occ_tmp);
     (void)hipFree(c->smap);
-    void* ptrs[] = {c->G, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->spart, c->kappa,
+    void* ptrs[] = {c->G, c->G_alt, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->spart, c->kappa,
                     c->zlist, c->phase, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
                     c->red_work, c->dflag};
     for (void* q : ptrs) (void)hipFree(q);
@@ -439,6 +466,7 @@ void release(rihvr_ctx* c) {
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     for (hipEvent_t e : c->ev_adj) (void)hipEventDestroy(e);
     if (c->ev_rhat) (void)hipEventDestroy(c->ev_rhat);
+    if (c->ev_scal) (void)hipEventDestroy(c->ev_scal);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -457,6 +485,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     RIHVR_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
     RIHVR_HIP(hipEventCreateWithFlags(&c->ev_rhat, hipEventDisableTiming));
+    RIHVR_HIP(hipEventCreateWithFlags(&c->ev_scal, hipEventDisableTiming));
     int zc = 32;
     if (const char* e = std::getenv("RIHVR_ZCHUNK")) zc = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("RIHVR_PROX_ROWS")) c->prox_rows = std::max(0, std::atoi(e));
@@ -476,6 +505,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     c->pipe_planes = std::min(c->Nz, std::max(16, pp));
     if (c->pipe_planes % c->zchunk_fwd || c->pipe_planes % c->zchunk_adj) c->pipe_planes = c->Nz;
     c->n_groups = (c->Nz + c->pipe_planes - 1) / c->pipe_planes;
+    if (const char* e = std::getenv("RIHVR_GROUP_ONE_STREAM")) c->groups_one_stream = std::atoi(e) != 0;
     c->ev_adj.assign(c->n_groups, nullptr);
     for (hipEvent_t& e : c->ev_adj) RIHVR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& b : c->Yb) RIHVR_TRY(dalloc(&b, c->V));
@@ -483,6 +513,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_TRY(dalloc(&c->occ_tmp, (size_t)c->Nz * c->H));
     RIHVR_TRY(dalloc(&c->smap, (size_t)c->Nz * (c->W / 16)));
     RIHVR_TRY(dalloc(&c->G, c->V));
+    RIHVR_TRY(dalloc(&c->G_alt, c->V));
     RIHVR_TRY(dalloc(&c->T, c->V));
     float2** holos[] = {&c->meas, &c->hE, &c->hC, &c->hP, &c->hX, &c->resid, &c->rhat, &c->htmp};
     for (float2** h : holos) RIHVR_TRY(dalloc(h, c->N));
@@ -644,6 +675,12 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
     if (n_it > 0) c->have_result = false;  // a snapshot lives in the ring slot the prox writes
     SolveState& s = c->ss;
     const int mode = s.mode;
+    // Without an objective target, the work of an accepted iteration (hologram momentum,
+    // R-hat and the next gradient into G_alt) is queued before the host has read the
+    // scalars of the prox and A, so the device does not idle while the host decides; a
+    // backtrack (rare) discards it.  The step it uses is the host's, chosen on the device
+    // by the restart test (a restart makes it 0).
+    const bool ahead = f_stop <= 0 && c->n_groups == 1;
     for (int i = 0; i < n_it; ++i) {
         double alpha0 = s.alpha1;
         int bt = 0;
@@ -651,6 +688,7 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
         float2* Yn = c->Yb[c->iy];
         float2* Y0 = c->Yb[c->iy0];
         float2* YM = c->Yb[c->iym];
+        bool queued = false;  // the accepted-iteration work is queued
         for (;;) {
             if (!s.g_ready && mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && c->n_groups > 1) {
                 RIHVR_TRY(pipelined_pass(c, Y0, YM, s.s_mom, s.t, Yn));
@@ -662,7 +700,28 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
                 RIHVR_TRY(apply_forward(c, Yn, c->hX));
             }
             s.g_ready = true;
-            RIHVR_TRY(fetch_scalars(c));
+            if (ahead) {
+                RIHVR_TRY(fetch_scalars_async(c));
+                const double a1 = (1.0 + std::sqrt(1.0 + 4.0 * alpha0 * alpha0)) / 2.0;
+                std::swap(c->hP, c->hC);
+                std::swap(c->hC, c->hX);
+                RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
+                    return holo_momentum(c->hC, c->hP, (alpha0 - 1) / a1, c->meas, c->hE, c->resid, c->N,
+                                         c->partial_holo, c->stream, c->scal + SL_PROX + PS_RESTART);
+                }));
+                RIHVR_TRY(timed(c, K_REDUCE, [&] {
+                    return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work,
+                                           c->scal + SL_RESID_M, c->stream);
+                }));
+                RIHVR_TRY(spectrum(c, c->resid));
+                if (i + 1 < n_it)
+                    for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
+                        RIHVR_TRY(adjoint_group(c, c->G_alt, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
+                queued = true;
+                RIHVR_TRY(wait_scalars(c));
+            } else {
+                RIHVR_TRY(fetch_scalars(c));
+            }
             const double* sc = c->h_scal;
             if (bt == 0 && s.iters > 0) {
                 // residual norm at the previous momentum point (:545-549), and the
@@ -690,6 +749,11 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
             const double f_resid = 0.5 * sc[SL_RESID_A];
             ratio = f_resid / (0.5 * s.prev_summag + lim);  // :403
             if (((ratio > 1.0) || (ratio < 0)) && (bt < 2)) {  // :423
+                if (queued) {  // discard the queued work: the hologram ring as it was
+                    std::swap(c->hC, c->hX);
+                    std::swap(c->hP, c->hC);
+                    queued = false;
+                }
                 s.t *= 0.5;
                 bt++;
                 s.total_bt++;
@@ -703,18 +767,28 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
         if (restart) alpha0 = 1;
         s.alpha1 = (1.0 + std::sqrt(1.0 + 4.0 * alpha0 * alpha0)) / 2.0;  // :476
         const double step = (alpha0 - 1) / s.alpha1;                       // :481
-        // hologram momentum by linearity (:494-498): prev <- cur, cur <- A y_k, est <- cur + s (cur - prev)
-        std::swap(c->hP, c->hC);
-        std::swap(c->hC, c->hX);
-        RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
-            return holo_momentum(c->hC, c->hP, step, c->meas, c->hE, c->resid, c->N, c->partial_holo, c->stream);
-        }));
-        RIHVR_TRY(timed(c, K_REDUCE, [&] {
-            return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work, c->scal + SL_RESID_M, c->stream);
-        }));
-        // the next gradient (:545-546): R-hat now, its planes with the next prox
-        RIHVR_TRY(spectrum(c, c->resid));
-        s.g_ready = false;
+        if (queued) {
+            // the queued work used this step (0 after a restart); G_alt is the next gradient
+            if (i + 1 < n_it) {
+                std::swap(c->G, c->G_alt);
+                s.g_ready = true;
+            } else {
+                s.g_ready = false;
+            }
+        } else {
+            // hologram momentum by linearity (:494-498): prev <- cur, cur <- A y_k, est <- cur + s (cur - prev)
+            std::swap(c->hP, c->hC);
+            std::swap(c->hC, c->hX);
+            RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
+                return holo_momentum(c->hC, c->hP, step, c->meas, c->hE, c->resid, c->N, c->partial_holo, c->stream);
+            }));
+            RIHVR_TRY(timed(c, K_REDUCE, [&] {
+                return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work, c->scal + SL_RESID_M, c->stream);
+            }));
+            // the next gradient (:545-546): R-hat now, its planes with the next prox
+            RIHVR_TRY(spectrum(c, c->resid));
+            s.g_ready = false;
+        }
         // rotate the prox-output ring: y_{k-2} <- y_{k-1} <- y_k
         const int old_m = c->iym;
         c->iym = c->iy0;
@@ -956,7 +1030,9 @@ rihvr_status rihvr_set_timing(rihvr_ctx* ctx, int32_t enable) {
         ctx->tm.total_ms[k] = 0;
         ctx->tm.launches[k] = 0;
     }
+    if (enable < 0 || enable >= 2 + K_COUNT) return RIHVR_E_INVALID_ARGUMENT;
     ctx->tm.on = enable != 0;
+    ctx->tm.only = enable >= 2 ? enable - 2 : -1;
     return RIHVR_OK;
 }
 
