@@ -129,6 +129,10 @@ struct rihvr_ctx {
     bool have_result = false;
     bool force_step = false;
     double forced_step = 0;
+    // testing knob (RIHVR_TEST_BACKTRACKS, a bit mask of iteration numbers 0..63): the
+    // first attempt of those iterations is rejected as if its ratio were above 1, so the
+    // backtracking paths can be exercised; never set in normal use
+    uint64_t test_backtracks = 0;
     Timer tm;
     SolveState ss;
 };
@@ -506,6 +510,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     if (c->pipe_planes % c->zchunk_fwd || c->pipe_planes % c->zchunk_adj) c->pipe_planes = c->Nz;
     c->n_groups = (c->Nz + c->pipe_planes - 1) / c->pipe_planes;
     if (const char* e = std::getenv("RIHVR_GROUP_ONE_STREAM")) c->groups_one_stream = std::atoi(e) != 0;
+    if (const char* e = std::getenv("RIHVR_TEST_BACKTRACKS")) c->test_backtracks = std::strtoull(e, nullptr, 0);
     c->ev_adj.assign(c->n_groups, nullptr);
     for (hipEvent_t& e : c->ev_adj) RIHVR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& b : c->Yb) RIHVR_TRY(dalloc(&b, c->V));
@@ -748,7 +753,8 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
             lim = sc[SL_PROX + PS_LS1] + sc[SL_PROX + PS_LS2] / (2 * s.t);  // :4381
             const double f_resid = 0.5 * sc[SL_RESID_A];
             ratio = f_resid / (0.5 * s.prev_summag + lim);  // :403
-            if (((ratio > 1.0) || (ratio < 0)) && (bt < 2)) {  // :423
+            const bool forced = bt == 0 && s.iters < 64 && ((c->test_backtracks >> s.iters) & 1ull);
+            if (((ratio > 1.0) || (ratio < 0) || forced) && (bt < 2)) {  // :423
                 if (queued) {  // discard the queued work: the hologram ring as it was
                     std::swap(c->hC, c->hX);
                     std::swap(c->hP, c->hC);

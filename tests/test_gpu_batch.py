@@ -83,3 +83,24 @@ def test_batch_mode6_and_device_arrays(pkg, frames, gpu_available):
         vh = np.zeros((3, 6, 64, 64), np.complex64)
         rec.reconstructBatch(holos, warm_from_previous=True, volumes=vh)
     np.testing.assert_array_equal(vd.cpu().numpy(), vh)
+
+
+def test_backtracks_with_work_queued_ahead(pkg, frames, gpu_available, monkeypatch):
+    """Without an objective target the solver queues each accepted iteration's work before it
+    reads the scalars; a backtrack discards that work.  Forced backtracks (iterations 1 and 4)
+    give the same volume as the path that waits for the scalars (an unreachable target)."""
+    c, holos = frames
+    c12 = dict(c, num_inverse_iterations=12)
+    monkeypatch.setenv("RIHVR_TEST_BACKTRACKS", str((1 << 1) | (1 << 4)))
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c12)) as rec:
+        st = rec.inverseReconstruct(holos[0])
+        ahead = rec.getVolume()
+        fs = rec.reconstructBatch(holos[:1], f_target=[1e-300], max_iterations=12)
+        waited = rec.getVolume()
+    assert st["total_backtracks"] == 2 and fs[0]["total_backtracks"] == 2
+    assert list(st["backtracks"][:6]) == [0, 1, 0, 0, 1, 0]
+    np.testing.assert_array_equal(ahead, waited)
+    monkeypatch.delenv("RIHVR_TEST_BACKTRACKS")
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c12)) as rec:
+        plain = rec.inverseReconstruct(holos[0])
+    assert plain["total_backtracks"] == 0
