@@ -123,6 +123,8 @@ template <int NTV> struct ProxWave {
     // y_{k-2} for the rows loaded next, and of the output's current content for the rows
     // written next
     uint64_t ow0[2] = {~0ull, ~0ull}, owm[2] = {~0ull, ~0ull}, oold[2] = {~0ull, ~0ull};
+    // the lines (bit l: columns 16l .. 16l + 15) that the wave's 64 lanes read
+    uint64_t wlines = ~0ull;
     float acc_ls1 = 0.f, acc_ls2 = 0.f, acc_rs = 0.f, acc_l1 = 0.f, acc_tv = 0.f, acc_nnz = 0.f;
 
     static __device__ __forceinline__ int wrap_row(int tau, int H) {
@@ -138,32 +140,50 @@ template <int NTV> struct ProxWave {
             (const __attribute__((address_space(4))) uint64_t*)map;
         return cmap[i];
     }
-    // occupancy words of y_{k-1} and y_{k-2} for row tau, one step before its loads
-    __device__ __forceinline__ void load_words(const ProxDev& a, int slot, int tau, int zrow) {
-        const int row = wrap_row(tau, a.H);
-        ow0[slot] = uniform_word(a.occ_y0, zrow + row);
-        owm[slot] = uniform_word(a.occ_ym, zrow + row);
+    // Plane bases (wave-uniform) and the wrapped rows of the next loads, kept as running
+    // counters so that no step divides by H.
+    const char* y0p;
+    const char* ymp;
+    const char* gp;
+    char* yp;
+    unsigned rowb;                // bytes per row
+    int row_ld, row_wd, row_old;  // rows of the next load_row, load_words and old-map read
+    static __device__ __forceinline__ int next_row(int r, int H) { return r + 1 == H ? 0 : r + 1; }
+
+    // occupancy words of y_{k-1} and y_{k-2} for row row_wd, one step before its loads
+    __device__ __forceinline__ void load_words(const ProxDev& a, int slot, int zrow) {
+        ow0[slot] = uniform_word(a.occ_y0, zrow + row_wd);
+        owm[slot] = uniform_word(a.occ_ym, zrow + row_wd);
+        row_wd = next_row(row_wd, a.H);
     }
-    // lines of y_{k-1} and y_{k-2} that are empty in their maps are zeros: not read
-    __device__ __forceinline__ void load_row(const ProxDev& a, int slot, int tau, size_t plane_off, int xcol) {
-        const int row = wrap_row(tau, a.H);
-        const size_t ro = plane_off + (size_t)row * a.W + xcol;
-        const int line = xcol >> 4;
+    // Element at byte offset xoff (per lane) of the row that starts off bytes into the
+    // plane at base (wave-uniform): the row address stays in scalar registers.
+    static __device__ __forceinline__ const float2& at(const char* base, unsigned off, unsigned xoff) {
+        return *(const float2*)(base + off + xoff);
+    }
+    // Row row_ld.  A row of y_{k-1} or y_{k-2} is read only when one of the wave's lines is
+    // set in its map: a line whose bit is clear holds +0, so a wave that reads the row with
+    // some of its lines clear reads zeros there, and a wave with all its lines clear takes
+    // zeros.
+    __device__ __forceinline__ void load_row(const ProxDev& a, int slot, unsigned xoff) {
+        const unsigned off = (unsigned)row_ld * rowb;
         const float2 zero = make_float2(0.0f, 0.0f);
-        pf[slot][0] = ((ow0[slot] >> line) & 1ull) ? a.Y0[ro] : zero;
-        pf[slot][1] = ((owm[slot] >> line) & 1ull) ? a.YM[ro] : zero;
-        pf[slot][2] = a.G[ro];
+        pf[slot][0] = (ow0[slot] & wlines) ? at(y0p, off, xoff) : zero;
+        pf[slot][1] = (owm[slot] & wlines) ? at(ymp, off, xoff) : zero;
+        pf[slot][2] = at(gp, off, xoff);
+        row_ld = next_row(row_ld, a.H);
     }
 
     // Step S of an unrolled block: row tau, ring slot S mod P, row parity S & 1.
     template <int S>
     __device__ __forceinline__ void step(const ProxDev& a, int tau, int r0, int r1,
-                                         size_t plane_off, int xcol, int ocol, bool out_lane, int tau_end,
+                                         unsigned xoff, int ocol, bool out_lane, int tau_end,
                                          int zrow, int strip) {
         constexpr int SLOT = S % P;
         constexpr int PAR = S & 1;
         constexpr int Q = PAR ^ 1;
         const int W = a.W;
+        const unsigned oofs = (unsigned)ocol * (unsigned)sizeof(float2);  // output lanes only
         // ---- row tau: momentum point x0 = y0 + s (y0 - ym), x-hat = x0 - t g ----
         {
             const float2 y0 = pf[PAR][0], ym = pf[PAR][1], g = pf[PAR][2];
@@ -173,9 +193,10 @@ template <int NTV> struct ProxWave {
             hist[2][SLOT] = g;
             hist[3][SLOT] = y0;
             // the output's current map for the row written in the next step
-            if (a.occ_old) oold[Q] = uniform_word(a.occ_old, zrow + wrap_row(tau - NTV, a.H));
-            if (tau + 2 <= tau_end) load_row(a, PAR, tau + 2, plane_off, xcol);
-            if (tau + 3 <= tau_end) load_words(a, Q, tau + 3, zrow);
+            if (a.occ_old) oold[Q] = uniform_word(a.occ_old, zrow + row_old);
+            row_old = next_row(row_old, a.H);
+            if (tau + 2 <= tau_end) load_row(a, PAR, xoff);
+            if (tau + 3 <= tau_end) load_words(a, Q, zrow);
         }
         // ---- TV levels: level k works on row tau - k ----
 #pragma unroll
@@ -200,9 +221,16 @@ template <int NTV> struct ProxWave {
             const float ayr = fmaf(a.sc, yu.x - y.x, go.z);
             const float ayi = fmaf(a.sc, yu.y - y.y, go.w);
             const float n2 = fmaf(axr, axr, fmaf(axi, axi, fmaf(ayr, ayr, ayi * ayi)));
-            // projection: divide by max(|a|, 1)
-            const float inv = __builtin_amdgcn_rsqf(fmaxf(n2, 1.0f));
-            gd[PAR][k] = make_float4(axr * inv, axi * inv, ayr * inv, ayi * inv);
+            // projection: divide by max(|a|, 1), i.e. by |a| where n2 > 1 and by exactly 1
+            // elsewhere; a wave with no lane above 1 (nearly every wave of a sparse solve)
+            // skips the reciprocal square root and the products
+            const bool proj = n2 > 1.0f;
+            if (__builtin_amdgcn_ballot_w64(proj)) {
+                const float inv = proj ? __builtin_amdgcn_rsqf(n2) : 1.0f;
+                gd[PAR][k] = make_float4(axr * inv, axi * inv, ayr * inv, ayi * inv);
+            } else {
+                gd[PAR][k] = make_float4(axr, axi, ayr, ayi);
+            }
             yh[PAR][k] = y;
         }
         // ---- final estimate, threshold and reductions: row o = tau - NTV - 1 ----
@@ -245,9 +273,9 @@ template <int NTV> struct ProxWave {
                 if ((threadIdx.x & 63) == 0) atomicOr((unsigned long long*)&a.occ_new[zrow + o], bits);
             }
             const bool keep = nz || ((oold[PAR] >> (ocol >> 4)) & 1ull);
-            if (out_lane && keep) a.Y[plane_off + (size_t)o * W + ocol] = yv;
+            if (out_lane && keep) *(float2*)(yp + (unsigned)o * rowb + oofs) = yv;
         }
-        if (o >= r0 && o < r1 && out_lane && !a.occ_new) a.Y[plane_off + (size_t)o * W + ocol] = yv;
+        if (o >= r0 && o < r1 && out_lane && !a.occ_new) *(float2*)(yp + (unsigned)o * rowb + oofs) = yv;
         // Every reduction term of row o is a product with (y - x0), (x0 - y), x0 or one of
         // the x0 differences, so it is zero (adding it changes no sum, which starts at +0)
         // when y, x0, y_{k-1} and the row above's x0 are zero in every lane of the wave,
@@ -279,15 +307,15 @@ template <int NTV> struct ProxWave {
 
     template <int... S>
     __device__ __forceinline__ void block(std::integer_sequence<int, S...>, const ProxDev& a,
-                                          int tau, int r0, int r1, size_t plane_off, int xcol, int ocol,
+                                          int tau, int r0, int r1, unsigned xoff, int ocol,
                                           bool out_lane, int tau_end, int zrow, int strip) {
-        (step<S>(a, tau + S, r0, r1, plane_off, xcol, ocol, out_lane, tau_end, zrow, strip), ...);
+        (step<S>(a, tau + S, r0, r1, xoff, ocol, out_lane, tau_end, zrow, strip), ...);
     }
     template <int... S>
     __device__ __forceinline__ void tail(std::integer_sequence<int, S...>, int rem, const ProxDev& a,
-                                         int tau, int r0, int r1, size_t plane_off,
-                                         int xcol, int ocol, bool out_lane, int tau_end, int zrow, int strip) {
-        ((S < rem ? step<S>(a, tau + S, r0, r1, plane_off, xcol, ocol, out_lane, tau_end, zrow, strip) : void()),
+                                         int tau, int r0, int r1,
+                                         unsigned xoff, int ocol, bool out_lane, int tau_end, int zrow, int strip) {
+        ((S < rem ? step<S>(a, tau + S, r0, r1, xoff, ocol, out_lane, tau_end, zrow, strip) : void()),
          ...);
     }
 };
@@ -316,12 +344,17 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     const int ocol = strip * geo.ow + (lane - NTV);  // output column of this lane
     const bool out_lane = (lane - NTV) >= 0 && (lane - NTV) < geo.ow && ocol < W;
     const int xcol = wrapi(strip * geo.ow - NTV + lane, W);
+    const unsigned xoff = (unsigned)xcol * (unsigned)sizeof(float2);  // byte offset in a row
     const int r0 = chunk * geo.rc;
     const int r1 = min(a.H, r0 + geo.rc);
     const size_t plane_off = (size_t)z * a.H * W;
     const int zrow = z * a.H;  // first row of plane z in the occupancy maps
 
     PW st;
+    // the wave's lines: the periodic wrap can add a line at the other end of the row
+    st.wlines = 0;
+    for (int l = 0; l < (W >> 4); ++l)
+        if (__builtin_amdgcn_ballot_w64((xcol >> 4) == l)) st.wlines |= 1ull << l;
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -332,18 +365,26 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     const int tau0 = r0 - (NTV > 0 ? NTV : 1);
     const int tau1 = r1 + NTV;  // inclusive
     // the first two rows; every step then loads the row two ahead (U is even)
-    st.load_words(a, 0, tau0, zrow);
-    st.load_words(a, 1, tau0 + 1, zrow);
-    st.load_row(a, 0, tau0, plane_off, xcol);
-    st.load_row(a, 1, tau0 + 1, plane_off, xcol);
-    st.load_words(a, 0, tau0 + 2, zrow);
+    st.y0p = (const char*)(a.Y0 + plane_off);
+    st.ymp = (const char*)(a.YM + plane_off);
+    st.gp = (const char*)(a.G + plane_off);
+    st.yp = (char*)(a.Y + plane_off);
+    st.rowb = (unsigned)W * (unsigned)sizeof(float2);
+    st.row_ld = PW::wrap_row(tau0, a.H);
+    st.row_wd = st.row_ld;
+    st.row_old = PW::wrap_row(tau0 - NTV, a.H);
+    st.load_words(a, 0, zrow);  // tau0
+    st.load_words(a, 1, zrow);  // tau0 + 1
+    st.load_row(a, 0, xoff);    // tau0
+    st.load_row(a, 1, xoff);    // tau0 + 1
+    st.load_words(a, 0, zrow);  // tau0 + 2
     st.oold[0] = PW::uniform_word(a.occ_old, zrow + PW::wrap_row(tau0 - NTV - 1, a.H));
     int tau = tau0;
     for (; tau + U - 1 <= tau1; tau += U)
-        st.block(std::make_integer_sequence<int, U>{}, a, tau, r0, r1, plane_off, xcol, ocol, out_lane,
+        st.block(std::make_integer_sequence<int, U>{}, a, tau, r0, r1, xoff, ocol, out_lane,
                  tau1, zrow, strip);
     if (tau <= tau1)
-        st.tail(std::make_integer_sequence<int, U - 1>{}, tau1 - tau + 1, a, tau, r0, r1, plane_off, xcol,
+        st.tail(std::make_integer_sequence<int, U - 1>{}, tau1 - tau + 1, a, tau, r0, r1, xoff,
                 ocol, out_lane, tau1, zrow, strip);
 
     const double s_ls1 = wave_sum((double)st.acc_ls1);
@@ -392,6 +433,8 @@ template <int NTV> static hipError_t launch_fl2d(const ProxDev& d, hipStream_t s
 }
 
 hipError_t prox_fl2d(const ProxLaunch& a, hipStream_t s) {
+    // row offsets inside a plane are 32-bit byte offsets
+    if ((size_t)a.W * a.H * sizeof(float2) > 0xffffffffull) return hipErrorInvalidValue;
     ProxDev d = make_dev(a);
     switch (a.n_tv) {
         case 0: return launch_fl2d<0>(d, s);
