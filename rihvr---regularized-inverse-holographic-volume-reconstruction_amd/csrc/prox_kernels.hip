@@ -123,8 +123,9 @@ template <int NTV> struct ProxWave {
     // y_{k-2} for the rows loaded next, and of the output's current content for the rows
     // written next
     uint64_t ow0[2] = {~0ull, ~0ull}, owm[2] = {~0ull, ~0ull}, oold[2] = {~0ull, ~0ull};
-    // the lines (bit l: columns 16l .. 16l + 15) that the wave's 64 lanes read
-    uint64_t wlines = ~0ull;
+    // the lines (bit l: columns 16l .. 16l + 15) that the wave's 64 lanes read, and those
+    // its output lanes write
+    uint64_t wlines = ~0ull, outlines = ~0ull;
     float acc_ls1 = 0.f, acc_ls2 = 0.f, acc_rs = 0.f, acc_l1 = 0.f, acc_tv = 0.f, acc_nnz = 0.f;
 
     static __device__ __forceinline__ int wrap_row(int tau, int H) {
@@ -248,32 +249,48 @@ template <int NTV> struct ProxWave {
             const float dvi = lane_from_right(gu.y) - gu.y + gb.w - gu.w;
             yv = make_float2(fmaf(-a.mu, dvr, xh.x), fmaf(-a.mu, dvi, xh.y));
         }
-        if (a.do_soft) yv = make_float2(soft_fast(yv.x, a.thr), soft_fast(yv.y, a.thr));
-        // zeros are stored as +0 (the soft threshold's v * 0 is -0 for negative v), so a
-        // line of zeros is a line of +0 whether it is written or skipped
-        yv = make_float2(yv.x + 0.0f, yv.y + 0.0f);
+        // Soft threshold.  A lane with |v| <= T in both parts gives zero, so a wave where every
+        // lane does (most waves of a sparse solve) skips the division; NaN stays on the full
+        // path.  Zeros are stored as +0 (the threshold's v * 0 is -0 for negative v), so a line
+        // of zeros is a line of +0 whether it is written or skipped.
+        bool live = true;  // wave-uniform: some lane may be non-zero
+        if (a.do_soft) {
+            const bool l = !(fabsf(yv.x) <= a.thr) || !(fabsf(yv.y) <= a.thr);
+            live = __builtin_amdgcn_ballot_w64(l) != 0ull;
+            if (live)
+                yv = make_float2(soft_fast(yv.x, a.thr) + 0.0f, soft_fast(yv.y, a.thr) + 0.0f);
+            else
+                yv = make_float2(0.0f, 0.0f);
+        } else {
+            yv = make_float2(yv.x + 0.0f, yv.y + 0.0f);
+        }
         const float2 x0o = hist[1][SO];
-        const float x0lr = lane_from_left(x0o.x);
-        const float x0li = lane_from_left(x0o.y);
         if (o >= r0 && o < r1 && a.occ_new) {  // row o: the new map and the skipped writes
-            const bool nz = out_lane && (yv.x != 0.0f || yv.y != 0.0f);
-            const uint64_t m = __ballot(nz);
-            if (m) {
-                // lanes NTV .. NTV + ow - 1 hold columns strip*ow + (lane - NTV): the lines
-                // touched by non-zero lanes, then one atomic OR per row and wave
-                const int c0 = strip * a.geo.ow - NTV;  // column of lane 0
-                uint64_t bits = 0;
-                const int l_last = min(c0 + NTV + a.geo.ow - 1, W - 1) >> 4;
-                for (int l = (c0 + NTV) >> 4; l <= l_last; ++l) {
-                    const int lo = max(16 * l - c0, NTV), hi = min(16 * l + 15 - c0, NTV + a.geo.ow - 1);
-                    if (lo > hi || lo > 63) continue;
-                    const uint64_t lanes = (hi >= 63 ? ~0ull : ((1ull << (hi + 1)) - 1ull)) & ~((1ull << lo) - 1ull);
-                    if (m & lanes) bits |= 1ull << l;
+            uint64_t m = 0ull;
+            if (live) {
+                m = __builtin_amdgcn_ballot_w64(out_lane && (yv.x != 0.0f || yv.y != 0.0f));
+                if (m) {
+                    // lanes NTV .. NTV + ow - 1 hold columns strip*ow + (lane - NTV): the lines
+                    // touched by non-zero lanes, then one atomic OR per row and wave
+                    const int c0 = strip * a.geo.ow - NTV;  // column of lane 0
+                    uint64_t bits = 0;
+                    const int l_last = min(c0 + NTV + a.geo.ow - 1, W - 1) >> 4;
+                    for (int l = (c0 + NTV) >> 4; l <= l_last; ++l) {
+                        const int lo = max(16 * l - c0, NTV), hi = min(16 * l + 15 - c0, NTV + a.geo.ow - 1);
+                        if (lo > hi || lo > 63) continue;
+                        const uint64_t lanes =
+                            (hi >= 63 ? ~0ull : ((1ull << (hi + 1)) - 1ull)) & ~((1ull << lo) - 1ull);
+                        if (m & lanes) bits |= 1ull << l;
+                    }
+                    if ((threadIdx.x & 63) == 0) atomicOr((unsigned long long*)&a.occ_new[zrow + o], bits);
                 }
-                if ((threadIdx.x & 63) == 0) atomicOr((unsigned long long*)&a.occ_new[zrow + o], bits);
             }
-            const bool keep = nz || ((oold[PAR] >> (ocol >> 4)) & 1ull);
-            if (out_lane && keep) *(float2*)(yp + (unsigned)o * rowb + oofs) = yv;
+            // a lane writes when its value is non-zero or its line holds an old non-zero; a
+            // wave with neither writes nothing
+            if (m | (oold[PAR] & outlines)) {
+                const bool keep = (yv.x != 0.0f || yv.y != 0.0f) || ((oold[PAR] >> (ocol >> 4)) & 1ull);
+                if (out_lane && keep) *(float2*)(yp + (unsigned)o * rowb + oofs) = yv;
+            }
         }
         if (o >= r0 && o < r1 && out_lane && !a.occ_new) *(float2*)(yp + (unsigned)o * rowb + oofs) = yv;
         // Every reduction term of row o is a product with (y - x0), (x0 - y), x0 or one of
@@ -286,22 +303,27 @@ template <int NTV> struct ProxWave {
         const uint32_t bitsor = (__float_as_uint(x0o.x) | __float_as_uint(x0o.y) | __float_as_uint(x0u.x) |
                                  __float_as_uint(x0u.y) | __float_as_uint(yv.x) | __float_as_uint(yv.y) |
                                  __float_as_uint(y0.x) | __float_as_uint(y0.y)) << 1;  // sign bits dropped
-        if (o >= r0 && o < r1 && __ballot(bitsor != 0u) && out_lane) {
-            const float2 g = hist[2][SO];
-            const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
-            acc_ls1 = fmaf(dr, g.x, fmaf(di, g.y, acc_ls1));
-            acc_ls2 = fmaf(dr, dr, fmaf(di, di, acc_ls2));
-            const float dre = (x0o.x - yv.x) * (yv.x - y0.x);
-            const float dim = (x0o.y - yv.y) * (yv.y - y0.y);
-            const int off = ((o * W) + ocol) & 1023;
-            acc_rs += off < 512 ? fmaf(2.0f, dim, dre) : dim;
-            // objective terms of x0: the hardware square root (about 1 ulp) feeds sums
-            // accumulated per wave and reported in double
-            const float m2 = fmaf(x0o.x, x0o.x, x0o.y * x0o.y);
-            acc_l1 += __builtin_amdgcn_sqrtf(m2);
-            acc_nnz += (x0o.x != 0.0f || x0o.y != 0.0f) ? 1.0f : 0.0f;
-            const float gxx = x0lr - x0o.x, gyx = x0u.x - x0o.x, gxy = x0li - x0o.y, gyy = x0u.y - x0o.y;
-            acc_tv += __builtin_amdgcn_sqrtf(fmaf(gxx, gxx, fmaf(gxy, gxy, fmaf(gyx, gyx, gyy * gyy))));
+        if (o >= r0 && o < r1 && __builtin_amdgcn_ballot_w64(bitsor != 0u)) {
+            // all lanes active here: the left neighbours' x0 for the TV term
+            const float x0lr = lane_from_left(x0o.x);
+            const float x0li = lane_from_left(x0o.y);
+            if (out_lane) {
+                const float2 g = hist[2][SO];
+                const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
+                acc_ls1 = fmaf(dr, g.x, fmaf(di, g.y, acc_ls1));
+                acc_ls2 = fmaf(dr, dr, fmaf(di, di, acc_ls2));
+                const float dre = (x0o.x - yv.x) * (yv.x - y0.x);
+                const float dim = (x0o.y - yv.y) * (yv.y - y0.y);
+                const int off = ((o * W) + ocol) & 1023;
+                acc_rs += off < 512 ? fmaf(2.0f, dim, dre) : dim;
+                // objective terms of x0: the hardware square root (about 1 ulp) feeds sums
+                // accumulated per wave and reported in double
+                const float m2 = fmaf(x0o.x, x0o.x, x0o.y * x0o.y);
+                acc_l1 += __builtin_amdgcn_sqrtf(m2);
+                acc_nnz += (x0o.x != 0.0f || x0o.y != 0.0f) ? 1.0f : 0.0f;
+                const float gxx = x0lr - x0o.x, gyx = x0u.x - x0o.x, gxy = x0li - x0o.y, gyy = x0u.y - x0o.y;
+                acc_tv += __builtin_amdgcn_sqrtf(fmaf(gxx, gxx, fmaf(gxy, gxy, fmaf(gyx, gyx, gyy * gyy))));
+            }
         }
     }
 
@@ -353,8 +375,11 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     PW st;
     // the wave's lines: the periodic wrap can add a line at the other end of the row
     st.wlines = 0;
-    for (int l = 0; l < (W >> 4); ++l)
+    st.outlines = 0;
+    for (int l = 0; l < (W >> 4); ++l) {
         if (__builtin_amdgcn_ballot_w64((xcol >> 4) == l)) st.wlines |= 1ull << l;
+        if (__builtin_amdgcn_ballot_w64(out_lane && (ocol >> 4) == l)) st.outlines |= 1ull << l;
+    }
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
