@@ -457,6 +457,52 @@ template <int NTV> static hipError_t launch_fl2d(const ProxDev& d, hipStream_t s
     return hipGetLastError();
 }
 
+template <int NTV> static int fl2d_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_prox_fl2d<NTV>, 64 * kProxWaves, 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+int prox_fl2d_auto_rows(int W, int H, int nz, int n_tv) {
+    int nb = 0;
+    switch (n_tv) {
+        case 0: nb = fl2d_blocks_per_cu<0>(); break;
+        case 1: nb = fl2d_blocks_per_cu<1>(); break;
+        case 2: nb = fl2d_blocks_per_cu<2>(); break;
+        case 3: nb = fl2d_blocks_per_cu<3>(); break;
+        case 4: nb = fl2d_blocks_per_cu<4>(); break;
+        case 5: nb = fl2d_blocks_per_cu<5>(); break;
+        case 6: nb = fl2d_blocks_per_cu<6>(); break;
+        case 7: nb = fl2d_blocks_per_cu<7>(); break;
+        case 8: nb = fl2d_blocks_per_cu<8>(); break;
+        case 9: nb = fl2d_blocks_per_cu<9>(); break;
+        case 10: nb = fl2d_blocks_per_cu<10>(); break;
+        default: return 0;
+    }
+    int dev = 0, cus = 0;
+    if (nb <= 0 || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 0;
+    const long slots = (long)nb * kProxWaves * cus;  // resident waves
+    const int nstrips = prox_geom(W, H, n_tv, H).nstrips;
+    long best_cost = -1;
+    int best_rc = 0;
+    for (int nch = 1; nch <= H; ++nch) {
+        const int rc = (H + nch - 1) / nch;
+        if (rc < 32 && nch > 1) break;
+        const int chunks = (H + rc - 1) / rc;
+        const long waves = (long)nstrips * chunks * nz;
+        const long rounds = (waves + slots - 1) / slots;
+        const long cost = rounds * (rc + 2 * n_tv + 1);
+        if (best_cost < 0 || cost < best_cost) {  // ties keep the longer chunk (less halo)
+            best_cost = cost;
+            best_rc = rc;
+        }
+    }
+    return best_rc;
+}
+
 hipError_t prox_fl2d(const ProxLaunch& a, hipStream_t s) {
     // row offsets inside a plane are 32-bit byte offsets
     if ((size_t)a.W * a.H * sizeof(float2) > 0xffffffffull) return hipErrorInvalidValue;

@@ -492,7 +492,10 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_HIP(hipEventCreateWithFlags(&c->ev_scal, hipEventDisableTiming));
     int zc = 32;
     if (const char* e = std::getenv("RIHVR_ZCHUNK")) zc = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("RIHVR_PROX_ROWS")) c->prox_rows = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("RIHVR_PROX_ROWS"))
+        c->prox_rows = std::max(0, std::atoi(e));
+    else if (params->num_tv_iterations >= 0 && params->num_tv_iterations <= 10)
+        c->prox_rows = prox_fl2d_auto_rows(c->W, c->H, c->Nz, params->num_tv_iterations);
     c->zchunk_fwd = std::min(c->Nz, zc);
     c->nparts = (c->Nz + c->zchunk_fwd - 1) / c->zchunk_fwd;
     int za = 16;
