@@ -221,16 +221,20 @@ template <int NTV> struct ProxWave {
             const float axi = fmaf(a.sc, lane_from_left(y.y) - y.y, go.y);
             const float ayr = fmaf(a.sc, yu.x - y.x, go.z);
             const float ayi = fmaf(a.sc, yu.y - y.y, go.w);
-            const float n2 = fmaf(axr, axr, fmaf(axi, axi, fmaf(ayr, ayr, ayi * ayi)));
-            // projection: divide by max(|a|, 1), i.e. by |a| where n2 > 1 and by exactly 1
-            // elsewhere; a wave with no lane above 1 (nearly every wave of a sparse solve)
-            // skips the reciprocal square root and the products
-            const bool proj = n2 > 1.0f;
-            if (__builtin_amdgcn_ballot_w64(proj)) {
-                const float inv = proj ? __builtin_amdgcn_rsqf(n2) : 1.0f;
-                gd[PAR][k] = make_float4(axr * inv, axi * inv, ayr * inv, ayi * inv);
-            } else {
-                gd[PAR][k] = make_float4(axr, axi, ayr, ayi);
+            // projection: divide by max(|a|, 1), i.e. by |a| where |a|^2 > 1 and by exactly 1
+            // elsewhere.  Components all within 0.5 give |a|^2 <= 1 (also after rounding), so a
+            // wave where every lane's are (nearly every wave of a sparse solve) neither forms
+            // |a|^2 nor projects; otherwise a wave with no lane above 1 skips the reciprocal
+            // square root and the products.
+            gd[PAR][k] = make_float4(axr, axi, ayr, ayi);
+            const float amax = fmaxf(fmaxf(fabsf(axr), fabsf(axi)), fmaxf(fabsf(ayr), fabsf(ayi)));
+            if (__builtin_amdgcn_ballot_w64(amax > 0.5f)) {
+                const float n2 = fmaf(axr, axr, fmaf(axi, axi, fmaf(ayr, ayr, ayi * ayi)));
+                const bool proj = n2 > 1.0f;
+                if (__builtin_amdgcn_ballot_w64(proj)) {
+                    const float inv = proj ? __builtin_amdgcn_rsqf(n2) : 1.0f;
+                    gd[PAR][k] = make_float4(axr * inv, axi * inv, ayr * inv, ayi * inv);
+                }
             }
             yh[PAR][k] = y;
         }
