@@ -18,7 +18,7 @@
 // and every neighbour in x is a lane shuffle.  Rows and columns outside the
 // strip's output range are recomputed halo (n_tv on each side in x, and at the
 // ends of the row chunk in y).  No workgroup barriers and no LDS: the per-lane
-// history of x-hat, x0, g and y_{k-1} lives in registers.
+// history of x-hat and x0 lives in registers.
 #include <utility>
 
 #include "device_common.hpp"
@@ -107,7 +107,7 @@ __device__ __forceinline__ float soft_fast(float v, float thr) {
 
 // One wave's state: the TV duals g_k (gx.re, gx.im, gy.re, gy.im) and y-hat_k of
 // the last two rows of every level, indexed [row parity][level]; a per-lane
-// history ring holds x-hat, x0, g and y_{k-1} of the last NTV + 3 rows.  The row
+// history ring holds x-hat and x0 of the last NTV + 3 rows.  The row
 // loop is unrolled by U (a multiple of the ring length and of 2), so every ring
 // slot and row parity is a compile-time constant and the ring is registers.
 template <int NTV> struct ProxWave {
@@ -117,8 +117,11 @@ template <int NTV> struct ProxWave {
     float2 yh[2][NTV + 1];
     // rows tau and tau + 1 of y_{k-1}, y_{k-2} and g, loaded two steps ahead
     float2 pf[2][3];
-    // x-hat, x0, g and y_{k-1} of the last P rows, indexed by compile-time slots: registers
-    float2 hist[4][P];
+    // x-hat and x0 of the last P rows, indexed by compile-time slots: registers
+    float2 hist[2][P];
+    // bit j: row tau + 2 - j of y_{k-1} may hold a non-zero in the wave's lines (its map says
+    // so, or it has no map), shifted in as each row is loaded
+    uint32_t y0rows = 0;
     // occupancy words (wave-uniform, kept in scalar registers), [row parity]: of y_{k-1} and
     // y_{k-2} for the rows loaded next, and of the output's current content for the rows
     // written next
@@ -191,12 +194,12 @@ template <int NTV> struct ProxWave {
             const float2 x0 = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
             hist[0][SLOT] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
             hist[1][SLOT] = x0;
-            hist[2][SLOT] = g;
-            hist[3][SLOT] = y0;
             // the output's current map for the row written in the next step
             if (a.occ_old) oold[Q] = uniform_word(a.occ_old, zrow + row_old);
             row_old = next_row(row_old, a.H);
-            if (tau + 2 <= tau_end) load_row(a, PAR, xoff);
+            const bool ld = tau + 2 <= tau_end;
+            y0rows = (y0rows << 1) | ((ld && (ow0[PAR] & wlines)) ? 1u : 0u);
+            if (ld) load_row(a, PAR, xoff);
             if (tau + 3 <= tau_end) load_words(a, Q, zrow);
         }
         // ---- TV levels: level k works on row tau - k ----
@@ -303,16 +306,19 @@ template <int NTV> struct ProxWave {
         // halo lanes included (they are the left neighbours).  Late in a solve almost every
         // row is.
         const float2 x0u = hist[1][SU];
-        const float2 y0 = hist[3][SO];
         const uint32_t bitsor = (__float_as_uint(x0o.x) | __float_as_uint(x0o.y) | __float_as_uint(x0u.x) |
-                                 __float_as_uint(x0u.y) | __float_as_uint(yv.x) | __float_as_uint(yv.y) |
-                                 __float_as_uint(y0.x) | __float_as_uint(y0.y)) << 1;  // sign bits dropped
-        if (o >= r0 && o < r1 && __builtin_amdgcn_ballot_w64(bitsor != 0u)) {
-            // all lanes active here: the left neighbours' x0 for the TV term
+                                 __float_as_uint(x0u.y) | __float_as_uint(yv.x) | __float_as_uint(yv.y)) << 1;
+        const bool y0row = (y0rows >> (NTV + 3)) & 1u;  // row o of y_{k-1} (sign bits dropped above)
+        if (o >= r0 && o < r1 && (y0row || __builtin_amdgcn_ballot_w64(bitsor != 0u))) {
+            // all lanes active here: the left neighbours' x0 for the TV term, and g and
+            // y_{k-1} of row o read again (they were loaded NTV + 1 rows ago; rows that reduce
+            // are few, so they are not kept in registers)
             const float x0lr = lane_from_left(x0o.x);
             const float x0li = lane_from_left(x0o.y);
+            const unsigned ro = (unsigned)o * rowb;
+            const float2 g = at(gp, ro, xoff);
+            const float2 y0 = at(y0p, ro, xoff);
             if (out_lane) {
-                const float2 g = hist[2][SO];
                 const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
                 acc_ls1 = fmaf(dr, g.x, fmaf(di, g.y, acc_ls1));
                 acc_ls2 = fmaf(dr, dr, fmaf(di, di, acc_ls2));
@@ -404,8 +410,10 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     st.row_old = PW::wrap_row(tau0 - NTV, a.H);
     st.load_words(a, 0, zrow);  // tau0
     st.load_words(a, 1, zrow);  // tau0 + 1
-    st.load_row(a, 0, xoff);    // tau0
-    st.load_row(a, 1, xoff);    // tau0 + 1
+    st.y0rows = (st.ow0[0] & st.wlines) ? 1u : 0u;
+    st.load_row(a, 0, xoff);  // tau0
+    st.y0rows = (st.y0rows << 1) | ((st.ow0[1] & st.wlines) ? 1u : 0u);
+    st.load_row(a, 1, xoff);  // tau0 + 1
     st.load_words(a, 0, zrow);  // tau0 + 2
     st.oold[0] = PW::uniform_word(a.occ_old, zrow + PW::wrap_row(tau0 - NTV - 1, a.H));
     int tau = tau0;
