@@ -224,6 +224,23 @@ def test_solve_other_sizes_match_oracle(pkg, orc, scenes, gpu_available, W, H, N
     assert rel(est, rest) < 1e-4
 
 
+@pytest.mark.parametrize("ntv", [1, 8, 10])
+def test_solve_mode7_tv_iterations_match_oracle(pkg, orc, scenes, gpu_available, ntv):
+    """Mode 7 with 1, 8 and 10 TV iterations on a 128x96 plane: the prox's halo (64 - 2 n_tv
+    output columns per wave, wrapped periodically at the plane's edges), several row chunks
+    per plane, and the occupancy maps of a sparse solve (rows of y_{k-1} whose lines are all
+    empty skipped, the map bits that stand for y_{k-1} in the reduction-skip test)."""
+    c = cfg(128, 96, 6, its=8, reg=0.1, tv=0.05, ntv=ntv)  # 2-3 % of the voxels non-zero
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 6, 2.0, 4.0, 21 + ntv, 0.01)
+    vol, st, ref, rst, (est, res, rest, rres) = _solve_both(pkg, orc, c, holo)
+    assert rel(vol, ref) < 1e-4, rel(vol, ref)
+    assert list(st["restarts"]) == list(rst["restarts"])
+    assert list(st["backtracks"]) == list(rst["backtracks"])
+    assert _support_mismatch(vol, ref, 0) <= max(2, int(1e-4 * vol.size))
+    np.testing.assert_allclose(st["f"], rst["f"], rtol=1e-4)
+    assert 0 < st["nnz"] < vol.size // 10  # sparse: the maps are in use
+
+
 def test_device_pointers_match_host(pkg, scenes, gpu_available):
     import torch
     c = cfg(64, 64, 4, its=3)
