@@ -72,10 +72,11 @@ struct ProxLaunch {
     int n_tv;
     int row_chunk;      // output rows per wave of the mode-7 prox (<= 0: default)
     double* partial;    // [nwaves][PS_COUNT]
-    // mode 7 only, all optional ([nz][H] words, see the line occupancy above): the maps of
+    // mode 7 only, all required ([nz][H] words, see the line occupancy above): the maps of
     // Y0 and YM (their clear lines are not read), of Y's current content (its zero lines
-    // with a clear bit are not rewritten; NULL: write every line) and the map of the new
-    // Y (OR-ed in; must start cleared).  The output's zeros are +0.
+    // with a clear bit are not rewritten) and the map of the new Y (OR-ed in; must start
+    // cleared).  A map with every bit set stands for a volume without one.  The output's
+    // zeros are +0.
     const uint64_t* occ_y0 = nullptr;
     const uint64_t* occ_ym = nullptr;
     const uint64_t* occ_old = nullptr;

@@ -70,7 +70,7 @@ struct ProxDev {
     ProxGeom geo;
     int nwaves;
     double* partial;
-    // line occupancy ([nz][H] words; NULL: none) of Y0, YM, Y's current content and new Y
+    // line occupancy ([nz][H] words, all required) of Y0, YM, Y's current content and new Y
     const uint64_t* occ_y0;
     const uint64_t* occ_ym;
     const uint64_t* occ_old;
@@ -119,9 +119,10 @@ template <int NTV> struct ProxWave {
     float2 pf[2][3];
     // x-hat and x0 of the last P rows, indexed by compile-time slots: registers
     float2 hist[2][P];
-    // bit j: row tau + 2 - j of y_{k-1} may hold a non-zero in the wave's lines (its map says
-    // so, or it has no map), shifted in as each row is loaded
-    uint32_t y0rows = 0;
+    // Row flags, shifted in as each row is loaded (after the load of row tau + 2, bit j is
+    // row tau + 2 - j): y0rows, y_{k-1} may hold a non-zero in the wave's lines (its map says
+    // so, or it has no map); xrows, y_{k-1} or y_{k-2} may, so x0 may be non-zero.
+    uint32_t y0rows = 0, xrows = 0;
     // occupancy words (wave-uniform, kept in scalar registers), [row parity]: of y_{k-1} and
     // y_{k-2} for the rows loaded next, and of the output's current content for the rows
     // written next
@@ -139,7 +140,6 @@ template <int NTV> struct ProxWave {
     // constant address space (its own counter, so it never waits on the vector loads in
     // flight for the rows ahead).
     static __device__ __forceinline__ uint64_t uniform_word(const uint64_t* map, int i) {
-        if (!map) return ~0ull;
         const __attribute__((address_space(4))) uint64_t* cmap =
             (const __attribute__((address_space(4))) uint64_t*)map;
         return cmap[i];
@@ -169,11 +169,19 @@ template <int NTV> struct ProxWave {
     // set in its map: a line whose bit is clear holds +0, so a wave that reads the row with
     // some of its lines clear reads zeros there, and a wave with all its lines clear takes
     // zeros.
-    __device__ __forceinline__ void load_row(const ProxDev& a, int slot, unsigned xoff) {
+    // A row with neither is not read at all (its x0 is +0 and is not formed from pf).
+    __device__ __forceinline__ void load_row(const ProxDev& a, int slot, unsigned xoff, bool ld) {
+        const bool f0 = ld && (ow0[slot] & wlines) != 0ull;
+        const bool fm = ld && (owm[slot] & wlines) != 0ull;
+        y0rows = (y0rows << 1) | (f0 ? 1u : 0u);
+        xrows = (xrows << 1) | ((f0 || fm) ? 1u : 0u);
+        if (!ld) return;
         const unsigned off = (unsigned)row_ld * rowb;
         const float2 zero = make_float2(0.0f, 0.0f);
-        pf[slot][0] = (ow0[slot] & wlines) ? at(y0p, off, xoff) : zero;
-        pf[slot][1] = (owm[slot] & wlines) ? at(ymp, off, xoff) : zero;
+        if (f0 || fm) {
+            pf[slot][0] = f0 ? at(y0p, off, xoff) : zero;
+            pf[slot][1] = fm ? at(ymp, off, xoff) : zero;
+        }
         pf[slot][2] = at(gp, off, xoff);
         row_ld = next_row(row_ld, a.H);
     }
@@ -190,16 +198,18 @@ template <int NTV> struct ProxWave {
         const unsigned oofs = (unsigned)ocol * (unsigned)sizeof(float2);  // output lanes only
         // ---- row tau: momentum point x0 = y0 + s (y0 - ym), x-hat = x0 - t g ----
         {
-            const float2 y0 = pf[PAR][0], ym = pf[PAR][1], g = pf[PAR][2];
-            const float2 x0 = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
+            const float2 g = pf[PAR][2];
+            float2 x0 = make_float2(0.0f, 0.0f);  // y0 = ym = +0 give x0 = +0
+            if ((xrows >> 1) & 1u) {              // row tau (its load was shifted in 2 rows ago)
+                const float2 y0 = pf[PAR][0], ym = pf[PAR][1];
+                x0 = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
+            }
             hist[0][SLOT] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
             hist[1][SLOT] = x0;
             // the output's current map for the row written in the next step
-            if (a.occ_old) oold[Q] = uniform_word(a.occ_old, zrow + row_old);
+            oold[Q] = uniform_word(a.occ_old, zrow + row_old);
             row_old = next_row(row_old, a.H);
-            const bool ld = tau + 2 <= tau_end;
-            y0rows = (y0rows << 1) | ((ld && (ow0[PAR] & wlines)) ? 1u : 0u);
-            if (ld) load_row(a, PAR, xoff);
+            load_row(a, PAR, xoff, tau + 2 <= tau_end);
             if (tau + 3 <= tau_end) load_words(a, Q, zrow);
         }
         // ---- TV levels: level k works on row tau - k ----
@@ -272,7 +282,7 @@ template <int NTV> struct ProxWave {
             yv = make_float2(yv.x + 0.0f, yv.y + 0.0f);
         }
         const float2 x0o = hist[1][SO];
-        if (o >= r0 && o < r1 && a.occ_new) {  // row o: the new map and the skipped writes
+        if (o >= r0 && o < r1) {  // row o: the new map and the skipped writes
             uint64_t m = 0ull;
             if (live) {
                 m = __builtin_amdgcn_ballot_w64(out_lane && (yv.x != 0.0f || yv.y != 0.0f));
@@ -299,17 +309,18 @@ template <int NTV> struct ProxWave {
                 if (out_lane && keep) *(float2*)(yp + (unsigned)o * rowb + oofs) = yv;
             }
         }
-        if (o >= r0 && o < r1 && out_lane && !a.occ_new) *(float2*)(yp + (unsigned)o * rowb + oofs) = yv;
         // Every reduction term of row o is a product with (y - x0), (x0 - y), x0 or one of
         // the x0 differences, so it is zero (adding it changes no sum, which starts at +0)
         // when y, x0, y_{k-1} and the row above's x0 are zero in every lane of the wave,
         // halo lanes included (they are the left neighbours).  Late in a solve almost every
         // row is.
-        const float2 x0u = hist[1][SU];
-        const uint32_t bitsor = (__float_as_uint(x0o.x) | __float_as_uint(x0o.y) | __float_as_uint(x0u.x) |
-                                 __float_as_uint(x0u.y) | __float_as_uint(yv.x) | __float_as_uint(yv.y)) << 1;
-        const bool y0row = (y0rows >> (NTV + 3)) & 1u;  // row o of y_{k-1} (sign bits dropped above)
-        if (o >= r0 && o < r1 && (y0row || __builtin_amdgcn_ballot_w64(bitsor != 0u))) {
+        // x0 of rows o and o - 1 and y_{k-1} of row o by their row flags, y by its value
+        const bool xflags = ((xrows >> (NTV + 3)) & 3u) != 0u || ((y0rows >> (NTV + 3)) & 1u) != 0u;
+        bool reduce = o >= r0 && o < r1 && xflags;
+        if (live && o >= r0 && o < r1 && !xflags)
+            reduce = __builtin_amdgcn_ballot_w64(((__float_as_uint(yv.x) | __float_as_uint(yv.y)) << 1) != 0u) != 0ull;
+        if (reduce) {
+            const float2 x0u = hist[1][SU];
             // all lanes active here: the left neighbours' x0 for the TV term, and g and
             // y_{k-1} of row o read again (they were loaded NTV + 1 rows ago; rows that reduce
             // are few, so they are not kept in registers)
@@ -410,10 +421,8 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     st.row_old = PW::wrap_row(tau0 - NTV, a.H);
     st.load_words(a, 0, zrow);  // tau0
     st.load_words(a, 1, zrow);  // tau0 + 1
-    st.y0rows = (st.ow0[0] & st.wlines) ? 1u : 0u;
-    st.load_row(a, 0, xoff);  // tau0
-    st.y0rows = (st.y0rows << 1) | ((st.ow0[1] & st.wlines) ? 1u : 0u);
-    st.load_row(a, 1, xoff);  // tau0 + 1
+    st.load_row(a, 0, xoff, true);  // tau0
+    st.load_row(a, 1, xoff, true);  // tau0 + 1
     st.load_words(a, 0, zrow);  // tau0 + 2
     st.oold[0] = PW::uniform_word(a.occ_old, zrow + PW::wrap_row(tau0 - NTV - 1, a.H));
     int tau = tau0;
@@ -516,8 +525,9 @@ int prox_fl2d_auto_rows(int W, int H, int nz, int n_tv) {
 }
 
 hipError_t prox_fl2d(const ProxLaunch& a, hipStream_t s) {
-    // row offsets inside a plane are 32-bit byte offsets
+    // row offsets inside a plane are 32-bit byte offsets; every map is required
     if ((size_t)a.W * a.H * sizeof(float2) > 0xffffffffull) return hipErrorInvalidValue;
+    if (!a.occ_y0 || !a.occ_ym || !a.occ_old || !a.occ_new) return hipErrorInvalidValue;
     ProxDev d = make_dev(a);
     switch (a.n_tv) {
         case 0: return launch_fl2d<0>(d, s);

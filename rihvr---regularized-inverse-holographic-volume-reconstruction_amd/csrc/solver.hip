@@ -98,6 +98,7 @@ struct rihvr_ctx {
     uint64_t* occ[3] = {nullptr, nullptr, nullptr};
     bool occ_valid[3] = {false, false, false};
     uint64_t* occ_tmp = nullptr;                  // the map being built by a prox
+    uint64_t* occ_ones = nullptr;                 // every line set: stands for "no map"
     uint8_t* smap = nullptr;                      // [Nz][W/16] strips of A's column pass
     float2* G = nullptr;                          // A^T r
     float2* G_alt = nullptr;                      // the next iteration's A^T r, formed ahead
@@ -365,14 +366,15 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
     int rows;
     const int ky = ring_index(c, Y);
     if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) {
-        // the new map is built in occ_tmp, then becomes Y's
-        a.occ_y0 = occ_of(c, Y0);
-        a.occ_ym = occ_of(c, YM);
-        if (ky >= 0) {
-            a.occ_old = c->occ_valid[ky] ? c->occ[ky] : nullptr;
-            a.occ_new = c->occ_tmp;
-            RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
-        }
+        // the new map is built in occ_tmp, then becomes Y's; a volume without a valid map
+        // passes the all-ones map (every line read, every line of Y rewritten)
+        const uint64_t* o0 = occ_of(c, Y0);
+        const uint64_t* om = occ_of(c, YM);
+        a.occ_y0 = o0 ? o0 : c->occ_ones;
+        a.occ_ym = om ? om : c->occ_ones;
+        a.occ_old = (ky >= 0 && c->occ_valid[ky]) ? c->occ[ky] : c->occ_ones;
+        a.occ_new = c->occ_tmp;
+        RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
         rows = prox_fl2d_num_waves(c->W, c->H, c->Nz, a.n_tv, c->prox_rows);
         if (ky >= 0) {
@@ -416,8 +418,8 @@ rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, fl
     const uint64_t* o0 = occ_of(c, Y0);
     const uint64_t* om = occ_of(c, YM);
     const int ky = ring_index(c, Yn);
-    const uint64_t* oold = (ky >= 0 && c->occ_valid[ky]) ? c->occ[ky] : nullptr;
-    if (ky >= 0) RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
+    const uint64_t* oold = (ky >= 0 && c->occ_valid[ky]) ? c->occ[ky] : c->occ_ones;
+    RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
     for (int g = 0, g0 = 0; g < c->n_groups; ++g, g0 += c->pipe_planes) {
         const int gz = std::min(c->pipe_planes, c->Nz - g0);
         const size_t off = (size_t)g0 * c->N;
@@ -429,10 +431,10 @@ rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, fl
         a.Y0 = Y0 + off; a.YM = YM + off; a.G = c->G + off; a.Y = Yn + off;
         a.nz = gz;
         a.partial = c->partial + (size_t)g0 * per_plane * PS_COUNT;
-        a.occ_y0 = o0 ? o0 + roff : nullptr;
-        a.occ_ym = om ? om + roff : nullptr;
-        a.occ_old = oold ? oold + roff : nullptr;
-        a.occ_new = ky >= 0 ? c->occ_tmp + roff : nullptr;
+        a.occ_y0 = (o0 ? o0 : c->occ_ones) + roff;
+        a.occ_ym = (om ? om : c->occ_ones) + roff;
+        a.occ_old = oold + roff;
+        a.occ_new = c->occ_tmp + roff;
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
         RIHVR_TRY(forward_group(c, Yn, g0, gz, c->stream, ky >= 0 ? c->occ_tmp : nullptr));
     }
@@ -460,6 +462,7 @@ void release(rihvr_ctx* c) {
     for (auto& b : c->Yb) (void)hipFree(b);
     for (auto& m : c->occ) (void)hipFree(m);
     (void)hipFree(c->occ_tmp);
+    (void)hipFree(c->occ_ones);
     (void)hipFree(c->smap);
     void* ptrs[] = {c->G, c->G_alt, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->spart, c->kappa,
                     c->zlist, c->phase, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
@@ -519,6 +522,8 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     for (auto& b : c->Yb) RIHVR_TRY(dalloc(&b, c->V));
     for (auto& m : c->occ) RIHVR_TRY(dalloc(&m, (size_t)c->Nz * c->H));
     RIHVR_TRY(dalloc(&c->occ_tmp, (size_t)c->Nz * c->H));
+    RIHVR_TRY(dalloc(&c->occ_ones, (size_t)c->Nz * c->H));
+    RIHVR_HIP(hipMemset(c->occ_ones, 0xff, (size_t)c->Nz * c->H * sizeof(uint64_t)));
     RIHVR_TRY(dalloc(&c->smap, (size_t)c->Nz * (c->W / 16)));
     RIHVR_TRY(dalloc(&c->G, c->V));
     RIHVR_TRY(dalloc(&c->G_alt, c->V));
