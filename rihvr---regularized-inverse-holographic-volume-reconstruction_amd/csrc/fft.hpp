@@ -54,6 +54,18 @@ RIHVR_PLAN(896, 16, 8, 7)
 RIHVR_PLAN(1024, 16, 8, 8)
 #undef RIHVR_PLAN
 
+// Plans for row transforms (one transform per 1 or more waves, several transforms per
+// workgroup).  Where the row plan has a smaller first radix than the column plan, a
+// thread holds fewer elements, so the row kernels -- which keep per-element data in
+// registers across planes -- fit more waves per SIMD.  Column kernels keep FftPlan (their
+// workgroup of 16 transforms must stay within 1024 threads).
+template <int L> struct RowPlan : FftPlan<L> {};
+template <> struct RowPlan<1024> {
+    using radices = Radices<8, 8, 8, 2>;
+    static constexpr int first = 8;
+    static constexpr int TPT = 1024 / 8;
+};
+
 // Multiply by W4 = e^{SIGN * i pi/2}: -i for the forward transform, +i inverse.
 template <int SIGN> __device__ __forceinline__ float2 mul_w4(float2 a) {
     return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
@@ -219,11 +231,11 @@ template <int L, int TPT, int SIGN, int NS, int R, int... REST> struct FftPasses
     }
 };
 
-template <int L, int SIGN, class Rad> struct FftRun;
-template <int L, int SIGN, int... R> struct FftRun<L, SIGN, Radices<R...>> {
+template <int L, int SIGN, class Rad, int TPT = FftPlan<L>::TPT> struct FftRun;
+template <int L, int SIGN, int... R, int TPT> struct FftRun<L, SIGN, Radices<R...>, TPT> {
     template <class Src, class Dst, class Lds>
     __device__ static __forceinline__ void run(const float2* tw, int t, Src&& src, Dst&& dst, Lds&& lds) {
-        FftPasses<L, FftPlan<L>::TPT, SIGN, 1, R...>::run(tw, t, src, dst, lds);
+        FftPasses<L, TPT, SIGN, 1, R...>::run(tw, t, src, dst, lds);
     }
 };
 
@@ -234,9 +246,9 @@ template <int R0> struct LastRadix<Radices<R0>> { static constexpr int value = R
 template <int R0, int R1, int... Rest> struct LastRadix<Radices<R0, R1, Rest...>> {
     static constexpr int value = LastRadix<Radices<R1, Rest...>>::value;
 };
-template <int L> struct LastPass {
-    static constexpr int R = LastRadix<typename FftPlan<L>::radices>::value;
-    static constexpr int TPT = FftPlan<L>::TPT;
+template <int L, class Plan = FftPlan<L>> struct LastPass {
+    static constexpr int R = LastRadix<typename Plan::radices>::value;
+    static constexpr int TPT = Plan::TPT;
     static constexpr int NBF = L / R;
     static constexpr int NB = (NBF + TPT - 1) / TPT;
 };

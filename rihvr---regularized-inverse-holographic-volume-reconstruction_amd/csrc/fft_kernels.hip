@@ -12,7 +12,7 @@ namespace rihvr {
 constexpr int kColStrip = 16;  // columns per column-kernel workgroup
 
 template <int L> struct RowCfg {
-    static constexpr int TPT = FftPlan<L>::TPT;
+    static constexpr int TPT = RowPlan<L>::TPT;
     static constexpr int RPW = (256 / TPT) > 0 ? (256 / TPT) : 1;  // rows per workgroup
     static constexpr int THREADS = TPT * RPW;
     // LDS row: element j at j + j/16 (one pad slot per 16 elements breaks the
@@ -56,7 +56,7 @@ struct RowArgs {
 template <int L, int SIGN, int MODE>
 __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
     using C = RowCfg<L>;
-    using Plan = FftPlan<L>;
+    using Plan = RowPlan<L>;
     __shared__ float2 tw_s[L];
     __shared__ float2 buf[C::RPW * C::LP];
     load_twiddles<L>(tw_s, a.twg);
@@ -72,13 +72,13 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
     if constexpr (MODE == ROW_PLAIN) {
         const float2* src_row = a.in + (size_t)rrow * L;
         float2* dst_row = a.out + (size_t)rrow * L;
-        FftRun<L, SIGN, typename Plan::radices>::run(
+        FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
             tw_s, t, [&](int, int, int j) { return src_row[j]; },
             [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
     } else if constexpr (MODE == ROW_SUMPART) {
         const float2* src_row = a.in + (size_t)rrow * L;
         float2* dst_row = a.out + (size_t)rrow * L;
-        FftRun<L, SIGN, typename Plan::radices>::run(
+        FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
             tw_s, t,
             [&](int, int, int j) {
                 float2 s = src_row[j];
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
         // A, second pass: forward row FFT of the column-transformed planes of a
         // chunk of planes; epilogue H_{-z} / (Nx Ny) summed over the chunk
         // (sparse_rs_phase_mult_kernel + sparse_add_plane_kernel)
-        using LPs = LastPass<L>;
+        using LPs = LastPass<L, Plan>;
         float2 acc[LPs::NB][LPs::R];
         double kp[LPs::NB][LPs::R];
         const double* krow_p = a.kappa + (size_t)rrow * L;
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
             if (!smask) continue;  // the same for every wave: no barrier is skipped by part
             const double mz = -(double)a.zlist[z];
             const float2* src_row = a.in + ((size_t)z * a.nrows + rrow) * L;
-            FftRun<L, SIGN, typename Plan::radices>::run(
+            FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
                 tw_s, t,
                 [&](int, int, int j) {
                     return ((smask >> (j >> 4)) & 1ull) ? src_row[j] : make_float2(0.0f, 0.0f);
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
         for (int z = z0; z < z1; ++z) {
             const double zz = (double)a.zlist[z];
             float2* dst_row = a.out + ((size_t)z * a.nrows + rrow) * L;
-            FftRun<L, SIGN, typename Plan::radices>::run(
+            FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
                 tw_s, t,
                 [&](int i, int r, int) {
                     float2 h = transfer(zz, kp[i][r]);
