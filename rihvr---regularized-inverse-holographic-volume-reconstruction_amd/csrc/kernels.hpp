@@ -117,6 +117,14 @@ hipError_t holo_momentum(const float2* cur, const float2* prev, double s_mom, co
 // resid = est - meas with partial sum |resid|^2
 hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, long n, double* partial,
                          hipStream_t s);
+// Spectrum of A's real estimate from nparts partial spectra [nparts][H][W] (stride apart):
+// est = (S(k) + conj S(-k)) / 2 with S = sum of the parts; partial[block] = sum |est - bspec|^2
+// over holo_spec_num_blocks(W, H) blocks (W and H even)
+int holo_spec_num_blocks(int W, int H);
+hipError_t holo_spec(const float2* parts, int nparts, long stride, int W, int H, const float2* bspec, float2* est,
+                     double* partial, hipStream_t s);
+// out = f * in
+hipError_t holo_scale(const float2* in, float2* out, long n, float f, hipStream_t s);
 
 // sums rows of partial [nrows][ncols] into out[ncols] (fixed order: deterministic);
 // work holds kReduceBlocks * ncols doubles

@@ -806,7 +806,8 @@ hipError_t momentum_out(const float2* Y, const float2* Y0, float s_mom, float2* 
 // ----------------------------------------------------------------------------
 // Hologram-sized kernels
 // ----------------------------------------------------------------------------
-int holo_num_blocks(long n) { return (int)((n + (long)kEwThreads * kEwItems - 1) / ((long)kEwThreads * kEwItems)); }
+constexpr int kHoloItems = kEwItems;  // elements per thread
+int holo_num_blocks(long n) { return (int)((n + (long)kEwThreads * kHoloItems - 1) / ((long)kEwThreads * kHoloItems)); }
 
 __global__ void __launch_bounds__(kEwThreads)
 k_holo_momentum(const float2* __restrict__ cur, const float2* __restrict__ prev, double s_mom,
@@ -817,8 +818,8 @@ k_holo_momentum(const float2* __restrict__ cur, const float2* __restrict__ prev,
     // host's alpha update gives, so the step can be chosen before the host has the sum
     if (restart_sum && *restart_sum > 0) s_mom = 0.0;
     double acc = 0;
-    for (int it = 0; it < kEwItems; ++it) {
-        long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
+    for (int it = 0; it < kHoloItems; ++it) {
+        long i = ((long)blockIdx.x * kHoloItems + it) * kEwThreads + threadIdx.x;
         if (i >= n) break;
         const float2 c = cur[i], p = prev[i], m = meas[i];
         // sparse_fistaUpdate_kernel on holograms: double step, float storage
@@ -845,8 +846,8 @@ k_holo_residual(const float2* __restrict__ est, const float2* __restrict__ meas,
                 double* partial) {
     __shared__ double red[kEwThreads / 64];
     double acc = 0;
-    for (int it = 0; it < kEwItems; ++it) {
-        long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
+    for (int it = 0; it < kHoloItems; ++it) {
+        long i = ((long)blockIdx.x * kHoloItems + it) * kEwThreads + threadIdx.x;
         if (i >= n) break;
         const float2 e = est[i], m = meas[i];
         float2 r = make_float2(e.x - m.x, e.y - m.y);
@@ -861,6 +862,67 @@ hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, l
                          hipStream_t s) {
     hipLaunchKernelGGL(k_holo_residual, dim3(holo_num_blocks(n)), dim3(kEwThreads), 0, s, est, meas, resid, n,
                        partial);
+    return hipGetLastError();
+}
+
+// The spectrum of A's real-valued estimate from the partial spectra S_p of its plane chunks:
+// est = Re(IFFT(S)) with S = sum_p S_p has FFT(est) / N = (S(k) + conj S(-k)) / 2 (the real part
+// of a transform is the Hermitian-symmetric half of its spectrum), written to est; with the
+// spectrum of the measurement bspec (FFT(b) / N), the residual's is est - bspec, and
+// partial[block] = sum |est - bspec|^2 (N times that is |A x - b|^2, by Parseval).  A thread
+// takes an element k of rows 0 .. H/2 whose mirror -k is not before it and writes both
+// (E(-k) = conj E(k)); n = (H/2 + 1) W elements are visited (H and W even).
+// One pair per thread, so that its loads are all in flight together.
+int holo_spec_num_blocks(int W, int H) { return (int)(((long)(H / 2 + 1) * W + kEwThreads - 1) / kEwThreads); }
+
+__global__ void __launch_bounds__(kEwThreads)
+k_holo_spec(const float2* __restrict__ parts, int nparts, long stride, int W, int H,
+            const float2* __restrict__ bspec, float2* __restrict__ est, long n, double* partial) {
+    __shared__ double red[kEwThreads / 64];
+    double acc = 0;
+    const long i = (long)blockIdx.x * kEwThreads + threadIdx.x;
+    const int ky = (int)(i / W), kx = (int)(i - (long)ky * W);
+    const long m = (long)(ky ? H - ky : 0) * W + (kx ? W - kx : 0);
+    if (i < n && m >= i) {  // else: outside, or the pair belongs to element m
+        float2 s1 = parts[i], s2 = parts[m];
+        for (int q = 1; q < nparts; ++q) {
+            s1 = cadd(s1, parts[(size_t)q * stride + i]);
+            s2 = cadd(s2, parts[(size_t)q * stride + m]);
+        }
+        const float2 e = make_float2(0.5f * (s1.x + s2.x), 0.5f * (s1.y - s2.y));
+        const float2 bi = bspec[i];
+        est[i] = e;
+        const float2 r = make_float2(e.x - bi.x, e.y - bi.y);
+        acc += (double)r.x * r.x + (double)r.y * r.y;
+        if (m != i) {
+            const float2 em = make_float2(e.x, -e.y);
+            const float2 bm = bspec[m];
+            est[m] = em;
+            const float2 rm = make_float2(em.x - bm.x, em.y - bm.y);
+            acc += (double)rm.x * rm.x + (double)rm.y * rm.y;
+        }
+    }
+    double v = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+hipError_t holo_spec(const float2* parts, int nparts, long stride, int W, int H, const float2* bspec, float2* est,
+                     double* partial, hipStream_t s) {
+    if ((W | H) & 1) return hipErrorInvalidValue;
+    const long n = (long)(H / 2 + 1) * W;
+    hipLaunchKernelGGL(k_holo_spec, dim3(holo_spec_num_blocks(W, H)), dim3(kEwThreads), 0, s, parts, nparts, stride,
+                       W, H, bspec, est, n, partial);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(kEwThreads) k_holo_scale(const float2* __restrict__ in, float2* out, long n, float f) {
+    const long i = (long)blockIdx.x * kEwThreads + threadIdx.x;
+    if (i < n) out[i] = make_float2(in[i].x * f, in[i].y * f);
+}
+
+hipError_t holo_scale(const float2* in, float2* out, long n, float f, hipStream_t s) {
+    hipLaunchKernelGGL(k_holo_scale, dim3((unsigned)((n + kEwThreads - 1) / kEwThreads)), dim3(kEwThreads), 0, s, in,
+                       out, n, f);
     return hipGetLastError();
 }
 

@@ -48,11 +48,12 @@ enum Slot { SL_PROX = 0, SL_RESID_A = 8, SL_RESID_M = 9, SL_COUNT = 16 };
 // DECLARE_TIMING / SAVE_TIMING phases, src/core/umnholo.h:79-124).
 enum KernelId {
     K_COLS_FWD = 0, K_ROWS_FWDACC, K_ROWS_SUMPARTS, K_COLS_REAL, K_PROX, K_HOLO_MOM, K_ROWS_PLAIN,
-    K_COLS_PLAIN, K_ROWS_ADJ, K_COLS_ADJ, K_REDUCE, K_MOM_OUT, K_COUNT
+    K_COLS_PLAIN, K_ROWS_ADJ, K_COLS_ADJ, K_REDUCE, K_MOM_OUT, K_HOLO_SPEC, K_COUNT
 };
 const char* const kKernelNames[K_COUNT] = {
     "fft_cols_fwd", "fft_rows_fwdacc", "fft_rows_sumparts", "fft_cols_real", "prox", "holo_momentum",
-    "fft_rows_plain", "fft_cols_plain", "fft_rows_adj", "fft_cols_adj", "reduce_partials", "momentum_out"};
+    "fft_rows_plain", "fft_cols_plain", "fft_rows_adj", "fft_cols_adj", "reduce_partials", "momentum_out",
+    "holo_spectrum"};
 
 struct Timer {
     bool on = false;
@@ -105,10 +106,15 @@ struct rihvr_ctx {
     float2* T = nullptr;                          // FFT scratch volume
     float2* out = nullptr;                        // points at the returned volume
     // holograms
-    // hE: estimate at the momentum point; hC/hP: A y_k, A y_{k-1}; hX: A of the prox
-    // output being tested (becomes hC once the iteration is accepted)
+    // Hologram-domain quantities of the iteration are kept as spectra FFT(.) / N (the
+    // estimates are real, so their spectra are Hermitian-symmetric): hE, the estimate at the
+    // momentum point; hC/hP, A y_k and A y_{k-1}; hX, A of the prox output being tested
+    // (becomes hC once the iteration is accepted); bspec, the measurement's; rhat then holds
+    // the residual's spectrum hE - bspec (rhat_scaled).  meas is the measurement itself;
+    // hspat and resid receive the spatial estimate and residual when they are asked for.
     float2 *meas = nullptr, *hE = nullptr, *hC = nullptr, *hP = nullptr, *hX = nullptr, *resid = nullptr,
-           *rhat = nullptr, *htmp = nullptr, *spart = nullptr;
+           *rhat = nullptr, *htmp = nullptr, *spart = nullptr, *bspec = nullptr, *hspat = nullptr;
+    bool rhat_scaled = false;  // rhat holds FFT(r) / N (else FFT(r))
     // tables
     double* kappa = nullptr;
     float* zlist = nullptr;
@@ -240,9 +246,16 @@ rihvr_status resolve_timing(rihvr_ctx* c) {
     return RIHVR_OK;
 }
 
+// the residual norms are reduced over spectra FFT(r) / N: |r|^2 = N sum |FFT(r) / N|^2
+void scale_resid_norms(rihvr_ctx* c) {
+    c->h_scal[SL_RESID_A] *= (double)c->N;
+    c->h_scal[SL_RESID_M] *= (double)c->N;
+}
+
 rihvr_status fetch_scalars(rihvr_ctx* c) {
     RIHVR_HIP(hipMemcpyAsync(c->h_scal, c->scal, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     RIHVR_HIP(hipStreamSynchronize(c->stream));
+    scale_resid_norms(c);
     return resolve_timing(c);
 }
 
@@ -254,6 +267,7 @@ rihvr_status fetch_scalars_async(rihvr_ctx* c) {
 }
 rihvr_status wait_scalars(rihvr_ctx* c) {
     RIHVR_HIP(hipEventSynchronize(c->ev_scal));
+    scale_resid_norms(c);
     return resolve_timing(c);
 }
 
@@ -294,13 +308,26 @@ rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipS
     return RIHVR_OK;
 }
 
-// The end of A: sum of the partial spectra, inverse row and column FFT, real part;
-// est, resid = est - meas and scal[SL_RESID_A] = sum |resid|^2 (calcResidual :2676-2723
-// + calcSumMagnitude)
+// The end of A (calcResidual :2676-2723 + calcSumMagnitude): the estimate is the real part
+// of the inverse transform of the summed partial spectra; it is kept as its spectrum
+// (FFT(est) / N) in est, and scal[SL_RESID_A] = sum |est - bspec|^2 = |A x - b|^2 / N
+// (Parseval; fetch_scalars scales it)
 rihvr_status forward_finish(rihvr_ctx* c, float2* est) {
-    RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] { return fft_rows_sumparts(c->W, c->spart, c->nparts, c->N, c->htmp, c->H, c->twiW, c->stream); }));
-    RIHVR_TRY(timed(c, K_COLS_REAL, [&] { return fft_cols_real(c->H, c->htmp, est, c->meas, c->resid, c->partial_holo, c->W, c->twiH, c->stream); }));
-    RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial_holo, c->W / 16, 1, c->red_work, c->scal + SL_RESID_A, c->stream); }));
+    RIHVR_TRY(timed(c, K_HOLO_SPEC, [&] {
+        return holo_spec(c->spart, c->nparts, c->N, c->W, c->H, c->bspec, est, c->partial_holo, c->stream);
+    }));
+    RIHVR_TRY(timed(c, K_REDUCE, [&] {
+        return reduce_partials(c->partial_holo, holo_spec_num_blocks(c->W, c->H), 1, c->red_work, c->scal + SL_RESID_A,
+                               c->stream);
+    }));
+    return RIHVR_OK;
+}
+
+// The spatial estimate and residual of a spectrum est_spec (FFT(est) / N): est = Re(IFFT(.)),
+// into hspat, and resid = est - meas (back_propagate_finish_kernel, RM_EST_TRUE)
+rihvr_status spatial_estimate(rihvr_ctx* c, const float2* est_spec) {
+    RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] { return fft_rows_sumparts(c->W, est_spec, 1, c->N, c->htmp, c->H, c->twiW, c->stream); }));
+    RIHVR_TRY(timed(c, K_COLS_REAL, [&] { return fft_cols_real(c->H, c->htmp, c->hspat, c->meas, c->resid, c->partial_holo, c->W, c->twiH, c->stream); }));
     return RIHVR_OK;
 }
 
@@ -311,10 +338,20 @@ rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
     return forward_finish(c, est);
 }
 
-// R-hat = FFT(r) (SparseGradient::reconstruct, sparse_gradient.cu:47-75); ev_rhat marks it
-rihvr_status spectrum(rihvr_ctx* c, const float2* r) {
+// R-hat = FFT(r) of a spatial residual (SparseGradient::reconstruct, sparse_gradient.cu:47-75)
+// into out; ev_rhat marks rhat
+rihvr_status spectrum(rihvr_ctx* c, const float2* r, float2* out) {
     RIHVR_TRY(timed(c, K_ROWS_PLAIN, [&] { return fft_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream); }));
-    RIHVR_TRY(timed(c, K_COLS_PLAIN, [&] { return fft_cols_plain(c->H, -1, c->htmp, c->rhat, c->W, 1, c->twfH, c->stream); }));
+    RIHVR_TRY(timed(c, K_COLS_PLAIN, [&] { return fft_cols_plain(c->H, -1, c->htmp, out, c->W, 1, c->twfH, c->stream); }));
+    if (out == c->rhat) {
+        c->rhat_scaled = false;
+        RIHVR_HIP(hipEventRecord(c->ev_rhat, c->stream));
+    }
+    return RIHVR_OK;
+}
+// rhat now holds the residual spectrum FFT(r) / N written by a hologram kernel
+rihvr_status rhat_written(rihvr_ctx* c) {
+    c->rhat_scaled = true;
     RIHVR_HIP(hipEventRecord(c->ev_rhat, c->stream));
     return RIHVR_OK;
 }
@@ -323,7 +360,7 @@ rihvr_status spectrum(rihvr_ctx* c, const float2* r) {
 // sparse_gradient.cu:125-165): row pass of H_z R-hat / N into T[g0..], column pass
 // times e^{+i phi_z} into G[g0..]
 rihvr_status adjoint_group(rihvr_ctx* c, float2* G, int g0, int gz, hipStream_t st) {
-    const float invN = (float)(1.0 / (double)c->N);
+    const float invN = c->rhat_scaled ? 1.0f : (float)(1.0 / (double)c->N);
     float2* Tg = c->T + (size_t)g0 * c->N;
     RIHVR_TRY(timed_on(c, K_ROWS_ADJ, st, [&] {
         return fft_rows_adj(c->W, c->rhat, c->kappa, c->zlist + g0, Tg, c->H, gz, std::min(c->zchunk_adj, gz), invN,
@@ -337,7 +374,7 @@ rihvr_status adjoint_group(rihvr_ctx* c, float2* G, int g0, int gz, hipStream_t 
 
 // G = A^T r on the main stream
 rihvr_status apply_adjoint(rihvr_ctx* c, const float2* r, float2* G) {
-    RIHVR_TRY(spectrum(c, r));
+    RIHVR_TRY(spectrum(c, r, c->rhat));
     for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
         RIHVR_TRY(adjoint_group(c, G, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
     return RIHVR_OK;
@@ -464,7 +501,8 @@ void release(rihvr_ctx* c) {
     (void)hipFree(c->occ_tmp);
     (void)hipFree(c->occ_ones);
     (void)hipFree(c->smap);
-    void* ptrs[] = {c->G, c->G_alt, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->spart, c->kappa,
+    void* ptrs[] = {c->G, c->G_alt, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->bspec,
+                    c->hspat, c->spart, c->kappa,
                     c->zlist, c->phase, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
                     c->red_work, c->dflag};
     for (void* q : ptrs) (void)hipFree(q);
@@ -528,7 +566,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_TRY(dalloc(&c->G, c->V));
     RIHVR_TRY(dalloc(&c->G_alt, c->V));
     RIHVR_TRY(dalloc(&c->T, c->V));
-    float2** holos[] = {&c->meas, &c->hE, &c->hC, &c->hP, &c->hX, &c->resid, &c->rhat, &c->htmp};
+    float2** holos[] = {&c->meas, &c->hE, &c->hC, &c->hP, &c->hX, &c->resid, &c->rhat, &c->htmp, &c->bspec, &c->hspat};
     for (float2** h : holos) RIHVR_TRY(dalloc(h, c->N));
     RIHVR_TRY(dalloc(&c->spart, (size_t)c->N * c->nparts));
     RIHVR_TRY(dalloc(&c->kappa, c->N));
@@ -544,7 +582,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     rows = std::max<long>(rows, momentum_out_num_blocks(c->W, c->H, c->Nz));
     c->partial_rows = rows;
     RIHVR_TRY(dalloc(&c->partial, (size_t)rows * PS_COUNT));
-    c->holo_rows = std::max<long>(holo_num_blocks(c->N), c->W / 16);
+    c->holo_rows = std::max<long>(std::max<long>(holo_num_blocks(c->N), c->W / 16), holo_spec_num_blocks(c->W, c->H));
     RIHVR_TRY(dalloc(&c->partial_holo, c->holo_rows));
     RIHVR_TRY(dalloc(&c->scal, SL_COUNT));
     RIHVR_TRY(dalloc(&c->red_work, (size_t)kReduceBlocks * PS_COUNT));
@@ -607,6 +645,9 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     RIHVR_HIP(hipMemcpyAsync(&hflag, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     RIHVR_HIP(hipStreamSynchronize(c->stream));
     if (hflag) return RIHVR_E_INVALID_DATA;
+    // the measurement's spectrum FFT(b) / N
+    RIHVR_TRY(spectrum(c, c->meas, c->bspec));
+    RIHVR_HIP(holo_scale(c->bspec, c->bspec, c->N, (float)(1.0 / (double)c->N), c->stream));
 
     // start point x1 = x0 = y0 = warm start or zero (:258-259, sparse_volume.cu:44-48);
     // a warm start that already is one of the ring volumes (the previous frame's
@@ -652,7 +693,11 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     c->ss.prev_summag = c->h_scal[SL_RESID_A];
     c->ss.f_initial = objective(c, c->ss.mode, c->ss.prev_summag, l1, tv);
     if (f_initial) *f_initial = c->ss.f_initial;
-    RIHVR_TRY(spectrum(c, c->resid));  // the gradient's planes are formed with the first prox
+    // the residual's spectrum; the gradient's planes are formed with the first prox
+    RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
+        return holo_residual(c->hE, c->bspec, c->rhat, c->N, c->partial_holo, c->stream);
+    }));
+    RIHVR_TRY(rhat_written(c));
     c->ss.t = 1.0 / (double)c->Nz;  // :247 (a forced step is overridden there too)
     c->ss.alpha1 = 1.0;
     c->ss.s_mom = 0.0f;             // x0 = Y0 for the first prox
@@ -719,14 +764,14 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
                 std::swap(c->hP, c->hC);
                 std::swap(c->hC, c->hX);
                 RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
-                    return holo_momentum(c->hC, c->hP, (alpha0 - 1) / a1, c->meas, c->hE, c->resid, c->N,
+                    return holo_momentum(c->hC, c->hP, (alpha0 - 1) / a1, c->bspec, c->hE, c->rhat, c->N,
                                          c->partial_holo, c->stream, c->scal + SL_PROX + PS_RESTART);
                 }));
                 RIHVR_TRY(timed(c, K_REDUCE, [&] {
                     return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work,
                                            c->scal + SL_RESID_M, c->stream);
                 }));
-                RIHVR_TRY(spectrum(c, c->resid));
+                RIHVR_TRY(rhat_written(c));
                 if (i + 1 < n_it)
                     for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
                         RIHVR_TRY(adjoint_group(c, c->G_alt, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
@@ -748,11 +793,8 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
                     if (st->f) st->f[i - 1] = f_prev;
                 }
                 if (f_stop > 0 && f_prev <= f_stop) {
-                    // keep the accepted point: its residual est - meas into resid again
-                    // (the forward pass above overwrote it)
-                    RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
-                        return holo_residual(c->hE, c->meas, c->resid, c->N, c->partial_holo, c->stream);
-                    }));
+                    // keep the accepted point: its estimate is still in hE (the forward pass
+                    // above wrote hX)
                     if (done) *done = i;
                     n_it = i;
                     break;
@@ -794,13 +836,13 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
             std::swap(c->hP, c->hC);
             std::swap(c->hC, c->hX);
             RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
-                return holo_momentum(c->hC, c->hP, step, c->meas, c->hE, c->resid, c->N, c->partial_holo, c->stream);
+                return holo_momentum(c->hC, c->hP, step, c->bspec, c->hE, c->rhat, c->N, c->partial_holo, c->stream);
             }));
             RIHVR_TRY(timed(c, K_REDUCE, [&] {
                 return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work, c->scal + SL_RESID_M, c->stream);
             }));
             // the next gradient (:545-546): R-hat now, its planes with the next prox
-            RIHVR_TRY(spectrum(c, c->resid));
+            RIHVR_TRY(rhat_written(c));
             s.g_ready = false;
         }
         // rotate the prox-output ring: y_{k-2} <- y_{k-1} <- y_k
@@ -1094,7 +1136,8 @@ rihvr_status rihvr_set_stepsize(rihvr_ctx* ctx, double step) {
 rihvr_status rihvr_get_estimate(rihvr_ctx* ctx, float* estimate, float* residual, int out_loc) {
     if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
     if (!ctx->have_result) return RIHVR_E_INVALID_STATE;
-    if (estimate) RIHVR_TRY(copy_out(ctx, estimate, ctx->hE, out_loc, ctx->N));
+    RIHVR_TRY(spatial_estimate(ctx, ctx->hE));
+    if (estimate) RIHVR_TRY(copy_out(ctx, estimate, ctx->hspat, out_loc, ctx->N));
     if (residual) RIHVR_TRY(copy_out(ctx, residual, ctx->resid, out_loc, ctx->N));
     return RIHVR_OK;
 }
@@ -1107,8 +1150,10 @@ rihvr_status rihvr_forward(rihvr_ctx* ctx, const float* volume, int vol_loc, flo
     ctx->occ_valid[0] = false;
     RIHVR_TRY(copy_in(ctx, v, volume, vol_loc, ctx->V));
     RIHVR_HIP(hipMemsetAsync(ctx->meas, 0, ctx->N * sizeof(float2), ctx->stream));
+    RIHVR_HIP(hipMemsetAsync(ctx->bspec, 0, ctx->N * sizeof(float2), ctx->stream));
     RIHVR_TRY(apply_forward(ctx, v, ctx->hE));
-    return copy_out(ctx, holo_out, ctx->hE, holo_loc, ctx->N);
+    RIHVR_TRY(spatial_estimate(ctx, ctx->hE));
+    return copy_out(ctx, holo_out, ctx->hspat, holo_loc, ctx->N);
 }
 
 rihvr_status rihvr_adjoint(rihvr_ctx* ctx, const float* holo, int holo_loc, float* volume_out, int vol_loc) {
