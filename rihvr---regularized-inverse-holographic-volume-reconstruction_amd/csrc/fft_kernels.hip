@@ -192,8 +192,10 @@ struct ColArgs {
     const float2* meas;
     float2* resid;
     double* partial;        // one double per workgroup: sum |r|^2
-    // FWD: line occupancy of the input ([nz][L] words, may be NULL) and the strip map written
+    // FWD: line occupancy of the input ([nz][L] row words and [nz] plane words, may be NULL)
+    // and the strip map written
     const uint64_t* occ;
+    const uint64_t* occ_planes;
     uint8_t* smap;
 };
 
@@ -201,6 +203,14 @@ template <int L, int SIGN, int MODE>
 __global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
     using C = ColCfg<L>;
     using Plan = FftPlan<L>;
+    if constexpr (MODE == COL_FWD) {
+        // a strip with no line set in its plane transforms to zeros: not written, and the
+        // workgroup leaves before anything else (uniform: every thread reads the same word)
+        if (a.occ_planes && !((a.occ_planes[blockIdx.y] >> blockIdx.x) & 1ull)) {
+            if (threadIdx.x == 0) a.smap[(size_t)blockIdx.y * (a.W / kColStrip) + blockIdx.x] = 0;
+            return;
+        }
+    }
     __shared__ float2 tw_s[L];
     __shared__ float2 buf[L * kColStrip];
     __shared__ double red_s[C::THREADS / 64 + 1];
@@ -401,9 +411,11 @@ hipError_t fft_cols_adj(int L, const float2* in, const float2* phase, float2* ou
 }
 
 hipError_t fft_cols_fwd(int L, const float2* in, const float2* phase, float2* out, int W, int nz,
-                        const float2* tw_fwd, const uint64_t* occ, uint8_t* smap, hipStream_t s) {
+                        const float2* tw_fwd, const uint64_t* occ, const uint64_t* occ_planes, uint8_t* smap,
+                        hipStream_t s) {
     ColArgs a{};
-    a.in = in; a.out = out; a.W = W; a.twg = tw_fwd; a.phase = phase; a.occ = occ; a.smap = smap;
+    a.in = in; a.out = out; a.W = W; a.twg = tw_fwd; a.phase = phase; a.occ = occ; a.occ_planes = occ_planes;
+    a.smap = smap;
     switch (L) {
 #define X(n) case n: return launch_cols_t<n, -1, COL_FWD>(a, nz, s);
         RIHVR_FOR_EACH_L(X)

@@ -8,7 +8,8 @@ namespace rihvr {
 // Line occupancy of a prox-output volume: one 64-bit word per (plane, row), bit l set when
 // the row's voxels 16l .. 16l+15 (one 128-byte line; W <= 1024) hold a non-zero value.  A
 // volume with a valid map holds exact zeros in every line whose bit is clear, so readers
-// may skip those lines and writers may skip zero lines that were already zero.
+// may skip those lines and writers may skip zero lines that were already zero.  A map is
+// [nz][H] row words followed by [nz] plane words, each the OR of its plane's row words.
 
 // ---- FFT (fft_kernels.hip) -------------------------------------------------
 bool fft_length_supported(int L);
@@ -21,11 +22,13 @@ hipError_t fft_rows_plain(int L, int sign, const float2* in, float2* out, int nr
 hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out,
                              int nrows, const float2* tw_inv, hipStream_t s);
 // A, first pass: forward column FFT (length L = H) of vol[z] * e^{-i phi_z}, nz planes of width W.
-// occ (may be NULL): vol's line occupancy, [nz][L] words; lines with a clear bit are not
-// read, and a 16-column strip with no set bit in any row is neither transformed nor
-// written.  smap[z][W/16] (always written) = 1 for the strips written to out.
+// occ (may be NULL): vol's line occupancy, [nz][L] row words, and occ_planes its [nz] plane
+// words; lines with a clear bit are not read, and a 16-column strip with no set bit in its
+// plane word is neither transformed nor written (its workgroup leaves at once).
+// smap[z][W/16] (always written) = 1 for the strips written to out.
 hipError_t fft_cols_fwd(int L, const float2* vol, const float2* phase, float2* out, int W, int nz,
-                        const float2* tw_fwd, const uint64_t* occ, uint8_t* smap, hipStream_t s);
+                        const float2* tw_fwd, const uint64_t* occ, const uint64_t* occ_planes, uint8_t* smap,
+                        hipStream_t s);
 // A, second pass: forward row FFT (length L = W), epilogue H_{-z} / N summed over zchunk
 // planes into parts[chunk] ([H][W] each); strips with smap 0 are read as zeros, planes
 // without any strip are skipped.
@@ -81,6 +84,7 @@ struct ProxLaunch {
     const uint64_t* occ_ym = nullptr;
     const uint64_t* occ_old = nullptr;
     uint64_t* occ_new = nullptr;
+    uint64_t* occ_new_planes = nullptr;  // [nz] plane words of the new map (OR-ed in)
 };
 // Number of per-wave partial rows the mode-7 prox writes for this geometry.
 // row_chunk: output rows per wave (<= 0: default 128)

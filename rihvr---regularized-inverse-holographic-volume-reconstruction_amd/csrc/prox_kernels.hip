@@ -75,6 +75,7 @@ struct ProxDev {
     const uint64_t* occ_ym;
     const uint64_t* occ_old;
     uint64_t* occ_new;
+    uint64_t* occ_new_planes;  // [nz] plane words of the new map
 };
 
 __device__ __forceinline__ float soft1(float v, float thr) {
@@ -130,6 +131,7 @@ template <int NTV> struct ProxWave {
     // the lines (bit l: columns 16l .. 16l + 15) that the wave's 64 lanes read, and those
     // its output lanes write
     uint64_t wlines = ~0ull, outlines = ~0ull;
+    uint64_t plane_bits = 0;  // lines this wave set in the new map (its plane word's share)
     float acc_ls1 = 0.f, acc_ls2 = 0.f, acc_rs = 0.f, acc_l1 = 0.f, acc_tv = 0.f, acc_nnz = 0.f;
 
     static __device__ __forceinline__ int wrap_row(int tau, int H) {
@@ -300,6 +302,7 @@ template <int NTV> struct ProxWave {
                         if (m & lanes) bits |= 1ull << l;
                     }
                     if ((threadIdx.x & 63) == 0) atomicOr((unsigned long long*)&a.occ_new[zrow + o], bits);
+                    plane_bits |= bits;
                 }
             }
             // a lane writes when its value is non-zero or its line holds an old non-zero; a
@@ -433,6 +436,7 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
         st.tail(std::make_integer_sequence<int, U - 1>{}, tau1 - tau + 1, a, tau, r0, r1, xoff,
                 ocol, out_lane, tau1, zrow, strip);
 
+    if (st.plane_bits && lane == 0) atomicOr((unsigned long long*)&a.occ_new_planes[z], st.plane_bits);
     const double s_ls1 = wave_sum((double)st.acc_ls1);
     const double s_ls2 = wave_sum((double)st.acc_ls2);
     const double s_rs = wave_sum((double)st.acc_rs);
@@ -469,6 +473,7 @@ static ProxDev make_dev(const ProxLaunch& a) {
     d.occ_ym = a.occ_ym;
     d.occ_old = a.occ_old;
     d.occ_new = a.occ_new;
+    d.occ_new_planes = a.occ_new_planes;
     return d;
 }
 
@@ -527,7 +532,7 @@ int prox_fl2d_auto_rows(int W, int H, int nz, int n_tv) {
 hipError_t prox_fl2d(const ProxLaunch& a, hipStream_t s) {
     // row offsets inside a plane are 32-bit byte offsets; every map is required
     if ((size_t)a.W * a.H * sizeof(float2) > 0xffffffffull) return hipErrorInvalidValue;
-    if (!a.occ_y0 || !a.occ_ym || !a.occ_old || !a.occ_new) return hipErrorInvalidValue;
+    if (!a.occ_y0 || !a.occ_ym || !a.occ_old || !a.occ_new || !a.occ_new_planes) return hipErrorInvalidValue;
     ProxDev d = make_dev(a);
     switch (a.n_tv) {
         case 0: return launch_fl2d<0>(d, s);

@@ -278,6 +278,10 @@ int ring_index(const rihvr_ctx* c, const float2* v) {
     return -1;
 }
 // the valid line-occupancy map of a ring volume, else NULL
+// words of a line-occupancy map: [Nz][H] row words, then [Nz] plane words
+size_t map_words(const rihvr_ctx* c) { return (size_t)c->Nz * c->H + c->Nz; }
+const uint64_t* map_planes(const rihvr_ctx* c, const uint64_t* map) { return map + (size_t)c->Nz * c->H; }
+
 const uint64_t* occ_of(const rihvr_ctx* c, const float2* v) {
     const int k = ring_index(c, v);
     return (k >= 0 && c->occ_valid[k]) ? c->occ[k] : nullptr;
@@ -298,8 +302,9 @@ rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipS
     float2* spg = c->spart + (size_t)(g0 / c->zchunk_fwd) * c->N;
     uint8_t* smg = c->smap + (size_t)g0 * (c->W / 16);
     const uint64_t* og = occ ? occ + (size_t)g0 * c->H : nullptr;
+    const uint64_t* opg = occ ? map_planes(c, occ) + g0 : nullptr;
     RIHVR_TRY(timed_on(c, K_COLS_FWD, st, [&] {
-        return fft_cols_fwd(c->H, vol + (size_t)g0 * c->N, c->phase + g0, Tg, c->W, gz, c->twfH, og, smg, st);
+        return fft_cols_fwd(c->H, vol + (size_t)g0 * c->N, c->phase + g0, Tg, c->W, gz, c->twfH, og, opg, smg, st);
     }));
     RIHVR_TRY(timed_on(c, K_ROWS_FWDACC, st, [&] {
         return fft_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, invN, c->twfW, smg,
@@ -411,7 +416,8 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
         a.occ_ym = om ? om : c->occ_ones;
         a.occ_old = (ky >= 0 && c->occ_valid[ky]) ? c->occ[ky] : c->occ_ones;
         a.occ_new = c->occ_tmp;
-        RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
+        a.occ_new_planes = c->occ_tmp + (size_t)c->Nz * c->H;
+        RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, map_words(c) * sizeof(uint64_t), c->stream));
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
         rows = prox_fl2d_num_waves(c->W, c->H, c->Nz, a.n_tv, c->prox_rows);
         if (ky >= 0) {
@@ -456,7 +462,7 @@ rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, fl
     const uint64_t* om = occ_of(c, YM);
     const int ky = ring_index(c, Yn);
     const uint64_t* oold = (ky >= 0 && c->occ_valid[ky]) ? c->occ[ky] : c->occ_ones;
-    RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
+    RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, map_words(c) * sizeof(uint64_t), c->stream));
     for (int g = 0, g0 = 0; g < c->n_groups; ++g, g0 += c->pipe_planes) {
         const int gz = std::min(c->pipe_planes, c->Nz - g0);
         const size_t off = (size_t)g0 * c->N;
@@ -472,6 +478,7 @@ rihvr_status pipelined_pass(rihvr_ctx* c, const float2* Y0, const float2* YM, fl
         a.occ_ym = (om ? om : c->occ_ones) + roff;
         a.occ_old = oold + roff;
         a.occ_new = c->occ_tmp + roff;
+        a.occ_new_planes = c->occ_tmp + (size_t)c->Nz * c->H + g0;
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
         RIHVR_TRY(forward_group(c, Yn, g0, gz, c->stream, ky >= 0 ? c->occ_tmp : nullptr));
     }
@@ -558,10 +565,10 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     c->ev_adj.assign(c->n_groups, nullptr);
     for (hipEvent_t& e : c->ev_adj) RIHVR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& b : c->Yb) RIHVR_TRY(dalloc(&b, c->V));
-    for (auto& m : c->occ) RIHVR_TRY(dalloc(&m, (size_t)c->Nz * c->H));
-    RIHVR_TRY(dalloc(&c->occ_tmp, (size_t)c->Nz * c->H));
-    RIHVR_TRY(dalloc(&c->occ_ones, (size_t)c->Nz * c->H));
-    RIHVR_HIP(hipMemset(c->occ_ones, 0xff, (size_t)c->Nz * c->H * sizeof(uint64_t)));
+    for (auto& m : c->occ) RIHVR_TRY(dalloc(&m, map_words(c)));
+    RIHVR_TRY(dalloc(&c->occ_tmp, map_words(c)));
+    RIHVR_TRY(dalloc(&c->occ_ones, map_words(c)));
+    RIHVR_HIP(hipMemset(c->occ_ones, 0xff, map_words(c) * sizeof(uint64_t)));
     RIHVR_TRY(dalloc(&c->smap, (size_t)c->Nz * (c->W / 16)));
     RIHVR_TRY(dalloc(&c->G, c->V));
     RIHVR_TRY(dalloc(&c->G_alt, c->V));
@@ -668,7 +675,7 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     // a zero start has the empty map; a warm start none (every line is read once)
     for (int k : {c->iy0, c->iym}) {
         c->occ_valid[k] = !warm;
-        if (!warm) RIHVR_HIP(hipMemsetAsync(c->occ[k], 0, (size_t)c->Nz * c->H * sizeof(uint64_t), c->stream));
+        if (!warm) RIHVR_HIP(hipMemsetAsync(c->occ[k], 0, map_words(c) * sizeof(uint64_t), c->stream));
     }
     RIHVR_HIP(hipMemsetAsync(c->hC, 0, c->N * sizeof(float2), c->stream));
     RIHVR_HIP(hipMemsetAsync(c->hP, 0, c->N * sizeof(float2), c->stream));
