@@ -88,7 +88,8 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
             [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
     } else if constexpr (MODE == ROW_FWDACC) {
         // A, second pass: forward row FFT of the column-transformed planes of a
-        // chunk of planes; epilogue H_{-z} / (Nx Ny) summed over the chunk
+        // chunk of planes; epilogue H_{-z} summed over the chunk (the 1 / (Nx Ny) is applied
+        // to the first pass's input)
         // (sparse_rs_phase_mult_kernel + sparse_add_plane_kernel)
         using LPs = LastPass<L, Plan>;
         float2 acc[LPs::NB][LPs::R];
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
                 },
                 [&](int i, int r, int, float2 v) {
                     float2 h = transfer(mz, kp[i][r]);
-                    acc[i][r] = cadd(acc[i][r], cscale(cmul(h, v), a.invN));
+                    acc[i][r] = cadd(acc[i][r], cmul(h, v));  // 1/N is in the first pass's input
                 },
                 lds);
             __syncthreads();
@@ -357,11 +358,10 @@ hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_s
 }
 
 hipError_t fft_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts,
-                          int H, int nz, int zchunk, float invN, const float2* tw_fwd, const uint8_t* smap,
-                          hipStream_t s) {
+                          int H, int nz, int zchunk, const float2* tw_fwd, const uint8_t* smap, hipStream_t s) {
     RowArgs a{};
     a.in = in; a.out = parts; a.nrows = H; a.twg = tw_fwd; a.kappa = kappa; a.zlist = zlist; a.nz = nz;
-    a.zchunk = zchunk; a.invN = invN; a.part_stride = (long)H * L; a.smap = smap;
+    a.zchunk = zchunk; a.part_stride = (long)H * L; a.smap = smap;
     const int gy = (nz + zchunk - 1) / zchunk;
     switch (L) {
 #define X(n) case n: return launch_rows_t<n, -1, ROW_FWDACC>(a, H, gy, s);

@@ -21,7 +21,8 @@ hipError_t fft_rows_plain(int L, int sign, const float2* in, float2* out, int nr
 // Inverse row FFT of sum_p parts[p] (nparts partial spectra, part_stride elements apart).
 hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out,
                              int nrows, const float2* tw_inv, hipStream_t s);
-// A, first pass: forward column FFT (length L = H) of vol[z] * e^{-i phi_z}, nz planes of width W.
+// A, first pass: forward column FFT (length L = H) of vol[z] * conj(phase[z]), nz planes of
+// width W (phase[z] = e^{i phi_z} / N for A).
 // occ (may be NULL): vol's line occupancy, [nz][L] row words, and occ_planes its [nz] plane
 // words; lines with a clear bit are not read, and a 16-column strip with no set bit in its
 // plane word is neither transformed nor written (its workgroup leaves at once).
@@ -29,12 +30,11 @@ hipError_t fft_rows_sumparts(int L, const float2* parts, int nparts, long part_s
 hipError_t fft_cols_fwd(int L, const float2* vol, const float2* phase, float2* out, int W, int nz,
                         const float2* tw_fwd, const uint64_t* occ, const uint64_t* occ_planes, uint8_t* smap,
                         hipStream_t s);
-// A, second pass: forward row FFT (length L = W), epilogue H_{-z} / N summed over zchunk
-// planes into parts[chunk] ([H][W] each); strips with smap 0 are read as zeros, planes
-// without any strip are skipped.
+// A, second pass: forward row FFT (length L = W), epilogue H_{-z} summed over zchunk planes
+// into parts[chunk] ([H][W] each); the 1/N of A is applied by the first pass's phase
+// factors.  Strips with smap 0 are read as zeros, planes without any strip are skipped.
 hipError_t fft_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts,
-                          int H, int nz, int zchunk, float invN, const float2* tw_fwd, const uint8_t* smap,
-                          hipStream_t s);
+                          int H, int nz, int zchunk, const float2* tw_fwd, const uint8_t* smap, hipStream_t s);
 // A^T, first pass: inverse row FFT of H_z * rhat / N, planes processed zchunk per workgroup.
 hipError_t fft_rows_adj(int L, const float2* rhat, const double* kappa, const float* zlist, float2* out,
                         int H, int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s);

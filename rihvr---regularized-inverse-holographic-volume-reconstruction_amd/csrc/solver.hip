@@ -119,6 +119,7 @@ struct rihvr_ctx {
     double* kappa = nullptr;
     float* zlist = nullptr;
     float2* phase = nullptr;
+    float2* phase_fwd = nullptr;  // phase / N: A's first pass applies A's 1/N with its phase
     float2 *twfW = nullptr, *twiW = nullptr, *twfH = nullptr, *twiH = nullptr;
     // reductions
     double* partial = nullptr;
@@ -297,17 +298,16 @@ void invalidate_occ(rihvr_ctx* c, const float2* v) {
 // occ: vol's line occupancy map (NULL: read every line); the strips the column pass
 // writes are recorded in smap for the row pass
 rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipStream_t st, const uint64_t* occ) {
-    const float invN = (float)(1.0 / (double)c->N);
     float2* Tg = c->T + (size_t)g0 * c->N;
     float2* spg = c->spart + (size_t)(g0 / c->zchunk_fwd) * c->N;
     uint8_t* smg = c->smap + (size_t)g0 * (c->W / 16);
     const uint64_t* og = occ ? occ + (size_t)g0 * c->H : nullptr;
     const uint64_t* opg = occ ? map_planes(c, occ) + g0 : nullptr;
     RIHVR_TRY(timed_on(c, K_COLS_FWD, st, [&] {
-        return fft_cols_fwd(c->H, vol + (size_t)g0 * c->N, c->phase + g0, Tg, c->W, gz, c->twfH, og, opg, smg, st);
+        return fft_cols_fwd(c->H, vol + (size_t)g0 * c->N, c->phase_fwd + g0, Tg, c->W, gz, c->twfH, og, opg, smg, st);
     }));
     RIHVR_TRY(timed_on(c, K_ROWS_FWDACC, st, [&] {
-        return fft_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, invN, c->twfW, smg,
+        return fft_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, c->twfW, smg,
                                st);
     }));
     return RIHVR_OK;
@@ -510,7 +510,7 @@ void release(rihvr_ctx* c) {
     (void)hipFree(c->smap);
     void* ptrs[] = {c->G, c->G_alt, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->bspec,
                     c->hspat, c->spart, c->kappa,
-                    c->zlist, c->phase, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
+                    c->zlist, c->phase, c->phase_fwd, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
                     c->red_work, c->dflag};
     for (void* q : ptrs) (void)hipFree(q);
     for (auto& p : c->tm.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -579,6 +579,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_TRY(dalloc(&c->kappa, c->N));
     RIHVR_TRY(dalloc(&c->zlist, c->Nz));
     RIHVR_TRY(dalloc(&c->phase, c->Nz));
+    RIHVR_TRY(dalloc(&c->phase_fwd, c->Nz));
     RIHVR_TRY(dalloc(&c->twfW, c->W));
     RIHVR_TRY(dalloc(&c->twiW, c->W));
     RIHVR_TRY(dalloc(&c->twfH, c->H));
@@ -614,16 +615,19 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
             kap[(size_t)y * c->W + x] = num / lambda;
         }
     std::vector<float> z(c->Nz);
-    std::vector<float2> ph(c->Nz);
+    std::vector<float2> ph(c->Nz), phf(c->Nz);
+    const double invN = 1.0 / (double)c->N;
     for (int i = 0; i < c->Nz; ++i) {
         float step = (float)i * params->plane_stepsize;
         z[i] = params->start_plane + step;
         float arg = (float)(2 * kPi * (double)z[i] / lambda);
         ph[i] = make_float2((float)std::cos((double)arg), (float)std::sin((double)arg));
+        phf[i] = make_float2((float)((double)ph[i].x * invN), (float)((double)ph[i].y * invN));
     }
     RIHVR_TRY(upload(c->kappa, kap.data(), sizeof(double) * c->N, c->stream));
     RIHVR_TRY(upload(c->zlist, z.data(), sizeof(float) * c->Nz, c->stream));
     RIHVR_TRY(upload(c->phase, ph.data(), sizeof(float2) * c->Nz, c->stream));
+    RIHVR_TRY(upload(c->phase_fwd, phf.data(), sizeof(float2) * c->Nz, c->stream));
     auto tfw = twiddles(c->W, -1), tiw = twiddles(c->W, +1), tfh = twiddles(c->H, -1), tih = twiddles(c->H, +1);
     RIHVR_TRY(upload(c->twfW, tfw.data(), sizeof(float2) * c->W, c->stream));
     RIHVR_TRY(upload(c->twiW, tiw.data(), sizeof(float2) * c->W, c->stream));
