@@ -89,9 +89,10 @@ struct ProxLaunch {
 // Number of per-wave partial rows the mode-7 prox writes for this geometry.
 // row_chunk: output rows per wave (<= 0: default 128)
 int prox_fl2d_num_waves(int W, int H, int nz, int n_tv, int row_chunk);
-// Output rows per wave for this geometry on the current device: the chunk length whose
-// rounds of resident waves (rounded up) times the rows a wave steps through (its chunk
-// plus the 2 n_tv + 1 halo rows) is least.  0 if the occupancy query fails.
+// Output rows per wave for this geometry on the current device: about 205 when that gives
+// at least two rounds of resident waves, else the chunk length whose rounds of resident
+// waves (rounded up) times the rows a wave steps through (its chunk plus the 2 n_tv + 1
+// halo rows) is least.  0 if the occupancy query fails.
 int prox_fl2d_auto_rows(int W, int H, int nz, int n_tv);
 bool prox_fl2d_supported(int n_tv);
 hipError_t prox_fl2d(const ProxLaunch& a, hipStream_t s);

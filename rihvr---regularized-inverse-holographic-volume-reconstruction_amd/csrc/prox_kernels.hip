@@ -18,7 +18,8 @@
 // and every neighbour in x is a lane shuffle.  Rows and columns outside the
 // strip's output range are recomputed halo (n_tv on each side in x, and at the
 // ends of the row chunk in y).  No workgroup barriers and no LDS: the per-lane
-// history of x-hat and x0 lives in registers.
+// history of x-hat lives in registers.
+#include <algorithm>
 #include <utility>
 
 #include "device_common.hpp"
@@ -108,7 +109,7 @@ __device__ __forceinline__ float soft_fast(float v, float thr) {
 
 // One wave's state: the TV duals g_k (gx.re, gx.im, gy.re, gy.im) and y-hat_k of
 // the last two rows of every level, indexed [row parity][level]; a per-lane
-// history ring holds x-hat and x0 of the last NTV + 3 rows.  The row
+// history ring holds x-hat of the last NTV + 3 rows.  The row
 // loop is unrolled by U (a multiple of the ring length and of 2), so every ring
 // slot and row parity is a compile-time constant and the ring is registers.
 template <int NTV> struct ProxWave {
@@ -116,10 +117,11 @@ template <int NTV> struct ProxWave {
     static constexpr int U = (P % 2 == 0) ? P : 2 * P;
     float4 gd[2][NTV + 1];
     float2 yh[2][NTV + 1];
-    // rows tau and tau + 1 of y_{k-1}, y_{k-2} and g, loaded two steps ahead
-    float2 pf[2][3];
-    // x-hat and x0 of the last P rows, indexed by compile-time slots: registers
-    float2 hist[2][P];
+    // rows tau and tau + 1 of g, loaded two steps ahead (y_{k-1} and y_{k-2} are read at their
+    // row's step, by the few rows that hold data)
+    float2 pfg[2];
+    // x-hat of the last P rows, indexed by compile-time slots: registers
+    float2 hist[1][P];
     // Row flags, shifted in as each row is loaded (after the load of row tau + 2, bit j is
     // row tau + 2 - j): y0rows, y_{k-1} may hold a non-zero in the wave's lines (its map says
     // so, or it has no map); xrows, y_{k-1} or y_{k-2} may, so x0 may be non-zero.
@@ -154,6 +156,7 @@ template <int NTV> struct ProxWave {
     char* yp;
     unsigned rowb;                // bytes per row
     int row_ld, row_wd, row_old;  // rows of the next load_row, load_words and old-map read
+    int row_cur;                  // row tau of the step
     static __device__ __forceinline__ int next_row(int r, int H) { return r + 1 == H ? 0 : r + 1; }
 
     // occupancy words of y_{k-1} and y_{k-2} for row row_wd, one step before its loads
@@ -167,24 +170,16 @@ template <int NTV> struct ProxWave {
     static __device__ __forceinline__ const float2& at(const char* base, unsigned off, unsigned xoff) {
         return *(const float2*)(base + off + xoff);
     }
-    // Row row_ld.  A row of y_{k-1} or y_{k-2} is read only when one of the wave's lines is
-    // set in its map: a line whose bit is clear holds +0, so a wave that reads the row with
-    // some of its lines clear reads zeros there, and a wave with all its lines clear takes
-    // zeros.
-    // A row with neither is not read at all (its x0 is +0 and is not formed from pf).
+    // Row row_ld: g, and the row flags of y_{k-1} and y_{k-2} (a row with one of the wave's
+    // lines set in either map is read at its step; a line whose bit is clear holds +0, so a
+    // wave that reads the row with some of its lines clear reads zeros there).
     __device__ __forceinline__ void load_row(const ProxDev& a, int slot, unsigned xoff, bool ld) {
         const bool f0 = ld && (ow0[slot] & wlines) != 0ull;
         const bool fm = ld && (owm[slot] & wlines) != 0ull;
         y0rows = (y0rows << 1) | (f0 ? 1u : 0u);
         xrows = (xrows << 1) | ((f0 || fm) ? 1u : 0u);
         if (!ld) return;
-        const unsigned off = (unsigned)row_ld * rowb;
-        const float2 zero = make_float2(0.0f, 0.0f);
-        if (f0 || fm) {
-            pf[slot][0] = f0 ? at(y0p, off, xoff) : zero;
-            pf[slot][1] = fm ? at(ymp, off, xoff) : zero;
-        }
-        pf[slot][2] = at(gp, off, xoff);
+        pfg[slot] = at(gp, (unsigned)row_ld * rowb, xoff);
         row_ld = next_row(row_ld, a.H);
     }
 
@@ -200,14 +195,15 @@ template <int NTV> struct ProxWave {
         const unsigned oofs = (unsigned)ocol * (unsigned)sizeof(float2);  // output lanes only
         // ---- row tau: momentum point x0 = y0 + s (y0 - ym), x-hat = x0 - t g ----
         {
-            const float2 g = pf[PAR][2];
+            const float2 g = pfg[PAR];
             float2 x0 = make_float2(0.0f, 0.0f);  // y0 = ym = +0 give x0 = +0
-            if ((xrows >> 1) & 1u) {              // row tau (its load was shifted in 2 rows ago)
-                const float2 y0 = pf[PAR][0], ym = pf[PAR][1];
+            if ((xrows >> 1) & 1u) {              // row tau (its flags were shifted in 2 rows ago)
+                const unsigned off = (unsigned)row_cur * rowb;
+                const float2 y0 = at(y0p, off, xoff), ym = at(ymp, off, xoff);
                 x0 = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
             }
+            row_cur = next_row(row_cur, a.H);
             hist[0][SLOT] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
-            hist[1][SLOT] = x0;
             // the output's current map for the row written in the next step
             oold[Q] = uniform_word(a.occ_old, zrow + row_old);
             row_old = next_row(row_old, a.H);
@@ -283,7 +279,6 @@ template <int NTV> struct ProxWave {
         } else {
             yv = make_float2(yv.x + 0.0f, yv.y + 0.0f);
         }
-        const float2 x0o = hist[1][SO];
         if (o >= r0 && o < r1) {  // row o: the new map and the skipped writes
             uint64_t m = 0ull;
             if (live) {
@@ -323,15 +318,25 @@ template <int NTV> struct ProxWave {
         if (live && o >= r0 && o < r1 && !xflags)
             reduce = __builtin_amdgcn_ballot_w64(((__float_as_uint(yv.x) | __float_as_uint(yv.y)) << 1) != 0u) != 0ull;
         if (reduce) {
-            const float2 x0u = hist[1][SU];
-            // all lanes active here: the left neighbours' x0 for the TV term, and g and
-            // y_{k-1} of row o read again (they were loaded NTV + 1 rows ago; rows that reduce
-            // are few, so they are not kept in registers)
-            const float x0lr = lane_from_left(x0o.x);
-            const float x0li = lane_from_left(x0o.y);
+            // all lanes active here.  g, y_{k-1} and x0 of row o and x0 of the row above are
+            // formed again from memory (rows that reduce are few, so they are not kept in
+            // registers; x0 = y0 + s (y0 - ym) exactly as at the row's load, +0 for a row
+            // without data), and the left neighbours' x0 for the TV term
             const unsigned ro = (unsigned)o * rowb;
+            const unsigned ru = (unsigned)(o > 0 ? o - 1 : a.H - 1) * rowb;
             const float2 g = at(gp, ro, xoff);
             const float2 y0 = at(y0p, ro, xoff);
+            float2 x0o = make_float2(0.0f, 0.0f), x0u = make_float2(0.0f, 0.0f);
+            if ((xrows >> (NTV + 3)) & 1u) {
+                const float2 ym = at(ymp, ro, xoff);
+                x0o = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
+            }
+            if ((xrows >> (NTV + 4)) & 1u) {
+                const float2 y0u = at(y0p, ru, xoff), ymu = at(ymp, ru, xoff);
+                x0u = make_float2(fmaf(a.s_mom, y0u.x - ymu.x, y0u.x), fmaf(a.s_mom, y0u.y - ymu.y, y0u.y));
+            }
+            const float x0lr = lane_from_left(x0o.x);
+            const float x0li = lane_from_left(x0o.y);
             if (out_lane) {
                 const float dr = yv.x - x0o.x, di = yv.y - x0o.y;
                 acc_ls1 = fmaf(dr, g.x, fmaf(di, g.y, acc_ls1));
@@ -366,8 +371,14 @@ template <int NTV> struct ProxWave {
     }
 };
 
+// Up to 5 TV iterations the kernel fits 5 waves per SIMD (96 VGPRs; the few registers that
+// do not fit are spilled only in the end-of-chunk steps, not in the main row loop); with more
+// its state needs more registers than 5 waves leave.
+template <int NTV> constexpr int prox_waves_per_simd() { return NTV <= 5 ? 5 : 1; }
+
 template <int NTV>
-__global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
+__global__ void __launch_bounds__(64 * kProxWaves)
+    __attribute__((amdgpu_waves_per_eu(prox_waves_per_simd<NTV>()))) k_prox_fl2d(ProxDev a) {
     using PW = ProxWave<NTV>;
     constexpr int P = PW::P;
     constexpr int U = PW::U;
@@ -421,6 +432,7 @@ __global__ void __launch_bounds__(64 * kProxWaves) k_prox_fl2d(ProxDev a) {
     st.rowb = (unsigned)W * (unsigned)sizeof(float2);
     st.row_ld = PW::wrap_row(tau0, a.H);
     st.row_wd = st.row_ld;
+    st.row_cur = st.row_ld;
     st.row_old = PW::wrap_row(tau0 - NTV, a.H);
     st.load_words(a, 0, zrow);  // tau0
     st.load_words(a, 1, zrow);  // tau0 + 1
@@ -512,6 +524,13 @@ int prox_fl2d_auto_rows(int W, int H, int nz, int n_tv) {
         return 0;
     const long slots = (long)nb * kProxWaves * cus;  // resident waves
     const int nstrips = prox_geom(W, H, n_tv, H).nstrips;
+    // Chunks of about 205 rows when they give at least two rounds of resident waves: waves
+    // with data run longer, and the dispatcher balances them only over several rounds (on
+    // MI355X at 1024^2 x 128, 205 rows was best or tied-best with 3, 4 and 5 waves per SIMD,
+    // and one round of 512-row chunks 7 % slower).  Smaller problems take the chunk that
+    // fills the device in the fewest rounds of the shortest waves.
+    const int nch_target = std::max(1, (H + 102) / 205);
+    if ((long)nstrips * nch_target * nz >= 2 * slots) return (H + nch_target - 1) / nch_target;
     long best_cost = -1;
     int best_rc = 0;
     for (int nch = 1; nch <= H; ++nch) {
