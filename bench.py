@@ -431,7 +431,8 @@ def main():
             "fp32_tflops": iteration_flops(W, H, Nz) * ips / 1e12,
             "fp32_frac": iteration_flops(W, H, Nz) * ips / 1e12 / FP32_PEAK_TFLOPS,
         },
-        "kernels_ms_per_iteration": {k: v[1] / n_after for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][1])},
+        "kernels_ms_per_iteration": {k: v[1] / n_after for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][1])
+                                     if v[0] > 0},
         "kernels_breakdown_source": f"every kernel timed over {n_after} iterations after the timed region",
         "gpu_ms_per_iteration_after": gpu_ms_after / n_after,
         "solver": {"iterations_total": args.warmup + args.steps + n_after,
