@@ -241,6 +241,24 @@ def test_solve_mode7_tv_iterations_match_oracle(pkg, orc, scenes, gpu_available,
     assert 0 < st["nnz"] < vol.size // 10  # sparse: the maps are in use
 
 
+def test_solve_complex_hologram_matches_oracle(pkg, orc, scenes, gpu_available):
+    """A hologram with an imaginary part (the API takes complex64): the estimate stays real,
+    so the residual's imaginary part is -Im(b) and its spectrum is not Hermitian-symmetric;
+    the solve, its objective and the returned estimate and residual follow the oracle."""
+    c = cfg(64, 64, 6, its=8, reg=0.05, tv=0.02)
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 6, 2.0, 4.0, 31, 0.01)
+    rng = np.random.default_rng(32)
+    holo = (holo + 1j * (0.1 * rng.standard_normal(holo.shape))).astype(np.complex64)
+    vol, st, ref, rst, (est, res, rest, rres) = _solve_both(pkg, orc, c, holo)
+    assert rel(vol, ref) < 1e-4, rel(vol, ref)
+    assert list(st["restarts"]) == list(rst["restarts"])
+    assert list(st["backtracks"]) == list(rst["backtracks"])
+    np.testing.assert_allclose(st["f"], rst["f"], rtol=1e-4)
+    assert np.all(est.imag == 0)
+    assert rel(est, rest) < 1e-4 and rel(res, rres) < 1e-3
+    np.testing.assert_allclose(res.imag, -holo.imag, rtol=0, atol=1e-6)
+
+
 def test_device_pointers_match_host(pkg, scenes, gpu_available):
     import torch
     c = cfg(64, 64, 4, its=3)
