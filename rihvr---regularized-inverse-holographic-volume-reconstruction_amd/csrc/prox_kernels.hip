@@ -143,7 +143,7 @@ template <int NTV> struct ProxWave {
     // A wave-uniform word of a map this kernel does not write: a scalar load through the
     // constant address space (its own counter, so it never waits on the vector loads in
     // flight for the rows ahead).
-    static __device__ __forceinline__ uint64_t uniform_word(const uint64_t* map, int i) {
+    static __device__ __forceinline__ uint64_t uniform_word(const uint64_t* map, unsigned i) {
         const __attribute__((address_space(4))) uint64_t* cmap =
             (const __attribute__((address_space(4))) uint64_t*)map;
         return cmap[i];
@@ -157,12 +157,16 @@ template <int NTV> struct ProxWave {
     unsigned rowb;                // bytes per row
     int row_ld, row_wd, row_old;  // rows of the next load_row, load_words and old-map read
     int row_cur;                  // row tau of the step
+    // this plane's row words of the maps of y_{k-1}, y_{k-2} and the output's current content
+    const uint64_t* m0;
+    const uint64_t* mm;
+    const uint64_t* mold;
     static __device__ __forceinline__ int next_row(int r, int H) { return r + 1 == H ? 0 : r + 1; }
 
     // occupancy words of y_{k-1} and y_{k-2} for row row_wd, one step before its loads
     __device__ __forceinline__ void load_words(const ProxDev& a, int slot, int zrow) {
-        ow0[slot] = uniform_word(a.occ_y0, zrow + row_wd);
-        owm[slot] = uniform_word(a.occ_ym, zrow + row_wd);
+        ow0[slot] = uniform_word(m0, (unsigned)row_wd);
+        owm[slot] = uniform_word(mm, (unsigned)row_wd);
         row_wd = next_row(row_wd, a.H);
     }
     // Element at byte offset xoff (per lane) of the row that starts off bytes into the
@@ -205,7 +209,7 @@ template <int NTV> struct ProxWave {
             row_cur = next_row(row_cur, a.H);
             hist[0][SLOT] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
             // the output's current map for the row written in the next step
-            oold[Q] = uniform_word(a.occ_old, zrow + row_old);
+            oold[Q] = uniform_word(mold, (unsigned)row_old);
             row_old = next_row(row_old, a.H);
             load_row(a, PAR, xoff, tau + 2 <= tau_end);
             if (tau + 3 <= tau_end) load_words(a, Q, zrow);
@@ -433,13 +437,16 @@ __global__ void __launch_bounds__(64 * kProxWaves)
     st.row_ld = PW::wrap_row(tau0, a.H);
     st.row_wd = st.row_ld;
     st.row_cur = st.row_ld;
+    st.m0 = a.occ_y0 + zrow;
+    st.mm = a.occ_ym + zrow;
+    st.mold = a.occ_old + zrow;
     st.row_old = PW::wrap_row(tau0 - NTV, a.H);
     st.load_words(a, 0, zrow);  // tau0
     st.load_words(a, 1, zrow);  // tau0 + 1
     st.load_row(a, 0, xoff, true);  // tau0
     st.load_row(a, 1, xoff, true);  // tau0 + 1
     st.load_words(a, 0, zrow);  // tau0 + 2
-    st.oold[0] = PW::uniform_word(a.occ_old, zrow + PW::wrap_row(tau0 - NTV - 1, a.H));
+    st.oold[0] = PW::uniform_word(st.mold, (unsigned)PW::wrap_row(tau0 - NTV - 1, a.H));
     int tau = tau0;
     for (; tau + U - 1 <= tau1; tau += U)
         st.block(std::make_integer_sequence<int, U>{}, a, tau, r0, r1, xoff, ocol, out_lane,
