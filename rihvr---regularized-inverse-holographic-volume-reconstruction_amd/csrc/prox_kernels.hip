@@ -164,10 +164,10 @@ template <int NTV> struct ProxWave {
     static __device__ __forceinline__ int next_row(int r, int H) { return r + 1 == H ? 0 : r + 1; }
 
     // occupancy words of y_{k-1} and y_{k-2} for row row_wd, one step before its loads
-    __device__ __forceinline__ void load_words(const ProxDev& a, int slot, int zrow) {
+    __device__ __forceinline__ void load_words(int slot, int H) {
         ow0[slot] = uniform_word(m0, (unsigned)row_wd);
         owm[slot] = uniform_word(mm, (unsigned)row_wd);
-        row_wd = next_row(row_wd, a.H);
+        row_wd = next_row(row_wd, H);
     }
     // Element at byte offset xoff (per lane) of the row that starts off bytes into the
     // plane at base (wave-uniform): the row address stays in scalar registers.
@@ -212,7 +212,7 @@ template <int NTV> struct ProxWave {
             oold[Q] = uniform_word(mold, (unsigned)row_old);
             row_old = next_row(row_old, a.H);
             load_row(a, PAR, xoff, tau + 2 <= tau_end);
-            if (tau + 3 <= tau_end) load_words(a, Q, zrow);
+            if (tau + 3 <= tau_end) load_words(Q, a.H);
         }
         // ---- TV levels: level k works on row tau - k ----
 #pragma unroll
@@ -441,11 +441,11 @@ __global__ void __launch_bounds__(64 * kProxWaves)
     st.mm = a.occ_ym + zrow;
     st.mold = a.occ_old + zrow;
     st.row_old = PW::wrap_row(tau0 - NTV, a.H);
-    st.load_words(a, 0, zrow);  // tau0
-    st.load_words(a, 1, zrow);  // tau0 + 1
+    st.load_words(0, a.H);  // tau0
+    st.load_words(1, a.H);  // tau0 + 1
     st.load_row(a, 0, xoff, true);  // tau0
     st.load_row(a, 1, xoff, true);  // tau0 + 1
-    st.load_words(a, 0, zrow);  // tau0 + 2
+    st.load_words(0, a.H);  // tau0 + 2
     st.oold[0] = PW::uniform_word(st.mold, (unsigned)PW::wrap_row(tau0 - NTV - 1, a.H));
     int tau = tau0;
     for (; tau + U - 1 <= tau1; tau += U)
