@@ -11,6 +11,8 @@
 // functor (epilogue: global stores, phase factor, z-sum accumulation).
 // Intermediate passes exchange data through one LDS buffer per transform.
 #pragma once
+#include <utility>
+
 #include "device_common.hpp"
 
 namespace rihvr {
@@ -51,14 +53,15 @@ RIHVR_PLAN(640, 16, 8, 5)
 RIHVR_PLAN(768, 16, 16, 3)
 RIHVR_PLAN(832, 13, 8, 8)
 RIHVR_PLAN(896, 16, 8, 7)
-RIHVR_PLAN(1024, 16, 8, 8)
+RIHVR_PLAN(1024, 32, 32)
 #undef RIHVR_PLAN
 
 // Plans for row transforms (one transform per 1 or more waves, several transforms per
 // workgroup).  Where the row plan has a smaller first radix than the column plan, a
 // thread holds fewer elements, so the row kernels -- which keep per-element data in
-// registers across planes -- fit more waves per SIMD.  Column kernels keep FftPlan (their
-// workgroup of 16 transforms must stay within 1024 threads).
+// registers across planes -- fit more waves per SIMD.  Column kernels use FftPlan (their
+// workgroup of 16 transforms must stay within 1024 threads), 32 x 32 at 1024 (one LDS
+// exchange instead of two).
 template <int L> struct RowPlan : FftPlan<L> {};
 template <> struct RowPlan<1024> {
     using radices = Radices<8, 8, 8, 2>;
@@ -159,6 +162,43 @@ template <int SIGN, int L> struct Dft<16, SIGN, L> {
             for (int k2 = 0; k2 < 4; ++k2) t[k1 + 4 * k2] = v[4 * k1 + k2];
 #pragma unroll
         for (int i = 0; i < 16; ++i) v[i] = t[i];
+    }
+};
+// multiply by W_32^m (m in 0..15) with compile-time m: even m are W_16^(m/2)
+template <int SIGN, int M> __device__ __forceinline__ float2 mul_w32(float2 a) {
+    if constexpr ((M & 1) == 0) {
+        return mul_w16<SIGN, M / 2>(a);
+    } else {
+        // cos and sin of 2 pi m / 32 for odd m = 1, 3, ..., 15
+        constexpr float c32[8] = {0.98078528040323043f, 0.83146961230254524f, 0.55557023301960218f,
+                                  0.19509032201612825f, -0.19509032201612825f, -0.55557023301960218f,
+                                  -0.83146961230254524f, -0.98078528040323043f};
+        constexpr float s32[8] = {0.19509032201612825f, 0.55557023301960218f, 0.83146961230254524f,
+                                  0.98078528040323043f, 0.98078528040323043f, 0.83146961230254524f,
+                                  0.55557023301960218f, 0.19509032201612825f};
+        constexpr float c = c32[M / 2];
+        constexpr float s = SIGN < 0 ? -s32[M / 2] : s32[M / 2];
+        return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+template <int SIGN, int L> struct Dft<32, SIGN, L> {
+    // radix 2 over two radix-16 DFTs: X[k] = E[k] + W32^k O[k], X[k + 16] = E[k] - W32^k O[k]
+    template <int... K>
+    __device__ static __forceinline__ void combine(float2 (&v)[32], const float2 (&e)[16], float2 (&o)[16],
+                                                   std::integer_sequence<int, K...>) {
+        ((o[K] = mul_w32<SIGN, K>(o[K])), ...);
+        ((v[K] = cadd(e[K], o[K]), v[K + 16] = csub(e[K], o[K])), ...);
+    }
+    __device__ static __forceinline__ void run(float2 (&v)[32], const float2* tw) {
+        float2 e[16], o[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        Dft<16, SIGN, L>::run(e, tw);
+        Dft<16, SIGN, L>::run(o, tw);
+        combine(v, e, o, std::make_integer_sequence<int, 16>{});
     }
 };
 // generic direct DFT (odd radices); roots from the L-point twiddle table

@@ -19,8 +19,19 @@ template <int L> struct RowCfg {
     // 16-element store stride of a radix-16 pass across the banks)
     static constexpr int LP = L + L / 16 + 1;
 };
-template <int L> struct ColCfg {
-    static constexpr int TPT = FftPlan<L>::TPT;
+enum ColMode { COL_PLAIN = 0, COL_ADJ = 1, COL_FWD = 2, COL_REAL = 3 };
+// Column plans: the dense passes use FftPlan (32 x 32 at 1024: one LDS exchange); the sparse
+// forward pass keeps 16 x 8 x 8 on 64 threads per column (its workgroups mostly leave at once
+// or load few rows, and more threads per column finish a strip sooner: measured 0.060 ms
+// against 0.066 ms with 32 x 32 at 1024^2 x 128).
+template <int L, int MODE> struct ColPlan : FftPlan<L> {};
+template <> struct ColPlan<1024, COL_FWD> {
+    using radices = Radices<16, 8, 8>;
+    static constexpr int first = 16;
+    static constexpr int TPT = 1024 / 16;
+};
+template <int L, int MODE> struct ColCfg {
+    static constexpr int TPT = ColPlan<L, MODE>::TPT;
     static constexpr int THREADS = TPT * kColStrip;
 };
 
@@ -180,8 +191,6 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
 // (or of a chunk of planes, accumulating).  LDS layout [j][16] keeps the 16
 // columns of one row contiguous.
 // ----------------------------------------------------------------------------
-enum ColMode { COL_PLAIN = 0, COL_ADJ = 1, COL_FWD = 2, COL_REAL = 3 };
-
 struct ColArgs {
     const float2* in;       // [planes][L][W]
     float2* out;            // PLAIN/ADJ: [planes][L][W]; FWDACC: [chunks][L][W]; REAL: est [L][W]
@@ -201,9 +210,9 @@ struct ColArgs {
 };
 
 template <int L, int SIGN, int MODE>
-__global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
-    using C = ColCfg<L>;
-    using Plan = FftPlan<L>;
+__global__ void __launch_bounds__((ColCfg<L, MODE>::THREADS)) k_cols(ColArgs a) {
+    using C = ColCfg<L, MODE>;
+    using Plan = ColPlan<L, MODE>;
     if constexpr (MODE == COL_FWD) {
         // a strip with no line set in its plane transforms to zeros: not written, and the
         // workgroup leaves before anything else (uniform: every thread reads the same word)
@@ -229,7 +238,7 @@ __global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
         float2* dst = a.out + (size_t)z * L * W + col;
         float2 ph = make_float2(1.0f, 0.0f);
         if constexpr (MODE == COL_ADJ) ph = a.phase[z];  // e^{+i phi_z} (sparse_phase_scale_kernel2)
-        FftRun<L, SIGN, typename Plan::radices>::run(
+        FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
             tw_s, t, [&](int, int, int j) { return src[(size_t)j * W]; },
             [&](int, int, int j, float2 v) {
                 if constexpr (MODE == COL_ADJ)
@@ -264,7 +273,7 @@ __global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
         const float px = ph.x, py = -ph.y;
         const float2* src = a.in + (size_t)z * L * W + col;
         float2* dst = a.out + (size_t)z * L * W + col;
-        FftRun<L, SIGN, typename Plan::radices>::run(
+        FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
             tw_s, t,
             [&](int, int, int j) {
                 if (!((rows_s[j >> 5] >> (j & 31)) & 1u)) return make_float2(0.0f, 0.0f);
@@ -275,7 +284,7 @@ __global__ void __launch_bounds__(ColCfg<L>::THREADS) k_cols(ColArgs a) {
     } else {  // COL_REAL: est = Re(IFFT(...)), r = est - meas, partial sum |r|^2
         const float2* src = a.in + col;
         double s = 0.0;
-        FftRun<L, SIGN, typename Plan::radices>::run(
+        FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
             tw_s, t, [&](int, int, int j) { return src[(size_t)j * W]; },
             [&](int, int, int j, float2 v) {
                 const size_t o = (size_t)j * W + col;
@@ -327,7 +336,7 @@ static hipError_t launch_rows_t(const RowArgs& a, int nrows_total_or_plane, int 
 template <int L, int SIGN, int MODE>
 static hipError_t launch_cols_t(const ColArgs& a, int gy, hipStream_t s) {
     dim3 grid(a.W / kColStrip, gy);
-    hipLaunchKernelGGL((k_cols<L, SIGN, MODE>), grid, dim3(ColCfg<L>::THREADS), 0, s, a);
+    hipLaunchKernelGGL((k_cols<L, SIGN, MODE>), grid, dim3(ColCfg<L, MODE>::THREADS), 0, s, a);
     return hipGetLastError();
 }
 
