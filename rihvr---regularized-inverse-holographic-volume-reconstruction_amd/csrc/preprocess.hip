@@ -19,24 +19,43 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kItems = 16;  // pixels per thread in the mean pass
+constexpr int kItems = 4;  // 16-byte groups per thread in the mean pass
+constexpr int kMaxGridYZ = 65535;
+constexpr int kPix = 4;    // output pixels per thread in the write pass when rows allow it
 
 template <typename T>
 __device__ __forceinline__ float grey(const T* p, long i) {
     return (float)p[i];
 }
 
-// per frame (blockIdx.y) and block: double sum of the rescaled pixels
+// per frame (blockIdx.y) and block: double sum of the rescaled pixels, read 16 bytes at a time
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_pre_sum(const T* __restrict__ frames, long N, float inv255,
                                                       double* __restrict__ partial) {
+    constexpr int G = 16 / sizeof(T);  // pixels per group
     __shared__ double red[kThreads / 64];
     const T* f = frames + (size_t)blockIdx.y * N;
     double s = 0.0;
+    const long ng = N / G;  // whole groups; frames start at multiples of N elements
+    const bool aligned = ((size_t)f % 16) == 0;
     for (int it = 0; it < kItems; ++it) {
-        const long i = ((long)blockIdx.x * kItems + it) * kThreads + threadIdx.x;
-        if (i < N) s += (double)(grey(f, i) * inv255);
+        const long q = ((long)blockIdx.x * kItems + it) * kThreads + threadIdx.x;
+        if (q >= ng) break;
+        if (aligned) {
+            union {
+                uint4 u;
+                T px[G];
+            } v;
+            v.u = ((const uint4*)f)[q];
+#pragma unroll
+            for (int k = 0; k < G; ++k) s += (double)((float)v.px[k] * inv255);
+        } else {
+#pragma unroll
+            for (int k = 0; k < G; ++k) s += (double)(grey(f, G * q + k) * inv255);
+        }
     }
+    // the last N mod G pixels
+    if (blockIdx.x == 0 && threadIdx.x < (unsigned)(N - G * ng)) s += (double)(grey(f, G * ng + threadIdx.x) * inv255);
     s = rihvr::block_sum(s, red);
     if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
@@ -57,39 +76,53 @@ struct Geometry {
     int rw, rh, pad, ow, oh;
 };
 
-// one thread per output pixel of every frame
+// real part of output pixel (x, y) of frame f
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_pre_write(const T* __restrict__ frames,
-                                                        const float* __restrict__ background, Geometry g,
-                                                        int n_frames, float inv255,
-                                                        const float* __restrict__ inv_mean,
-                                                        float2* __restrict__ out) {
+__device__ __forceinline__ float pre_pixel(const T* __restrict__ frames, const float* __restrict__ background,
+                                           const Geometry& g, int f, int x, int y, float inv255,
+                                           const float* __restrict__ inv_mean) {
 #pragma clang fp contract(off)  // the host function's separate float roundings
-    const long per = (long)g.ow * g.oh;
-    const long total = per * n_frames;
-    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < total; i += (long)gridDim.x * kThreads) {
-        const int f = (int)(i / per);
-        const long r = i - (long)f * per;
-        const int y = (int)(r / g.ow);
-        const int x = (int)(r - (long)y * g.ow);
-        const int cx = x - g.pad, cy = y - g.pad;
-        float v = 0.0f;
-        if (cx >= 0 && cx < g.rw && cy >= 0 && cy < g.rh) {
-            const int sx = g.src_x0 + cx, sy = g.src_y0 + cy;
-            if (sx >= 0 && sx < g.W && sy >= 0 && sy < g.H) {
-                const long si = (long)sy * g.W + sx;
-                const float h = grey(frames + (size_t)f * g.W * g.H, si) * inv255;
-                if (background) {
-                    const float b = background[si] * inv255;
-                    const float sq = sqrtf(b);
-                    const float sb = sq < 0.1f ? 0.1f : sq;  // std::max(sqrt(b), 0.1f)
-                    v = (h - b) / sb;
-                } else {
-                    v = h * inv_mean[f] - 1.0f;
-                }
+    const int cx = x - g.pad, cy = y - g.pad;
+    float v = 0.0f;
+    if (cx >= 0 && cx < g.rw && cy >= 0 && cy < g.rh) {
+        const int sx = g.src_x0 + cx, sy = g.src_y0 + cy;
+        if (sx >= 0 && sx < g.W && sy >= 0 && sy < g.H) {
+            const long si = (long)sy * g.W + sx;
+            const float h = grey(frames + (size_t)f * g.W * g.H, si) * inv255;
+            if (background) {
+                const float b = background[si] * inv255;
+                const float sq = sqrtf(b);
+                const float sb = sq < 0.1f ? 0.1f : sq;  // std::max(sqrt(b), 0.1f)
+                v = (h - b) / sb;
+            } else {
+                v = h * inv_mean[f] - 1.0f;
             }
         }
-        out[i] = make_float2(v, 0.0f);
+    }
+    return v;
+}
+
+// output pixels PIX * (blockIdx.x * kThreads + threadIdx.x) + [0, PIX) of row blockIdx.y of frame
+// blockIdx.z + f0; PIX = kPix needs ow % kPix == 0 and a 16-byte aligned output (two complex per store)
+template <typename T, int PIX>
+__global__ void __launch_bounds__(kThreads) k_pre_write(const T* __restrict__ frames,
+                                                        const float* __restrict__ background, Geometry g,
+                                                        int f0, float inv255,
+                                                        const float* __restrict__ inv_mean,
+                                                        float2* __restrict__ out) {
+    const int x = PIX * (blockIdx.x * kThreads + threadIdx.x);
+    if (x >= g.ow) return;
+    const int y = blockIdx.y;
+    const int f = f0 + blockIdx.z;
+    float v[PIX];
+#pragma unroll
+    for (int k = 0; k < PIX; ++k) v[k] = pre_pixel(frames, background, g, f, x + k, y, inv255, inv_mean);
+    float2* o = out + ((size_t)f * g.oh + y) * g.ow + x;
+    if constexpr (PIX == 1) {
+        *o = make_float2(v[0], 0.0f);
+    } else {
+#pragma unroll
+        for (int k = 0; k < PIX; k += 2) ((float4*)o)[k / 2] = make_float4(v[k], 0.0f, v[k + 1], 0.0f);
     }
 }
 
@@ -117,17 +150,29 @@ rihvr_status run(const T* frames, const float* bg, const Geometry& g, int n, flo
     DevBuf hm, hp;
     PRE_HIP(hipMalloc(&hm.p, (size_t)n * sizeof(float)));
     if (!bg) {
-        const int blocks = (int)((N + (long)kThreads * kItems - 1) / ((long)kThreads * kItems));
+        const long per_block = (long)kThreads * kItems * (16 / sizeof(T));
+        const int blocks = (int)std::max(1L, (N + per_block - 1) / per_block);
         PRE_HIP(hipMalloc(&hp.p, (size_t)n * blocks * sizeof(double)));
-        hipLaunchKernelGGL(k_pre_sum<T>, dim3(blocks, n), dim3(kThreads), 0, 0, frames, N, inv255, (double*)hp.p);
+        for (int f0 = 0; f0 < n; f0 += kMaxGridYZ)
+            hipLaunchKernelGGL(k_pre_sum<T>, dim3(blocks, std::min(n - f0, kMaxGridYZ)), dim3(kThreads), 0, 0,
+                               frames + (size_t)f0 * N, N, inv255, (double*)hp.p + (size_t)f0 * blocks);
         hipLaunchKernelGGL(k_pre_mean, dim3((n + 63) / 64), dim3(64), 0, 0, (const double*)hp.p, blocks, n, N,
                            (float*)hm.p);
         PRE_HIP(hipGetLastError());
     }
-    const long total = (long)g.ow * g.oh * n;
-    const unsigned grid = (unsigned)std::max(1L, std::min((total + kThreads - 1) / kThreads, 256L * 1024L));
-    hipLaunchKernelGGL(k_pre_write<T>, dim3(grid), dim3(kThreads), 0, 0, frames, bg, g, n, inv255,
-                       (const float*)hm.p, (float2*)out_dev);
+    // frames in launches of at most kMaxGridYZ (the grid's y and z limit)
+    const bool wide = g.ow % kPix == 0 && (size_t)out_dev % 16 == 0;
+    const int pix = wide ? kPix : 1;
+    for (int f0 = 0; f0 < n; f0 += kMaxGridYZ) {
+        const dim3 grid((unsigned)((g.ow + kThreads * pix - 1) / (kThreads * pix)), (unsigned)g.oh,
+                        (unsigned)std::min(n - f0, kMaxGridYZ));
+        if (wide)
+            hipLaunchKernelGGL((k_pre_write<T, kPix>), grid, dim3(kThreads), 0, 0, frames, bg, g, f0, inv255,
+                               (const float*)hm.p, (float2*)out_dev);
+        else
+            hipLaunchKernelGGL((k_pre_write<T, 1>), grid, dim3(kThreads), 0, 0, frames, bg, g, f0, inv255,
+                               (const float*)hm.p, (float2*)out_dev);
+    }
     PRE_HIP(hipGetLastError());
     PRE_HIP(hipDeviceSynchronize());
     return RIHVR_OK;
@@ -151,6 +196,7 @@ rihvr_status rihvr_prepare_holograms(const rihvr_file_params* fp, const void* fr
     *out_width = cg.ow;
     *out_height = cg.oh;
     if (!out) return RIHVR_OK;
+    if (cg.oh > kMaxGridYZ) return RIHVR_E_INVALID_ARGUMENT;  // one output row per grid row
     const Geometry g{width, height, cg.src_x0, cg.src_y0, cg.rw, cg.rh, cg.pad, cg.ow, cg.oh};
     const size_t in_bytes = (size_t)n_frames * width * height * (size_t)pixel_type;
     const size_t out_bytes = (size_t)n_frames * cg.ow * cg.oh * 8;

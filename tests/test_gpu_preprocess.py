@@ -2,7 +2,8 @@
 frames) against rihvr_prepare_hologram, the host function the CPU tests pin against a
 numpy restatement: every frame bit for bit, for 8-bit, 16-bit and float frames, mean
 normalisation and background removal, the centre/size and rectangle crops (inside the
-image and past its borders), zero padding, host and device buffers; and the errors."""
+image and past its borders), zero padding, odd frame and output sizes, host and device
+buffers; and the errors."""
 from __future__ import annotations
 
 import numpy as np
@@ -51,6 +52,21 @@ def test_device_equals_host(pkg, gpu_available, dtype, case, remove_bg):
         bg[0, :5] = 0.0  # sqrt(B) below 0.1 takes the floor
     got = pkg.prepare_holograms(fp, fr, bg)
     for k in range(3):
+        exp = pkg.prepare_hologram(fp, fr[k].astype(np.float32), bg)
+        np.testing.assert_array_equal(got[k].view(np.uint32), exp.view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+@pytest.mark.parametrize("remove_bg", [False, True])
+def test_odd_sizes_equal_host(pkg, gpu_available, dtype, remove_bg):
+    # 37 x 45 frames: a frame size that is no multiple of 16 bytes (the mean pass's unaligned
+    # frames and its tail), and an odd output width 35 (one pixel per thread in the write pass)
+    fp = _fp(pkg, remove_bg=remove_bg, center=(20, 18), size=33, pad=1)
+    fr = frames_of(dtype, 5, 37, 45, seed=11)
+    bg = (np.random.default_rng(12).random((37, 45)) * 200).astype(np.float32) if remove_bg else None
+    got = pkg.prepare_holograms(fp, fr, bg)
+    assert got.shape == (5, 35, 35)
+    for k in range(5):
         exp = pkg.prepare_hologram(fp, fr[k].astype(np.float32), bg)
         np.testing.assert_array_equal(got[k].view(np.uint32), exp.view(np.uint32))
 
