@@ -6,6 +6,7 @@
 // sum, one for the statistics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -17,35 +18,69 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kItems = 32;  // pixels per thread in the moment pass
+constexpr int kBatch = 8;   // frame loads issued before their adds in the sum pass
 
 // B = (f_0 + f_1 + ... + f_{n-1}) * scale, summed in frame order in float as the
-// reference's running CuMat sum does; scale = float(1/n) as cv::Mat /= n.
-__global__ void __launch_bounds__(kThreads) k_bg_sum(const float* __restrict__ frames, int n, long N, float scale,
-                                                     float* __restrict__ bg) {
+// reference's running CuMat sum does; scale = float(1/n) as cv::Mat /= n.  V = float4
+// takes four pixels per thread (N % 4 == 0, 16-byte aligned frames), V = float one.
+__device__ __forceinline__ float vadd(float a, float b) { return a + b; }
+__device__ __forceinline__ float4 vadd(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float vscale(float a, float s) { return a * s; }
+__device__ __forceinline__ float4 vscale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
+
+template <typename V>
+__global__ void __launch_bounds__(kThreads) k_bg_sum(const V* __restrict__ frames, int n, long NV, float scale,
+                                                     V* __restrict__ bg) {
     const long i = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= N) return;
-    float s = frames[i];
-    for (int k = 1; k < n; ++k) s += frames[(size_t)k * N + i];
-    bg[i] = s * scale;
+    if (i >= NV) return;
+    V s = frames[i];
+    int k = 1;
+    for (; k + kBatch <= n; k += kBatch) {
+        V v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) v[j] = frames[(size_t)(k + j) * NV + i];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) s = vadd(s, v[j]);
+    }
+    for (; k < n; ++k) s = vadd(s, frames[(size_t)k * NV + i]);
+    bg[i] = vscale(s, scale);
 }
 
-// Per frame (blockIdx.y) and block: sum of d and d^2 over its pixels, d = f - B in
+// Per frame (k0 + blockIdx.y) and block: sum of d and d^2 over its pixels, d = f - B in
 // float (the Mat subtraction), accumulated in double (cv::meanStdDev).  Block 0 of
-// frame row n (blockIdx.y == n) sums B itself for cv::mean(B).
-__global__ void __launch_bounds__(kThreads) k_bg_moments(const float* __restrict__ frames, int n, long N,
-                                                         const float* __restrict__ bg, double* __restrict__ partial) {
+// frame row n (k == n) sums B itself for cv::mean(B).  V as in k_bg_sum.
+__device__ __forceinline__ void moments_add(float f, float b, double& s1, double& s2) {
+    const float d = f - b;
+    s1 += (double)d;
+    s2 += (double)d * (double)d;
+}
+__device__ __forceinline__ void moments_add(float4 f, float4 b, double& s1, double& s2) {
+    moments_add(f.x, b.x, s1, s2);
+    moments_add(f.y, b.y, s1, s2);
+    moments_add(f.z, b.z, s1, s2);
+    moments_add(f.w, b.w, s1, s2);
+}
+__device__ __forceinline__ double pixel_sum(float b) { return (double)b; }
+__device__ __forceinline__ double pixel_sum(float4 b) {
+    return (((double)b.x + (double)b.y) + (double)b.z) + (double)b.w;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kThreads) k_bg_moments(const V* __restrict__ frames, int n, int k0, long NV,
+                                                         const V* __restrict__ bg, double* __restrict__ partial) {
+    constexpr int items = kItems / (sizeof(V) / sizeof(float));
     __shared__ double red[kThreads / 64];
-    const int k = blockIdx.y;
+    const int k = k0 + blockIdx.y;
     double s1 = 0.0, s2 = 0.0;
-    for (int it = 0; it < kItems; ++it) {
-        const long i = ((long)blockIdx.x * kItems + it) * kThreads + threadIdx.x;
-        if (i >= N) break;
+    for (int it = 0; it < items; ++it) {
+        const long i = ((long)blockIdx.x * items + it) * kThreads + threadIdx.x;
+        if (i >= NV) break;
         if (k < n) {
-            const float d = frames[(size_t)k * N + i] - bg[i];
-            s1 += (double)d;
-            s2 += (double)d * (double)d;
+            moments_add(frames[(size_t)k * NV + i], bg[i], s1, s2);
         } else {
-            s1 += (double)bg[i];
+            s1 += pixel_sum(bg[i]);
         }
     }
     const double a = rihvr::block_sum(s1, red);
@@ -124,14 +159,30 @@ rihvr_status rihvr_background_mean(const float* frames, int32_t n_frames, int32_
     if (st) return st;
     BG_HIP(hipMalloc(&hb.p, N * sizeof(float)));
     float* dbg = (float*)hb.p;
-    const unsigned g1 = (unsigned)((N + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_bg_sum, dim3(g1), dim3(kThreads), 0, 0, df, n_frames, N, (float)(1.0 / n_frames), dbg);
+    // four pixels per thread when every frame starts on a 16-byte boundary (hipMalloc'd B does)
+    const bool wide = N % 4 == 0 && (size_t)df % 16 == 0;
+    const long NV = wide ? N / 4 : N;
+    const float scale = (float)(1.0 / n_frames);
+    const unsigned g1 = (unsigned)((NV + kThreads - 1) / kThreads);
+    if (wide)
+        hipLaunchKernelGGL(k_bg_sum<float4>, dim3(g1), dim3(kThreads), 0, 0, (const float4*)df, n_frames, NV, scale,
+                           (float4*)dbg);
+    else
+        hipLaunchKernelGGL(k_bg_sum<float>, dim3(g1), dim3(kThreads), 0, 0, df, n_frames, NV, scale, dbg);
     BG_HIP(hipGetLastError());
     const unsigned g2 = (unsigned)((N + (long)kThreads * kItems - 1) / ((long)kThreads * kItems));
     const size_t np = (size_t)(n_frames + 1) * g2 * 2;
     BG_HIP(hipMalloc(&hp.p, np * sizeof(double)));
-    hipLaunchKernelGGL(k_bg_moments, dim3(g2, n_frames + 1), dim3(kThreads), 0, 0, df, n_frames, N, dbg,
-                       (double*)hp.p);
+    // frame rows 0..n (row n: B itself) in launches of at most 65535 (the grid's y limit)
+    for (int k0 = 0; k0 <= n_frames; k0 += 65535) {
+        const dim3 grid(g2, (unsigned)std::min(n_frames + 1 - k0, 65535));
+        if (wide)
+            hipLaunchKernelGGL(k_bg_moments<float4>, grid, dim3(kThreads), 0, 0, (const float4*)df, n_frames, k0, NV,
+                               (const float4*)dbg, (double*)hp.p);
+        else
+            hipLaunchKernelGGL(k_bg_moments<float>, grid, dim3(kThreads), 0, 0, df, n_frames, k0, NV, dbg,
+                               (double*)hp.p);
+    }
     BG_HIP(hipGetLastError());
     std::vector<double> part(np);
     BG_HIP(hipMemcpy(part.data(), hp.p, np * sizeof(double), hipMemcpyDeviceToHost));

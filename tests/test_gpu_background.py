@@ -25,8 +25,11 @@ def _frames(n, h=48, w=64, seed=3):
     return np.clip(base[None] + rng.normal(0, 12, size=(n, h, w)), 0, 255).round().astype(np.uint8)
 
 
-def test_background_mean_matches_oracle(pkg, bgo, gpu_available):
-    f = _frames(7).astype(np.float32)
+@pytest.mark.parametrize("n, h, w", [(7, 48, 64), (21, 48, 64), (21, 37, 45)])
+def test_background_mean_matches_oracle(pkg, bgo, gpu_available, n, h, w):
+    # 21 frames run the sum pass's batches of 8 loads and its remainder; 37 x 45 pixels
+    # (no multiple of 4) take the one-pixel-per-thread passes
+    f = _frames(n, h, w).astype(np.float32)
     bg, st = pkg.background_mean(f)
     rbg, rst = bgo.mean_background(f)
     np.testing.assert_array_equal(bg, rbg)  # same float summation order
