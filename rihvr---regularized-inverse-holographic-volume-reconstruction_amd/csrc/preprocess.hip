@@ -76,47 +76,51 @@ struct Geometry {
     int rw, rh, pad, ow, oh;
 };
 
-// real part of output pixel (x, y) of frame f
-template <typename T>
-__device__ __forceinline__ float pre_pixel(const T* __restrict__ frames, const float* __restrict__ background,
-                                           const Geometry& g, int f, int x, int y, float inv255,
-                                           const float* __restrict__ inv_mean) {
-#pragma clang fp contract(off)  // the host function's separate float roundings
-    const int cx = x - g.pad, cy = y - g.pad;
-    float v = 0.0f;
-    if (cx >= 0 && cx < g.rw && cy >= 0 && cy < g.rh) {
-        const int sx = g.src_x0 + cx, sy = g.src_y0 + cy;
-        if (sx >= 0 && sx < g.W && sy >= 0 && sy < g.H) {
-            const long si = (long)sy * g.W + sx;
-            const float h = grey(frames + (size_t)f * g.W * g.H, si) * inv255;
-            if (background) {
-                const float b = background[si] * inv255;
-                const float sq = sqrtf(b);
-                const float sb = sq < 0.1f ? 0.1f : sq;  // std::max(sqrt(b), 0.1f)
-                v = (h - b) / sb;
-            } else {
-                v = h * inv_mean[f] - 1.0f;
-            }
-        }
-    }
-    return v;
-}
-
 // output pixels PIX * (blockIdx.x * kThreads + threadIdx.x) + [0, PIX) of row blockIdx.y of frame
-// blockIdx.z + f0; PIX = kPix needs ow % kPix == 0 and a 16-byte aligned output (two complex per store)
+// blockIdx.z + f0; PIX = kPix needs ow % kPix == 0 and a 16-byte aligned output (two complex per store).
+// Every pixel's loads are issued unconditionally (pixel 0 of the frame stands in for a pixel outside
+// the crop or the image) so that they overlap; the select afterwards keeps zero for those pixels.
 template <typename T, int PIX>
 __global__ void __launch_bounds__(kThreads) k_pre_write(const T* __restrict__ frames,
                                                         const float* __restrict__ background, Geometry g,
                                                         int f0, float inv255,
                                                         const float* __restrict__ inv_mean,
                                                         float2* __restrict__ out) {
+#pragma clang fp contract(off)  // the host function's separate float roundings
     const int x = PIX * (blockIdx.x * kThreads + threadIdx.x);
     if (x >= g.ow) return;
     const int y = blockIdx.y;
     const int f = f0 + blockIdx.z;
-    float v[PIX];
+    const T* fr = frames + (size_t)f * g.W * g.H;
+    const int cy = y - g.pad, sy = g.src_y0 + cy;
+    const bool row_in = cy >= 0 && cy < g.rh && sy >= 0 && sy < g.H;
+    bool in[PIX];
+    long si[PIX];
 #pragma unroll
-    for (int k = 0; k < PIX; ++k) v[k] = pre_pixel(frames, background, g, f, x + k, y, inv255, inv_mean);
+    for (int k = 0; k < PIX; ++k) {
+        const int cx = x + k - g.pad, sx = g.src_x0 + cx;
+        in[k] = row_in && cx >= 0 && cx < g.rw && sx >= 0 && sx < g.W;
+        si[k] = in[k] ? (long)sy * g.W + sx : 0;
+    }
+    float h[PIX], v[PIX];
+#pragma unroll
+    for (int k = 0; k < PIX; ++k) h[k] = grey(fr, si[k]) * inv255;
+    if (background) {
+        float bk[PIX];
+#pragma unroll
+        for (int k = 0; k < PIX; ++k) bk[k] = background[si[k]];
+#pragma unroll
+        for (int k = 0; k < PIX; ++k) {
+            const float b = bk[k] * inv255;
+            const float sq = sqrtf(b);
+            const float sb = sq < 0.1f ? 0.1f : sq;  // std::max(sqrt(b), 0.1f)
+            v[k] = in[k] ? (h[k] - b) / sb : 0.0f;
+        }
+    } else {
+        const float im = inv_mean[f];
+#pragma unroll
+        for (int k = 0; k < PIX; ++k) v[k] = in[k] ? h[k] * im - 1.0f : 0.0f;
+    }
     float2* o = out + ((size_t)f * g.oh + y) * g.ow + x;
     if constexpr (PIX == 1) {
         *o = make_float2(v[0], 0.0f);
