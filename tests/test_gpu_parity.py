@@ -2,7 +2,7 @@
 
 Tolerances (relative L2 over the complex arrays):
   operators A, A^T and one prox call : 2e-5 (FP32 FFT / stencil rounding only)
-  full solves                          : 1e-4, identical
+  full solves                          : 1e-4 (the parity bar for a solve), identical
                                          thresholded support except voxels whose
                                          magnitude is within 1e-4 of the threshold,
                                          identical backtrack/restart decisions.
