@@ -309,8 +309,9 @@ rihvr_status rihvr_prepare_hologram(const rihvr_file_params* fp, const float* im
  * of raw grey values (pixel_type RIHVR_PIXEL_U8, _U16 or _F32) at in_loc, background
  * [height][width] float32 raw grey values at in_loc when fp->remove_background; out:
  * [n][H'][W'] complex64 at out_loc (call with out == NULL for the size).  Each frame equals
- * rihvr_prepare_hologram's output for it; the errors are that function's, and
- * RIHVR_E_UNKNOWN_TYPE for another pixel_type.  Runs on the current HIP device. */
+ * rihvr_prepare_hologram's output for it; the errors are that function's,
+ * RIHVR_E_UNKNOWN_TYPE for another pixel_type, and RIHVR_E_INVALID_ARGUMENT for an output
+ * taller than 65535 rows (the size query still answers).  Runs on the current HIP device. */
 #define RIHVR_PIXEL_U8 1
 #define RIHVR_PIXEL_U16 2
 #define RIHVR_PIXEL_F32 4
