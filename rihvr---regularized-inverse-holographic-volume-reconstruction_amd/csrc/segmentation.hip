@@ -62,7 +62,9 @@ __device__ __forceinline__ float reloaded_intensity(float stored) {
 __global__ void __launch_bounds__(kThreads) k_seg_scatter(const int32_t* __restrict__ plane,
                                                           const int32_t* __restrict__ index,
                                                           const float* __restrict__ inten, long n, int Nz, long P,
-                                                          uint32_t* __restrict__ key, uint32_t* __restrict__ maxbits,
+                                                          int W, uint32_t* __restrict__ key,
+                                                          uint32_t* __restrict__ rowflag,
+                                                          uint32_t* __restrict__ maxbits,
                                                           uint32_t* __restrict__ bad) {
     uint32_t m = 0;
     for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) {
@@ -74,6 +76,7 @@ __global__ void __launch_bounds__(kThreads) k_seg_scatter(const int32_t* __restr
             continue;
         }
         atomicMax(&key[(size_t)z * P + idx], intensity_key(v));
+        rowflag[((size_t)z * P + idx) / W] = 1u;  // the row holds a non-zero key
         if (v > 0.0f) m = max(m, __float_as_uint(v));  // v >= 0: bits order as values; NaN skipped
     }
     m = wave_max_u32(m);
@@ -81,8 +84,9 @@ __global__ void __launch_bounds__(kThreads) k_seg_scatter(const int32_t* __restr
 }
 
 // the same from a dense complex volume: intensity = re^2 + im^2 of the non-zero voxels
-__global__ void __launch_bounds__(kThreads) k_seg_dense(const float2* __restrict__ vol, long N,
-                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ maxbits) {
+__global__ void __launch_bounds__(kThreads) k_seg_dense(const float2* __restrict__ vol, long N, int W,
+                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ rowflag,
+                                                        uint32_t* __restrict__ maxbits) {
 #pragma clang fp contract(off)  // re^2 + im^2 with two roundings, as the file's writer computes it
     uint32_t m = 0;
     for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) {
@@ -93,6 +97,7 @@ __global__ void __launch_bounds__(kThreads) k_seg_dense(const float2* __restrict
             const float v = reloaded_intensity(stored);
             k = intensity_key(v);
             if (v > 0.0f) m = max(m, __float_as_uint(v));
+            rowflag[i / W] = 1u;  // the row holds a non-zero key
         }
         key[i] = k;
     }
@@ -119,16 +124,24 @@ __device__ __forceinline__ uint64_t valid_bits(int wi, int R, int W) {
 
 // binary = intensity > thr (float compared in double, binarize_kernel :39-50); the key
 // becomes the value of the masked volume (mask_kernel :52-61): intensity or 0.  One wave
-// per row at a time, kUnroll 64-bit words per step (their loads issued together).
+// per row at a time, kUnroll 64-bit words per step (their loads issued together).  A row
+// whose flag says that all its keys are 0 only gets zero words when skip_empty (the host
+// sets it when 0 > thr is false, so such a row has no voxel above the threshold and no key
+// to rewrite).
 constexpr int kUnroll = 8;
 __global__ void __launch_bounds__(kThreads) k_seg_binarize(uint32_t* __restrict__ key_value_bits,
-                                                           uint64_t* __restrict__ bits, int W, int R, int rows,
-                                                           double thr) {
+                                                           uint64_t* __restrict__ bits,
+                                                           const uint32_t* __restrict__ rowflag, int W, int R,
+                                                           int rows, double thr, int skip_empty) {
     const int lane = threadIdx.x & 63;
     const int waves = gridDim.x * (kThreads / 64);
     for (int row = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); row < rows; row += waves) {
         uint32_t* kr = key_value_bits + (size_t)row * W;
         uint64_t* br = bits + (size_t)row * R;
+        if (skip_empty && rowflag[row] == 0u) {  // wave-uniform: one row per wave
+            for (int w = lane; w < R; w += 64) br[w] = 0ull;
+            continue;
+        }
         for (int w0 = 0; w0 < R; w0 += kUnroll) {
             uint32_t k[kUnroll];
 #pragma unroll
@@ -445,6 +458,7 @@ struct rihvr_segmentation {
     uint32_t* key = nullptr;     // keys, then values (float bits)
     uint64_t* bits = nullptr;    // binary volume, bit-packed rows
     uint64_t* bits2 = nullptr;   // the dilation during the closing
+    uint32_t* rowflag = nullptr; // per row (z * H + y) of the volume: 1 if a key in it is non-zero
     uint32_t* work = nullptr;    // N words: root ranks, or dense copies for the caller
     uint32_t* lab = nullptr;     // labels (object voxels only)
     uint32_t* counts = nullptr;  // per scan block
@@ -619,6 +633,7 @@ rihvr_status rihvr_segmentation_create(int32_t width, int32_t height, int32_t nu
     if (e == hipSuccess) e = hipMalloc((void**)&s->key, (size_t)N * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->bits, (size_t)s->nwords * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&s->bits2, (size_t)s->nwords * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->rowflag, (size_t)height * num_planes * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->work, (size_t)N * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->lab, (size_t)N * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->counts, (size_t)s->nblocks * 4);
@@ -639,7 +654,7 @@ void rihvr_segmentation_destroy(rihvr_segmentation* s) {
     if (!s) return;
     DeviceGuard g(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (void* p : {(void*)s->key, (void*)s->bits, (void*)s->bits2, (void*)s->work, (void*)s->lab, (void*)s->counts,
+    for (void* p : {(void*)s->key, (void*)s->bits, (void*)s->bits2, (void*)s->rowflag, (void*)s->work, (void*)s->lab, (void*)s->counts,
                     (void*)s->offsets, (void*)s->small, s->list})
         if (p) (void)hipFree(p);
     for (int t = 0; t < T_COUNT; ++t)
@@ -658,6 +673,7 @@ rihvr_status rihvr_segmentation_load(rihvr_segmentation* s, int64_t count, const
     rihvr_status st = begin_timer(s, T_LOAD);
     if (st) return st;
     SEG_HIP(hipMemsetAsync(s->key, 0, (size_t)s->N * 4, s->stream));
+    SEG_HIP(hipMemsetAsync(s->rowflag, 0, (size_t)s->H * s->Nz * 4, s->stream));
     SEG_HIP(hipMemsetAsync(s->small, 0, 16 * sizeof(uint32_t), s->stream));
     if (count > 0) {
         const int32_t* dp = plane;
@@ -676,7 +692,7 @@ rihvr_status rihvr_segmentation_load(rihvr_segmentation* s, int64_t count, const
             dv = (const float*)(base + 2 * b);
         }
         hipLaunchKernelGGL(k_seg_scatter, dim3(grid_for(count)), dim3(kThreads), 0, s->stream, dp, di, dv,
-                           (long)count, s->Nz, s->P, s->key, s->small, s->small + 1);
+                           (long)count, s->Nz, s->P, s->W, s->key, s->rowflag, s->small, s->small + 1);
         SEG_HIP(hipGetLastError());
     }
     st = end_timer(s, T_LOAD);
@@ -699,8 +715,9 @@ rihvr_status rihvr_segmentation_load_dense(rihvr_segmentation* s, const float* v
     rihvr_status st = begin_timer(s, T_LOAD);
     if (st) return st;
     SEG_HIP(hipMemsetAsync(s->small, 0, 16 * sizeof(uint32_t), s->stream));
+    SEG_HIP(hipMemsetAsync(s->rowflag, 0, (size_t)s->H * s->Nz * 4, s->stream));
     hipLaunchKernelGGL(k_seg_dense, dim3(grid_for(s->N)), dim3(kThreads), 0, s->stream, (const float2*)dv, s->N,
-                       s->key, s->small);
+                       s->W, s->key, s->rowflag, s->small);
     SEG_HIP(hipGetLastError());
     st = end_timer(s, T_LOAD);
     if (st) return st;
@@ -723,7 +740,7 @@ rihvr_status rihvr_segmentation_binarize(rihvr_segmentation* s, double min_inten
     if (st) return st;
     const int rows = s->H * s->Nz;
     hipLaunchKernelGGL(k_seg_binarize, dim3((unsigned)std::min(rows / 4 + 1, 64 * 1024)), dim3(kThreads), 0,
-                       s->stream, s->key, s->bits, s->W, s->R, rows, thr);
+                       s->stream, s->key, s->bits, s->rowflag, s->W, s->R, rows, thr, !(0.0 > thr) ? 1 : 0);
     SEG_HIP(hipGetLastError());
     st = end_timer(s, T_BINARIZE);
     if (st) return st;
