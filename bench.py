@@ -236,13 +236,15 @@ def segmentation_bench(pkg, scenes, steps: int, warmup: int, iterations: int = 1
     dt = (time.perf_counter() - t0) / steps
     fg = int(seg.mask().sum())
     seg.destroy()
-    # compulsory HBM bytes per stage (N voxels, bit volume N/8 bytes): load reads the
-    # complex volume and writes the u32 keys; binarize reads the keys, writes the values
-    # and the bit volume; close reads and writes the bit volume twice; label reads the bit
-    # volume three times (labels are touched only at object voxels); extract reads it
-    # three times (counts, roots, scatter)
-    stage_bytes = {"load": 12.0 * N, "binarize": 8.0 * N + N / 8, "close": 4 * N / 8, "label": 3 * N / 8,
-                   "extract": 3 * N / 8}
+    # compulsory HBM bytes per stage (N voxels, bit volume N/8 bytes, H*Nz row flags of 4
+    # bytes): load reads the complex volume and the row flags (keys are written only where
+    # non-zero); binarize reads the row flags and writes the bit volume (the keys of the
+    # flagged rows, data-dependent, are not counted); close reads and writes the bit volume
+    # twice; label reads the bit volume three times (labels are touched only at object
+    # voxels); extract reads it three times (counts, roots, scatter)
+    rows = H * Nz
+    stage_bytes = {"load": 8.0 * N + 4.0 * rows, "binarize": 4.0 * rows + N / 8, "close": 4 * N / 8,
+                   "label": 3 * N / 8, "extract": 3 * N / 8}
     dom = max(stage, key=stage.get)
     achieved = stage_bytes[dom] / (stage[dom] * 1e-3) / 1e9
     out = {"metric": "segmentation volumes/s (1024^2x128 reconstruction -> centroids)",

@@ -84,25 +84,61 @@ __global__ void __launch_bounds__(kThreads) k_seg_scatter(const int32_t* __restr
 }
 
 // the same from a dense complex volume: intensity = re^2 + im^2 of the non-zero voxels
+__device__ __forceinline__ void dense_voxel(float re, float im, long i, int W, uint32_t* __restrict__ key,
+                                            uint32_t* __restrict__ rowflag, uint32_t& m) {
+#pragma clang fp contract(off)  // re^2 + im^2 with two roundings, as the file's writer computes it
+    if (re != 0.0f || im != 0.0f) {  // zero keys are already in place (k_seg_clear_rows)
+        const float stored = re * re + im * im;
+        const float v = reloaded_intensity(stored);
+        key[i] = intensity_key(v);
+        if (v > 0.0f) m = max(m, __float_as_uint(v));
+        rowflag[i / W] = 1u;  // the row holds a non-zero key
+    }
+}
+
+// PAIRS: two voxels per 16-byte load (the volume 16-byte aligned), the last voxel of an odd
+// N by thread 0 of block 0; else one voxel per load
+template <bool PAIRS>
 __global__ void __launch_bounds__(kThreads) k_seg_dense(const float2* __restrict__ vol, long N, int W,
                                                         uint32_t* __restrict__ key, uint32_t* __restrict__ rowflag,
                                                         uint32_t* __restrict__ maxbits) {
-#pragma clang fp contract(off)  // re^2 + im^2 with two roundings, as the file's writer computes it
     uint32_t m = 0;
-    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) {
-        const float2 c = vol[i];
-        uint32_t k = 0;
-        if (c.x != 0.0f || c.y != 0.0f) {
-            const float stored = c.x * c.x + c.y * c.y;
-            const float v = reloaded_intensity(stored);
-            k = intensity_key(v);
-            if (v > 0.0f) m = max(m, __float_as_uint(v));
-            rowflag[i / W] = 1u;  // the row holds a non-zero key
+    const long stride = (long)gridDim.x * kThreads;
+    if constexpr (PAIRS) {
+        const float4* v4 = (const float4*)vol;
+        for (long q = (long)blockIdx.x * kThreads + threadIdx.x; q < N / 2; q += stride) {
+            const float4 c = v4[q];
+            dense_voxel(c.x, c.y, 2 * q, W, key, rowflag, m);
+            dense_voxel(c.z, c.w, 2 * q + 1, W, key, rowflag, m);
         }
-        key[i] = k;
+        if ((N & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+            const float2 c = vol[N - 1];
+            dense_voxel(c.x, c.y, N - 1, W, key, rowflag, m);
+        }
+    } else {
+        for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += stride) {
+            const float2 c = vol[i];
+            dense_voxel(c.x, c.y, i, W, key, rowflag, m);
+        }
     }
     m = wave_max_u32(m);
     if ((threadIdx.x & 63) == 0 && m) atomicMax(maxbits, m);
+}
+
+// Between loads the key array is zero outside the rows the last load flagged (binarize
+// rewrites only non-zero keys, and nothing else writes keys), so a load first zeroes the
+// keys of the flagged rows and clears their flags, then writes only its non-zero keys.
+// One wave per row at a time.
+__global__ void __launch_bounds__(kThreads) k_seg_clear_rows(uint32_t* __restrict__ key,
+                                                             uint32_t* __restrict__ rowflag, int W, int rows) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (kThreads / 64);
+    for (int row = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); row < rows; row += waves) {
+        if (rowflag[row] == 0u) continue;  // wave-uniform
+        uint32_t* kr = key + (size_t)row * W;
+        for (int x = lane; x < W; x += 64) kr[x] = 0u;
+        if (lane == 0) rowflag[row] = 0u;
+    }
 }
 
 __device__ __forceinline__ int isqrt_floor(int n) {
@@ -486,6 +522,13 @@ rihvr_status stage_buffer(rihvr_segmentation* s, size_t bytes) {
     return RIHVR_OK;
 }
 
+hipError_t clear_rows(rihvr_segmentation* s) {
+    const int rows = s->H * s->Nz;
+    hipLaunchKernelGGL(k_seg_clear_rows, dim3((unsigned)std::min(rows / 4 + 1, 64 * 1024)), dim3(kThreads), 0,
+                       s->stream, s->key, s->rowflag, s->W, rows);
+    return hipGetLastError();
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int d) {
@@ -641,6 +684,10 @@ rihvr_status rihvr_segmentation_create(int32_t width, int32_t height, int32_t nu
     if (e == hipSuccess) e = hipMalloc((void**)&s->small, 16 * sizeof(uint32_t));
     for (int t = 0; t < T_COUNT && e == hipSuccess; ++t)
         for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&s->ev[t][k]);
+    // the loads rely on zero keys outside the flagged rows
+    if (e == hipSuccess) e = hipMemsetAsync(s->key, 0, (size_t)N * 4, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->rowflag, 0, (size_t)height * num_planes * 4, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
         fail(e);
         rihvr_segmentation_destroy(s);
@@ -672,8 +719,7 @@ rihvr_status rihvr_segmentation_load(rihvr_segmentation* s, int64_t count, const
     std::fill(s->timed, s->timed + T_COUNT, false);
     rihvr_status st = begin_timer(s, T_LOAD);
     if (st) return st;
-    SEG_HIP(hipMemsetAsync(s->key, 0, (size_t)s->N * 4, s->stream));
-    SEG_HIP(hipMemsetAsync(s->rowflag, 0, (size_t)s->H * s->Nz * 4, s->stream));
+    SEG_HIP(clear_rows(s));
     SEG_HIP(hipMemsetAsync(s->small, 0, 16 * sizeof(uint32_t), s->stream));
     if (count > 0) {
         const int32_t* dp = plane;
@@ -715,9 +761,13 @@ rihvr_status rihvr_segmentation_load_dense(rihvr_segmentation* s, const float* v
     rihvr_status st = begin_timer(s, T_LOAD);
     if (st) return st;
     SEG_HIP(hipMemsetAsync(s->small, 0, 16 * sizeof(uint32_t), s->stream));
-    SEG_HIP(hipMemsetAsync(s->rowflag, 0, (size_t)s->H * s->Nz * 4, s->stream));
-    hipLaunchKernelGGL(k_seg_dense, dim3(grid_for(s->N)), dim3(kThreads), 0, s->stream, (const float2*)dv, s->N,
-                       s->W, s->key, s->rowflag, s->small);
+    SEG_HIP(clear_rows(s));
+    if ((size_t)dv % 16 == 0)
+        hipLaunchKernelGGL(k_seg_dense<true>, dim3(grid_for(s->N / 2)), dim3(kThreads), 0, s->stream,
+                           (const float2*)dv, s->N, s->W, s->key, s->rowflag, s->small);
+    else
+        hipLaunchKernelGGL(k_seg_dense<false>, dim3(grid_for(s->N)), dim3(kThreads), 0, s->stream,
+                           (const float2*)dv, s->N, s->W, s->key, s->rowflag, s->small);
     SEG_HIP(hipGetLastError());
     st = end_timer(s, T_LOAD);
     if (st) return st;

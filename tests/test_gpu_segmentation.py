@@ -159,6 +159,39 @@ def test_dense_loader(pkg, so, gpu_available):
     seg.destroy()
 
 
+def test_dense_loads_reuse_odd_unaligned(pkg, so, gpu_available):
+    # one context, three loads: an odd voxel count (the tail of the two-voxel loads), a device
+    # volume starting 8 bytes past a 16-byte boundary (one voxel per load), and a sparser
+    # volume after a denser one (the keys of the rows the last load flagged are cleared)
+    import torch
+    shape = (3, 7, 9)
+    Nz, H, W = shape
+    seg = pkg.SparseSegmentation(W, H, Nz)
+    rng = np.random.default_rng(21)
+    for k, (density, unaligned) in enumerate([(0.3, False), (0.3, True), (0.02, False)]):
+        vol = np.zeros(shape, np.complex64)
+        sel = rng.random(shape) < density
+        vol[sel] = (rng.standard_normal(sel.sum()) + 1j * rng.standard_normal(sel.sum())).astype(np.complex64)
+        if unaligned:
+            buf = torch.zeros(vol.size + 1, dtype=torch.complex64, device="cuda")
+            buf[1:] = torch.from_numpy(vol.ravel()).cuda()
+            dev = buf[1:].view(shape)
+            assert dev.data_ptr() % 16 == 8
+            seg.loadDense(dev)
+        else:
+            seg.loadDense(vol)
+        mx, thr = seg.binarize(10.0)
+        values = seg.values()
+        z, y, x = np.nonzero(vol)
+        v = vol[z, y, x]
+        inten = (v.real * v.real + v.imag * v.imag).astype(np.float32)
+        ref = so.segment(z.astype(np.int32), (y * W + x).astype(np.int32), inten, W, H, Nz, min_intensity=10.0,
+                         close_size=1)
+        assert mx == ref["max_intensity"] and thr == ref["thr"], k
+        np.testing.assert_array_equal(values.view(np.uint32), ref["weight"].view(np.uint32))
+    seg.destroy()
+
+
 def test_device_lists(pkg, so, gpu_available):
     import torch
     shape = (5, 20, 24)
