@@ -167,7 +167,8 @@ __device__ __forceinline__ uint64_t valid_bits(int wi, int R, int W) {
 constexpr int kUnroll = 8;
 __global__ void __launch_bounds__(kThreads) k_seg_binarize(uint32_t* __restrict__ key_value_bits,
                                                            uint64_t* __restrict__ bits,
-                                                           const uint32_t* __restrict__ rowflag, int W, int R,
+                                                           const uint32_t* __restrict__ rowflag,
+                                                           uint32_t* __restrict__ rowbits, int W, int R,
                                                            int rows, double thr, int skip_empty) {
     const int lane = threadIdx.x & 63;
     const int waves = gridDim.x * (kThreads / 64);
@@ -176,8 +177,10 @@ __global__ void __launch_bounds__(kThreads) k_seg_binarize(uint32_t* __restrict_
         uint64_t* br = bits + (size_t)row * R;
         if (skip_empty && rowflag[row] == 0u) {  // wave-uniform: one row per wave
             for (int w = lane; w < R; w += 64) br[w] = 0ull;
+            if (lane == 0) rowbits[row] = 0u;
             continue;
         }
+        uint64_t any = 0;  // wave-uniform: OR of the row's words
         for (int w0 = 0; w0 < R; w0 += kUnroll) {
             uint32_t k[kUnroll];
 #pragma unroll
@@ -193,8 +196,10 @@ __global__ void __launch_bounds__(kThreads) k_seg_binarize(uint32_t* __restrict_
                 if (k[u]) kr[x] = b ? __float_as_uint(v) : 0u;  // key 0 is already value 0
                 const uint64_t word = __ballot(b);
                 if (lane == 0 && w0 + u < R) br[w0 + u] = word;
+                any |= word;
             }
         }
+        if (lane == 0) rowbits[row] = any != 0ull ? 1u : 0u;
     }
 }
 
@@ -209,10 +214,13 @@ __device__ __forceinline__ uint64_t xdilate(uint64_t prev, uint64_t cur, uint64_
 // the ball's rows (dy, dz) of the row's x-dilation by w(dy, dz) = floor(sqrt(r^2 - dy^2 -
 // dz^2)).  Dilation (complement = 0): rows and words outside the volume are empty.
 // Erosion (complement = 1) = NOT dilation of (NOT src with everything outside set): a
-// voxel survives only if its whole ball lies inside the volume and inside src.
+// voxel survives only if its whole ball lies inside the volume and inside src.  rowbits
+// (dilation only, may be NULL): per row of src, 0 if all its words are 0 (such rows add
+// nothing and are not read).
 __global__ void __launch_bounds__(kThreads) k_seg_morph(const uint64_t* __restrict__ src,
-                                                        uint64_t* __restrict__ dst, int W, int R, int H, int Nz,
-                                                        int r, int complement) {
+                                                        uint64_t* __restrict__ dst,
+                                                        const uint32_t* __restrict__ rowbits, int W, int R, int H,
+                                                        int Nz, int r, int complement) {
     const uint32_t nwords = (uint32_t)R * H * Nz;
     const uint64_t flip = complement ? ~0ull : 0ull;  // src words are read as src ^ flip
     for (uint32_t w = blockIdx.x * kThreads + threadIdx.x; w < nwords; w += gridDim.x * kThreads) {
@@ -235,6 +243,7 @@ __global__ void __launch_bounds__(kThreads) k_seg_morph(const uint64_t* __restri
                     acc |= flip;
                     continue;
                 }
+                if (rowbits && rowbits[(size_t)zz * H + yy] == 0u) continue;
                 const uint64_t* rowp = src + ((size_t)zz * H + yy) * R;
                 const uint64_t cur = rowp[wi] ^ flip;
                 const uint64_t prev = wi > 0 ? rowp[wi - 1] ^ flip : flip;
@@ -495,6 +504,8 @@ struct rihvr_segmentation {
     uint64_t* bits = nullptr;    // binary volume, bit-packed rows
     uint64_t* bits2 = nullptr;   // the dilation during the closing
     uint32_t* rowflag = nullptr; // per row (z * H + y) of the volume: 1 if a key in it is non-zero
+    uint32_t* rowbits = nullptr; // per row of the binarized bit volume: 1 if a bit in it is set
+    bool rowbits_valid = false;  // rowbits describes `bits` (from binarize until the first closing)
     uint32_t* work = nullptr;    // N words: root ranks, or dense copies for the caller
     uint32_t* lab = nullptr;     // labels (object voxels only)
     uint32_t* counts = nullptr;  // per scan block
@@ -677,6 +688,7 @@ rihvr_status rihvr_segmentation_create(int32_t width, int32_t height, int32_t nu
     if (e == hipSuccess) e = hipMalloc((void**)&s->bits, (size_t)s->nwords * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&s->bits2, (size_t)s->nwords * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&s->rowflag, (size_t)height * num_planes * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->rowbits, (size_t)height * num_planes * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->work, (size_t)N * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->lab, (size_t)N * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&s->counts, (size_t)s->nblocks * 4);
@@ -701,7 +713,7 @@ void rihvr_segmentation_destroy(rihvr_segmentation* s) {
     if (!s) return;
     DeviceGuard g(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (void* p : {(void*)s->key, (void*)s->bits, (void*)s->bits2, (void*)s->rowflag, (void*)s->work, (void*)s->lab, (void*)s->counts,
+    for (void* p : {(void*)s->key, (void*)s->bits, (void*)s->bits2, (void*)s->rowflag, (void*)s->rowbits, (void*)s->work, (void*)s->lab, (void*)s->counts,
                     (void*)s->offsets, (void*)s->small, s->list})
         if (p) (void)hipFree(p);
     for (int t = 0; t < T_COUNT; ++t)
@@ -790,7 +802,8 @@ rihvr_status rihvr_segmentation_binarize(rihvr_segmentation* s, double min_inten
     if (st) return st;
     const int rows = s->H * s->Nz;
     hipLaunchKernelGGL(k_seg_binarize, dim3((unsigned)std::min(rows / 4 + 1, 64 * 1024)), dim3(kThreads), 0,
-                       s->stream, s->key, s->bits, s->rowflag, s->W, s->R, rows, thr, !(0.0 > thr) ? 1 : 0);
+                       s->stream, s->key, s->bits, s->rowflag, s->rowbits, s->W, s->R, rows, thr,
+                       !(0.0 > thr) ? 1 : 0);
     SEG_HIP(hipGetLastError());
     st = end_timer(s, T_BINARIZE);
     if (st) return st;
@@ -798,6 +811,7 @@ rihvr_status rihvr_segmentation_binarize(rihvr_segmentation* s, double min_inten
     if (max_intensity) *max_intensity = mx;
     if (threshold) *threshold = thr;
     s->stage = BINARIZED;
+    s->rowbits_valid = true;
     s->timed[T_CLOSE] = s->timed[T_LABEL] = s->timed[T_EXTRACT] = false;
     return RIHVR_OK;
 }
@@ -811,10 +825,13 @@ rihvr_status rihvr_segmentation_close(rihvr_segmentation* s, int32_t close_size)
     rihvr_status st = begin_timer(s, T_CLOSE);
     if (st) return st;
     const unsigned grid = grid_for(s->nwords);
-    hipLaunchKernelGGL(k_seg_morph, dim3(grid), dim3(kThreads), 0, s->stream, s->bits, s->bits2, s->W, s->R, s->H,
-                       s->Nz, close_size, 0);
-    hipLaunchKernelGGL(k_seg_morph, dim3(grid), dim3(kThreads), 0, s->stream, s->bits2, s->bits, s->W, s->R, s->H,
-                       s->Nz, close_size, 1);
+    // a closing can set bits in rows that were empty: the row flags hold for the first one only
+    const uint32_t* rb = s->rowbits_valid ? s->rowbits : nullptr;
+    s->rowbits_valid = false;
+    hipLaunchKernelGGL(k_seg_morph, dim3(grid), dim3(kThreads), 0, s->stream, s->bits, s->bits2, rb, s->W, s->R,
+                       s->H, s->Nz, close_size, 0);
+    hipLaunchKernelGGL(k_seg_morph, dim3(grid), dim3(kThreads), 0, s->stream, s->bits2, s->bits,
+                       (const uint32_t*)nullptr, s->W, s->R, s->H, s->Nz, close_size, 1);
     SEG_HIP(hipGetLastError());
     st = end_timer(s, T_CLOSE);
     if (st) return st;

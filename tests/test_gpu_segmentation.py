@@ -192,6 +192,30 @@ def test_dense_loads_reuse_odd_unaligned(pkg, so, gpu_available):
     seg.destroy()
 
 
+def test_two_closings(pkg, so, gpu_available):
+    # the first closing after binarize skips the rows binarize found empty; a second closing
+    # may see bits in such rows and must read every row
+    shape = (8, 24, 70)
+    Nz, H, W = shape
+    rng = np.random.default_rng(5)
+    vol = np.zeros(shape, np.complex64)
+    sel = rng.random(shape) < 0.02
+    vol[sel] = (rng.standard_normal(sel.sum()) + 1j * rng.standard_normal(sel.sum())).astype(np.complex64)
+    vol[3, 5, 10:60:3] = 4 + 0j  # dotted lines with empty rows between them
+    vol[3, 9, 10:60:3] = 4 + 0j
+    seg = pkg.SparseSegmentation(W, H, Nz, dense=vol)
+    seg.binarize(5.0)
+    seg.close(2)
+    seg.close(2)
+    z, y, x = np.nonzero(vol)
+    v = vol[z, y, x]
+    inten = (v.real * v.real + v.imag * v.imag).astype(np.float32)
+    binary, _w, _m, _t = so.binarize(z.astype(np.int32), (y * W + x).astype(np.int32), inten, W, H, Nz, 5.0)
+    exp = so.close(so.close(binary, 2), 2)
+    np.testing.assert_array_equal(seg.mask().reshape(shape) != 0, exp)
+    seg.destroy()
+
+
 def test_device_lists(pkg, so, gpu_available):
     import torch
     shape = (5, 20, 24)
