@@ -40,6 +40,17 @@ def test_background_mean_matches_oracle(pkg, bgo, gpu_available, n, h, w):
     assert e.value.status == 3
 
 
+def test_background_mean_many_frames(pkg, bgo, gpu_available):
+    # 65,537 frames: the statistics pass's frame rows (frames + 1) take two launches of at
+    # most 65,535 grid rows each
+    f = np.random.default_rng(8).integers(0, 256, size=(65537, 4, 4)).astype(np.float32)
+    bg, st = pkg.background_mean(f)
+    rbg, rst = bgo.mean_background(f)
+    np.testing.assert_array_equal(bg, rbg)
+    for k in ("alpha", "beta", "minimum", "average"):
+        assert st[k] == pytest.approx(rst[k], rel=1e-12, abs=1e-12), k
+
+
 def test_background_median5_matches_oracle(pkg, bgo, gpu_available):
     f = _frames(5, seed=4).astype(np.float32)
     np.testing.assert_array_equal(pkg.background_median5(f), bgo.median5(f))

@@ -71,6 +71,17 @@ def test_odd_sizes_equal_host(pkg, gpu_available, dtype, remove_bg):
         np.testing.assert_array_equal(got[k].view(np.uint32), exp.view(np.uint32))
 
 
+def test_many_frames(pkg, gpu_available):
+    # 65,537 frames: the mean and write passes take two launches of at most 65,535 frames
+    fp = _fp(pkg)
+    fr = frames_of(np.uint8, 65537, 4, 4, seed=13)
+    got = pkg.prepare_holograms(fp, fr)
+    assert got.shape == (65537, 4, 4)
+    for k in (0, 1, 65534, 65535, 65536):
+        exp = pkg.prepare_hologram(fp, fr[k].astype(np.float32))
+        np.testing.assert_array_equal(got[k].view(np.uint32), exp.view(np.uint32))
+
+
 def test_device_buffers_and_batch_solve(pkg, gpu_available):
     import torch
     fp = _fp(pkg, pad=32)
