@@ -713,8 +713,8 @@ void rihvr_segmentation_destroy(rihvr_segmentation* s) {
     if (!s) return;
     DeviceGuard g(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (void* p : {(void*)s->key, (void*)s->bits, (void*)s->bits2, (void*)s->rowflag, (void*)s->rowbits, (void*)s->work, (void*)s->lab, (void*)s->counts,
-                    (void*)s->offsets, (void*)s->small, s->list})
+    for (void* p : {(void*)s->key, (void*)s->bits, (void*)s->bits2, (void*)s->rowflag, (void*)s->rowbits,
+                    (void*)s->work, (void*)s->lab, (void*)s->counts, (void*)s->offsets, (void*)s->small, s->list})
         if (p) (void)hipFree(p);
     for (int t = 0; t < T_COUNT; ++t)
         for (int k = 0; k < 2; ++k)
