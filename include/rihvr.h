@@ -309,7 +309,10 @@ rihvr_status rihvr_prepare_hologram(const rihvr_file_params* fp, const float* im
  * of raw grey values (pixel_type RIHVR_PIXEL_U8, _U16 or _F32) at in_loc, background
  * [height][width] float32 raw grey values at in_loc when fp->remove_background; out:
  * [n][H'][W'] complex64 at out_loc (call with out == NULL for the size).  Each frame equals
- * rihvr_prepare_hologram's output for it; the errors are that function's,
+ * rihvr_prepare_hologram's output for it bit for bit for 8-bit frames, whose double sum for the
+ * mean is exact in any order; for 16-bit and float frames the device adds the mean's terms in
+ * block order and the host in raster order, so a value may differ by the rounding of that sum
+ * (at most 1 float ulp).  The errors are that function's,
  * RIHVR_E_UNKNOWN_TYPE for another pixel_type, and RIHVR_E_INVALID_ARGUMENT for an output
  * taller than 65535 rows (the size query still answers).  Runs on the current HIP device. */
 #define RIHVR_PIXEL_U8 1

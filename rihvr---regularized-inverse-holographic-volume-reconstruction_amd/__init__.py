@@ -403,11 +403,23 @@ def prepare_holograms(fp, frames, background=None, out=None):
     bkeep = None
     bptr = None
     if background is not None:
+        # the background goes where the frames are: a contiguous float32 (h, w) CUDA tensor on
+        # the frames' device with device frames, a float32 (h, w) host array with host frames
         if floc == DEVICE:
+            import torch
+            if (not _is_torch(background) or not background.is_cuda or background.dtype != torch.float32
+                    or not background.is_contiguous() or tuple(background.shape) != (h, w)
+                    or background.device != keep.device):
+                raise ValueError(f"with device frames the background must be a contiguous float32 CUDA tensor "
+                                 f"of shape {(h, w)} on {keep.device}")
             bkeep = background
             bptr = ctypes.c_void_p(background.data_ptr())
         else:
+            if _is_torch(background):
+                raise ValueError("with host frames the background must be a host array")
             bkeep = np.ascontiguousarray(background, np.float32)
+            if bkeep.shape != (h, w):
+                raise ValueError(f"background shape {bkeep.shape} != frame shape {(h, w)}")
             bptr = ctypes.c_void_p(bkeep.ctypes.data)
     ow, oh = ctypes.c_int32(), ctypes.c_int32()
     L = lib()

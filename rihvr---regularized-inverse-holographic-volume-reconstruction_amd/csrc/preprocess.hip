@@ -3,8 +3,11 @@
 // rescale /255 (hologram.cu:401-439) -> I/mean - 1 (normalize_mean0, :332-358) or
 // (I - B)/max(sqrt(B), 0.1) (removeBackground, :1365-1432) -> crop (:768-880) -> zero pad
 // (:882-945), written as complex64 holograms ready for the solver.  The arithmetic is
-// rihvr_prepare_hologram's (float products and quotients, the mean summed in double),
-// so each frame equals the host function's output.  HBM-bound: one pass over the frames
+// rihvr_prepare_hologram's (float products and quotients, the mean summed in double).
+// The mean's terms are added in block order here and in raster order on the host: for
+// 8-bit frames that double sum is exact, so each frame equals the host function's output
+// bit for bit; for 16-bit and float frames a value may differ from it by the rounding of
+// the sum (at most 1 float ulp).  HBM-bound: one pass over the frames
 // for the means, one pass writing the holograms.
 #include <hip/hip_runtime.h>
 

@@ -156,3 +156,19 @@ int main() {
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "7", (r.returncode, r.stdout, r.stderr)
+
+
+def test_prepare_holograms_rejects_a_mismatched_background(pkg):
+    """The background must match the frames' location, type and shape: a wrong host shape
+    or a torch tensor with host frames is refused before any library call."""
+    import torch
+    fp = pkg.FileParams()
+    fp.roi_size = -1
+    fp.remove_background = 1
+    frames = np.zeros((2, 16, 16), np.uint8)
+    with pytest.raises(ValueError):
+        pkg.prepare_holograms(fp, frames, np.zeros((16, 8), np.float32))
+    with pytest.raises(ValueError):
+        pkg.prepare_holograms(fp, frames, np.zeros((8, 16, 16), np.float32))
+    with pytest.raises(ValueError):
+        pkg.prepare_holograms(fp, frames, torch.zeros((16, 16)))

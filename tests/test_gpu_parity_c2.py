@@ -1,0 +1,125 @@
+"""GPU parity at full size and through a FISTA restart.
+
+The 1024^2 x 128 workload (plain FISTA + 2D TV, mode 7, 100 FISTA x 5 TV iterations,
+seeded 500 tracers + 20 rods) is solved on the GPU through the C ABI and by the FP32
+oracle on the host cores in the same process:
+  - relative L2 over the returned complex volume <= 1e-4;
+  - identical restart and backtrack decisions, objective per iteration within rtol 1e-4;
+  - thresholded support: every voxel non-zero on one side and zero on the other must be a
+    threshold crossing, i.e. its non-zero value is below BAND_REL * t * lambda_1 in every
+    component (a value that large or larger arises only from a prox input within that
+    distance of the threshold t * lambda_1);
+  - the error of both against the FP64 oracle's solve of the same hologram (committed
+    fixture tests/golden/c2_oracle_f64.npz, written by tests/golden/make_c2_golden.py) is
+    printed; it is dominated by the float rounding of the phase arguments (Q2) and is
+    bounded here by 2e-2.
+"""
+import pathlib
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+BAND_REL = 1e-3
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-30))
+
+
+def support_report(a, b, thr):
+    """Voxels whose support differs, and the largest component magnitude of the non-zero
+    side among them, relative to the threshold thr."""
+    na = (a.real != 0) | (a.imag != 0)
+    nb = (b.real != 0) | (b.imag != 0)
+    diff = na != nb
+    n = int(diff.sum())
+    if n == 0:
+        return 0, 0.0
+    v = np.where(na[diff], a[diff], b[diff])
+    return n, float(np.max(np.maximum(np.abs(v.real), np.abs(v.imag))) / thr)
+
+
+def sparse_to_dense(fix, shape, dtype):
+    out = np.zeros(int(np.prod(shape)), dtype)
+    out[fix["idx"]] = fix["val"]
+    return out.reshape(shape)
+
+
+@pytest.mark.timeout(1200)
+def test_solve_c2_matches_oracle(pkg, orc, scenes, gpu_available):
+    import torch
+    c = dict(scenes.C2, mode=7)
+    holo = scenes.c2_hologram()
+    W, H, Nz = c["width"], c["height"], c["num_planes"]
+    print(f"C2 case: {W}x{H}x{Nz}, {c['num_inverse_iterations']} FISTA x {c['num_tv_iterations']} TV", flush=True)
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        st = rec.inverseReconstruct(torch.from_numpy(holo).cuda())
+        vol = rec.getVolume()
+    print(f"GPU solve done: nnz {st['nnz']} f_final {st['f_final']:.9g}", flush=True)
+    t0 = time.perf_counter()
+    ref, _, _, rst = orc.solve(orc.make_params(**c), holo, None, "f32")
+    print(f"FP32 oracle solve on {orc.get_threads()} host threads: {time.perf_counter() - t0:.1f} s", flush=True)
+    thr = float(np.float32(c["regularization_param"])) / Nz  # t * lambda_1 at t = 1/Nz
+    r = rel(vol, ref)
+    n_mis, band = support_report(vol, ref, thr)
+    f64 = np.load(GOLDEN / "c2_oracle_f64.npz")
+    assert complex(f64["holo_sum"]) == complex(np.sum(holo.astype(np.complex128))), "fixture is for another input"
+    ref64 = sparse_to_dense(f64, vol.shape, np.complex128)
+    r64_gpu, r64_orc = rel(vol, ref64), rel(ref, ref64)
+    print(f"C2 rel-L2 GPU vs FP32 oracle {r:.3e}; support mismatches {n_mis} of {vol.size} "
+          f"(largest {band:.2e} x threshold); nnz {st['nnz']} vs {rst['nnz']}; restarts "
+          f"{int(np.sum(st['restarts']))} vs {int(np.sum(rst['restarts']))}; backtracks "
+          f"{st['total_backtracks']} vs {rst['total_backtracks']}; f_final {st['f_final']:.9g} vs "
+          f"{rst['f_final']:.9g}; FP64 oracle: GPU {r64_gpu:.3e}, FP32 oracle {r64_orc:.3e}", flush=True)
+    assert r <= 1e-4, r
+    assert list(st["restarts"]) == list(rst["restarts"])
+    assert list(st["backtracks"]) == list(rst["backtracks"])
+    np.testing.assert_allclose(st["f"], rst["f"], rtol=1e-4)
+    assert n_mis == 0 or band <= BAND_REL, (n_mis, band)
+    assert r64_gpu <= 2e-2 and r64_orc <= 2e-2
+
+
+def test_solve_c1_support_is_identical(pkg, orc, scenes, gpu_available):
+    """The 256^2 x 32 workload: the same support rule as above, with its FP64 statement."""
+    c = dict(scenes.C1, mode=7)
+    holo = scenes.c1_hologram()
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        st = rec.inverseReconstruct(holo)
+        vol = rec.getVolume()
+    ref, _, _, rst = orc.solve(orc.make_params(**c), holo, None, "f32")
+    ref64, _, _, _ = orc.solve(orc.make_params(**c), holo, None, "f64")
+    thr = float(np.float32(c["regularization_param"])) / c["num_planes"]
+    n_mis, band = support_report(vol, ref, thr)
+    print(f"C1 rel-L2 GPU vs FP32 oracle {rel(vol, ref):.3e}; support mismatches {n_mis} (largest "
+          f"{band:.2e} x threshold); FP64 oracle: GPU {rel(vol, ref64):.3e}, FP32 oracle {rel(ref, ref64):.3e}",
+          flush=True)
+    assert rel(vol, ref) <= 1e-4
+    assert n_mis == 0 or band <= BAND_REL, (n_mis, band)
+    assert rel(vol, ref64) <= 2e-2
+
+
+def test_restart_sum_and_decision_match_oracle(pkg, orc, scenes, gpu_available):
+    """A 32^2 x 2 solve whose restart test fires (from iteration 41): the per-iteration
+    restart reduction (the reference's weights, :4393-4482) and the decisions match the
+    FP32 oracle, so the weights of the GPU reduction are checked where they matter."""
+    c = dict(width=32, height=32, num_planes=2, start_plane=1000.0, plane_stepsize=50.0, wavelength=0.633,
+             resolution=2.0, num_inverse_iterations=60, regularization_param=0.01, regularization_TV_param=0.01,
+             num_tv_iterations=5, mode=7)
+    holo, _ = scenes.disks_scene(scenes.optics_of(c), 3, 2.0, 4.0, 3, 0.01)
+    with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec:
+        st = rec.inverseReconstruct(holo)
+        vol = rec.getVolume()
+    ref, _, _, rst = orc.solve(orc.make_params(**c), holo, None, "f32")
+    print(f"restarts GPU {np.flatnonzero(st['restarts']).tolist()} oracle {np.flatnonzero(rst['restarts']).tolist()}",
+          flush=True)
+    assert int(np.sum(rst["restarts"])) >= 1
+    assert list(st["restarts"]) == list(rst["restarts"])
+    rs, rrs = np.asarray(st["restart_sum"]), np.asarray(rst["restart_sum"])
+    big = np.abs(rrs) > 1e-3 * np.abs(rrs).max()
+    np.testing.assert_allclose(rs[big], rrs[big], rtol=1e-3)
+    assert np.all(np.sign(rs[big]) == np.sign(rrs[big]))
+    assert rel(vol, ref) <= 1e-4

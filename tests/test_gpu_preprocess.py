@@ -122,3 +122,22 @@ def test_errors(pkg, gpu_available):
     assert e.value.status == 7                       # non-square crop
     with pytest.raises(ValueError):
         pkg.prepare_holograms(_fp(pkg), fr.astype(np.int64))
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.float32])
+def test_full_size_frames_match_numpy_oracle(pkg, gpu_available, dtype):
+    """1024^2 16-bit and float frames straight against the numpy restatement of the input
+    contract (tests/test_io_cpu.py): the device sums a frame's mean in block order, the host
+    in raster order, and for these pixel types the double sum is not exact, so values agree
+    with the host function to 1 float ulp and with numpy to FP32 rounding."""
+    from test_io_cpu import _prepare_np
+    fp = _fp(pkg, pad=0)
+    fr = frames_of(dtype, 2, 1024, 1024, seed=21)
+    got = pkg.prepare_holograms(fp, fr)
+    for k in range(2):
+        exp_np = _prepare_np(fr[k].astype(np.float32))
+        np.testing.assert_allclose(got[k], exp_np, rtol=2e-6, atol=2e-6)
+        host = pkg.prepare_hologram(fp, fr[k].astype(np.float32))
+        ulps = np.abs(got[k].real.view(np.int32).astype(np.int64) - host.real.view(np.int32).astype(np.int64))
+        assert int(ulps.max()) <= 1, int(ulps.max())
+        assert np.all(got[k].imag == 0)
