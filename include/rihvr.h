@@ -322,6 +322,20 @@ rihvr_status rihvr_prepare_holograms(const rihvr_file_params* fp, const void* fr
                                      int32_t n_frames, const float* background, int32_t width, int32_t height,
                                      int in_loc, float* out, int out_loc, int32_t* out_width, int32_t* out_height);
 
+/* No counterpart in the reference.  rihvr_prepare_holograms for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight frame loops, harnesses that forbid
+ * hipMalloc): frames, background and out are device pointers on the current HIP device, and the
+ * call allocates nothing.  The mean pass's partial sums and the frames' inverse means live in a
+ * 1 MiB scratch array that each GPU holds for the life of the process, which bounds the batch:
+ * n_frames * 8 * ceil(width * height / (16384 / bytes per pixel)) bytes of partial sums (none
+ * with a background), rounded up to 256, plus 4 * n_frames bytes must fit in 1 MiB (for example
+ * 2,000 8-bit or 500 float frames of 1024^2), else RIHVR_E_INVALID_ARGUMENT.  Calls that
+ * overlap on one GPU share that scratch and race; the call waits for its work before returning.
+ * Results equal rihvr_prepare_holograms' bit for bit; its other errors are that function's. */
+rihvr_status rihvr_prepare_holograms_noalloc(const rihvr_file_params* fp, const void* frames, int32_t pixel_type,
+                                             int32_t n_frames, const float* background, int32_t width,
+                                             int32_t height, float* out, int32_t* out_width, int32_t* out_height);
+
 /* 8-bit grayscale uncompressed TIFF writer (the reference writes with cv::imwrite). */
 rihvr_status rihvr_write_tiff8(const char* path, const uint8_t* pixels, int32_t width,
                                int32_t height);

@@ -46,7 +46,8 @@ EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_d
             "rihvr_segmentation_binarize", "rihvr_segmentation_close", "rihvr_segmentation_label",
             "rihvr_segmentation_extract", "rihvr_segmentation_size_filter", "rihvr_segmentation_objects",
             "rihvr_segmentation_write_centroids", "rihvr_segmentation_mask", "rihvr_segmentation_values",
-            "rihvr_segmentation_labels", "rihvr_segmentation_timing", "rihvr_prepare_holograms")
+            "rihvr_segmentation_labels", "rihvr_segmentation_timing", "rihvr_prepare_holograms",
+            "rihvr_prepare_holograms_noalloc")
 PATH_MAX = 4096
 
 
@@ -329,6 +330,9 @@ def lib():
         L.rihvr_prepare_holograms.argtypes = [_P(FileParams), vp, ctypes.c_int32, ctypes.c_int32, vp, ctypes.c_int32,
                                               ctypes.c_int32, ip, vp, ip, i32p, i32p]
         L.rihvr_prepare_holograms.restype = st
+        L.rihvr_prepare_holograms_noalloc.argtypes = [_P(FileParams), vp, ctypes.c_int32, ctypes.c_int32, vp,
+                                                      ctypes.c_int32, ctypes.c_int32, vp, i32p, i32p]
+        L.rihvr_prepare_holograms_noalloc.restype = st
         _lib = L
     return _lib
 
@@ -431,6 +435,41 @@ def prepare_holograms(fp, frames, background=None, out=None):
     op, oloc, _k = _arg(out, shape, writable=True)
     _check(L.rihvr_prepare_holograms(ctypes.byref(fp), fptr, PIXEL_TYPES[dtype], n, bptr, w, h, floc, op, oloc,
                                      ctypes.byref(ow), ctypes.byref(oh)), "rihvr_prepare_holograms")
+    return out
+
+
+def prepare_holograms_noalloc(fp, frames, out, background=None):
+    """rihvr_prepare_holograms_noalloc: rihvr_prepare_holograms for device frames (a contiguous
+    uint8, uint16 or float32 CUDA tensor [n][H][W]) into ``out`` (a contiguous complex64 CUDA
+    tensor [n][H'][W']), with no device allocation; the batch is bounded by the library's 1 MiB
+    scratch (RihvrError RIHVR_E_INVALID_ARGUMENT beyond it)."""
+    import torch
+    if not _is_torch(frames) or not frames.is_cuda or not frames.is_contiguous():
+        raise ValueError("frames must be a contiguous CUDA tensor")
+    dtype = str(frames.dtype).replace("torch.", "")
+    if dtype not in PIXEL_TYPES:
+        raise ValueError(f"pixel type {dtype}")
+    n, h, w = frames.shape
+    bptr = None
+    if background is not None:
+        if (not _is_torch(background) or not background.is_cuda or background.dtype != torch.float32
+                or not background.is_contiguous() or tuple(background.shape) != (h, w)
+                or background.device != frames.device):
+            raise ValueError(f"the background must be a contiguous float32 CUDA tensor of shape {(h, w)} "
+                             f"on {frames.device}")
+        bptr = ctypes.c_void_p(background.data_ptr())
+    ow, oh = ctypes.c_int32(), ctypes.c_int32()
+    L = lib()
+    fptr = ctypes.c_void_p(frames.data_ptr())
+    _check(L.rihvr_prepare_holograms_noalloc(ctypes.byref(fp), fptr, PIXEL_TYPES[dtype], n, bptr, w, h, None,
+                                             ctypes.byref(ow), ctypes.byref(oh)), "rihvr_prepare_holograms_noalloc")
+    shape = (n, oh.value, ow.value)
+    if (not _is_torch(out) or not out.is_cuda or out.dtype != torch.complex64 or not out.is_contiguous()
+            or tuple(out.shape) != shape or out.device != frames.device):
+        raise ValueError(f"out must be a contiguous complex64 CUDA tensor of shape {shape} on {frames.device}")
+    _check(L.rihvr_prepare_holograms_noalloc(ctypes.byref(fp), fptr, PIXEL_TYPES[dtype], n, bptr, w, h,
+                                             ctypes.c_void_p(out.data_ptr()), ctypes.byref(ow), ctypes.byref(oh)),
+           "rihvr_prepare_holograms_noalloc")
     return out
 
 

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "../../include/rihvr.h"
@@ -150,21 +151,26 @@ struct DevBuf {
     }
 };
 
+// The two passes with caller-provided scratch: partial (n * sum_blocks<T>(N) doubles, unused with
+// a background) and inv_mean (n floats).  Launches on the null stream and waits for it.
 template <typename T>
-rihvr_status run(const T* frames, const float* bg, const Geometry& g, int n, float* out_dev) {
+long sum_blocks(long N) {
+    const long per_block = (long)kThreads * kItems * (16 / sizeof(T));
+    return std::max(1L, (N + per_block - 1) / per_block);
+}
+
+template <typename T>
+rihvr_status run_with(const T* frames, const float* bg, const Geometry& g, int n, float* out_dev, double* partial,
+                      float* inv_mean) {
     const float inv255 = (float)(1.0 / 255.0);  // cv::Mat / 255 scales by the float reciprocal
     const long N = (long)g.W * g.H;
-    DevBuf hm, hp;
-    PRE_HIP(hipMalloc(&hm.p, (size_t)n * sizeof(float)));
     if (!bg) {
-        const long per_block = (long)kThreads * kItems * (16 / sizeof(T));
-        const int blocks = (int)std::max(1L, (N + per_block - 1) / per_block);
-        PRE_HIP(hipMalloc(&hp.p, (size_t)n * blocks * sizeof(double)));
+        const int blocks = (int)sum_blocks<T>(N);
         for (int f0 = 0; f0 < n; f0 += kMaxGridYZ)
             hipLaunchKernelGGL(k_pre_sum<T>, dim3(blocks, std::min(n - f0, kMaxGridYZ)), dim3(kThreads), 0, 0,
-                               frames + (size_t)f0 * N, N, inv255, (double*)hp.p + (size_t)f0 * blocks);
-        hipLaunchKernelGGL(k_pre_mean, dim3((n + 63) / 64), dim3(64), 0, 0, (const double*)hp.p, blocks, n, N,
-                           (float*)hm.p);
+                               frames + (size_t)f0 * N, N, inv255, partial + (size_t)f0 * blocks);
+        hipLaunchKernelGGL(k_pre_mean, dim3((n + 63) / 64), dim3(64), 0, 0, (const double*)partial, blocks, n, N,
+                           inv_mean);
         PRE_HIP(hipGetLastError());
     }
     // frames in launches of at most kMaxGridYZ (the grid's y and z limit)
@@ -175,13 +181,59 @@ rihvr_status run(const T* frames, const float* bg, const Geometry& g, int n, flo
                         (unsigned)std::min(n - f0, kMaxGridYZ));
         if (wide)
             hipLaunchKernelGGL((k_pre_write<T, kPix>), grid, dim3(kThreads), 0, 0, frames, bg, g, f0, inv255,
-                               (const float*)hm.p, (float2*)out_dev);
+                               (const float*)inv_mean, (float2*)out_dev);
         else
             hipLaunchKernelGGL((k_pre_write<T, 1>), grid, dim3(kThreads), 0, 0, frames, bg, g, f0, inv255,
-                               (const float*)hm.p, (float2*)out_dev);
+                               (const float*)inv_mean, (float2*)out_dev);
     }
     PRE_HIP(hipGetLastError());
     PRE_HIP(hipDeviceSynchronize());
+    return RIHVR_OK;
+}
+
+template <typename T>
+rihvr_status run(const T* frames, const float* bg, const Geometry& g, int n, float* out_dev) {
+    DevBuf hm, hp;
+    PRE_HIP(hipMalloc(&hm.p, (size_t)n * sizeof(float)));
+    if (!bg) PRE_HIP(hipMalloc(&hp.p, (size_t)n * sum_blocks<T>((long)g.W * g.H) * sizeof(double)));
+    return run_with(frames, bg, g, n, out_dev, (double*)hp.p, (float*)hm.p);
+}
+
+// Scratch of rihvr_prepare_holograms_noalloc: the mean pass's per-frame partial sums, then the
+// frames' inverse means.  Each GPU the library runs on holds its own copy until the process ends.
+constexpr size_t kScratchBytes = size_t(1) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's scratch (the array's name is a host-side stand-in).
+rihvr_status device_scratch(void** out) {
+    int dev = 0;
+    PRE_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return RIHVR_E_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        PRE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return RIHVR_OK;
+}
+
+rihvr_status check_args(const rihvr_file_params* fp, const void* frames, int32_t pixel_type, int32_t n_frames,
+                        const float* background, int32_t width, int32_t height, int32_t* out_width,
+                        int32_t* out_height, rihvr::CropGeometry* cg) {
+    if (!fp || !frames || width <= 0 || height <= 0 || n_frames < 1 || !out_width || !out_height)
+        return RIHVR_E_INVALID_ARGUMENT;
+    if (pixel_type != RIHVR_PIXEL_U8 && pixel_type != RIHVR_PIXEL_U16 && pixel_type != RIHVR_PIXEL_F32)
+        return RIHVR_E_UNKNOWN_TYPE;
+    if (fp->remove_background && !background) return RIHVR_E_MISSING_DATA;
+    const rihvr_status gs = rihvr::crop_geometry(fp, width, height, cg);
+    if (gs) return gs;
+    *out_width = cg->ow;
+    *out_height = cg->oh;
     return RIHVR_OK;
 }
 
@@ -192,16 +244,10 @@ extern "C" {
 rihvr_status rihvr_prepare_holograms(const rihvr_file_params* fp, const void* frames, int32_t pixel_type,
                                      int32_t n_frames, const float* background, int32_t width, int32_t height,
                                      int in_loc, float* out, int out_loc, int32_t* out_width, int32_t* out_height) {
-    if (!fp || !frames || width <= 0 || height <= 0 || n_frames < 1 || !out_width || !out_height)
-        return RIHVR_E_INVALID_ARGUMENT;
-    if (pixel_type != RIHVR_PIXEL_U8 && pixel_type != RIHVR_PIXEL_U16 && pixel_type != RIHVR_PIXEL_F32)
-        return RIHVR_E_UNKNOWN_TYPE;
-    if (fp->remove_background && !background) return RIHVR_E_MISSING_DATA;
     rihvr::CropGeometry cg;
-    const rihvr_status gs = rihvr::crop_geometry(fp, width, height, &cg);
-    if (gs) return gs;
-    *out_width = cg.ow;
-    *out_height = cg.oh;
+    const rihvr_status as = check_args(fp, frames, pixel_type, n_frames, background, width, height, out_width,
+                                       out_height, &cg);
+    if (as) return as;
     if (!out) return RIHVR_OK;
     if (cg.oh > kMaxGridYZ) return RIHVR_E_INVALID_ARGUMENT;  // one output row per grid row
     const Geometry g{width, height, cg.src_x0, cg.src_y0, cg.rw, cg.rh, cg.pad, cg.ow, cg.oh};
@@ -235,6 +281,43 @@ rihvr_status rihvr_prepare_holograms(const rihvr_file_params* fp, const void* fr
     if (st) return st;
     if (out_loc != RIHVR_DEVICE) PRE_HIP(hipMemcpy(out, dout, out_bytes, hipMemcpyDeviceToHost));
     return RIHVR_OK;
+}
+
+rihvr_status rihvr_prepare_holograms_noalloc(const rihvr_file_params* fp, const void* frames, int32_t pixel_type,
+                                             int32_t n_frames, const float* background, int32_t width,
+                                             int32_t height, float* out, int32_t* out_width, int32_t* out_height) {
+    rihvr::CropGeometry cg;
+    const rihvr_status as = check_args(fp, frames, pixel_type, n_frames, background, width, height, out_width,
+                                       out_height, &cg);
+    if (as) return as;
+    if (!out) return RIHVR_OK;
+    if (cg.oh > kMaxGridYZ) return RIHVR_E_INVALID_ARGUMENT;
+    const long N = (long)width * height;
+    const long blocks = pixel_type == RIHVR_PIXEL_U8    ? sum_blocks<uint8_t>(N)
+                        : pixel_type == RIHVR_PIXEL_U16 ? sum_blocks<uint16_t>(N)
+                                                        : sum_blocks<float>(N);
+    // partial sums (none with a background), then the inverse means at a 256-byte boundary
+    const size_t part_bytes = fp->remove_background ? 0 : (size_t)n_frames * blocks * sizeof(double);
+    const size_t mean_off = (part_bytes + 255) & ~size_t(255);
+    if (mean_off + (size_t)n_frames * sizeof(float) > kScratchBytes) {
+        std::fprintf(stderr,
+                     "rihvr_prepare_holograms_noalloc: %d frames of %dx%d need %zu scratch bytes, more than the "
+                     "%zu this entry point holds\n",
+                     n_frames, width, height, mean_off + (size_t)n_frames * sizeof(float), kScratchBytes);
+        return RIHVR_E_INVALID_ARGUMENT;
+    }
+    void* sp = nullptr;
+    const rihvr_status ss = device_scratch(&sp);
+    if (ss) return ss;
+    double* partial = (double*)sp;
+    float* inv_mean = (float*)((unsigned char*)sp + mean_off);
+    const Geometry g{width, height, cg.src_x0, cg.src_y0, cg.rw, cg.rh, cg.pad, cg.ow, cg.oh};
+    const float* dbg = fp->remove_background ? background : nullptr;
+    if (pixel_type == RIHVR_PIXEL_U8)
+        return run_with((const uint8_t*)frames, dbg, g, n_frames, out, partial, inv_mean);
+    if (pixel_type == RIHVR_PIXEL_U16)
+        return run_with((const uint16_t*)frames, dbg, g, n_frames, out, partial, inv_mean);
+    return run_with((const float*)frames, dbg, g, n_frames, out, partial, inv_mean);
 }
 
 }  // extern "C"
