@@ -495,6 +495,98 @@ rihvr_status rihvr_segmentation_labels(rihvr_segmentation* seg, uint32_t* out, i
  * label and extract calls, in that order: ms[5]. */
 rihvr_status rihvr_segmentation_timing(rihvr_segmentation* seg, double* ms);
 
+/* ---- Learned warm start.  No counterpart in the reference, whose solver always starts from
+ * zero (sparse_compressive_holo.cu:258-259, sparse_volume.cu:44-48).  A 3D encoder-decoder
+ * network maps the back-propagated volume A^T b of a hologram (SparseGradient::reconstruct,
+ * sparse_gradient.cu:47-165, with r = b) to a start volume for FISTA; it is trained on the
+ * solver's own converged volumes.  Architecture (fixed): 4 levels of 16, 32, 64, 128 channels,
+ * one 3x3x3 convolution + ReLU per level on the way down (2x2x2 average pooling between levels)
+ * and on the way up (nearest 2x2x2 upsampling concatenated with the skip), a 1x1x1 head to
+ * (Re, Im).  Input scaled by in_scale, output by out_scale.  Activations are BF16, convolutions
+ * run on MFMA with FP32 accumulation, master weights and Adam state are FP32.  Volumes are
+ * [Nz][H][W] complex64 on the current HIP device; Nz, H, W must be multiples of 8. */
+typedef struct rihvr_net rihvr_net;
+
+/* Number of FP32 parameters: per convolution e0, e1, e2, e3, d2, d1, d0 the weights
+ * [27][Cout][Cin] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1); e0's Cin is 16 of which channels 0, 1
+ * are Re, Im) then the bias [Cout]; then the head's [2][16] weights and [2] bias. */
+rihvr_status rihvr_net_param_count(int64_t* count);
+/* Allocates a network's inference buffers for Nz x H x W volumes (about 40 bytes per voxel;
+ * training buffers, about 3 times that, are added by the first gradient call).  Parameters
+ * start at zero: call rihvr_net_init or rihvr_net_set_params. */
+rihvr_status rihvr_net_create(int32_t num_planes, int32_t height, int32_t width, rihvr_net** out);
+rihvr_status rihvr_net_destroy(rihvr_net* net);
+/* He-normal weights from a seeded generator, zero biases; resets Adam. */
+rihvr_status rihvr_net_init(rihvr_net* net, uint64_t seed);
+/* Host FP32 parameters (rihvr_net_param_count of them) and the input/output scales; resets Adam. */
+rihvr_status rihvr_net_set_params(rihvr_net* net, const float* params, float in_scale, float out_scale);
+rihvr_status rihvr_net_get_params(rihvr_net* net, float* params, float* in_scale, float* out_scale);
+/* volume_out = the network's start volume for volume_in (device pointers). */
+rihvr_status rihvr_net_predict(rihvr_net* net, const float* volume_in, float* volume_out);
+/* Loss weight: voxels where the target is non-zero count 1 + pos_weight times (default 0: plain
+ * mean squared error).  The solver's volumes are sparse (a few voxels in 10^4 are non-zero), so
+ * the plain error is minimised by predicting zeros; the weight keeps the objects in the loss. */
+rihvr_status rihvr_net_set_loss_weight(rihvr_net* net, float pos_weight);
+/* Loss sum_v w_v |out_v - target_v|^2 / (2 V out_scale^2) (w_v = 1 + pos_weight where
+ * target_v != 0, else 1) and its gradient with respect to every parameter (kept for
+ * rihvr_net_adam_step; copied to the host array grads when not NULL). */
+rihvr_status rihvr_net_loss_grad(rihvr_net* net, const float* volume_in, const float* target, double* loss,
+                                 float* grads);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the gradient of the last loss_grad. */
+rihvr_status rihvr_net_adam_step(rihvr_net* net, float lr);
+/* rihvr_net_loss_grad then rihvr_net_adam_step. */
+rihvr_status rihvr_net_train_step(rihvr_net* net, const float* volume_in, const float* target, float lr,
+                                  double* loss);
+/* Waits for the network's queued work. */
+rihvr_status rihvr_net_sync(rihvr_net* net);
+
+/* The network's convolution kernels on device tensors (BF16 as uint16 bit patterns,
+ * channels-last [D][H][W][C], channel counts multiples of 16, zero padding):
+ *   forward: out[v][co] = act(bias[co] + sum_tap sum_ci in[v + tap][ci] w[tap][co][ci]), where the
+ *            input is in0 (c0 channels) followed by in1 (c1 channels at half resolution, read at
+ *            (z/2, y/2, x/2); NULL when c1 = 0); act = ReLU when relu != 0; bias may be NULL;
+ *   dgrad:   grad_in[v][ci] = sum_tap sum_co grad_out[v - tap][co] w[tap][co][ci], with weight_t the
+ *            flipped transpose wT[26 - tap][ci][co] (FP32 out);
+ *   wgrad:   grad_weight_bias = dW[tap][co][ci] = sum_v grad_out[v][co] in[v + tap][ci], then
+ *            db[co] = sum_v grad_out[v][co] (FP32, 27*cout*cin + cout values). */
+rihvr_status rihvr_conv3d_bf16_forward(const uint16_t* in0, int32_t c0, const uint16_t* in1, int32_t c1, int32_t depth,
+                                       int32_t height, int32_t width, const uint16_t* weight, const float* bias,
+                                       int32_t cout, int32_t relu, uint16_t* out);
+rihvr_status rihvr_conv3d_bf16_dgrad(const uint16_t* grad_out, int32_t cout, int32_t depth, int32_t height,
+                                     int32_t width, const uint16_t* weight_t, int32_t cin, float* grad_in);
+rihvr_status rihvr_conv3d_bf16_wgrad(const uint16_t* in0, int32_t c0, const uint16_t* in1, int32_t c1, int32_t depth,
+                                     int32_t height, int32_t width, const uint16_t* grad_out, int32_t cout,
+                                     float* grad_weight_bias);
+
+/* Result of rihvr_reconstruct_learned. */
+typedef struct rihvr_learned_stats {
+    int32_t iterations;     /* FISTA iterations run from the chosen start */
+    int32_t fallback;       /* 1: the network's start was rejected and the zero start used */
+    int32_t reached;        /* 1: the objective target was reached */
+    int32_t total_backtracks;
+    double  resid_net;      /* 0.5 ||A x_net - b||^2 of the network's start */
+    double  resid_adjoint;  /* 0.5 ||A (t A^T b) - b||^2 of the A^T b start, t = 1/Nz (zero for a
+                               real hologram: A A^T = Nz I on real holograms) */
+    double  f_net;          /* objective (calcObjectiveFunction) at the network's start */
+    double  f_adjoint;      /* objective at the A^T b start */
+    double  f_zero;         /* objective at the zero start, 0.5 ||b||^2 */
+    double  f_start;        /* objective at the chosen start */
+    double  f_final;        /* objective at the returned volume */
+    double  ms_adjoint;     /* wall time of A^T b */
+    double  ms_network;     /* wall time of the network's prediction and the residual test */
+    double  ms_solve;       /* wall time of FISTA from the chosen start */
+} rihvr_learned_stats;
+
+/* Reconstructs one hologram with the network's start: A^T b -> network -> x_net; when the
+ * objective at x_net exceeds the objective at the A^T b start (t A^T b, t = 1/Nz) or at the
+ * zero start, the zero start of the reference is used instead (fallback); then FISTA (the
+ * context's mode and parameters)
+ * until the objective at the momentum point is <= f_target (f_target <= 0: no target) or
+ * max_iterations (<= 0: num_inverse_iterations).  The network must have been created for the
+ * context's Nz x H x W.  The result is read as after rihvr_reconstruct. */
+rihvr_status rihvr_reconstruct_learned(rihvr_ctx* ctx, rihvr_net* net, const float* holo, int holo_loc,
+                                       double f_target, int32_t max_iterations, rihvr_learned_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,7 +47,11 @@ EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_d
             "rihvr_segmentation_extract", "rihvr_segmentation_size_filter", "rihvr_segmentation_objects",
             "rihvr_segmentation_write_centroids", "rihvr_segmentation_mask", "rihvr_segmentation_values",
             "rihvr_segmentation_labels", "rihvr_segmentation_timing", "rihvr_prepare_holograms",
-            "rihvr_prepare_holograms_noalloc")
+            "rihvr_prepare_holograms_noalloc", "rihvr_net_param_count", "rihvr_net_create", "rihvr_net_destroy",
+            "rihvr_net_init", "rihvr_net_set_params", "rihvr_net_get_params", "rihvr_net_predict",
+            "rihvr_net_loss_grad", "rihvr_net_adam_step", "rihvr_net_train_step", "rihvr_net_sync",
+            "rihvr_conv3d_bf16_forward", "rihvr_conv3d_bf16_dgrad", "rihvr_conv3d_bf16_wgrad",
+            "rihvr_reconstruct_learned", "rihvr_net_set_loss_weight")
 PATH_MAX = 4096
 
 
@@ -179,6 +183,30 @@ class Object(ctypes.Structure):
         return dict(centroid=list(self.centroid), weighted_centroid=list(self.weighted_centroid),
                     max_value=self.max_value, mean_value=self.mean_value, num_voxels=self.num_voxels,
                     label=self.label, bounds=list(self.bounds))
+
+
+class LearnedStats(ctypes.Structure):
+    """Mirror of ``rihvr_learned_stats`` (include/rihvr.h)."""
+
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("fallback", ctypes.c_int32),
+        ("reached", ctypes.c_int32),
+        ("total_backtracks", ctypes.c_int32),
+        ("resid_net", ctypes.c_double),
+        ("resid_adjoint", ctypes.c_double),
+        ("f_net", ctypes.c_double),
+        ("f_adjoint", ctypes.c_double),
+        ("f_zero", ctypes.c_double),
+        ("f_start", ctypes.c_double),
+        ("f_final", ctypes.c_double),
+        ("ms_adjoint", ctypes.c_double),
+        ("ms_network", ctypes.c_double),
+        ("ms_solve", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def make_params(width, height, num_planes, start_plane, plane_stepsize, wavelength, resolution,
@@ -333,6 +361,29 @@ def lib():
         L.rihvr_prepare_holograms_noalloc.argtypes = [_P(FileParams), vp, ctypes.c_int32, ctypes.c_int32, vp,
                                                       ctypes.c_int32, ctypes.c_int32, vp, i32p, i32p]
         L.rihvr_prepare_holograms_noalloc.restype = st
+        i32 = ctypes.c_int32
+        L.rihvr_net_param_count.argtypes = [i64p]
+        L.rihvr_net_create.argtypes = [i32, i32, i32, _P(vp)]
+        L.rihvr_net_destroy.argtypes = [vp]
+        L.rihvr_net_init.argtypes = [vp, ctypes.c_uint64]
+        L.rihvr_net_set_params.argtypes = [vp, vp, ctypes.c_float, ctypes.c_float]
+        L.rihvr_net_get_params.argtypes = [vp, vp, _P(ctypes.c_float), _P(ctypes.c_float)]
+        L.rihvr_net_predict.argtypes = [vp, vp, vp]
+        L.rihvr_net_loss_grad.argtypes = [vp, vp, vp, _P(ctypes.c_double), vp]
+        L.rihvr_net_adam_step.argtypes = [vp, ctypes.c_float]
+        L.rihvr_net_set_loss_weight.argtypes = [vp, ctypes.c_float]
+        L.rihvr_net_train_step.argtypes = [vp, vp, vp, ctypes.c_float, _P(ctypes.c_double)]
+        L.rihvr_net_sync.argtypes = [vp]
+        L.rihvr_conv3d_bf16_forward.argtypes = [vp, i32, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp]
+        L.rihvr_conv3d_bf16_dgrad.argtypes = [vp, i32, i32, i32, i32, vp, i32, vp]
+        L.rihvr_conv3d_bf16_wgrad.argtypes = [vp, i32, vp, i32, i32, i32, i32, vp, i32, vp]
+        L.rihvr_reconstruct_learned.argtypes = [vp, vp, vp, ip, ctypes.c_double, i32, _P(LearnedStats)]
+        for name in ("rihvr_net_param_count", "rihvr_net_create", "rihvr_net_destroy", "rihvr_net_init",
+                     "rihvr_net_set_params", "rihvr_net_get_params", "rihvr_net_predict", "rihvr_net_loss_grad",
+                     "rihvr_net_adam_step", "rihvr_net_train_step", "rihvr_net_sync", "rihvr_conv3d_bf16_forward",
+                     "rihvr_conv3d_bf16_dgrad", "rihvr_conv3d_bf16_wgrad", "rihvr_reconstruct_learned",
+                     "rihvr_net_set_loss_weight"):
+            getattr(L, name).restype = st
         _lib = L
     return _lib
 
@@ -594,6 +645,15 @@ class SparseCompressiveHolo:
         self.last_stats = sb.as_dict()
         return self.last_stats
 
+    def reconstructLearned(self, net: "WarmStartNet", holo, f_target=0.0, max_iterations=0) -> dict:
+        """rihvr_reconstruct_learned: the network's start (A^T b -> net), the fallback test
+        against the A^T b start, then FISTA to f_target (<= 0: none) or max_iterations."""
+        hp, hl, _k = _arg(holo, self.shape_holo)
+        st = LearnedStats()
+        _check(lib().rihvr_reconstruct_learned(self._ctx, net._net, hp, hl, float(f_target), int(max_iterations),
+                                               ctypes.byref(st)), "rihvr_reconstruct_learned")
+        return st.as_dict()
+
     def reconstructBatch(self, holos, warm_start=None, warm_from_previous=False, f_target=None,
                          max_iterations=0, volumes=None) -> list:
         """rihvr_reconstruct_batch: frames [n][H][W] one after the other; see the
@@ -723,6 +783,155 @@ class SparseCompressiveHolo:
 
 
 # -- segmentation and centroids (the segmentation executable) ---------------
+# -- learned warm start ------------------------------------------------------------
+def net_param_count() -> int:
+    n = ctypes.c_int64()
+    _check(lib().rihvr_net_param_count(ctypes.byref(n)), "rihvr_net_param_count")
+    return n.value
+
+
+def _dev(t, dtype, shape=None, what="tensor"):
+    """Device pointer of a contiguous CUDA tensor of the given torch dtype (and shape)."""
+    if not _is_torch(t) or not t.is_cuda or not t.is_contiguous() or t.dtype != dtype:
+        raise ValueError(f"{what} must be a contiguous {dtype} CUDA tensor")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what} shape {tuple(t.shape)} != {tuple(shape)}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class WarmStartNet:
+    """The learned warm start's network (rihvr_net_*): volumes are [Nz][H][W] complex64 CUDA
+    tensors; parameters travel as float32 numpy arrays."""
+
+    def __init__(self, num_planes, height, width, seed=None):
+        self.shape = (int(num_planes), int(height), int(width))
+        self._net = ctypes.c_void_p()
+        _check(lib().rihvr_net_create(*self.shape, ctypes.byref(self._net)), "rihvr_net_create")
+        self.nparams = net_param_count()
+        if seed is not None:
+            self.init(seed)
+
+    def destroy(self):
+        if self._net:
+            lib().rihvr_net_destroy(self._net)
+            self._net = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def init(self, seed: int):
+        _check(lib().rihvr_net_init(self._net, int(seed)), "rihvr_net_init")
+
+    def set_params(self, params, in_scale=1.0, out_scale=1.0):
+        a = np.ascontiguousarray(params, np.float32)
+        if a.size != self.nparams:
+            raise ValueError(f"{a.size} parameters, expected {self.nparams}")
+        _check(lib().rihvr_net_set_params(self._net, ctypes.c_void_p(a.ctypes.data), float(in_scale),
+                                          float(out_scale)), "rihvr_net_set_params")
+
+    def get_params(self):
+        a = np.empty(self.nparams, np.float32)
+        i, o = ctypes.c_float(), ctypes.c_float()
+        _check(lib().rihvr_net_get_params(self._net, ctypes.c_void_p(a.ctypes.data), ctypes.byref(i),
+                                          ctypes.byref(o)), "rihvr_net_get_params")
+        return a, i.value, o.value
+
+    def save(self, path, **meta):
+        p, i, o = self.get_params()
+        np.savez(path, params=p, in_scale=np.float32(i), out_scale=np.float32(o), shape=np.array(self.shape),
+                 **{k: np.asarray(v) for k, v in meta.items()})
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.set_params(z["params"], float(z["in_scale"]), float(z["out_scale"]))
+
+    def predict(self, vol, out=None):
+        import torch
+        if out is None:
+            out = torch.empty(self.shape, dtype=torch.complex64, device=vol.device)
+        _check(lib().rihvr_net_predict(self._net, _dev(vol, torch.complex64, self.shape, "vol"),
+                                       _dev(out, torch.complex64, self.shape, "out")), "rihvr_net_predict")
+        return out
+
+    def loss_grad(self, vol, target, with_grads=True):
+        import torch
+        loss = ctypes.c_double()
+        g = np.empty(self.nparams, np.float32) if with_grads else None
+        _check(lib().rihvr_net_loss_grad(self._net, _dev(vol, torch.complex64, self.shape, "vol"),
+                                         _dev(target, torch.complex64, self.shape, "target"), ctypes.byref(loss),
+                                         ctypes.c_void_p(g.ctypes.data) if g is not None else None),
+               "rihvr_net_loss_grad")
+        return loss.value, g
+
+    def set_loss_weight(self, pos_weight):
+        _check(lib().rihvr_net_set_loss_weight(self._net, float(pos_weight)), "rihvr_net_set_loss_weight")
+
+    def adam_step(self, lr):
+        _check(lib().rihvr_net_adam_step(self._net, float(lr)), "rihvr_net_adam_step")
+
+    def train_step(self, vol, target, lr) -> float:
+        import torch
+        loss = ctypes.c_double()
+        _check(lib().rihvr_net_train_step(self._net, _dev(vol, torch.complex64, self.shape, "vol"),
+                                          _dev(target, torch.complex64, self.shape, "target"), float(lr),
+                                          ctypes.byref(loss)), "rihvr_net_train_step")
+        return loss.value
+
+    def sync(self):
+        _check(lib().rihvr_net_sync(self._net), "rihvr_net_sync")
+
+
+def conv3d_bf16_forward(in0, weight, bias=None, in1=None, relu=False):
+    """rihvr_conv3d_bf16_forward on int16 CUDA tensors of BF16 bit patterns: in0 [D][H][W][C0],
+    in1 [D/2][H/2][W/2][C1] or None, weight [27][Cout][C0+C1], bias float32 [Cout] or None."""
+    import torch
+    D, H, W, C0 = in0.shape
+    C1 = 0 if in1 is None else in1.shape[3]
+    cout = weight.shape[1]
+    out = torch.empty((D, H, W, cout), dtype=torch.int16, device=in0.device)
+    _check(lib().rihvr_conv3d_bf16_forward(
+        _dev(in0, torch.int16), C0, None if in1 is None else _dev(in1, torch.int16), C1, D, H, W,
+        _dev(weight, torch.int16, (27, cout, C0 + C1)), None if bias is None else _dev(bias, torch.float32, (cout,)),
+        cout, 1 if relu else 0, _dev(out, torch.int16)), "rihvr_conv3d_bf16_forward")
+    return out
+
+
+def conv3d_bf16_dgrad(grad_out, weight_t):
+    """rihvr_conv3d_bf16_dgrad: grad_out [D][H][W][Cout] (BF16 bits), weight_t [27][Cin][Cout]."""
+    import torch
+    D, H, W, cout = grad_out.shape
+    cin = weight_t.shape[1]
+    out = torch.empty((D, H, W, cin), dtype=torch.float32, device=grad_out.device)
+    _check(lib().rihvr_conv3d_bf16_dgrad(_dev(grad_out, torch.int16), cout, D, H, W,
+                                         _dev(weight_t, torch.int16, (27, cin, cout)), cin,
+                                         _dev(out, torch.float32)), "rihvr_conv3d_bf16_dgrad")
+    return out
+
+
+def conv3d_bf16_wgrad(in0, grad_out, in1=None):
+    """rihvr_conv3d_bf16_wgrad: returns (dW [27][Cout][Cin] float32, db [Cout] float32)."""
+    import torch
+    D, H, W, C0 = in0.shape
+    C1 = 0 if in1 is None else in1.shape[3]
+    cout = grad_out.shape[3]
+    out = torch.empty(27 * cout * (C0 + C1) + cout, dtype=torch.float32, device=in0.device)
+    _check(lib().rihvr_conv3d_bf16_wgrad(_dev(in0, torch.int16), C0,
+                                         None if in1 is None else _dev(in1, torch.int16), C1, D, H, W,
+                                         _dev(grad_out, torch.int16, (D, H, W, cout)), cout, _dev(out, torch.float32)),
+           "rihvr_conv3d_bf16_wgrad")
+    n = 27 * cout * (C0 + C1)
+    return out[:n].reshape(27, cout, C0 + C1), out[n:]
+
+
 def read_sparse_voxels(path, width, height, num_planes):
     """SparseVolume::loadData (sparse_volume.cu:562-682) of an "x, y, z, value" file:
     -> (plane int32[n], index int32[n] = y*W + x, intensity float32[n])."""

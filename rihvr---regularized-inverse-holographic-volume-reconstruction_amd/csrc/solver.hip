@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 
 #include "../../include/rihvr.h"
 #include "kernels.hpp"
+#include "warmstart.hpp"
 
 using namespace rihvr;
 
@@ -716,6 +718,11 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     return RIHVR_OK;
 }
 
+// reduce_partials on the context's stream into out (timed as the solver's reductions)
+rihvr_status reduce_partials_on(rihvr_ctx* c, const double* partial, int rows, int cols, double* out) {
+    return timed(c, K_REDUCE, [&] { return reduce_partials(partial, rows, cols, c->red_work, out, c->stream); });
+}
+
 // objective of the current momentum point Y0 + s (Y0 - YM) with residual norm summag
 rihvr_status momentum_objective(rihvr_ctx* c, double summag, double* f) {
     RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
@@ -992,6 +999,79 @@ rihvr_status batch_impl(rihvr_ctx* c, const float* holos, int holo_loc, int n_fr
     return RIHVR_OK;
 }
 
+// The learned start: A^T b (the adjoint of the measurement, SparseGradient::reconstruct with
+// r = b) -> network -> x_net in a ring slot.  The objective (calcObjectiveFunction) at x_net is
+// compared with the objectives at the A^T b start t A^T b (t = 1/Nz; its data residual is zero
+// for a real hologram since A A^T = Nz I there) and at the reference's zero start (0.5 ||b||^2):
+// x_net is used only when it is below both, else the zero start.  Then FISTA to the target.
+rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int holo_loc, double f_target,
+                          int max_iterations, rihvr_learned_stats* out) {
+    if (!holo || !rihvr::net::net_matches(net, c->Nz, c->H, c->W)) return RIHVR_E_INVALID_ARGUMENT;
+    using clk = std::chrono::steady_clock;
+    auto ms = [](clk::time_point a, clk::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    rihvr_learned_stats st{};
+    const int max_it = max_iterations > 0 ? max_iterations : c->p.num_inverse_iterations;
+    c->have_result = false;
+    c->ss = SolveState{};
+    c->ss.mode = effective_mode(c);
+    const int mode = c->ss.mode;
+    const auto t0 = clk::now();
+    RIHVR_TRY(copy_in(c, c->meas, holo, holo_loc, c->N));
+    RIHVR_HIP(hipMemsetAsync(c->dflag, 0, sizeof(int), c->stream));
+    RIHVR_HIP(check_finite(c->meas, c->N, c->dflag, c->stream));
+    int hflag = 0;
+    RIHVR_HIP(hipMemcpyAsync(&hflag, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    RIHVR_HIP(hipStreamSynchronize(c->stream));
+    if (hflag) return RIHVR_E_INVALID_DATA;
+    RIHVR_TRY(apply_adjoint(c, c->meas, c->G));  // R-hat = FFT(b) (unscaled), G = A^T b
+    RIHVR_HIP(hipStreamSynchronize(c->stream));
+    const auto t1 = clk::now();
+    float2* xw = c->Yb[1];
+    RIHVR_TRY(rihvr::net::predict(net, (const float*)c->G, (float*)xw, c->stream));
+    invalidate_occ(c, xw);
+    // the measurement's spectrum FFT(b) / N, and 0.5 ||b||^2 = 0.5 N sum |FFT(b) / N|^2
+    RIHVR_HIP(holo_scale(c->rhat, c->bspec, c->N, (float)(1.0 / (double)c->N), c->stream));
+    RIHVR_HIP(hipMemsetAsync(c->hC, 0, c->N * sizeof(float2), c->stream));
+    RIHVR_HIP(holo_residual(c->bspec, c->hC, c->htmp, c->N, c->partial_holo, c->stream));
+    RIHVR_TRY(reduce_partials_on(c, c->partial_holo, holo_num_blocks(c->N), 1, c->scal + SL_RESID_M));
+    RIHVR_TRY(fetch_scalars(c));
+    st.f_zero = 0.5 * c->h_scal[SL_RESID_M];
+    // objective and data residual of a start volume X
+    auto volume_objective = [&](const float2* X, double* resid, double* f) -> rihvr_status {
+        RIHVR_TRY(apply_forward(c, X, c->hE));
+        RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
+            return momentum_out(X, X, 0.0f, nullptr, c->W, c->H, c->Nz, tv3d(c), c->partial, c->stream);
+        }));
+        RIHVR_TRY(reduce_partials_on(c, c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
+                                     c->scal + SL_PROX));
+        RIHVR_TRY(fetch_scalars(c));
+        *resid = 0.5 * c->h_scal[SL_RESID_A];
+        *f = objective(c, mode, c->h_scal[SL_RESID_A], c->h_scal[SL_PROX + PS_L1X0], c->h_scal[SL_PROX + PS_TVX0]);
+        return RIHVR_OK;
+    };
+    RIHVR_TRY(volume_objective(xw, &st.resid_net, &st.f_net));
+    RIHVR_HIP(holo_scale(c->G, c->G_alt, c->V, (float)(1.0 / (double)c->Nz), c->stream));
+    RIHVR_TRY(volume_objective(c->G_alt, &st.resid_adjoint, &st.f_adjoint));
+    st.fallback = (st.f_net <= st.f_adjoint && st.f_net <= st.f_zero) ? 0 : 1;
+    const auto t2 = clk::now();
+    rihvr_frame_stats fs{};
+    RIHVR_TRY(solve_frame(c, holo, holo_loc, st.fallback ? nullptr : (const float*)xw, RIHVR_DEVICE, f_target,
+                          max_it, &fs));
+    const auto t3 = clk::now();
+    st.iterations = fs.iterations;
+    st.reached = fs.reached;
+    st.total_backtracks = fs.total_backtracks;
+    st.f_start = fs.f_initial;
+    st.f_final = fs.f_final;
+    st.ms_adjoint = ms(t0, t1);
+    st.ms_network = ms(t1, t2);
+    st.ms_solve = ms(t2, t3);
+    if (out) *out = st;
+    return RIHVR_OK;
+}
+
 rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, const float* warm, int warm_loc,
                               rihvr_stats* st) {
     const int n_it = c->p.num_inverse_iterations;
@@ -1051,6 +1131,12 @@ rihvr_status rihvr_destroy(rihvr_ctx* ctx) {
     release(ctx);
     delete ctx;
     return RIHVR_OK;
+}
+
+rihvr_status rihvr_reconstruct_learned(rihvr_ctx* ctx, rihvr_net* net, const float* holo, int holo_loc,
+                                       double f_target, int32_t max_iterations, rihvr_learned_stats* stats) {
+    if (!ctx || !net) return RIHVR_E_INVALID_ARGUMENT;
+    return learned_impl(ctx, net, holo, holo_loc, f_target, max_iterations, stats);
 }
 
 rihvr_status rihvr_reconstruct(rihvr_ctx* ctx, const float* holo, int holo_loc, const float* warm, int warm_loc,

@@ -1,0 +1,1042 @@
+// warmstart.hip -- the learned warm start: a 3D encoder-decoder network that maps the
+// back-propagated volume A^T b of a hologram to a start volume for FISTA, with its BF16
+// training and inference kernels for gfx950.
+//
+// Network (4 levels, 16-32-64-128 channels, 3x3x3 convolutions, zero padding):
+//   x16 = in_scale * (Re, Im) of A^T b, as 16 BF16 channels (channels 2..15 zero)
+//   e0 = relu(conv(x16))          16   level 0
+//   e1 = relu(conv(pool(e0)))     32   level 1      pool = 2x2x2 average
+//   e2 = relu(conv(pool(e1)))     64   level 2
+//   e3 = relu(conv(pool(e2)))    128   level 3
+//   d2 = relu(conv(e2 ++ up(e3)))  64  level 2      up = nearest 2x2x2, ++ = channel concat
+//   d1 = relu(conv(e1 ++ up(d2)))  32  level 1
+//   d0 = relu(conv(e0 ++ up(d1)))  16  level 0
+//   out = out_scale * (Wo d0 + bo)  (1x1x1, 16 -> 2 = Re, Im of the start volume)
+// Activations are BF16, channels-last [z][y][x][c]; every 3x3x3 convolution is an implicit
+// GEMM on v_mfma_f32_16x16x16_bf16 with FP32 accumulation (M = 16 voxels, N = 16 output
+// channels, K = 16 input channels of one tap).  The backward pass uses the same forward kernel
+// for the input gradient (taps flipped, weights transposed) and an MFMA weight-gradient kernel
+// (K = voxels) whose per-chunk partials are summed in a fixed order.  Loss: mean squared error
+// against the solver's converged volume; optimiser: Adam on FP32 master weights.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/rihvr.h"
+#include "warmstart.hpp"
+
+namespace rihvr {
+namespace net {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLevels = 4;
+constexpr int kNumConv = 7;  // e0 e1 e2 e3 d2 d1 d0
+constexpr int kTaps = 27;
+constexpr int kFwdThreads = 256;   // 4 waves, 64 voxels each
+constexpr int kFwdTile = 256;      // voxels of a plane per workgroup
+constexpr int kMaxCB = 64;         // output channels per workgroup
+constexpr int kEw = 256;           // elementwise block size
+constexpr float kB1 = 0.9f, kB2 = 0.999f, kEps = 1e-8f;
+
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+
+struct ConvGeo {
+    int D, H, W;  // full-resolution grid of the output and of in0
+    int C0, C1;   // channels of in0 (full resolution) and of in1 (half resolution, read upsampled)
+    int Cout;
+    int CB;       // output channels per workgroup (16, 32, 48 or 64; divides Cout)
+};
+
+// A operand: 4 input channels (c .. c+3 of the concatenated input) of voxel (z, y, x), zero
+// outside the grid.  Channels below C0 come from in0, the others from in1 at (z/2, y/2, x/2).
+__device__ __forceinline__ s4 load_in4(const unsigned short* __restrict__ in0, const unsigned short* __restrict__ in1,
+                                       const ConvGeo& g, int z, int y, int x, int c) {
+    s4 v = {0, 0, 0, 0};
+    if (z < 0 || z >= g.D || y < 0 || y >= g.H || x < 0 || x >= g.W) return v;
+    if (c < g.C0) {
+        v = *(const s4*)(in0 + (((size_t)z * g.H + y) * g.W + x) * g.C0 + c);
+    } else {
+        const size_t h = (((size_t)(z >> 1) * (g.H >> 1) + (y >> 1)) * (g.W >> 1) + (x >> 1));
+        v = *(const s4*)(in1 + h * g.C1 + (c - g.C0));
+    }
+    return v;
+}
+
+// out[v][co] = act(bias[co] + sum_tap sum_ci in[v + tap][ci] * w[tap][co][ci]).
+// Grid: (ceil(H*W / 256), D, Cout / CB); 4 waves, each wave 4 M-tiles of 16 voxels (flat
+// in-plane index) x CB / 16 N-tiles.  The weights of one dz slab (9 taps) and one 16-channel
+// chunk are staged in LDS.  OUTF32: raw FP32 sums (input gradient), else BF16 with bias/relu.
+template <bool OUTF32>
+__global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* __restrict__ in0,
+                                                          const unsigned short* __restrict__ in1, ConvGeo g,
+                                                          const unsigned short* __restrict__ w,
+                                                          const float* __restrict__ bias, int relu,
+                                                          void* __restrict__ out) {
+    __shared__ s4 wl[9 * kMaxCB * 4];  // [tap of the slab][co][4 groups of 4 ci]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = lane >> 4, r16 = lane & 15;
+    const int z = blockIdx.y;
+    const int co0 = blockIdx.z * g.CB;
+    const int NT = g.CB >> 4;
+    const int plane = g.H * g.W;
+    const int Cin = g.C0 + g.C1;
+    const int vbase = blockIdx.x * kFwdTile + wave * 64;
+    int ya[4], xa[4];
+    bool va[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int v = vbase + m * 16 + r16;
+        va[m] = v < plane;
+        ya[m] = va[m] ? v / g.W : 0;
+        xa[m] = va[m] ? v - ya[m] * g.W : 0;
+    }
+    f4 acc[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+    const int nchunks = Cin >> 4;
+    for (int c = 0; c < nchunks; ++c) {
+        for (int dz = -1; dz <= 1; ++dz) {
+            __syncthreads();
+            const int t0 = (dz + 1) * 9;
+            for (int e = threadIdx.x; e < 9 * g.CB * 4; e += kFwdThreads) {
+                const int qq = e & 3, co = (e >> 2) % g.CB, tt = (e >> 2) / g.CB;
+                wl[(tt * kMaxCB + co) * 4 + qq] =
+                    *(const s4*)(w + ((size_t)(t0 + tt) * g.Cout + co0 + co) * Cin + c * 16 + qq * 4);
+            }
+            __syncthreads();
+            const int zz = z + dz;
+            for (int tt = 0; tt < 9; ++tt) {
+                const int dy = tt / 3 - 1, dx = tt % 3 - 1;
+                s4 a[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    a[m] = s4{0, 0, 0, 0};
+                    if (va[m]) a[m] = load_in4(in0, in1, g, zz, ya[m] + dy, xa[m] + dx, c * 16 + q * 4);
+                }
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    if (n < NT) {
+                        const s4 b = wl[(tt * kMaxCB + n * 16 + r16) * 4 + q];
+#pragma unroll
+                        for (int m = 0; m < 4; ++m)
+                            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+    // D[i][j]: lane (q, r16) item i holds voxel 4q + i of the M-tile, output channel r16 of the N-tile
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        if (n >= NT) continue;
+        const int co = co0 + n * 16 + r16;
+        const float b = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int v = vbase + m * 16 + q * 4 + i;
+                if (v >= plane) continue;
+                const size_t o = ((size_t)z * plane + v) * g.Cout + co;
+                float val = acc[m][n][i] + b;
+                if (relu) val = fmaxf(val, 0.0f);
+                if constexpr (OUTF32)
+                    ((float*)out)[o] = val;
+                else
+                    ((unsigned short*)out)[o] = f2bf(val);
+            }
+        }
+    }
+}
+
+// Weight (and bias) gradient partials of one chunk of rows (z, y):
+//   part[chunk][tap][co][ci] = sum_u g[u][co] * in[u + tap][ci],  part[chunk][NW + co] = sum_u g[u][co]
+// (NW = 27 Cout Cin; the bias sum is written by the ci-tile 0 workgroups).  One wave per
+// workgroup; grid (Cout/16, Cin/16, nchunks).  Per 16-voxel row segment the gradient tile is
+// staged transposed ([co][x]) and the input window of the 9 (dz, dy) rows over x0-1 .. x0+16
+// as loaded ([dzdy][x][ci]); MFMA: M = co, N = ci, K = the 16 voxels of the segment.
+__global__ void __launch_bounds__(64) k_conv_wgrad(const unsigned short* __restrict__ in0,
+                                                   const unsigned short* __restrict__ in1, ConvGeo g,
+                                                   const unsigned short* __restrict__ gout, int nchunks,
+                                                   float* __restrict__ part) {
+    __shared__ unsigned short ga[16 * 16];         // [co][x]
+    __shared__ unsigned short win[9 * 18 * 16];    // [dz dy][x + 1][ci]
+    const int lane = threadIdx.x;
+    const int q = lane >> 4, r16 = lane & 15;
+    const int co0 = blockIdx.x * 16, ci0 = blockIdx.y * 16;
+    const int chunk = blockIdx.z;
+    const int Cin = g.C0 + g.C1;
+    const long rows = (long)g.D * g.H;
+    const long r0 = rows * chunk / nchunks, r1 = rows * (chunk + 1) / nchunks;
+    f4 acc[kTaps];
+#pragma unroll
+    for (int t = 0; t < kTaps; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.0f;
+    for (long r = r0; r < r1; ++r) {
+        const int z = (int)(r / g.H), y = (int)(r - (long)z * g.H);
+        for (int x0 = 0; x0 < g.W; x0 += 16) {
+            __syncthreads();
+            {
+                const int xi = lane >> 2, cg = (lane & 3) * 4;
+                s4 v = {0, 0, 0, 0};
+                if (x0 + xi < g.W) v = *(const s4*)(gout + (((size_t)z * g.H + y) * g.W + x0 + xi) * g.Cout + co0 + cg);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) ga[(cg + j) * 16 + xi] = (unsigned short)v[j];
+            }
+            for (int e = lane; e < 9 * 18 * 4; e += 64) {
+                const int combo = e / 72, rem = e - combo * 72;
+                const int xi = rem >> 2, cg = (rem & 3) * 4;
+                const s4 v = load_in4(in0, in1, g, z + combo / 3 - 1, y + combo % 3 - 1, x0 - 1 + xi, ci0 + cg);
+                *(s4*)(win + (combo * 18 + xi) * 16 + cg) = v;
+            }
+            __syncthreads();
+            const s4 a = *(const s4*)(ga + r16 * 16 + q * 4);
+            if (blockIdx.y == 0) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bsum += bf2f((unsigned short)a[j]);
+            }
+#pragma unroll
+            for (int combo = 0; combo < 9; ++combo) {
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    s4 b;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) b[j] = (short)win[(combo * 18 + q * 4 + j + dx + 1) * 16 + r16];
+                    const int t = combo * 3 + dx + 1;
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const size_t NW = (size_t)kTaps * g.Cout * Cin;
+    float* p = part + (size_t)chunk * (NW + g.Cout);
+#pragma unroll
+    for (int t = 0; t < kTaps; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) p[((size_t)t * g.Cout + co0 + q * 4 + i) * Cin + ci0 + r16] = acc[t][i];
+    if (blockIdx.y == 0) {
+        bsum += __shfl_xor(bsum, 16);
+        bsum += __shfl_xor(bsum, 32);
+        if (lane < 16) p[NW + co0 + lane] = bsum;
+    }
+}
+
+// out[i] = sum over chunks (in order) of part[chunk][i]
+__global__ void k_sum_chunks(const float* __restrict__ part, long n, int nchunks, float* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int c = 0; c < nchunks; ++c) s += part[(size_t)c * n + i];
+    out[i] = s;
+}
+
+// x16[v] = (in_scale Re, in_scale Im, 0 x 14) in BF16
+__global__ void k_input16(const float2* __restrict__ in, long V, float scale, unsigned short* __restrict__ x16) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const float2 a = in[v];
+    uint4 lo, hi = make_uint4(0, 0, 0, 0);
+    lo.x = (unsigned)f2bf(a.x * scale) | ((unsigned)f2bf(a.y * scale) << 16);
+    lo.y = lo.z = lo.w = 0;
+    uint4* o = (uint4*)(x16 + v * 16);
+    o[0] = lo;
+    o[1] = hi;
+}
+
+// p[u][c] = (sum of the 8 children e[2u + (a, b, c)][c]) / 8 (FP32 sum, BF16 out); thread per (u, 4 channels)
+__global__ void k_pool(const unsigned short* __restrict__ e, int D, int H, int W, int C,
+                       unsigned short* __restrict__ p) {
+    const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1, C4 = C >> 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)D2 * H2 * W2 * C4;
+    if (i >= n) return;
+    const int c = (int)(i % C4) * 4;
+    long u = i / C4;
+    const int x = (int)(u % W2), y = (int)((u / W2) % H2), z = (int)(u / ((long)W2 * H2));
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int d = 0; d < 2; ++d) {
+                const s4 v = *(const s4*)(e + (((size_t)(2 * z + a) * H + 2 * y + b) * W + 2 * x + d) * C + c);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) s[j] += bf2f((unsigned short)v[j]);
+            }
+    s4 o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = (short)f2bf(s[j] * 0.125f);
+    *(s4*)(p + (size_t)u * C + c) = o;
+}
+
+// Pre-activation gradient of a ReLU layer at level k ([V][C], BF16):
+//   post[v][c] = skip[v][c]                            (skip: [V][sk_stride], channels 0..C-1)
+//              + pool[parent(v)][c] / 8                (pool: [V/8][C], the average pool's adjoint)
+//              + sum over the 8 children of split[child][off + c]   (split: [8V][Ct], the upsample's adjoint)
+//   pre = post * (act > 0)
+struct Finalize {
+    const unsigned short* act;
+    const float* skip;
+    int sk_stride;
+    const float* pool;
+    const float* split;
+    int off, Ct;
+    int D, H, W, C;
+    unsigned short* out;
+};
+__global__ void k_finalize(Finalize f) {
+    const int C4 = f.C >> 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)f.D * f.H * f.W * C4;
+    if (i >= n) return;
+    const int c = (int)(i % C4) * 4;
+    const long v = i / C4;
+    const int x = (int)(v % f.W), y = (int)((v / f.W) % f.H), z = (int)(v / ((long)f.W * f.H));
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    if (f.skip) {
+        const float4 a = *(const float4*)(f.skip + (size_t)v * f.sk_stride + c);
+        s[0] += a.x; s[1] += a.y; s[2] += a.z; s[3] += a.w;
+    }
+    if (f.pool) {
+        const long pu = (((long)(z >> 1) * (f.H >> 1) + (y >> 1)) * (f.W >> 1) + (x >> 1));
+        const float4 a = *(const float4*)(f.pool + (size_t)pu * f.C + c);
+        s[0] += a.x * 0.125f; s[1] += a.y * 0.125f; s[2] += a.z * 0.125f; s[3] += a.w * 0.125f;
+    }
+    if (f.split) {
+        const int H2 = f.H * 2, W2 = f.W * 2;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int d = 0; d < 2; ++d) {
+                    const size_t ch = (((size_t)(2 * z + a) * H2 + 2 * y + b) * W2 + 2 * x + d);
+                    const float4 t = *(const float4*)(f.split + ch * f.Ct + f.off + c);
+                    s[0] += t.x; s[1] += t.y; s[2] += t.z; s[3] += t.w;
+                }
+    }
+    const s4 act = *(const s4*)(f.act + (size_t)v * f.C + c);
+    s4 o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = (short)f2bf(bf2f((unsigned short)act[j]) > 0.0f ? s[j] : 0.0f);
+    *(s4*)(f.out + (size_t)v * f.C + c) = o;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Output head: raw o_c = bo[c] + sum_k wo[c][k] d0[v][k]; out[v] = out_scale * (o_0, o_1).
+// With a target, lpart[block] = sum over the block's voxels of w |o - target / out_scale|^2 with
+// w = 1 + pos_weight where the target is non-zero, else 1.
+__global__ void __launch_bounds__(kEw) k_head_fwd(const unsigned short* __restrict__ d0, long V,
+                                                  const float* __restrict__ wo, const float* __restrict__ bo,
+                                                  float out_scale, float2* __restrict__ out,
+                                                  const float2* __restrict__ target, float pos_weight,
+                                                  double* __restrict__ lpart) {
+    __shared__ double red[kEw / 64];
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (v < V) {
+        const s4* p = (const s4*)(d0 + v * 16);
+        float o0 = bo[0], o1 = bo[1];
+#pragma unroll
+        for (int k4 = 0; k4 < 4; ++k4) {
+            const s4 a = p[k4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float h = bf2f((unsigned short)a[j]);
+                o0 += wo[k4 * 4 + j] * h;
+                o1 += wo[16 + k4 * 4 + j] * h;
+            }
+        }
+        if (out) out[v] = make_float2(o0 * out_scale, o1 * out_scale);
+        if (target) {
+            const float2 t = target[v];
+            const float e0 = o0 - t.x / out_scale, e1 = o1 - t.y / out_scale;
+            const float wv = (t.x != 0.0f || t.y != 0.0f) ? 1.0f + pos_weight : 1.0f;
+            l = (double)wv * ((double)e0 * e0 + (double)e1 * e1);
+        }
+    }
+    if (!lpart) return;
+    l = wave_sum_d(l);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < kEw / 64; ++k) s += red[k];
+        lpart[blockIdx.x] = s;
+    }
+}
+
+// Head backward for loss L = sum_v w_v |o - t/out_scale|^2 / (2 V):  g_c = w_v (o_c - t_c/out_scale) / V;
+// hpart[block][0..31] = sum g_c d0[k] (dWo), [32..33] = sum g_c (dbo); gd0 = (sum_c g_c wo[c][k]) * (d0 > 0)
+__global__ void __launch_bounds__(kEw) k_head_bwd(const unsigned short* __restrict__ d0, long V,
+                                                  const float* __restrict__ wo, const float* __restrict__ bo,
+                                                  float out_scale, const float2* __restrict__ target,
+                                                  float pos_weight, unsigned short* __restrict__ gd0,
+                                                  float* __restrict__ hpart) {
+    __shared__ float red[34][kEw / 64];
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float h[16];
+    float g0 = 0.f, g1 = 0.f;
+    if (v < V) {
+        const s4* p = (const s4*)(d0 + v * 16);
+#pragma unroll
+        for (int k4 = 0; k4 < 4; ++k4) {
+            const s4 a = p[k4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[k4 * 4 + j] = bf2f((unsigned short)a[j]);
+        }
+        float o0 = bo[0], o1 = bo[1];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            o0 += wo[k] * h[k];
+            o1 += wo[16 + k] * h[k];
+        }
+        const float2 t = target[v];
+        const float inv = ((t.x != 0.0f || t.y != 0.0f) ? 1.0f + pos_weight : 1.0f) / (float)V;
+        g0 = (o0 - t.x / out_scale) * inv;
+        g1 = (o1 - t.y / out_scale) * inv;
+        s4 o[4];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float gk = h[k] > 0.0f ? g0 * wo[k] + g1 * wo[16 + k] : 0.0f;
+            o[k >> 2][k & 3] = (short)f2bf(gk);
+        }
+        s4* q = (s4*)(gd0 + v * 16);
+#pragma unroll
+        for (int k4 = 0; k4 < 4; ++k4) q[k4] = o[k4];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) h[k] = 0.0f;
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const float a = wave_sum_f(g0 * h[k]);
+        const float b = wave_sum_f(g1 * h[k]);
+        if ((threadIdx.x & 63) == 0) {
+            red[k][w] = a;
+            red[16 + k][w] = b;
+        }
+    }
+    const float s0 = wave_sum_f(g0), s1 = wave_sum_f(g1);
+    if ((threadIdx.x & 63) == 0) {
+        red[32][w] = s0;
+        red[33][w] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 34) {
+        float s = 0.f;
+        for (int k = 0; k < kEw / 64; ++k) s += red[threadIdx.x][k];
+        hpart[(size_t)blockIdx.x * 34 + threadIdx.x] = s;
+    }
+}
+
+// Adam on FP32 master parameters: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+// p -= lr * (m / bc1) / (sqrt(v / bc2) + eps)
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long n, float lr, float bc1, float bc2) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = kB1 * m[i] + (1.0f - kB1) * gi;
+    const float vi = kB2 * v[i] + (1.0f - kB2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + kEps);
+}
+
+// BF16 copies of a layer's weights: w[t][co][ci] and the input-gradient weights wT[26-t][ci][co]
+__global__ void k_prep_weights(const float* __restrict__ p, int Cout, int Cin, unsigned short* __restrict__ w,
+                               unsigned short* __restrict__ wT) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)kTaps * Cout * Cin;
+    if (i >= n) return;
+    const int ci = (int)(i % Cin), co = (int)((i / Cin) % Cout), t = (int)(i / ((long)Cin * Cout));
+    const unsigned short b = f2bf(p[i]);
+    w[i] = b;
+    wT[((size_t)(kTaps - 1 - t) * Cin + ci) * Cout + co] = b;
+}
+
+// ----------------------------------------------------------------------------
+// Host launchers
+// ----------------------------------------------------------------------------
+int pick_cb(int Cout) {
+    for (int cb : {64, 48, 32, 16})
+        if (Cout % cb == 0) return cb;
+    return 0;
+}
+
+hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short* in1, int C1, int D, int H, int W,
+                        const unsigned short* w, const float* bias, int Cout, int relu, void* out, bool out_f32,
+                        hipStream_t s) {
+    if (C0 <= 0 || C0 % 16 || C1 < 0 || C1 % 16 || Cout <= 0 || Cout % 16 || D <= 0 || H <= 0 || W <= 0)
+        return hipErrorInvalidValue;
+    if (C1 > 0 && (D % 2 || H % 2 || W % 2 || !in1)) return hipErrorInvalidValue;
+    if (D > 65535) return hipErrorInvalidValue;
+    ConvGeo g{D, H, W, C0, C1, Cout, pick_cb(Cout)};
+    const long plane = (long)H * W;
+    if (plane > (1L << 30)) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((plane + kFwdTile - 1) / kFwdTile), (unsigned)D, (unsigned)(Cout / g.CB));
+    if (out_f32)
+        hipLaunchKernelGGL(k_conv_fwd<true>, grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+    else
+        hipLaunchKernelGGL(k_conv_fwd<false>, grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+    return hipGetLastError();
+}
+
+// rows per chunk so that about 8192 waves run and the partials stay within cap floats
+int wgrad_chunks(long rows, int Cout, int Cin, long cap) {
+    const long tiles = (long)(Cout / 16) * (Cin / 16);
+    const long per = (long)kTaps * Cout * Cin + Cout;
+    long n = std::max(1L, (8192 + tiles - 1) / tiles);
+    n = std::min(n, rows);
+    n = std::min(n, std::max(1L, cap / per));
+    return (int)n;
+}
+
+hipError_t conv_wgrad(const unsigned short* in0, int C0, const unsigned short* in1, int C1, int D, int H, int W,
+                      const unsigned short* gout, int Cout, float* part, long part_cap, float* dw_db, hipStream_t s) {
+    if (C0 <= 0 || C0 % 16 || C1 < 0 || C1 % 16 || Cout <= 0 || Cout % 16 || D <= 0 || H <= 0 || W <= 0)
+        return hipErrorInvalidValue;
+    if (C1 > 0 && (D % 2 || H % 2 || W % 2 || !in1)) return hipErrorInvalidValue;
+    const int Cin = C0 + C1;
+    const long rows = (long)D * H;
+    const int nch = wgrad_chunks(rows, Cout, Cin, part_cap);
+    const long per = (long)kTaps * Cout * Cin + Cout;
+    if ((long)nch * per > part_cap) return hipErrorInvalidValue;
+    ConvGeo g{D, H, W, C0, C1, Cout, 16};
+    hipLaunchKernelGGL(k_conv_wgrad, dim3(Cout / 16, Cin / 16, nch), dim3(64), 0, s, in0, in1, g, gout, nch, part);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, s, (const float*)part, per,
+                       nch, dw_db);
+    return hipGetLastError();
+}
+
+unsigned blocks_of(long n, int bs = kEw) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace net
+}  // namespace rihvr
+
+using namespace rihvr::net;
+
+// ----------------------------------------------------------------------------
+// The network object
+// ----------------------------------------------------------------------------
+namespace {
+
+constexpr int kChan[kLevels] = {16, 32, 64, 128};
+
+struct Layer {
+    int level, C0, C1, Cout;  // C0: in0 channels, C1: in1 (half resolution) channels
+    size_t w_off, b_off;       // offsets in the flat parameter array
+    unsigned short *w = nullptr, *wT = nullptr;
+};
+
+#define NET_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            std::fprintf(stderr, "rihvr: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                                \
+            return e_ == hipErrorOutOfMemory ? RIHVR_E_OUT_OF_MEMORY : RIHVR_E_DEVICE;              \
+        }                                                                                          \
+    } while (0)
+
+// layer order of the flat parameter array: e0 e1 e2 e3 d2 d1 d0 (each W [27][Cout][Cin] then
+// b [Cout]), then the head Wo [2][16], bo [2]
+void layer_shapes(Layer* L, size_t* head_off, size_t* total) {
+    const int spec[kNumConv][4] = {
+        {0, 16, 0, 16},   // e0: x16 (2 channels used of 16)
+        {1, 16, 0, 32},   // e1: pool(e0)
+        {2, 32, 0, 64},   // e2: pool(e1)
+        {3, 64, 0, 128},  // e3: pool(e2)
+        {2, 64, 128, 64}, // d2: e2 ++ up(e3)
+        {1, 32, 64, 32},  // d1: e1 ++ up(d2)
+        {0, 16, 32, 16},  // d0: e0 ++ up(d1)
+    };
+    size_t off = 0;
+    for (int i = 0; i < kNumConv; ++i) {
+        L[i].level = spec[i][0];
+        L[i].C0 = spec[i][1];
+        L[i].C1 = spec[i][2];
+        L[i].Cout = spec[i][3];
+        L[i].w_off = off;
+        off += (size_t)kTaps * L[i].Cout * (L[i].C0 + L[i].C1);
+        L[i].b_off = off;
+        off += L[i].Cout;
+    }
+    *head_off = off;
+    *total = off + 34;
+}
+
+}  // namespace
+
+struct rihvr_net {
+    int D = 0, H = 0, W = 0;
+    int Dk[kLevels], Hk[kLevels], Wk[kLevels];
+    long Vk[kLevels];
+    float in_scale = 1.0f, out_scale = 1.0f;
+    float pos_weight = 0.0f;  // loss weight 1 + pos_weight where the target is non-zero
+    hipStream_t stream = nullptr;
+    Layer L[kNumConv];
+    size_t head_off = 0, nparams = 0;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    int adam_t = 0;
+    bool weights_ready = false;
+    // activations
+    unsigned short* x16 = nullptr;
+    unsigned short* e[kLevels] = {};
+    unsigned short* p[kLevels] = {};  // p[1..3]
+    unsigned short* d[3] = {};        // d0, d1, d2
+    // training buffers (allocated at the first gradient call)
+    bool train_ready = false;
+    unsigned short* ge[kLevels] = {};
+    unsigned short* gd[3] = {};
+    float* dgd[3] = {};  // input gradients of the decoder convolutions d0, d1, d2
+    float* dge[kLevels] = {};  // input gradients of e1, e2, e3 (= gradients of p1, p2, p3)
+    float* wpart = nullptr;
+    long wpart_cap = 0;
+    float* hpart = nullptr;
+    double* lpart = nullptr;
+    double* h_lpart = nullptr;
+    long head_blocks = 0;
+};
+
+namespace {
+
+template <class T> rihvr_status nalloc(T** ptr, size_t count) {
+    NET_HIP(hipMalloc((void**)ptr, std::max<size_t>(1, count) * sizeof(T)));
+    return RIHVR_OK;
+}
+
+void net_release(rihvr_net* n) {
+    if (!n) return;
+    std::vector<void*> ptrs = {n->params, n->grads, n->adam_m, n->adam_v, n->x16, n->wpart, n->hpart, n->lpart};
+    for (int k = 0; k < kLevels; ++k) {
+        ptrs.push_back(n->e[k]);
+        ptrs.push_back(n->p[k]);
+        ptrs.push_back(n->ge[k]);
+        ptrs.push_back(n->dge[k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+        ptrs.push_back(n->d[k]);
+        ptrs.push_back(n->gd[k]);
+        ptrs.push_back(n->dgd[k]);
+    }
+    for (auto& l : n->L) {
+        ptrs.push_back(l.w);
+        ptrs.push_back(l.wT);
+    }
+    for (void* q : ptrs) (void)hipFree(q);
+    if (n->h_lpart) (void)hipHostFree(n->h_lpart);
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+}
+
+rihvr_status prep_weights(rihvr_net* n) {
+    for (auto& l : n->L) {
+        const int Cin = l.C0 + l.C1;
+        const long cnt = (long)kTaps * l.Cout * Cin;
+        hipLaunchKernelGGL(k_prep_weights, dim3(blocks_of(cnt)), dim3(kEw), 0, n->stream, n->params + l.w_off, l.Cout,
+                           Cin, l.w, l.wT);
+    }
+    NET_HIP(hipGetLastError());
+    n->weights_ready = true;
+    return RIHVR_OK;
+}
+
+// the convolution of layer i on its inputs
+rihvr_status layer_forward(rihvr_net* n, int i, const unsigned short* in0, const unsigned short* in1,
+                           unsigned short* out) {
+    const Layer& l = n->L[i];
+    const int k = l.level;
+    NET_HIP(conv_forward(in0, l.C0, in1, l.C1, n->Dk[k], n->Hk[k], n->Wk[k], l.w, n->params + l.b_off, l.Cout, 1, out,
+                         false, n->stream));
+    return RIHVR_OK;
+}
+
+rihvr_status pool(rihvr_net* n, int k, const unsigned short* e, unsigned short* p) {
+    const long cnt = n->Vk[k + 1] * (kChan[k] / 4);
+    hipLaunchKernelGGL(k_pool, dim3(blocks_of(cnt)), dim3(kEw), 0, n->stream, e, n->Dk[k], n->Hk[k], n->Wk[k],
+                       kChan[k], p);
+    NET_HIP(hipGetLastError());
+    return RIHVR_OK;
+}
+
+// forward pass up to d0 (the head is applied by the caller)
+rihvr_status forward_body(rihvr_net* n, const float* in) {
+    if (!n->weights_ready) {
+        const rihvr_status st = prep_weights(n);
+        if (st) return st;
+    }
+    hipLaunchKernelGGL(k_input16, dim3(blocks_of(n->Vk[0])), dim3(kEw), 0, n->stream, (const float2*)in, n->Vk[0],
+                       n->in_scale, n->x16);
+    NET_HIP(hipGetLastError());
+    rihvr_status st;
+    if ((st = layer_forward(n, 0, n->x16, nullptr, n->e[0]))) return st;
+    for (int k = 1; k < kLevels; ++k) {
+        if ((st = pool(n, k - 1, n->e[k - 1], n->p[k]))) return st;
+        if ((st = layer_forward(n, k, n->p[k], nullptr, n->e[k]))) return st;
+    }
+    if ((st = layer_forward(n, 4, n->e[2], n->e[3], n->d[2]))) return st;
+    if ((st = layer_forward(n, 5, n->e[1], n->d[2], n->d[1]))) return st;
+    if ((st = layer_forward(n, 6, n->e[0], n->d[1], n->d[0]))) return st;
+    return RIHVR_OK;
+}
+
+rihvr_status alloc_training(rihvr_net* n) {
+    if (n->train_ready) return RIHVR_OK;
+    rihvr_status st;
+    for (int k = 0; k < kLevels; ++k)
+        if ((st = nalloc(&n->ge[k], (size_t)n->Vk[k] * kChan[k]))) return st;
+    for (int k = 0; k < 3; ++k) {
+        if ((st = nalloc(&n->gd[k], (size_t)n->Vk[k] * kChan[k]))) return st;
+        // d_k's input is e_k ++ up(d_{k+1} or e3): C_k + C_{k+1} channels at level k
+        if ((st = nalloc(&n->dgd[k], (size_t)n->Vk[k] * (kChan[k] + kChan[k + 1])))) return st;
+    }
+    for (int k = 1; k < kLevels; ++k)
+        if ((st = nalloc(&n->dge[k], (size_t)n->Vk[k] * kChan[k - 1]))) return st;
+    // weight-gradient partials: at least one chunk of the largest layer, at most 64 M floats
+    long maxper = 0;
+    for (auto& l : n->L) maxper = std::max(maxper, (long)kTaps * l.Cout * (l.C0 + l.C1) + l.Cout);
+    n->wpart_cap = std::max(maxper, std::min(64L << 20, maxper * 1024));
+    if ((st = nalloc(&n->wpart, (size_t)n->wpart_cap))) return st;
+    if ((st = nalloc(&n->grads, n->nparams))) return st;
+    if ((st = nalloc(&n->adam_m, n->nparams))) return st;
+    if ((st = nalloc(&n->adam_v, n->nparams))) return st;
+    NET_HIP(hipMemsetAsync(n->adam_m, 0, n->nparams * sizeof(float), n->stream));
+    NET_HIP(hipMemsetAsync(n->adam_v, 0, n->nparams * sizeof(float), n->stream));
+    n->adam_t = 0;
+    if ((st = nalloc(&n->hpart, (size_t)n->head_blocks * 34))) return st;
+    n->train_ready = true;
+    return RIHVR_OK;
+}
+
+rihvr_status layer_backward(rihvr_net* n, int i, const unsigned short* in0, const unsigned short* in1,
+                            const unsigned short* gout, float* din) {
+    const Layer& l = n->L[i];
+    const int k = l.level;
+    NET_HIP(conv_wgrad(in0, l.C0, in1, l.C1, n->Dk[k], n->Hk[k], n->Wk[k], gout, l.Cout, n->wpart, n->wpart_cap,
+                       n->grads + l.w_off, n->stream));
+    if (din)
+        NET_HIP(conv_forward(gout, l.Cout, nullptr, 0, n->Dk[k], n->Hk[k], n->Wk[k], l.wT, nullptr, l.C0 + l.C1, 0,
+                             din, true, n->stream));
+    return RIHVR_OK;
+}
+
+rihvr_status finalize(rihvr_net* n, int k, const unsigned short* act, const float* skip, int sk_stride,
+                      const float* pool_g, const float* split, int off, int Ct, unsigned short* out) {
+    Finalize f{act, skip, sk_stride, pool_g, split, off, Ct, n->Dk[k], n->Hk[k], n->Wk[k], kChan[k], out};
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_of(n->Vk[k] * (kChan[k] / 4))), dim3(kEw), 0, n->stream, f);
+    NET_HIP(hipGetLastError());
+    return RIHVR_OK;
+}
+
+// loss and every parameter gradient for one (input, target) pair, into n->grads
+rihvr_status loss_grad(rihvr_net* n, const float* in, const float* target, double* loss) {
+    rihvr_status st;
+    if ((st = alloc_training(n))) return st;
+    if ((st = forward_body(n, in))) return st;
+    const float* wo = n->params + n->head_off;
+    const float* bo = wo + 32;
+    hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)n->head_blocks), dim3(kEw), 0, n->stream, n->d[0], n->Vk[0], wo, bo,
+                       n->out_scale, (float2*)nullptr, (const float2*)target, n->pos_weight, n->lpart);
+    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)n->head_blocks), dim3(kEw), 0, n->stream, n->d[0], n->Vk[0], wo, bo,
+                       n->out_scale, (const float2*)target, n->pos_weight, n->gd[0], n->hpart);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(64), 0, n->stream, (const float*)n->hpart, 34L,
+                       (int)n->head_blocks, n->grads + n->head_off);
+    NET_HIP(hipGetLastError());
+    // decoder
+    if ((st = layer_backward(n, 6, n->e[0], n->d[1], n->gd[0], n->dgd[0]))) return st;
+    if ((st = finalize(n, 1, n->d[1], nullptr, 0, nullptr, n->dgd[0], kChan[0], kChan[0] + kChan[1], n->gd[1])))
+        return st;
+    if ((st = layer_backward(n, 5, n->e[1], n->d[2], n->gd[1], n->dgd[1]))) return st;
+    if ((st = finalize(n, 2, n->d[2], nullptr, 0, nullptr, n->dgd[1], kChan[1], kChan[1] + kChan[2], n->gd[2])))
+        return st;
+    if ((st = layer_backward(n, 4, n->e[2], n->e[3], n->gd[2], n->dgd[2]))) return st;
+    if ((st = finalize(n, 3, n->e[3], nullptr, 0, nullptr, n->dgd[2], kChan[2], kChan[2] + kChan[3], n->ge[3])))
+        return st;
+    // encoder
+    if ((st = layer_backward(n, 3, n->p[3], nullptr, n->ge[3], n->dge[3]))) return st;
+    if ((st = finalize(n, 2, n->e[2], n->dgd[2], kChan[2] + kChan[3], n->dge[3], nullptr, 0, 0, n->ge[2]))) return st;
+    if ((st = layer_backward(n, 2, n->p[2], nullptr, n->ge[2], n->dge[2]))) return st;
+    if ((st = finalize(n, 1, n->e[1], n->dgd[1], kChan[1] + kChan[2], n->dge[2], nullptr, 0, 0, n->ge[1]))) return st;
+    if ((st = layer_backward(n, 1, n->p[1], nullptr, n->ge[1], n->dge[1]))) return st;
+    if ((st = finalize(n, 0, n->e[0], n->dgd[0], kChan[0] + kChan[1], n->dge[1], nullptr, 0, 0, n->ge[0]))) return st;
+    if ((st = layer_backward(n, 0, n->x16, nullptr, n->ge[0], nullptr))) return st;
+    NET_HIP(hipMemcpyAsync(n->h_lpart, n->lpart, n->head_blocks * sizeof(double), hipMemcpyDeviceToHost, n->stream));
+    NET_HIP(hipStreamSynchronize(n->stream));
+    if (loss) {
+        double s = 0.0;
+        for (long b = 0; b < n->head_blocks; ++b) s += n->h_lpart[b];
+        *loss = s / (2.0 * (double)n->Vk[0]);
+    }
+    return RIHVR_OK;
+}
+
+rihvr_status adam_step(rihvr_net* n, float lr) {
+    if (!n->train_ready) return RIHVR_E_INVALID_STATE;
+    n->adam_t += 1;
+    const float bc1 = (float)(1.0 - std::pow((double)kB1, n->adam_t));
+    const float bc2 = (float)(1.0 - std::pow((double)kB2, n->adam_t));
+    hipLaunchKernelGGL(k_adam, dim3(blocks_of((long)n->nparams)), dim3(kEw), 0, n->stream, n->params, n->grads,
+                       n->adam_m, n->adam_v, (long)n->nparams, lr, bc1, bc2);
+    NET_HIP(hipGetLastError());
+    return prep_weights(n);
+}
+
+// splitmix64 + Box-Muller normals for the initial weights
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() { return std::sqrt(-2.0 * std::log(uniform())) * std::cos(2.0 * 3.14159265358979323846 * uniform()); }
+};
+
+}  // namespace
+
+namespace rihvr {
+namespace net {
+
+rihvr_status predict(rihvr_net* n, const float* in_dev, float* out_dev, hipStream_t wait_on) {
+    if (!n || !in_dev || !out_dev) return RIHVR_E_INVALID_ARGUMENT;
+    if (wait_on) {
+        hipEvent_t ev;
+        NET_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        NET_HIP(hipEventRecord(ev, wait_on));
+        NET_HIP(hipStreamWaitEvent(n->stream, ev, 0));
+        NET_HIP(hipEventDestroy(ev));
+    }
+    rihvr_status st;
+    if ((st = forward_body(n, in_dev))) return st;
+    const float* wo = n->params + n->head_off;
+    hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)n->head_blocks), dim3(kEw), 0, n->stream, n->d[0], n->Vk[0], wo,
+                       wo + 32, n->out_scale, (float2*)out_dev, (const float2*)nullptr, 0.0f, (double*)nullptr);
+    NET_HIP(hipGetLastError());
+    NET_HIP(hipStreamSynchronize(n->stream));
+    return RIHVR_OK;
+}
+
+bool net_matches(const rihvr_net* n, int D, int H, int W) { return n && n->D == D && n->H == H && n->W == W; }
+
+}  // namespace net
+}  // namespace rihvr
+
+extern "C" {
+
+rihvr_status rihvr_net_param_count(int64_t* count) {
+    if (!count) return RIHVR_E_INVALID_ARGUMENT;
+    Layer L[kNumConv];
+    size_t head = 0, total = 0;
+    layer_shapes(L, &head, &total);
+    *count = (int64_t)total;
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_net_create(int32_t num_planes, int32_t height, int32_t width, rihvr_net** out) {
+    if (!out) return RIHVR_E_INVALID_ARGUMENT;
+    *out = nullptr;
+    const int f = 1 << (kLevels - 1);
+    if (num_planes <= 0 || height <= 0 || width <= 0 || num_planes % f || height % f || width % f)
+        return RIHVR_E_INVALID_ARGUMENT;
+    if (num_planes > 65535 || (long)height * width > (1L << 30)) return RIHVR_E_INVALID_ARGUMENT;
+    rihvr_net* n = new rihvr_net();
+    n->D = num_planes;
+    n->H = height;
+    n->W = width;
+    for (int k = 0; k < kLevels; ++k) {
+        n->Dk[k] = num_planes >> k;
+        n->Hk[k] = height >> k;
+        n->Wk[k] = width >> k;
+        n->Vk[k] = (long)n->Dk[k] * n->Hk[k] * n->Wk[k];
+    }
+    layer_shapes(n->L, &n->head_off, &n->nparams);
+    n->head_blocks = (n->Vk[0] + kEw - 1) / kEw;
+    auto fail = [&](rihvr_status s) {
+        net_release(n);
+        delete n;
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) return fail(RIHVR_E_DEVICE);
+    rihvr_status st;
+    if ((st = nalloc(&n->params, n->nparams))) return fail(st);
+    if (hipMemset(n->params, 0, n->nparams * sizeof(float)) != hipSuccess) return fail(RIHVR_E_DEVICE);
+    for (auto& l : n->L) {
+        const size_t cnt = (size_t)kTaps * l.Cout * (l.C0 + l.C1);
+        if ((st = nalloc(&l.w, cnt)) || (st = nalloc(&l.wT, cnt))) return fail(st);
+    }
+    if ((st = nalloc(&n->x16, (size_t)n->Vk[0] * 16))) return fail(st);
+    for (int k = 0; k < kLevels; ++k)
+        if ((st = nalloc(&n->e[k], (size_t)n->Vk[k] * kChan[k]))) return fail(st);
+    for (int k = 1; k < kLevels; ++k)
+        if ((st = nalloc(&n->p[k], (size_t)n->Vk[k] * kChan[k - 1]))) return fail(st);
+    for (int k = 0; k < 3; ++k)
+        if ((st = nalloc(&n->d[k], (size_t)n->Vk[k] * kChan[k]))) return fail(st);
+    if ((st = nalloc(&n->lpart, (size_t)n->head_blocks))) return fail(st);
+    if (hipHostMalloc((void**)&n->h_lpart, n->head_blocks * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(RIHVR_E_OUT_OF_MEMORY);
+    *out = n;
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_net_destroy(rihvr_net* net) {
+    if (!net) return RIHVR_OK;
+    (void)hipStreamSynchronize(net->stream);
+    net_release(net);
+    delete net;
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_net_init(rihvr_net* net, uint64_t seed) {
+    if (!net) return RIHVR_E_INVALID_ARGUMENT;
+    std::vector<float> p(net->nparams, 0.0f);
+    Rng rng{seed};
+    for (int i = 0; i < kNumConv; ++i) {
+        const Layer& l = net->L[i];
+        const int Cin = l.C0 + l.C1;
+        // He initialisation over the used fan-in (the input layer reads 2 of its 16 channels)
+        const int used = i == 0 ? 2 : Cin;
+        const double sd = std::sqrt(2.0 / (double)(kTaps * used));
+        for (int t = 0; t < kTaps; ++t)
+            for (int co = 0; co < l.Cout; ++co)
+                for (int ci = 0; ci < used; ++ci)
+                    p[l.w_off + ((size_t)t * l.Cout + co) * Cin + ci] = (float)(sd * rng.normal());
+    }
+    const double sdo = std::sqrt(1.0 / 16.0);
+    for (int k = 0; k < 32; ++k) p[net->head_off + k] = (float)(sdo * rng.normal());
+    NET_HIP(hipMemcpy(net->params, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (net->train_ready) {
+        NET_HIP(hipMemset(net->adam_m, 0, net->nparams * sizeof(float)));
+        NET_HIP(hipMemset(net->adam_v, 0, net->nparams * sizeof(float)));
+        net->adam_t = 0;
+    }
+    net->weights_ready = false;
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_net_set_params(rihvr_net* net, const float* params, float in_scale, float out_scale) {
+    if (!net || !params || !(in_scale > 0) || !(out_scale > 0)) return RIHVR_E_INVALID_ARGUMENT;
+    NET_HIP(hipMemcpy(net->params, params, net->nparams * sizeof(float), hipMemcpyHostToDevice));
+    net->in_scale = in_scale;
+    net->out_scale = out_scale;
+    net->weights_ready = false;
+    if (net->train_ready) {
+        NET_HIP(hipMemset(net->adam_m, 0, net->nparams * sizeof(float)));
+        NET_HIP(hipMemset(net->adam_v, 0, net->nparams * sizeof(float)));
+        net->adam_t = 0;
+    }
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_net_set_loss_weight(rihvr_net* net, float pos_weight) {
+    if (!net || !(pos_weight >= 0.0f)) return RIHVR_E_INVALID_ARGUMENT;
+    net->pos_weight = pos_weight;
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_net_get_params(rihvr_net* net, float* params, float* in_scale, float* out_scale) {
+    if (!net) return RIHVR_E_INVALID_ARGUMENT;
+    if (params) {
+        NET_HIP(hipStreamSynchronize(net->stream));
+        NET_HIP(hipMemcpy(params, net->params, net->nparams * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    if (in_scale) *in_scale = net->in_scale;
+    if (out_scale) *out_scale = net->out_scale;
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_net_predict(rihvr_net* net, const float* volume_in, float* volume_out) {
+    return rihvr::net::predict(net, volume_in, volume_out, nullptr);
+}
+
+rihvr_status rihvr_net_loss_grad(rihvr_net* net, const float* volume_in, const float* target, double* loss,
+                                 float* grads) {
+    if (!net || !volume_in || !target) return RIHVR_E_INVALID_ARGUMENT;
+    const rihvr_status st = loss_grad(net, volume_in, target, loss);
+    if (st) return st;
+    if (grads) NET_HIP(hipMemcpy(grads, net->grads, net->nparams * sizeof(float), hipMemcpyDeviceToHost));
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_net_adam_step(rihvr_net* net, float lr) {
+    if (!net || !(lr > 0)) return RIHVR_E_INVALID_ARGUMENT;
+    return adam_step(net, lr);
+}
+
+rihvr_status rihvr_net_train_step(rihvr_net* net, const float* volume_in, const float* target, float lr,
+                                  double* loss) {
+    if (!net || !volume_in || !target || !(lr > 0)) return RIHVR_E_INVALID_ARGUMENT;
+    rihvr_status st = loss_grad(net, volume_in, target, loss);
+    if (st) return st;
+    return adam_step(net, lr);
+}
+
+rihvr_status rihvr_net_sync(rihvr_net* net) {
+    if (!net) return RIHVR_E_INVALID_ARGUMENT;
+    NET_HIP(hipStreamSynchronize(net->stream));
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_conv3d_bf16_forward(const uint16_t* in0, int32_t c0, const uint16_t* in1, int32_t c1, int32_t depth,
+                                       int32_t height, int32_t width, const uint16_t* weight, const float* bias,
+                                       int32_t cout, int32_t relu, uint16_t* out) {
+    if (!in0 || !weight || !out) return RIHVR_E_INVALID_ARGUMENT;
+    const hipError_t e = conv_forward(in0, c0, in1, c1, depth, height, width, weight, bias, cout, relu, out, false, 0);
+    if (e == hipErrorInvalidValue) return RIHVR_E_INVALID_ARGUMENT;
+    NET_HIP(e);
+    NET_HIP(hipDeviceSynchronize());
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_conv3d_bf16_dgrad(const uint16_t* grad_out, int32_t cout, int32_t depth, int32_t height,
+                                     int32_t width, const uint16_t* weight_t, int32_t cin, float* grad_in) {
+    if (!grad_out || !weight_t || !grad_in) return RIHVR_E_INVALID_ARGUMENT;
+    const hipError_t e =
+        conv_forward(grad_out, cout, nullptr, 0, depth, height, width, weight_t, nullptr, cin, 0, grad_in, true, 0);
+    if (e == hipErrorInvalidValue) return RIHVR_E_INVALID_ARGUMENT;
+    NET_HIP(e);
+    NET_HIP(hipDeviceSynchronize());
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_conv3d_bf16_wgrad(const uint16_t* in0, int32_t c0, const uint16_t* in1, int32_t c1, int32_t depth,
+                                     int32_t height, int32_t width, const uint16_t* grad_out, int32_t cout,
+                                     float* grad_weight_bias) {
+    if (!in0 || !grad_out || !grad_weight_bias) return RIHVR_E_INVALID_ARGUMENT;
+    if (c0 <= 0 || c0 % 16 || c1 < 0 || c1 % 16 || cout <= 0 || cout % 16) return RIHVR_E_INVALID_ARGUMENT;
+    const long per = (long)kTaps * cout * (c0 + c1) + cout;
+    const long cap = std::max(per, std::min(16L << 20, per * 256));
+    float* part = nullptr;
+    NET_HIP(hipMalloc((void**)&part, cap * sizeof(float)));
+    const hipError_t e = conv_wgrad(in0, c0, in1, c1, depth, height, width, grad_out, cout, part, cap,
+                                    grad_weight_bias, 0);
+    const hipError_t s = hipDeviceSynchronize();
+    (void)hipFree(part);
+    if (e == hipErrorInvalidValue) return RIHVR_E_INVALID_ARGUMENT;
+    NET_HIP(e);
+    NET_HIP(s);
+    return RIHVR_OK;
+}
+
+}  // extern "C"
