@@ -39,8 +39,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int kLevels = 4;
 constexpr int kNumConv = 7;  // e0 e1 e2 e3 d2 d1 d0
 constexpr int kTaps = 27;
-constexpr int kFwdThreads = 256;   // 4 waves, 64 voxels each
-constexpr int kFwdTile = 256;      // voxels of a plane per workgroup
+constexpr int kFwdThreads = 256;   // 4 waves, one output row of 64 voxels each
 constexpr int kMaxCB = 64;         // output channels per workgroup
 constexpr int kEw = 256;           // elementwise block size
 constexpr float kB1 = 0.9f, kB2 = 0.999f, kEps = 1e-8f;
@@ -76,42 +75,70 @@ __device__ __forceinline__ s4 load_in4(const unsigned short* __restrict__ in0, c
 }
 
 // out[v][co] = act(bias[co] + sum_tap sum_ci in[v + tap][ci] * w[tap][co][ci]).
-// Grid: (ceil(H*W / 256), D, Cout / CB); 4 waves, each wave 4 M-tiles of 16 voxels (flat
-// in-plane index) x CB / 16 N-tiles.  The weights of one dz slab (9 taps) and one 16-channel
-// chunk are staged in LDS.  OUTF32: raw FP32 sums (input gradient), else BF16 with bias/relu.
-template <bool OUTF32>
+// A workgroup computes a tile of 4 ROWS rows x 64 columns of one plane for CB output channels:
+// wave w takes rows y0 + w ROWS .. + ROWS - 1, each as 4 M-tiles of 16 voxels, CB / 16 N-tiles.
+// Per 16-channel chunk of the input and per dz, the (4 ROWS + 2) x 66 voxel halo of the tile in
+// plane z + dz is staged in LDS (coalesced 16-byte loads, zero outside the grid; in1's channels
+// read at half resolution) with the weights of that dz slab (9 taps), so the 9 taps read their A
+// and B fragments from LDS.  ROWS = 2 (for CB = 16, whose stages hold little MFMA work) doubles
+// the work per stage.  Grid: (ceil(W/64) * Cout/CB, ceil(H/(4 ROWS)), D).
+// OUTF32: raw FP32 sums (input gradient), else BF16 with bias and optional ReLU.
+constexpr int kHX = 66;  // halo columns: 64 + 2
+template <bool OUTF32, int ROWS>
 __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* __restrict__ in0,
                                                           const unsigned short* __restrict__ in1, ConvGeo g,
                                                           const unsigned short* __restrict__ w,
                                                           const float* __restrict__ bias, int relu,
                                                           void* __restrict__ out) {
-    __shared__ s4 wl[9 * kMaxCB * 4];  // [tap of the slab][co][4 groups of 4 ci]
+    constexpr int TY = 4 * ROWS, HY = TY + 2, HALO = HY * kHX;
+    constexpr int MT = 4 * ROWS;                       // M-tiles per wave
+    constexpr int NTMAX = ROWS == 1 ? 4 : 1;           // N-tiles per wave at most
+    __shared__ uint4 halo[HALO * 2];     // [row][x][16 ch] as two 16-byte halves per voxel
+    __shared__ s4 wl[9 * kMaxCB * 4];    // [tap of the slab][co][4 groups of 4 ci]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = lane >> 4, r16 = lane & 15;
-    const int z = blockIdx.y;
-    const int co0 = blockIdx.z * g.CB;
+    const int nxt = (g.W + 63) >> 6;
+    const int xt = blockIdx.x % nxt, cb = blockIdx.x / nxt;
+    const int x0 = xt * 64, y0 = blockIdx.y * TY, z = blockIdx.z;
+    const int co0 = cb * g.CB;
     const int NT = g.CB >> 4;
-    const int plane = g.H * g.W;
     const int Cin = g.C0 + g.C1;
-    const int vbase = blockIdx.x * kFwdTile + wave * 64;
-    int ya[4], xa[4];
-    bool va[4];
+    const int H2 = g.H >> 1, W2 = g.W >> 1;
+    f4 acc[MT][NTMAX];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const int v = vbase + m * 16 + r16;
-        va[m] = v < plane;
-        ya[m] = va[m] ? v / g.W : 0;
-        xa[m] = va[m] ? v - ya[m] * g.W : 0;
-    }
-    f4 acc[4][4];
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < 4; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int n = 0; n < NTMAX; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     const int nchunks = Cin >> 4;
+    const unsigned short* hs = (const unsigned short*)halo;
     for (int c = 0; c < nchunks; ++c) {
+        const bool hi = c * 16 >= g.C0;
         for (int dz = -1; dz <= 1; ++dz) {
-            __syncthreads();
+            const int zz = z + dz;
+            if (zz < 0 || zz >= g.D) continue;  // a plane outside the grid adds zeros (uniform)
+            __syncthreads();                    // the previous stage's LDS reads are done
+            constexpr int kPer = (HALO * 2 + kFwdThreads - 1) / kFwdThreads;
+            uint4 v[kPer];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int e = threadIdx.x + k * kFwdThreads;
+                const int half = e & 1, vox = e >> 1;
+                const int rx = vox % kHX, ry = vox / kHX;
+                const int yy = y0 + ry - 1, xx = x0 + rx - 1;
+                v[k] = make_uint4(0, 0, 0, 0);
+                if (e < HALO * 2 && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W) {
+                    if (!hi)
+                        v[k] = *(const uint4*)(in0 + (((size_t)zz * g.H + yy) * g.W + xx) * g.C0 + c * 16 + half * 8);
+                    else
+                        v[k] = *(const uint4*)(in1 + (((size_t)(zz >> 1) * H2 + (yy >> 1)) * W2 + (xx >> 1)) * g.C1 +
+                                               (c * 16 - g.C0) + half * 8);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int e = threadIdx.x + k * kFwdThreads;
+                if (e < HALO * 2) halo[e] = v[k];
+            }
             const int t0 = (dz + 1) * 9;
             for (int e = threadIdx.x; e < 9 * g.CB * 4; e += kFwdThreads) {
                 const int qq = e & 3, co = (e >> 2) % g.CB, tt = (e >> 2) / g.CB;
@@ -119,40 +146,43 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
                     *(const s4*)(w + ((size_t)(t0 + tt) * g.Cout + co0 + co) * Cin + c * 16 + qq * 4);
             }
             __syncthreads();
-            const int zz = z + dz;
+#pragma unroll
             for (int tt = 0; tt < 9; ++tt) {
                 const int dy = tt / 3 - 1, dx = tt % 3 - 1;
-                s4 a[4];
+                s4 a[MT];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    a[m] = s4{0, 0, 0, 0};
-                    if (va[m]) a[m] = load_in4(in0, in1, g, zz, ya[m] + dy, xa[m] + dx, c * 16 + q * 4);
+                for (int m = 0; m < MT; ++m) {
+                    const int row = (wave * ROWS + (m >> 2) + dy + 1) * kHX;
+                    a[m] = *(const s4*)(hs + (size_t)(row + (m & 3) * 16 + r16 + dx + 1) * 16 + q * 4);
                 }
 #pragma unroll
-                for (int n = 0; n < 4; ++n) {
+                for (int n = 0; n < NTMAX; ++n) {
                     if (n < NT) {
                         const s4 b = wl[(tt * kMaxCB + n * 16 + r16) * 4 + q];
 #pragma unroll
-                        for (int m = 0; m < 4; ++m)
+                        for (int m = 0; m < MT; ++m)
                             acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n], 0, 0, 0);
                     }
                 }
             }
         }
     }
-    // D[i][j]: lane (q, r16) item i holds voxel 4q + i of the M-tile, output channel r16 of the N-tile
+    // D[i][j]: lane (q, r16) item i holds voxel x0 + 16 (m & 3) + 4q + i of row y0 + wave ROWS + m / 4,
+    // output channel co0 + 16n + r16
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
+    for (int n = 0; n < NTMAX; ++n) {
         if (n >= NT) continue;
         const int co = co0 + n * 16 + r16;
         const float b = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
+        for (int m = 0; m < MT; ++m) {
+            const int y = y0 + wave * ROWS + (m >> 2);
+            if (y >= g.H) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int v = vbase + m * 16 + q * 4 + i;
-                if (v >= plane) continue;
-                const size_t o = ((size_t)z * plane + v) * g.Cout + co;
+                const int x = x0 + (m & 3) * 16 + q * 4 + i;
+                if (x >= g.W) continue;
+                const size_t o = (((size_t)z * g.H + y) * g.W + x) * g.Cout + co;
                 float val = acc[m][n][i] + b;
                 if (relu) val = fmaxf(val, 0.0f);
                 if constexpr (OUTF32)
@@ -256,6 +286,85 @@ __global__ void k_input16(const float2* __restrict__ in, long V, float scale, un
     uint4* o = (uint4*)(x16 + v * 16);
     o[0] = lo;
     o[1] = hi;
+}
+
+// The input layer e0 = relu(conv(x) + b) straight from the complex64 volume: its two real input
+// channels (in_scale Re, in_scale Im in BF16) make K = 27 taps x 2 channels, packed as K = 64
+// (taps 27..31 zero) into 4 MFMA K-steps: step s, lane group q holds taps 8s + 2q and 8s + 2q + 1.
+// A workgroup computes 8 rows x 64 columns of one plane (wave w: rows 2w, 2w + 1, 4 M-tiles each)
+// from the 3 x 10 x 66 halo of (Re, Im) pairs staged in LDS.  wp: [4][16 co][16 k] BF16 with
+// k = 2 (tap - 8s) + channel.  Grid: (ceil(W/64), ceil(H/8), D).
+__global__ void __launch_bounds__(kFwdThreads) k_conv_first(const float2* __restrict__ in, int D, int H, int W,
+                                                            float scale, const unsigned short* __restrict__ wp,
+                                                            const float* __restrict__ bias,
+                                                            unsigned short* __restrict__ out) {
+    constexpr int HY = 10, HX = 66, HALO = 3 * HY * HX;
+    __shared__ unsigned halo[HALO];  // (Re, Im) BF16 pair per voxel
+    __shared__ s4 wls[4 * 16 * 4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = lane >> 4, r16 = lane & 15;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 8, z = blockIdx.z;
+    for (int e = threadIdx.x; e < HALO; e += kFwdThreads) {
+        const int rx = e % HX, ry = (e / HX) % HY, dzi = e / (HX * HY);
+        const int zz = z + dzi - 1, yy = y0 + ry - 1, xx = x0 + rx - 1;
+        unsigned v = 0;
+        if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            const float2 a = in[((size_t)zz * H + yy) * W + xx];
+            v = (unsigned)f2bf(a.x * scale) | ((unsigned)f2bf(a.y * scale) << 16);
+        }
+        halo[e] = v;
+    }
+    for (int e = threadIdx.x; e < 4 * 16 * 4; e += kFwdThreads) wls[e] = *(const s4*)(wp + e * 4);
+    __syncthreads();
+    f4 acc[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const s4 b = wls[(st * 16 + r16) * 4 + q];
+        int off[2];
+        bool ok[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int t = 8 * st + 2 * q + h;
+            ok[h] = t < 27;
+            const int tc = ok[h] ? t : 0;
+            off[h] = ((tc / 9) * HY + (tc / 3) % 3) * HX + (tc % 3);  // halo offset of the tap at (row 0, x 0)
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int base = (wave * 2 + (m >> 2)) * HX + (m & 3) * 16 + r16;
+            const unsigned lo = ok[0] ? halo[base + off[0]] : 0u;
+            const unsigned hi = ok[1] ? halo[base + off[1]] : 0u;
+            s4 a;
+            a[0] = (short)(lo & 0xffffu);
+            a[1] = (short)(lo >> 16);
+            a[2] = (short)(hi & 0xffffu);
+            a[3] = (short)(hi >> 16);
+            acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[m], 0, 0, 0);
+        }
+    }
+    const float bv = bias[r16];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int y = y0 + wave * 2 + (m >> 2);
+        if (y >= H) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + (m & 3) * 16 + q * 4 + i;
+            if (x >= W) continue;
+            out[(((size_t)z * H + y) * W + x) * 16 + r16] = f2bf(fmaxf(acc[m][i] + bv, 0.0f));
+        }
+    }
+}
+
+// the input layer's packed weights: wp[s][co][k] = w[t = 8s + k/2][co][k % 2] (0 for t >= 27)
+__global__ void k_pack_first(const float* __restrict__ p, unsigned short* __restrict__ wp) {
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i >= 4 * 16 * 16) return;
+    const int k = i & 15, co = (i >> 4) & 15, st = i >> 8;
+    const int t = 8 * st + (k >> 1);
+    wp[i] = t < 27 ? f2bf(p[((size_t)t * 16 + co) * 16 + (k & 1)]) : (unsigned short)0;
 }
 
 // p[u][c] = (sum of the 8 children e[2u + (a, b, c)][c]) / 8 (FP32 sum, BF16 out); thread per (u, 4 channels)
@@ -494,11 +603,22 @@ hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short*
     ConvGeo g{D, H, W, C0, C1, Cout, pick_cb(Cout)};
     const long plane = (long)H * W;
     if (plane > (1L << 30)) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((plane + kFwdTile - 1) / kFwdTile), (unsigned)D, (unsigned)(Cout / g.CB));
-    if (out_f32)
-        hipLaunchKernelGGL(k_conv_fwd<true>, grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
-    else
-        hipLaunchKernelGGL(k_conv_fwd<false>, grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+    const long gx = (long)((W + 63) / 64) * (Cout / g.CB);
+    const int rows = g.CB == 16 ? 2 : 1;  // rows per wave
+    const int ty = 4 * rows;
+    if (gx > 0x7fffffffL || (H + ty - 1) / ty > 65535) return hipErrorInvalidValue;
+    dim3 grid((unsigned)gx, (unsigned)((H + ty - 1) / ty), (unsigned)D);
+    if (out_f32) {
+        if (rows == 2)
+            hipLaunchKernelGGL((k_conv_fwd<true, 2>), grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+        else
+            hipLaunchKernelGGL((k_conv_fwd<true, 1>), grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+    } else {
+        if (rows == 2)
+            hipLaunchKernelGGL((k_conv_fwd<false, 2>), grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+        else
+            hipLaunchKernelGGL((k_conv_fwd<false, 1>), grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+    }
     return hipGetLastError();
 }
 
@@ -600,8 +720,9 @@ struct rihvr_net {
     float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
     int adam_t = 0;
     bool weights_ready = false;
+    unsigned short* w0p = nullptr;  // the input layer's packed weights (k_conv_first)
     // activations
-    unsigned short* x16 = nullptr;
+    unsigned short* x16 = nullptr;  // the 16-channel input (training: the input layer's weight gradient)
     unsigned short* e[kLevels] = {};
     unsigned short* p[kLevels] = {};  // p[1..3]
     unsigned short* d[3] = {};        // d0, d1, d2
@@ -628,7 +749,8 @@ template <class T> rihvr_status nalloc(T** ptr, size_t count) {
 
 void net_release(rihvr_net* n) {
     if (!n) return;
-    std::vector<void*> ptrs = {n->params, n->grads, n->adam_m, n->adam_v, n->x16, n->wpart, n->hpart, n->lpart};
+    std::vector<void*> ptrs = {n->params, n->grads, n->adam_m, n->adam_v, n->x16, n->wpart, n->hpart, n->lpart,
+                               n->w0p};
     for (int k = 0; k < kLevels; ++k) {
         ptrs.push_back(n->e[k]);
         ptrs.push_back(n->p[k]);
@@ -656,6 +778,7 @@ rihvr_status prep_weights(rihvr_net* n) {
         hipLaunchKernelGGL(k_prep_weights, dim3(blocks_of(cnt)), dim3(kEw), 0, n->stream, n->params + l.w_off, l.Cout,
                            Cin, l.w, l.wT);
     }
+    hipLaunchKernelGGL(k_pack_first, dim3(4), dim3(256), 0, n->stream, n->params + n->L[0].w_off, n->w0p);
     NET_HIP(hipGetLastError());
     n->weights_ready = true;
     return RIHVR_OK;
@@ -679,17 +802,21 @@ rihvr_status pool(rihvr_net* n, int k, const unsigned short* e, unsigned short* 
     return RIHVR_OK;
 }
 
-// forward pass up to d0 (the head is applied by the caller)
-rihvr_status forward_body(rihvr_net* n, const float* in) {
+// forward pass up to d0 (the head is applied by the caller); with_x16 also writes the 16-channel
+// input the input layer's weight gradient reads
+rihvr_status forward_body(rihvr_net* n, const float* in, bool with_x16) {
     if (!n->weights_ready) {
         const rihvr_status st = prep_weights(n);
         if (st) return st;
     }
-    hipLaunchKernelGGL(k_input16, dim3(blocks_of(n->Vk[0])), dim3(kEw), 0, n->stream, (const float2*)in, n->Vk[0],
-                       n->in_scale, n->x16);
+    if (with_x16)
+        hipLaunchKernelGGL(k_input16, dim3(blocks_of(n->Vk[0])), dim3(kEw), 0, n->stream, (const float2*)in,
+                           n->Vk[0], n->in_scale, n->x16);
+    hipLaunchKernelGGL(k_conv_first, dim3((unsigned)((n->W + 63) / 64), (unsigned)((n->H + 7) / 8), (unsigned)n->D),
+                       dim3(kFwdThreads), 0, n->stream, (const float2*)in, n->D, n->H, n->W, n->in_scale, n->w0p,
+                       n->params + n->L[0].b_off, n->e[0]);
     NET_HIP(hipGetLastError());
     rihvr_status st;
-    if ((st = layer_forward(n, 0, n->x16, nullptr, n->e[0]))) return st;
     for (int k = 1; k < kLevels; ++k) {
         if ((st = pool(n, k - 1, n->e[k - 1], n->p[k]))) return st;
         if ((st = layer_forward(n, k, n->p[k], nullptr, n->e[k]))) return st;
@@ -703,6 +830,7 @@ rihvr_status forward_body(rihvr_net* n, const float* in) {
 rihvr_status alloc_training(rihvr_net* n) {
     if (n->train_ready) return RIHVR_OK;
     rihvr_status st;
+    if ((st = nalloc(&n->x16, (size_t)n->Vk[0] * 16))) return st;
     for (int k = 0; k < kLevels; ++k)
         if ((st = nalloc(&n->ge[k], (size_t)n->Vk[k] * kChan[k]))) return st;
     for (int k = 0; k < 3; ++k) {
@@ -752,7 +880,7 @@ rihvr_status finalize(rihvr_net* n, int k, const unsigned short* act, const floa
 rihvr_status loss_grad(rihvr_net* n, const float* in, const float* target, double* loss) {
     rihvr_status st;
     if ((st = alloc_training(n))) return st;
-    if ((st = forward_body(n, in))) return st;
+    if ((st = forward_body(n, in, true))) return st;
     const float* wo = n->params + n->head_off;
     const float* bo = wo + 32;
     hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)n->head_blocks), dim3(kEw), 0, n->stream, n->d[0], n->Vk[0], wo, bo,
@@ -829,7 +957,7 @@ rihvr_status predict(rihvr_net* n, const float* in_dev, float* out_dev, hipStrea
         NET_HIP(hipEventDestroy(ev));
     }
     rihvr_status st;
-    if ((st = forward_body(n, in_dev))) return st;
+    if ((st = forward_body(n, in_dev, false))) return st;
     const float* wo = n->params + n->head_off;
     hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)n->head_blocks), dim3(kEw), 0, n->stream, n->d[0], n->Vk[0], wo,
                        wo + 32, n->out_scale, (float2*)out_dev, (const float2*)nullptr, 0.0f, (double*)nullptr);
@@ -886,7 +1014,7 @@ rihvr_status rihvr_net_create(int32_t num_planes, int32_t height, int32_t width,
         const size_t cnt = (size_t)kTaps * l.Cout * (l.C0 + l.C1);
         if ((st = nalloc(&l.w, cnt)) || (st = nalloc(&l.wT, cnt))) return fail(st);
     }
-    if ((st = nalloc(&n->x16, (size_t)n->Vk[0] * 16))) return fail(st);
+    if ((st = nalloc(&n->w0p, 4 * 16 * 16))) return fail(st);
     for (int k = 0; k < kLevels; ++k)
         if ((st = nalloc(&n->e[k], (size_t)n->Vk[k] * kChan[k]))) return fail(st);
     for (int k = 1; k < kLevels; ++k)
