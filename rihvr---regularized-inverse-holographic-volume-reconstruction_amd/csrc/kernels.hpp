@@ -6,7 +6,9 @@
 namespace rihvr {
 
 // Line occupancy of a prox-output volume: one 64-bit word per (plane, row), bit l set when
-// the row's voxels 16l .. 16l+15 (one 128-byte line; W <= 1024) hold a non-zero value.  A
+// the row's voxels l L .. l L + L - 1 hold a non-zero value, L = 16 voxels (one 128-byte line)
+// for W <= 1024, else the least power of two giving at most 64 lines (32 up to 2048, 64 up to
+// 4096; only the fixed-length FFT kernels, W <= 1024, read the maps besides the prox).  A
 // volume with a valid map holds exact zeros in every line whose bit is clear, so readers
 // may skip those lines and writers may skip zero lines that were already zero.  A map is
 // [nz][H] row words followed by [nz] plane words, each the OR of its plane's row words.
@@ -48,6 +50,26 @@ hipError_t fft_cols_adj(int L, const float2* in, const float2* phase, float2* ou
 // partial[blockIdx.x] = sum |resid|^2.
 hipError_t fft_cols_real(int L, const float2* in, float2* est, const float2* meas, float2* resid,
                          double* partial, int W, const float2* tw_inv, hipStream_t s);
+
+// ---- FFTs of any length 2..4096 (fft_generic.hip) -----------------------------------
+// Row transforms of length L with a plan chosen at run time; transforms along y run between
+// tiled transposes ([z][R][C] -> [z][C][R]).  Same roles as the fixed-length launchers above.
+bool gen_fft_supported(int L);
+hipError_t gen_rows_plain(int L, int sign, const float2* in, float2* out, long nrows, const float2* tw, hipStream_t s);
+hipError_t gen_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out, int nrows,
+                             const float2* tw_inv, hipStream_t s);
+hipError_t gen_rows_adj(int L, const float2* rhat, const double* kappa, const float* zlist, float2* out, int nrows,
+                        int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s);
+hipError_t gen_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts, int nrows,
+                           int nz, int zchunk, const float2* tw_fwd, hipStream_t s);
+// out[z][c][r] = in[z][r][c] (times phase[z], conjugated with conj; phase may be NULL)
+hipError_t gen_transpose(const float2* in, float2* out, int R, int C, int nz, const float2* phase, bool conj,
+                         hipStream_t s);
+// in [C][R] (a transposed estimate): est = (Re, 0) [R][C], resid = est - meas,
+// partial[block] = sum |resid|^2 over gen_transpose_real_blocks(R, C) blocks
+int gen_transpose_real_blocks(int R, int C);
+hipError_t gen_transpose_real(const float2* in, float2* est, const float2* meas, float2* resid, double* partial, int R,
+                              int C, hipStream_t s);
 
 // ---- prox and elementwise (prox_kernels.hip) --------------------------------
 // Reduction slots written by the prox kernels (per wave partials, summed by reduce_partials).
@@ -103,6 +125,10 @@ hipError_t prox_l1(const ProxLaunch& a, hipStream_t s);
 // Mode 6 (3D TV, z-periodic): three kernels, partials [nblocks][PS_COUNT];
 // scratch is a volume-sized buffer distinct from a.Y.
 int prox_fl3d_num_blocks(int W, int H, int nz);
+// Mode 7 for any n_tv (multi-pass, the duals in memory): work holds 4 volumes
+// (x-hat, y-hat, g_x, g_y); partials [nblocks][PS_COUNT].
+int prox_tv2_num_blocks(int W, int H, int nz);
+hipError_t prox_tv2_general(const ProxLaunch& a, float2* work, hipStream_t s);
 hipError_t prox_fl3d(const ProxLaunch& a, float2* scratch, hipStream_t s);
 
 // out = Y + s (Y - Y0) (the returned momentum point) with partials

@@ -68,6 +68,7 @@ struct ProxDev {
     int W, H, nz;
     float s_mom, t, mu, sc, thr;
     int do_soft;
+    int lsh;  // a map line is 2^lsh voxels of a row: 16 up to W = 1024, then 32, 64 (64 lines per row)
     ProxGeom geo;
     int nwaves;
     double* partial;
@@ -130,7 +131,7 @@ template <int NTV> struct ProxWave {
     // y_{k-2} for the rows loaded next, and of the output's current content for the rows
     // written next
     uint64_t ow0[2] = {~0ull, ~0ull}, owm[2] = {~0ull, ~0ull}, oold[2] = {~0ull, ~0ull};
-    // the lines (bit l: columns 16l .. 16l + 15) that the wave's 64 lanes read, and those
+    // the lines (bit l: columns l 2^lsh .. (l + 1) 2^lsh - 1) that the wave's 64 lanes read, and those
     // its output lanes write
     uint64_t wlines = ~0ull, outlines = ~0ull;
     uint64_t plane_bits = 0;  // lines this wave set in the new map (its plane word's share)
@@ -292,9 +293,10 @@ template <int NTV> struct ProxWave {
                     // touched by non-zero lanes, then one atomic OR per row and wave
                     const int c0 = strip * a.geo.ow - NTV;  // column of lane 0
                     uint64_t bits = 0;
-                    const int l_last = min(c0 + NTV + a.geo.ow - 1, W - 1) >> 4;
-                    for (int l = (c0 + NTV) >> 4; l <= l_last; ++l) {
-                        const int lo = max(16 * l - c0, NTV), hi = min(16 * l + 15 - c0, NTV + a.geo.ow - 1);
+                    const int l_last = min(c0 + NTV + a.geo.ow - 1, W - 1) >> a.lsh;
+                    for (int l = (c0 + NTV) >> a.lsh; l <= l_last; ++l) {
+                        const int lo = max((l << a.lsh) - c0, NTV);
+                        const int hi = min((l << a.lsh) + (1 << a.lsh) - 1 - c0, NTV + a.geo.ow - 1);
                         if (lo > hi || lo > 63) continue;
                         const uint64_t lanes =
                             (hi >= 63 ? ~0ull : ((1ull << (hi + 1)) - 1ull)) & ~((1ull << lo) - 1ull);
@@ -307,7 +309,7 @@ template <int NTV> struct ProxWave {
             // a lane writes when its value is non-zero or its line holds an old non-zero; a
             // wave with neither writes nothing
             if (m | (oold[PAR] & outlines)) {
-                const bool keep = (yv.x != 0.0f || yv.y != 0.0f) || ((oold[PAR] >> (ocol >> 4)) & 1ull);
+                const bool keep = (yv.x != 0.0f || yv.y != 0.0f) || ((oold[PAR] >> (ocol >> a.lsh)) & 1ull);
                 if (out_lane && keep) *(float2*)(yp + (unsigned)o * rowb + oofs) = yv;
             }
         }
@@ -415,9 +417,9 @@ __global__ void __launch_bounds__(64 * kProxWaves)
     // the wave's lines: the periodic wrap can add a line at the other end of the row
     st.wlines = 0;
     st.outlines = 0;
-    for (int l = 0; l < (W >> 4); ++l) {
-        if (__builtin_amdgcn_ballot_w64((xcol >> 4) == l)) st.wlines |= 1ull << l;
-        if (__builtin_amdgcn_ballot_w64(out_lane && (ocol >> 4) == l)) st.outlines |= 1ull << l;
+    for (int l = 0; l <= ((W - 1) >> a.lsh); ++l) {
+        if (__builtin_amdgcn_ballot_w64((xcol >> a.lsh) == l)) st.wlines |= 1ull << l;
+        if (__builtin_amdgcn_ballot_w64(out_lane && (ocol >> a.lsh) == l)) st.outlines |= 1ull << l;
     }
 #pragma unroll
     for (int p = 0; p < 2; ++p)
@@ -485,6 +487,8 @@ static ProxDev make_dev(const ProxLaunch& a) {
     d.sc = a.reg_tv != 0.0 ? (float)(1 / (4 * 3 * a.t * a.reg_tv)) : 0.0f;
     d.thr = (float)(a.t * a.reg);
     d.do_soft = a.reg > 0.0;
+    d.lsh = 4;
+    while (((a.W - 1) >> d.lsh) >= 64) ++d.lsh;
     d.geo = prox_geom(a.W, a.H, a.n_tv, a.row_chunk);
     d.nwaves = d.geo.nstrips * d.geo.nchunks * a.nz;
     d.partial = a.partial;
@@ -771,6 +775,162 @@ hipError_t prox_fl3d(const ProxLaunch& a, float2* scratch, hipStream_t s) {
         src ^= 1;
     }
     hipLaunchKernelGGL(k_fl3d_final, eg, dim3(kEwThreads), 0, s, d, buf[src]);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// Mode 7 for any number of TV iterations (the fused k_prox_fl2d holds its per-level state in
+// registers and is built for up to kMaxNtv): proximalFusedLasso2d (:3652-3732) as separate
+// passes over the volume with the duals in memory, each pass the reference kernel's arithmetic
+// (float storage, double where it promotes):
+//   k_tv2_pre:      x0, x-hat = x0 - t g (sparse_get_xhat_kernel :3478-3489), duals cleared,
+//                   L1 / 2D TV / nnz of x0
+//   k_tv2_estimate: yhat = xhat - mu div(g) (tv_denoised_estimate_2d_kernel :3491-3513)
+//   k_tv2_project:  g = (g + sc grad yhat) / max(|.|, 1) (tv_gradient_projection_2d_kernel
+//                   :3539-3597), in place (each voxel reads only its own duals)
+//   k_tv2_final:    y = soft(yhat) and the line-search / restart sums (:3065-3110, :4307-4482)
+// ----------------------------------------------------------------------------
+struct Tv2Buf {
+    float2* xh;
+    float2* yh;
+    float2* gx;
+    float2* gy;
+};
+
+__global__ void __launch_bounds__(kEwThreads) k_tv2_pre(ProxDev a, Tv2Buf b, double t) {
+    __shared__ double red[kEwThreads / 64];
+    const int W = a.W, H = a.H;
+    const long N = (long)W * H;
+    const long n = N * a.nz;
+    double l1 = 0, tv = 0, nnz = 0;
+    const float s = a.s_mom;
+    auto mom = [&](long i) {
+        const float2 y0 = a.Y0[i], ym = a.YM[i];
+        return make_float2(fmaf(s, y0.x - ym.x, y0.x), fmaf(s, y0.y - ym.y, y0.y));
+    };
+    for (int it = 0; it < kEwItems; ++it) {
+        long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
+        if (i >= n) break;
+        const float2 x0 = mom(i), g = a.G[i];
+        b.xh[i] = make_float2((float)((double)x0.x - t * (double)g.x), (float)((double)x0.y - t * (double)g.y));
+        b.gx[i] = make_float2(0.0f, 0.0f);
+        b.gy[i] = make_float2(0.0f, 0.0f);
+        if (x0.x != 0.0f || x0.y != 0.0f) {
+            l1 += (double)sqrtf(x0.x * x0.x + x0.y * x0.y);
+            nnz += 1.0;
+        }
+        const long zo = (i / N) * N;
+        const int p = (int)(i - zo);
+        const int y = p / W, x = p - y * W;
+        const int xm = x > 0 ? x - 1 : W - 1, ym = y > 0 ? y - 1 : H - 1;
+        const float2 qa = mom(zo + (long)y * W + xm), qb = mom(zo + (long)ym * W + x);
+        const float gxx = qa.x - x0.x, gyx = qb.x - x0.x, gxy = qa.y - x0.y, gyy = qb.y - x0.y;
+        tv += (double)sqrtf(gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy);
+    }
+    double* pp = a.partial + (size_t)blockIdx.x * PS_COUNT;
+    double r;
+    r = block_sum(l1, red);
+    if (threadIdx.x == 0) pp[PS_L1X0] = r;
+    r = block_sum(tv, red);
+    if (threadIdx.x == 0) pp[PS_TVX0] = r;
+    r = block_sum(nnz, red);
+    if (threadIdx.x == 0) pp[PS_NNZX0] = r;
+}
+
+// yhat = xhat - mu (gx[x+1] - gx[x] + gy[y+1] - gy[y]), periodic; thread = voxel
+__global__ void __launch_bounds__(256) k_tv2_estimate(Tv2Buf b, int W, int H, double mu) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    const long o = (long)blockIdx.z * W * H;
+    const int xp = x < W - 1 ? x + 1 : 0, yp = y < H - 1 ? y + 1 : 0;
+    const long i = o + (long)y * W + x;
+    const float2 gxi = b.gx[i], gyi = b.gy[i];
+    const float2 gxp = b.gx[o + (long)y * W + xp], gyp = b.gy[o + (long)yp * W + x];
+    const float dx = gxp.x - gxi.x + gyp.x - gyi.x;
+    const float dy = gxp.y - gxi.y + gyp.y - gyi.y;
+    const float2 xh = b.xh[i];
+    b.yh[i] = make_float2((float)((double)xh.x - mu * (double)dx), (float)((double)xh.y - mu * (double)dy));
+}
+
+// g = (g + sc (yhat[x-1] - yhat[x], yhat[y-1] - yhat[y])) / max(|.|, 1) over (re, im) x (x, y)
+__global__ void __launch_bounds__(256) k_tv2_project(Tv2Buf b, int W, int H, double sc) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    const long o = (long)blockIdx.z * W * H;
+    const int xm = x > 0 ? x - 1 : W - 1, ym = y > 0 ? y - 1 : H - 1;
+    const long i = o + (long)y * W + x;
+    const float2 c = b.yh[i], a0 = b.yh[o + (long)y * W + xm], a1 = b.yh[o + (long)ym * W + x];
+    float gxx = a0.x - c.x, gyx = a1.x - c.x, gxy = a0.y - c.y, gyy = a1.y - c.y;
+    const float2 gxi = b.gx[i], gyi = b.gy[i];
+    gxx = (float)((double)gxi.x + sc * (double)gxx);
+    gyx = (float)((double)gyi.x + sc * (double)gyx);
+    gxy = (float)((double)gxi.y + sc * (double)gxy);
+    gyy = (float)((double)gyi.y + sc * (double)gyy);
+    float nrm = gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy;
+    nrm = (float)fmax(sqrt((double)nrm), 1.0);
+    b.gx[i] = make_float2(gxx / nrm, gxy / nrm);
+    b.gy[i] = make_float2(gyx / nrm, gyy / nrm);
+}
+
+__global__ void __launch_bounds__(kEwThreads) k_tv2_final(ProxDev a, const float2* __restrict__ yh) {
+    __shared__ double red[kEwThreads / 64];
+    const long N = (long)a.W * a.H;
+    const long n = N * a.nz;
+    double ls1 = 0, ls2 = 0, rs = 0, nnzy = 0;
+    const float s = a.s_mom;
+    for (int it = 0; it < kEwItems; ++it) {
+        long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
+        if (i >= n) break;
+        const float2 y0 = a.Y0[i], ym = a.YM[i], g = a.G[i], v = yh[i];
+        const float2 x0 = make_float2(fmaf(s, y0.x - ym.x, y0.x), fmaf(s, y0.y - ym.y, y0.y));
+        float2 yv = a.do_soft ? make_float2(soft1(v.x, a.thr), soft1(v.y, a.thr)) : v;
+        if (yv.x == 0.0f) yv.x = 0.0f;  // +0 for a zero of either sign
+        if (yv.y == 0.0f) yv.y = 0.0f;
+        a.Y[i] = yv;
+        const float dr = yv.x - x0.x, di = yv.y - x0.y;
+        ls1 += (double)(dr * g.x) + (double)(di * g.y);
+        ls2 += (double)dr * dr + (double)di * di;
+        const float dre = (x0.x - yv.x) * (yv.x - y0.x);
+        const float dim = (x0.y - yv.y) * (yv.y - y0.y);
+        const int off = (int)((i % N) & 1023);
+        rs += off < 512 ? (double)dre + 2.0 * (double)dim : (double)dim;
+        nnzy += (yv.x != 0.0f || yv.y != 0.0f) ? 1.0 : 0.0;
+    }
+    double* pp = a.partial + (size_t)blockIdx.x * PS_COUNT;
+    double r;
+    r = block_sum(ls1, red);
+    if (threadIdx.x == 0) pp[PS_LS1] = r;
+    r = block_sum(ls2, red);
+    if (threadIdx.x == 0) pp[PS_LS2] = r;
+    r = block_sum(rs, red);
+    if (threadIdx.x == 0) pp[PS_RESTART] = r;
+    r = block_sum(nnzy, red);
+    if (threadIdx.x == 0) {
+        pp[PS_NNZY] = r;
+        pp[7] = 0.0;
+    }
+}
+
+int prox_tv2_num_blocks(int W, int H, int nz) { return prox_l1_num_blocks(W, H, nz); }
+
+hipError_t prox_tv2_general(const ProxLaunch& a, float2* work, hipStream_t s) {
+    if (!work) return hipErrorInvalidValue;
+    ProxDev d = make_dev(a);
+    const size_t V = (size_t)a.W * a.H * a.nz;
+    Tv2Buf b{work, work + V, work + 2 * V, work + 3 * V};
+    const dim3 eg(prox_tv2_num_blocks(a.W, a.H, a.nz));
+    const dim3 sg((a.W + 63) / 64, (a.H + 3) / 4, a.nz);
+    const double mu = a.t * a.reg_tv;
+    const double sc = 1 / (4 * 3 * a.t * a.reg_tv);
+    hipLaunchKernelGGL(k_tv2_pre, eg, dim3(kEwThreads), 0, s, d, b, a.t);
+    for (int k = 0; k < a.n_tv; ++k) {
+        hipLaunchKernelGGL(k_tv2_estimate, sg, dim3(256), 0, s, b, a.W, a.H, mu);
+        hipLaunchKernelGGL(k_tv2_project, sg, dim3(256), 0, s, b, a.W, a.H, sc);
+    }
+    hipLaunchKernelGGL(k_tv2_estimate, sg, dim3(256), 0, s, b, a.W, a.H, mu);
+    hipLaunchKernelGGL(k_tv2_final, eg, dim3(kEwThreads), 0, s, d, (const float2*)b.yh);
     return hipGetLastError();
 }
 

@@ -106,6 +106,11 @@ struct rihvr_ctx {
     float2* G = nullptr;                          // A^T r
     float2* G_alt = nullptr;                      // the next iteration's A^T r, formed ahead
     float2* T = nullptr;                          // FFT scratch volume
+    // FFTs of lengths outside the fixed-length kernels' set: rows with run-time plans between
+    // transposes (fft_generic.hip); T2 holds the transposed planes
+    bool gen = false;
+    float2* T2 = nullptr;
+    float2* tvbuf = nullptr;                      // 4 volumes for mode 7 with n_tv above the fused prox's limit
     float2* out = nullptr;                        // points at the returned volume
     // holograms
     // Hologram-domain quantities of the iteration are kept as spectra FFT(.) / N (the
@@ -161,8 +166,9 @@ rihvr_status validate(const rihvr_params* p) {
     if (p->mode != RIHVR_MODE_FASTA_L1 && p->mode != RIHVR_MODE_FASTA_FUSED_LASSO &&
         p->mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D)
         return RIHVR_E_INVALID_ARGUMENT;
-    if (!fft_length_supported(p->width) || !fft_length_supported(p->height)) return RIHVR_E_INVALID_ARGUMENT;
-    if (p->mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && !prox_fl2d_supported(p->num_tv_iterations))
+    // any length the fixed-length FFT kernels or the run-time plans cover (up to 4096)
+    if (!(fft_length_supported(p->width) && fft_length_supported(p->height)) &&
+        !(gen_fft_supported(p->width) && gen_fft_supported(p->height)))
         return RIHVR_E_INVALID_ARGUMENT;
     return RIHVR_OK;
 }
@@ -302,6 +308,19 @@ void invalidate_occ(rihvr_ctx* c, const float2* v) {
 rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipStream_t st, const uint64_t* occ) {
     float2* Tg = c->T + (size_t)g0 * c->N;
     float2* spg = c->spart + (size_t)(g0 / c->zchunk_fwd) * c->N;
+    if (c->gen) {
+        // columns as rows of the transposed planes ([z][W][H]); every line is read
+        float2* Tt = c->T2 + (size_t)g0 * c->N;
+        RIHVR_TRY(timed_on(c, K_COLS_FWD, st, [&] {
+            hipError_t e = gen_transpose(vol + (size_t)g0 * c->N, Tt, c->H, c->W, gz, c->phase_fwd + g0, true, st);
+            if (e == hipSuccess) e = gen_rows_plain(c->H, -1, Tt, Tt, (long)gz * c->W, c->twfH, st);
+            if (e == hipSuccess) e = gen_transpose(Tt, Tg, c->W, c->H, gz, nullptr, false, st);
+            return e;
+        }));
+        return timed_on(c, K_ROWS_FWDACC, st, [&] {
+            return gen_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, c->twfW, st);
+        });
+    }
     uint8_t* smg = c->smap + (size_t)g0 * (c->W / 16);
     const uint64_t* og = occ ? occ + (size_t)g0 * c->H : nullptr;
     const uint64_t* opg = occ ? map_planes(c, occ) + g0 : nullptr;
@@ -333,6 +352,18 @@ rihvr_status forward_finish(rihvr_ctx* c, float2* est) {
 // The spatial estimate and residual of a spectrum est_spec (FFT(est) / N): est = Re(IFFT(.)),
 // into hspat, and resid = est - meas (back_propagate_finish_kernel, RM_EST_TRUE)
 rihvr_status spatial_estimate(rihvr_ctx* c, const float2* est_spec) {
+    if (c->gen) {
+        RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] {
+            return gen_rows_sumparts(c->W, est_spec, 1, c->N, c->htmp, c->H, c->twiW, c->stream);
+        }));
+        return timed(c, K_COLS_REAL, [&] {
+            hipError_t e = gen_transpose(c->htmp, c->T2, c->H, c->W, 1, nullptr, false, c->stream);
+            if (e == hipSuccess) e = gen_rows_plain(c->H, 1, c->T2, c->T2, c->W, c->twiH, c->stream);
+            if (e == hipSuccess)
+                e = gen_transpose_real(c->T2, c->hspat, c->meas, c->resid, c->partial_holo, c->H, c->W, c->stream);
+            return e;
+        });
+    }
     RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] { return fft_rows_sumparts(c->W, est_spec, 1, c->N, c->htmp, c->H, c->twiW, c->stream); }));
     RIHVR_TRY(timed(c, K_COLS_REAL, [&] { return fft_cols_real(c->H, c->htmp, c->hspat, c->meas, c->resid, c->partial_holo, c->W, c->twiH, c->stream); }));
     return RIHVR_OK;
@@ -348,6 +379,20 @@ rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
 // R-hat = FFT(r) of a spatial residual (SparseGradient::reconstruct, sparse_gradient.cu:47-75)
 // into out; ev_rhat marks rhat
 rihvr_status spectrum(rihvr_ctx* c, const float2* r, float2* out) {
+    if (c->gen) {
+        RIHVR_TRY(timed(c, K_ROWS_PLAIN, [&] { return gen_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream); }));
+        RIHVR_TRY(timed(c, K_COLS_PLAIN, [&] {
+            hipError_t e = gen_transpose(c->htmp, c->T2, c->H, c->W, 1, nullptr, false, c->stream);
+            if (e == hipSuccess) e = gen_rows_plain(c->H, -1, c->T2, c->T2, c->W, c->twfH, c->stream);
+            if (e == hipSuccess) e = gen_transpose(c->T2, out, c->W, c->H, 1, nullptr, false, c->stream);
+            return e;
+        }));
+        if (out == c->rhat) {
+            c->rhat_scaled = false;
+            RIHVR_HIP(hipEventRecord(c->ev_rhat, c->stream));
+        }
+        return RIHVR_OK;
+    }
     RIHVR_TRY(timed(c, K_ROWS_PLAIN, [&] { return fft_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream); }));
     RIHVR_TRY(timed(c, K_COLS_PLAIN, [&] { return fft_cols_plain(c->H, -1, c->htmp, out, c->W, 1, c->twfH, c->stream); }));
     if (out == c->rhat) {
@@ -369,6 +414,19 @@ rihvr_status rhat_written(rihvr_ctx* c) {
 rihvr_status adjoint_group(rihvr_ctx* c, float2* G, int g0, int gz, hipStream_t st) {
     const float invN = c->rhat_scaled ? 1.0f : (float)(1.0 / (double)c->N);
     float2* Tg = c->T + (size_t)g0 * c->N;
+    if (c->gen) {
+        float2* Tt = c->T2 + (size_t)g0 * c->N;
+        RIHVR_TRY(timed_on(c, K_ROWS_ADJ, st, [&] {
+            return gen_rows_adj(c->W, c->rhat, c->kappa, c->zlist + g0, Tg, c->H, gz, std::min(c->zchunk_adj, gz), invN,
+                                c->twiW, st);
+        }));
+        return timed_on(c, K_COLS_ADJ, st, [&] {
+            hipError_t e = gen_transpose(Tg, Tt, c->H, c->W, gz, nullptr, false, st);
+            if (e == hipSuccess) e = gen_rows_plain(c->H, 1, Tt, Tt, (long)gz * c->W, c->twiH, st);
+            if (e == hipSuccess) e = gen_transpose(Tt, G + (size_t)g0 * c->N, c->W, c->H, gz, c->phase + g0, false, st);
+            return e;
+        });
+    }
     RIHVR_TRY(timed_on(c, K_ROWS_ADJ, st, [&] {
         return fft_rows_adj(c->W, c->rhat, c->kappa, c->zlist + g0, Tg, c->H, gz, std::min(c->zchunk_adj, gz), invN,
                             c->twiW, st);
@@ -409,7 +467,12 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
     a.partial = c->partial;
     int rows;
     const int ky = ring_index(c, Y);
-    if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) {
+    if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && c->tvbuf) {
+        // more TV iterations than the fused kernel holds: the multi-pass form (no maps)
+        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_tv2_general(a, c->tvbuf, c->stream); }));
+        rows = prox_tv2_num_blocks(c->W, c->H, c->Nz);
+        if (ky >= 0) c->occ_valid[ky] = false;
+    } else if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) {
         // the new map is built in occ_tmp, then becomes Y's; a volume without a valid map
         // passes the all-ones map (every line read, every line of Y rewritten)
         const uint64_t* o0 = occ_of(c, Y0);
@@ -434,7 +497,7 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
         RIHVR_TRY(timed(c, K_PROX, [&] { return prox_l1(a, c->stream); }));
         rows = prox_l1_num_blocks(c->W, c->H, c->Nz);
     }
-    if (mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D && ky >= 0) c->occ_valid[ky] = false;
+    if ((mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D || c->tvbuf) && ky >= 0) c->occ_valid[ky] = false;
     RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial, rows, PS_COUNT, c->red_work, c->scal + SL_PROX, c->stream); }));
     return RIHVR_OK;
 }
@@ -510,7 +573,7 @@ void release(rihvr_ctx* c) {
     (void)hipFree(c->occ_tmp);
     (void)hipFree(c->occ_ones);
     (void)hipFree(c->smap);
-    void* ptrs[] = {c->G, c->G_alt, c->T, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->bspec,
+    void* ptrs[] = {c->G, c->G_alt, c->T, c->T2, c->tvbuf, c->meas, c->hE, c->hC, c->hP, c->hX, c->resid, c->rhat, c->htmp, c->bspec,
                     c->hspat, c->spart, c->kappa,
                     c->zlist, c->phase, c->phase_fwd, c->twfW, c->twiW, c->twfH, c->twiH, c->partial, c->partial_holo, c->scal,
                     c->red_work, c->dflag};
@@ -561,6 +624,10 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     pp = ((pp + 15) / 16) * 16;
     c->pipe_planes = std::min(c->Nz, std::max(16, pp));
     if (c->pipe_planes % c->zchunk_fwd || c->pipe_planes % c->zchunk_adj) c->pipe_planes = c->Nz;
+    c->gen = !(fft_length_supported(c->W) && fft_length_supported(c->H));
+    const bool tv_general = params->mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D &&
+                            !prox_fl2d_supported(params->num_tv_iterations);
+    if (c->gen || tv_general) c->pipe_planes = c->Nz;  // the plane-group schedule needs the fused kernels
     c->n_groups = (c->Nz + c->pipe_planes - 1) / c->pipe_planes;
     if (const char* e = std::getenv("RIHVR_GROUP_ONE_STREAM")) c->groups_one_stream = std::atoi(e) != 0;
     if (const char* e = std::getenv("RIHVR_TEST_BACKTRACKS")) c->test_backtracks = std::strtoull(e, nullptr, 0);
@@ -575,6 +642,8 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_TRY(dalloc(&c->G, c->V));
     RIHVR_TRY(dalloc(&c->G_alt, c->V));
     RIHVR_TRY(dalloc(&c->T, c->V));
+    if (c->gen) RIHVR_TRY(dalloc(&c->T2, c->V));
+    if (tv_general) RIHVR_TRY(dalloc(&c->tvbuf, 4 * (size_t)c->V));
     float2** holos[] = {&c->meas, &c->hE, &c->hC, &c->hP, &c->hX, &c->resid, &c->rhat, &c->htmp, &c->bspec, &c->hspat};
     for (float2** h : holos) RIHVR_TRY(dalloc(h, c->N));
     RIHVR_TRY(dalloc(&c->spart, (size_t)c->N * c->nparts));
@@ -590,9 +659,11 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
                                prox_l1_num_blocks(c->W, c->H, c->Nz));
     rows = std::max<long>(rows, prox_fl3d_num_blocks(c->W, c->H, c->Nz));
     rows = std::max<long>(rows, momentum_out_num_blocks(c->W, c->H, c->Nz));
+    rows = std::max<long>(rows, prox_tv2_num_blocks(c->W, c->H, c->Nz));
     c->partial_rows = rows;
     RIHVR_TRY(dalloc(&c->partial, (size_t)rows * PS_COUNT));
     c->holo_rows = std::max<long>(std::max<long>(holo_num_blocks(c->N), c->W / 16), holo_spec_num_blocks(c->W, c->H));
+    c->holo_rows = std::max<long>(c->holo_rows, gen_transpose_real_blocks(c->H, c->W));
     RIHVR_TRY(dalloc(&c->partial_holo, c->holo_rows));
     RIHVR_TRY(dalloc(&c->scal, SL_COUNT));
     RIHVR_TRY(dalloc(&c->red_work, (size_t)kReduceBlocks * PS_COUNT));

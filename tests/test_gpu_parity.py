@@ -281,6 +281,9 @@ def test_error_codes(pkg, gpu_available):
     with pytest.raises(pkg.RihvrError) as e:
         pkg.SparseCompressiveHolo(pkg.make_params(**cfg(32, 32, 2, mode=3)))
     assert e.value.status == 7  # modes other than 5, 6, 7: HOLO_ERROR_INVALID_ARGUMENT (:178-189)
+    with pytest.raises(pkg.RihvrError) as e:
+        pkg.SparseCompressiveHolo(pkg.make_params(**cfg(4112, 32, 2)))  # rows beyond the 4096 FFT limit
+    assert e.value.status == 7
     with pkg.SparseCompressiveHolo(pkg.make_params(**cfg(32, 32, 2))) as rec:
         bad = np.zeros((32, 32), np.complex64)
         bad[3, 4] = np.nan
