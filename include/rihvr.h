@@ -392,6 +392,24 @@ rihvr_status rihvr_background_mean(const float* frames, int32_t n_frames, int32_
 rihvr_status rihvr_background_median5(const float* frames5, int32_t width, int32_t height, int frames_loc,
                                       float* out);
 
+/* The reference's correlation background (ccbg.cpp computeBackgrounds): for each frame f of
+ * frames [n][H][W] (raw grey values, float32 at frames_loc), the other frames ranked by their
+ * normalised cross-correlation with f on images reduced by resize_factor (k x k box averages,
+ * cv::resize INTER_AREA at resizePct = 1/k; W and H multiples of k); for m = 1 .. min(max_frames,
+ * n - 1) the entropy of f minus the mean of its m best frames (256 bins of the rounded value,
+ * clamped to 0..255); the first m where the 3-point-smoothed entropy falls, clamped to
+ * [min_frames, min(max_frames, n - 1)] (the reference reads past its ranking when min_frames
+ * >= n; here the count stops at n - 1); the background is the mean of that many full-resolution
+ * frames as 8 bits (convertTo: rounded to even, saturated).  backgrounds: [n][H][W] on the host;
+ * enhanced (may be NULL): normalize(f - background, 0..255, NORM_MINMAX) as 8 bits; frames_used
+ * (may be NULL): [n].  The correlations are summed in double (the reference in float), so frames
+ * whose correlations agree to float rounding may rank in another order; equal correlations rank
+ * by frame number.  n < 2: RIHVR_E_MISSING_DATA. */
+rihvr_status rihvr_background_correlation(const float* frames, int32_t n_frames, int32_t width, int32_t height,
+                                          int frames_loc, int32_t min_frames, int32_t max_frames,
+                                          int32_t resize_factor, uint8_t* backgrounds, uint8_t* enhanced,
+                                          int32_t* frames_used);
+
 /* ---- Segmentation and object centroids (the reference's segmentation executable,
  * src/core/main_segmentation.cu:83-125): SparseSegmentation (sparse_segmentation.cu)
  * binarize -> close -> labelConnectedComponents, then ObjectCloud (object_cloud.cu)

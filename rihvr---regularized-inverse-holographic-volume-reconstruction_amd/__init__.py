@@ -51,7 +51,7 @@ EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_d
             "rihvr_net_init", "rihvr_net_set_params", "rihvr_net_get_params", "rihvr_net_predict",
             "rihvr_net_loss_grad", "rihvr_net_adam_step", "rihvr_net_train_step", "rihvr_net_sync",
             "rihvr_conv3d_bf16_forward", "rihvr_conv3d_bf16_dgrad", "rihvr_conv3d_bf16_wgrad",
-            "rihvr_reconstruct_learned", "rihvr_net_set_loss_weight")
+            "rihvr_reconstruct_learned", "rihvr_net_set_loss_weight", "rihvr_background_correlation")
 PATH_MAX = 4096
 
 
@@ -372,6 +372,8 @@ def lib():
         L.rihvr_net_loss_grad.argtypes = [vp, vp, vp, _P(ctypes.c_double), vp]
         L.rihvr_net_adam_step.argtypes = [vp, ctypes.c_float]
         L.rihvr_net_set_loss_weight.argtypes = [vp, ctypes.c_float]
+        L.rihvr_background_correlation.argtypes = [vp, i32, i32, i32, ip, i32, i32, i32, vp, vp, vp]
+        L.rihvr_background_correlation.restype = st
         L.rihvr_net_train_step.argtypes = [vp, vp, vp, ctypes.c_float, _P(ctypes.c_double)]
         L.rihvr_net_sync.argtypes = [vp]
         L.rihvr_conv3d_bf16_forward.argtypes = [vp, i32, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp]
@@ -567,6 +569,20 @@ def background_median5(frames5):
     _check(lib().rihvr_background_median5(fp, w, h, loc, ctypes.c_void_p(out.ctypes.data)),
            "rihvr_background_median5")
     return out
+
+
+def background_correlation(frames, min_frames=5, max_frames=100, resize_factor=4, enhanced=False):
+    """rihvr_background_correlation (ccbg.cpp): frames [n][H][W] raw grey values (numpy, or a
+    float32 CUDA tensor) -> (backgrounds uint8 [n][H][W], frames used [n], enhanced or None)."""
+    fp, loc, _keep, (n, h, w) = _frames_arg(frames)
+    bg = np.empty((n, h, w), np.uint8)
+    used = np.empty(n, np.int32)
+    enh = np.empty((n, h, w), np.uint8) if enhanced else None
+    _check(lib().rihvr_background_correlation(fp, n, w, h, loc, int(min_frames), int(max_frames), int(resize_factor),
+                                              ctypes.c_void_p(bg.ctypes.data),
+                                              ctypes.c_void_p(enh.ctypes.data) if enhanced else None,
+                                              ctypes.c_void_p(used.ctypes.data)), "rihvr_background_correlation")
+    return bg, used, enh
 
 
 def write_hologram(path, holo, lo=None, hi=None):

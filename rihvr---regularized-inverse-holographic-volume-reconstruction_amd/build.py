@@ -23,7 +23,7 @@ OUT = PKG / "_build"
 LIB = OUT / "librihvr.so"
 # executables linked against the library: main source -> file name under _build/
 EXECUTABLES = {"cli_main.cpp": "sparse-inverse-recon", "make_background_main.cpp": "make-background",
-               "segmentation_main.cpp": "segmentation"}
+               "segmentation_main.cpp": "segmentation", "ccbg_main.cpp": "ccbg"}
 CLI = OUT / EXECUTABLES["cli_main.cpp"]
 
 ARCH = "gfx950"

@@ -3,12 +3,14 @@
 // HoloSequence::useBackgroundSubtraction (src/core/holo_sequence.cu:309-437) and
 // the median of five frames (median5plane_kernel, main_create_background.cu:13-94).
 // Elementwise and reduction work, HBM-bound: one pass over the frames for the
-// sum, one for the statistics.
+// sum, one for the statistics.  And the reference's correlation background (ccbg.cpp):
+// per frame, the mean of its best-correlated frames, as many as an entropy rule picks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <numeric>
 #include <vector>
 
 #include "../../include/rihvr.h"
@@ -115,6 +117,100 @@ __global__ void __launch_bounds__(kThreads) k_median5(const float* __restrict__ 
     out[i] = a2;
 }
 
+// ---- correlation background (ccbg.cpp) -------------------------------------------
+// Box average of k x k pixels (cv::resize INTER_AREA at an integer factor): the block's values
+// added row by row in float, times float(1 / k^2).  out: [n][H/k][W/k]
+__global__ void k_cc_resize(const float* __restrict__ f, int W, int H, int k, long n_out, float* __restrict__ out) {
+    const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n_out) return;
+    const int w = W / k, h = H / k;
+    const int x = (int)(i % w), y = (int)((i / w) % h);
+    const long fr = i / ((long)w * h);
+    const float* src = f + (size_t)fr * W * H + (size_t)(y * k) * W + x * k;
+    float s = 0.0f;
+    for (int dy = 0; dy < k; ++dy)
+        for (int dx = 0; dx < k; ++dx) s += src[(size_t)dy * W + dx];
+    out[i] = s * (1.0f / (float)(k * k));
+}
+
+// mean[a] = sum_p x[a][p] / P in double (one block per frame)
+__global__ void __launch_bounds__(kThreads) k_cc_mean(const float* __restrict__ x, long P, double* __restrict__ mean) {
+    __shared__ double red[kThreads / 64];
+    const float* xa = x + (size_t)blockIdx.x * P;
+    double s = 0.0;
+    for (long p = threadIdx.x; p < P; p += kThreads) s += (double)xa[p];
+    s = rihvr::block_sum(s, red);
+    if (threadIdx.x == 0) mean[blockIdx.x] = s / (double)P;
+}
+
+// gram[a][b] = sum_p (x[a][p] - m_a)(x[b][p] - m_b) in double, for a <= b (block (b, a))
+__global__ void __launch_bounds__(kThreads) k_cc_gram(const float* __restrict__ x, long P, int n,
+                                                      const double* __restrict__ mean, double* __restrict__ gram) {
+    __shared__ double red[kThreads / 64];
+    const int a = blockIdx.y, b = blockIdx.x;
+    if (a > b) return;  // uniform per block
+    const float* xa = x + (size_t)a * P;
+    const float* xb = x + (size_t)b * P;
+    const double ma = mean[a], mb = mean[b];
+    double s = 0.0;
+    for (long p = threadIdx.x; p < P; p += kThreads) s += ((double)xa[p] - ma) * ((double)xb[p] - mb);
+    s = rihvr::block_sum(s, red);
+    if (threadIdx.x == 0) gram[(size_t)a * n + b] = gram[(size_t)b * n + a] = s;
+}
+
+// Entropy histograms of frame f = blockIdx.y against the running mean of its ranked frames:
+// for m = 0 .. mmax-1, avg = (x[order[0]] + ... + x[order[m]]) (float, in order) / float(m + 1),
+// v = x[f] - avg, bin = clamp(round(v), 0, 255) (std::round: halves away from zero).
+// hist: [n][mmax][256] counts (cleared by the caller).  One thread per pixel; LDS counts for
+// 32 values of m at a time.
+constexpr int kCcChunk = 32;
+__global__ void __launch_bounds__(kThreads) k_cc_hist(const float* __restrict__ x, long P, const int* __restrict__ order,
+                                                      int mmax, unsigned* __restrict__ hist) {
+    __shared__ unsigned h[kCcChunk * 256];
+    const int f = blockIdx.y;
+    const long p = (long)blockIdx.x * kThreads + threadIdx.x;
+    const bool in = p < P;
+    const float xf = in ? x[(size_t)f * P + p] : 0.0f;
+    const int* ord = order + (size_t)f * mmax;
+    float bg = 0.0f;
+    for (int m0 = 0; m0 < mmax; m0 += kCcChunk) {
+        const int m1 = min(mmax, m0 + kCcChunk);
+        for (int e = threadIdx.x; e < kCcChunk * 256; e += kThreads) h[e] = 0u;
+        __syncthreads();
+        for (int m = m0; m < m1; ++m) {
+            if (in) {
+                bg += x[(size_t)ord[m] * P + p];
+                const float avg = bg / (float)(m + 1);
+                const float v = xf - avg;
+                const int b = min(255, max(0, (int)roundf(v)));
+                atomicAdd(&h[(m - m0) * 256 + b], 1u);
+            }
+        }
+        __syncthreads();
+        unsigned* g = hist + ((size_t)f * mmax + m0) * 256;
+        for (int e = threadIdx.x; e < (m1 - m0) * 256; e += kThreads)
+            if (h[e]) atomicAdd(&g[e], h[e]);
+        __syncthreads();
+    }
+}
+
+// Backgrounds: bgF = (sum of the first used[f] ranked full frames, in order, in float) / float(used[f]);
+// out8 = saturate(round half to even(bgF)) (Mat::convertTo to 8U); bgF kept for the enhanced image.
+__global__ void k_cc_final(const float* __restrict__ full, long N, const int* __restrict__ order, int mmax,
+                           const int* __restrict__ used, float* __restrict__ bgf, unsigned char* __restrict__ out8) {
+    const int f = blockIdx.y;
+    const long p = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= N) return;
+    const int u = used[f];
+    const int* ord = order + (size_t)f * mmax;
+    float s = 0.0f;
+    for (int k = 0; k < u; ++k) s += full[(size_t)ord[k] * N + p];
+    const float b = s / (float)u;
+    if (bgf) bgf[(size_t)f * N + p] = b;
+    const float r = rintf(b);
+    out8[(size_t)f * N + p] = (unsigned char)fminf(fmaxf(r, 0.0f), 255.0f);
+}
+
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() {
@@ -210,6 +306,128 @@ rihvr_status rihvr_background_mean(const float* frames, int32_t n_frames, int32_
         stats->beta = 255 / (2 * mean_std * scale1);
         stats->minimum = 0.0;  // bg_min_grey (holo_sequence.cu:23)
         stats->average = sb / (double)N;
+    }
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_background_correlation(const float* frames, int32_t n_frames, int32_t width, int32_t height,
+                                          int frames_loc, int32_t min_frames, int32_t max_frames,
+                                          int32_t resize_factor, uint8_t* backgrounds, uint8_t* enhanced,
+                                          int32_t* frames_used) {
+    if (!frames || !backgrounds || width <= 0 || height <= 0 || resize_factor < 1 || max_frames < 1 ||
+        min_frames < 1)
+        return RIHVR_E_INVALID_ARGUMENT;
+    if (n_frames < 2) return RIHVR_E_MISSING_DATA;
+    if (width % resize_factor || height % resize_factor) return RIHVR_E_INVALID_ARGUMENT;
+    const int n = n_frames;
+    const long N = (long)width * height;
+    const int k = resize_factor;
+    const long P = (long)(width / k) * (height / k);
+    DevBuf hf, hx, hm, hg, ho, hh, hu, hb, h8;
+    const float* df = nullptr;
+    rihvr_status st = device_frames(frames, (size_t)n * N, frames_loc, hf, &df);
+    if (st) return st;
+    // correlation images and the centred Gram matrix (computeCorrelation on the resized frames)
+    BG_HIP(hipMalloc(&hx.p, (size_t)n * P * sizeof(float)));
+    const long nres = (long)n * P;
+    hipLaunchKernelGGL(k_cc_resize, dim3((unsigned)((nres + kThreads - 1) / kThreads)), dim3(kThreads), 0, 0, df,
+                       width, height, k, nres, (float*)hx.p);
+    BG_HIP(hipMalloc(&hm.p, n * sizeof(double)));
+    BG_HIP(hipMalloc(&hg.p, (size_t)n * n * sizeof(double)));
+    hipLaunchKernelGGL(k_cc_mean, dim3(n), dim3(kThreads), 0, 0, (const float*)hx.p, P, (double*)hm.p);
+    if (n > 65535) return RIHVR_E_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(k_cc_gram, dim3(n, n), dim3(kThreads), 0, 0, (const float*)hx.p, P, n, (const double*)hm.p,
+                       (double*)hg.p);
+    BG_HIP(hipGetLastError());
+    std::vector<double> gram((size_t)n * n);
+    BG_HIP(hipMemcpy(gram.data(), hg.p, gram.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // ranking: the other frames by correlation, highest first (ties: lower frame number first)
+    const int mmax = std::min(max_frames, n - 1);
+    std::vector<int> order((size_t)n * mmax);
+    for (int f = 0; f < n; ++f) {
+        std::vector<std::pair<double, int>> sc;
+        for (int j = 0; j < n; ++j) {
+            if (j == f) continue;
+            const double c = gram[(size_t)f * n + j] / (std::sqrt(gram[(size_t)f * n + f] * gram[(size_t)j * n + j]) + 1e-8);
+            sc.emplace_back(c, j);
+        }
+        std::stable_sort(sc.begin(), sc.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        for (int m = 0; m < mmax; ++m) order[(size_t)f * mmax + m] = sc[m].second;
+    }
+    BG_HIP(hipMalloc(&ho.p, order.size() * sizeof(int)));
+    BG_HIP(hipMemcpy(ho.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    // entropy of the frame minus each running mean (computeEntropy on the resized images)
+    const size_t nh = (size_t)n * mmax * 256;
+    BG_HIP(hipMalloc(&hh.p, nh * sizeof(unsigned)));
+    BG_HIP(hipMemset(hh.p, 0, nh * sizeof(unsigned)));
+    hipLaunchKernelGGL(k_cc_hist, dim3((unsigned)((P + kThreads - 1) / kThreads), n), dim3(kThreads), 0, 0,
+                       (const float*)hx.p, P, (const int*)ho.p, mmax, (unsigned*)hh.p);
+    BG_HIP(hipGetLastError());
+    std::vector<unsigned> hist(nh);
+    BG_HIP(hipMemcpy(hist.data(), hh.p, nh * sizeof(unsigned), hipMemcpyDeviceToHost));
+    std::vector<int> used(n);
+    for (int f = 0; f < n; ++f) {
+        // entropy in float as computeEntropy: p = float(h) / N; e -= p * log(p)
+        std::vector<float> ent(mmax);
+        for (int m = 0; m < mmax; ++m) {
+            float e = 0.0f;
+            const unsigned* hm_ = hist.data() + ((size_t)f * mmax + m) * 256;
+            for (int b = 0; b < 256; ++b)
+                if (hm_[b] > 0) {
+                    const float pr = float(hm_[b]) / (float)P;
+                    e -= pr * std::log(pr);
+                }
+            ent[m] = e;
+        }
+        // 3-point moving average, first negative gradient, clamp to [min - 1, mmax - 1]
+        std::vector<float> sm(mmax);
+        for (int i = 0; i < mmax; ++i) {
+            float sum = ent[i];
+            int cnt = 1;
+            if (i > 0) { sum += ent[i - 1]; ++cnt; }
+            if (i + 1 < mmax) { sum += ent[i + 1]; ++cnt; }
+            sm[i] = sum / cnt;
+        }
+        int opt = 1;
+        for (int i = 1; i < mmax; ++i)
+            if (sm[i] - sm[i - 1] < 0) {
+                opt = i;
+                break;
+            }
+        opt = std::max(min_frames - 1, std::min(opt, mmax - 1));
+        opt = std::min(opt, mmax - 1);  // the reference reads past its ranking when min_frames >= n
+        used[f] = opt + 1;
+        if (frames_used) frames_used[f] = opt + 1;
+    }
+    BG_HIP(hipMalloc(&hu.p, n * sizeof(int)));
+    BG_HIP(hipMemcpy(hu.p, used.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    BG_HIP(hipMalloc(&h8.p, (size_t)n * N));
+    if (enhanced) BG_HIP(hipMalloc(&hb.p, (size_t)n * N * sizeof(float)));
+    hipLaunchKernelGGL(k_cc_final, dim3((unsigned)((N + kThreads - 1) / kThreads), n), dim3(kThreads), 0, 0, df, N,
+                       (const int*)ho.p, mmax, (const int*)hu.p, (float*)hb.p, (unsigned char*)h8.p);
+    BG_HIP(hipGetLastError());
+    BG_HIP(hipMemcpy(backgrounds, h8.p, (size_t)n * N, hipMemcpyDeviceToHost));
+    if (enhanced) {
+        // enhanced = normalize(I - bgF, 0, 255, NORM_MINMAX) as 8 bits, on the host
+        std::vector<float> bgf((size_t)N), img((size_t)N);
+        for (int f = 0; f < n; ++f) {
+            BG_HIP(hipMemcpy(bgf.data(), (const float*)hb.p + (size_t)f * N, N * sizeof(float), hipMemcpyDeviceToHost));
+            BG_HIP(hipMemcpy(img.data(), df + (size_t)f * N, N * sizeof(float), hipMemcpyDeviceToHost));
+            float lo = INFINITY, hi = -INFINITY;
+            for (long i = 0; i < N; ++i) {
+                const float d = img[i] - bgf[i];
+                img[i] = d;
+                lo = std::min(lo, d);
+                hi = std::max(hi, d);
+            }
+            const double sc = hi > lo ? 255.0 / ((double)hi - (double)lo) : 0.0;
+            const double sh = -(double)lo * sc;
+            uint8_t* o = enhanced + (size_t)f * N;
+            for (long i = 0; i < N; ++i) {
+                const float v = std::nearbyint((float)((double)img[i] * sc + sh));
+                o[i] = (uint8_t)std::min(255.0f, std::max(0.0f, v));
+            }
+        }
     }
     return RIHVR_OK;
 }

@@ -115,3 +115,63 @@ def test_make_background_error_paths(pkg, tmp_path):
     assert r.returncode == 255
     assert "Median background calculation requires at least 5 frames" in r.stdout
     assert "Detected HoloError: 11" in r.stdout
+
+
+# ---- correlation background (ccbg.cpp) ------------------------------------------------
+def test_box_resize_is_the_block_mean():
+    from oracle import background as B
+    rng = np.random.default_rng(3)
+    f = rng.integers(0, 256, size=(2, 8, 12)).astype(np.float32)
+    got = B.box_resize(f, 4)
+    exp = f.reshape(2, 2, 4, 3, 4).mean(axis=(2, 4))
+    np.testing.assert_allclose(got, exp, rtol=0, atol=1e-5)
+    assert np.array_equal(B.box_resize(f, 1), f)
+
+
+def test_entropy_known_answers():
+    from oracle import background as B
+    assert B.entropy_f32(np.bincount([7] * 100, minlength=256), 100) == 0.0
+    two = np.bincount([0] * 50 + [9] * 50, minlength=256)
+    assert abs(float(B.entropy_f32(two, 100)) - np.log(2.0)) < 1e-6
+
+
+def test_frame_count_rule_by_hand():
+    from oracle import background as B
+    # entropies 1, 2, 3, 2, 1: smoothed 1.5, 2, 2.333, 2, 1.5 -> first fall at index 3
+    assert B.pick_frame_count([1, 2, 3, 2, 1], 1, 5) == 3
+    # rising throughout: no fall -> 1, then the clamps
+    assert B.pick_frame_count([1, 2, 3, 4, 5], 1, 5) == 1
+    assert B.pick_frame_count([1, 2, 3, 4, 5], 4, 5) == 3
+    assert B.pick_frame_count([1, 2, 3, 2, 1], 9, 5) == 4  # min_frames beyond the ranking
+
+
+def test_ranking_groups_similar_frames():
+    """Frames 0-2 are pattern A plus noise, 3-5 pattern B plus noise: each frame's two best
+    matches are the rest of its group; a frame and an affine copy of it correlate to 1."""
+    from oracle import background as B
+    rng = np.random.default_rng(4)
+    a, b = rng.random((16, 16)) * 200, rng.random((16, 16)) * 200
+    fr = np.stack([a + rng.normal(0, 3, a.shape) for _ in range(3)] +
+                  [b + rng.normal(0, 3, b.shape) for _ in range(3)]).astype(np.float32)
+    order = B.ranking(fr)
+    for i in range(6):
+        assert set(order[i][:2]) == ({0, 1, 2} - {i} if i < 3 else {3, 4, 5} - {i})
+    x = np.stack([a, 2 * a + 7]).astype(np.float32)
+    xm = x.reshape(2, -1).astype(np.float64)
+    xm -= xm.mean(axis=1, keepdims=True)
+    c = (xm[0] @ xm[1]) / (np.sqrt((xm[0] @ xm[0]) * (xm[1] @ xm[1])) + 1e-8)
+    assert abs(c - 1.0) < 1e-9
+
+
+def test_correlation_background_of_a_static_scene():
+    """A static background with a spot that moves every frame: with enough frames averaged
+    the spot drops out, and the frames used respect the clamps."""
+    from oracle import background as B
+    rng = np.random.default_rng(5)
+    base = rng.integers(60, 180, size=(32, 32)).astype(np.float32)
+    fr = np.repeat(base[None], 10, axis=0).copy()
+    for i in range(10):
+        fr[i, 3 * i % 30: 3 * i % 30 + 2, 5:7] = 255
+    bg, used, _ = B.correlation_backgrounds(fr, 3, 8, 4)
+    assert np.all((used >= 3) & (used <= 8))
+    assert np.abs(bg.astype(int) - base.astype(int)).max() <= 255 // 3 + 1

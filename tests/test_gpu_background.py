@@ -104,3 +104,50 @@ def test_make_background_median_end_to_end(pkg, bgo, tmp_path, gpu_available):
         exp = bgo.to_u8(bgo.median5(frames[stack].astype(np.float32)))
         np.testing.assert_array_equal(read_png(out / f"background_{oid:04d}.png"), exp)
     assert sorted(q.name for q in out.glob("background_*.png")) == [f"background_{o:04d}.png" for o, _ in sched]
+
+
+def _moving_spot_frames(n, h, w, seed):
+    """A static scene with a spot that moves each frame and camera noise (8-bit values)."""
+    rng = np.random.default_rng(seed)
+    base = rng.integers(50, 190, size=(h, w)).astype(np.float64)
+    fr = np.clip(base[None] + rng.normal(0, 6, size=(n, h, w)), 0, 255)
+    for i in range(n):
+        y, x = (7 * i) % (h - 4), (11 * i) % (w - 4)
+        fr[i, y:y + 3, x:x + 3] = 250
+    return fr.round().astype(np.float32)
+
+
+@pytest.mark.parametrize("n, h, w, k, lo, hi", [(12, 64, 64, 4, 2, 8), (7, 48, 32, 1, 1, 100), (16, 96, 80, 2, 5, 12)])
+@pytest.mark.parametrize("device", [False, True])
+def test_background_correlation_matches_oracle(pkg, bgo, gpu_available, n, h, w, k, lo, hi, device):
+    print(f"ccbg case {n}x{h}x{w} k={k} min={lo} max={hi} device={device}", flush=True)
+    fr = _moving_spot_frames(n, h, w, seed=n + k)
+    arg = fr
+    if device:
+        import torch
+        arg = torch.from_numpy(fr).cuda()
+    bg, used, enh = pkg.background_correlation(arg, lo, hi, k, enhanced=True)
+    rbg, rused, renh = bgo.correlation_backgrounds(fr, lo, hi, k, enhanced=True)
+    assert list(used) == list(rused)
+    np.testing.assert_array_equal(bg, rbg)
+    np.testing.assert_array_equal(enh, renh)
+
+
+def test_ccbg_end_to_end(pkg, bgo, tmp_path, gpu_available):
+    fr = _moving_spot_frames(9, 64, 64, seed=8).astype(np.uint8)
+    inp = tmp_path / "in"
+    inp.mkdir()
+    for i in range(9):
+        (inp / f"im_{i + 1:04d}.tif").write_bytes(tiff_bytes(fr[i]))
+    out = tmp_path / "bgs"
+    p = tmp_path / "params.txt"
+    p.write_text(f"# ccbg parameters\ninputFolder = {inp}\noutputFolder = {out}\nfilePattern = im_%04d.tif\n"
+                 f"startIdx = 1\nendIdx = 9\nminFrames = 3\nmaxFrames = 6\nresizePct = 0.25\nsaveEnhanced = true\n")
+    exe = _exe(pkg).parent / "ccbg"
+    r = subprocess.run([str(exe), str(p)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rbg, rused, renh = bgo.correlation_backgrounds(fr.astype(np.float32), 3, 6, 4, enhanced=True)
+    for i in range(9):
+        np.testing.assert_array_equal(read_png(out / f"background_{i + 1:04d}.png"), rbg[i])
+        np.testing.assert_array_equal(read_png(out / f"enhanced_{i + 1:04d}.png"), renh[i])
+        assert f"Saved background for image {i + 1} using {rused[i]} frames + enhanced image" in r.stdout
