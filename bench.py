@@ -101,17 +101,32 @@ def cpu_baseline(cfg: dict, holo, iters_hi=2, iters_lo=1):
                         f"{iters_lo}-iteration solve, {times[iters_hi] + times[iters_lo]:.1f} s total)"))
 
 
+WEIGHTS = os.path.join(ROOT, "rihvr---regularized-inverse-holographic-volume-reconstruction_amd", "weights",
+                       "warmstart_c2.npz")
+
+
+def load_net(pkg, shape):
+    """The committed warm-start network (trained at the 1024^2 x 128 workload's optics)."""
+    with np.load(WEIGHTS, allow_pickle=False) as z:
+        params, i_s, o_s = z["params"], float(z["in_scale"]), float(z["out_scale"])
+    net = pkg.WarmStartNet(*shape)
+    net.set_params(params, i_s, o_s)
+    return net
+
+
 def frames_leg(pkg, scenes, cfg, n_frames: int, tol: float = 1e-3, seed: int = 20261203):
     """s/frame to an objective tolerance on a time sequence at the workload's optics:
-    plain cold-start FISTA for num_inverse_iterations gives f_plain per frame; then
-    cold start and warm start (the previous frame's volume, kept on the device) both
-    run until f <= (1 + tol) f_plain or the same cap.  Frames are device-resident."""
+    plain cold-start FISTA for num_inverse_iterations gives f_plain per frame; then the cold
+    start, the learned warm start (rihvr_reconstruct_learned with the committed network: A^T b,
+    network, the start test, FISTA) and the previous frame's volume as start all run until
+    f <= (1 + tol) f_plain or the same cap.  Frames are device-resident."""
     import torch
     optics = scenes.optics_of(cfg)
     holos = scenes.sequence(optics, n_frames, n_tracers=500, n_rods=20, seed=seed, sigma=0.02)
     hd = torch.from_numpy(holos).cuda()
     cap = cfg["num_inverse_iterations"]
     rec = pkg.SparseCompressiveHolo(pkg.make_params(**cfg))
+    net = load_net(pkg, (cfg["num_planes"], cfg["height"], cfg["width"]))
     try:
         def run(**kw):
             torch.cuda.synchronize()
@@ -123,25 +138,117 @@ def frames_leg(pkg, scenes, cfg, n_frames: int, tol: float = 1e-3, seed: int = 2
         target = [(1.0 + tol) * f["f_final"] for f in plain]
         cold, t_cold = run(f_target=target, max_iterations=cap)
         warm, t_warm = run(f_target=target, max_iterations=cap, warm_from_previous=True)
+        learned = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(n_frames):
+            learned.append(rec.reconstructLearned(net, hd[k], f_target=target[k], max_iterations=cap))
+        torch.cuda.synchronize()
+        t_learned = time.perf_counter() - t0
     finally:
+        net.destroy()
         rec.destroy()
 
     def leg(fs, t):
         return {"s_per_frame": t / n_frames, "iterations": [f["iterations"] for f in fs],
                 "reached": [f["reached"] for f in fs], "f_final": [f["f_final"] for f in fs]}
-    # frame 0 has no predecessor: the warm-start comparison is over frames 1..n-1
+    # frame 0 has no predecessor: the temporal comparison is over frames 1..n-1
     it_cold = sum(f["iterations"] for f in cold[1:])
     it_warm = sum(f["iterations"] for f in warm[1:])
+    lrn = leg(learned, t_learned)
+    lrn.update(fallback=[f["fallback"] for f in learned],
+               ms_network=[f["ms_network"] for f in learned], ms_adjoint=[f["ms_adjoint"] for f in learned],
+               ms_solve=[f["ms_solve"] for f in learned])
     return {
         "workload": (f"{n_frames} frames, {cfg['width']}x{cfg['height']}x{cfg['num_planes']} at the same optics, "
                      "500 tracers + 20 rods advected by a seeded divergence-free 16-mode flow (rms 3 px/frame)"),
         "tolerance": f"f <= (1 + {tol}) * f_plain({cap})",
-        "warm_start": "previous frame's returned volume (temporal); not a learned predictor",
         "plain_fixed": dict(leg(plain, t_plain), iterations_each=cap),
         "plain_to_tol": leg(cold, t_cold),
+        "learned_to_tol": lrn,
         "warm_previous_to_tol": leg(warm, t_warm),
+        "learned_speedup_vs_plain_to_tol": (t_cold / t_learned) if t_learned else None,
+        "learned_speedup_vs_plain_fixed": (t_plain / t_learned) if t_learned else None,
         "iterations_ratio_cold_over_warm_frames_1_on": (it_cold / it_warm) if it_warm else None,
+        "network": "weights/warmstart_c2.npz (BF16 MFMA encoder-decoder, 16-128 channels, 4 levels)",
     }
+
+
+def c5_bench(pkg, scenes, n_frames: int, tol: float = 1e-3, seed: int = 910001):
+    """A stream of holograms at the 1024^2 x 128 workload's optics with 300-800 tracers and
+    rods each (independent scenes, not a sequence): plain cold start against the learned
+    start, both to (1 + tol) f_plain(100)."""
+    import torch
+    ws = entry._load_submodule("warmstart")
+    with np.load(WEIGHTS, allow_pickle=False) as z:
+        params, i_s, o_s = z["params"], float(z["in_scale"]), float(z["out_scale"])
+    with pkg.WarmStartNet(128, 1024, 1024) as net:
+        net.set_params(params, i_s, o_s)
+        s = ws.evaluate(net, "c2", 1024, n_frames, seed, log=None)
+    rows = s.pop("rows")
+    s["iterations_cold_mean"] = float(np.mean(s["cold_iterations"]))
+    s["iterations_learned_mean"] = float(np.mean(s["learned_iterations"]))
+    s["ms_network_mean"] = float(np.mean([r["ms_network"] for r in rows]))
+    return {"metric": "s/frame to (1 + 1e-3) f_plain(100), learned warm start vs plain, 1024^2x128 FP32",
+            "value": s["learned_s_per_frame"], "unit": "s/frame", "n_gpus": 1, "steps": n_frames, "warmup": 0,
+            "ms_per_step": 1e3 * s["learned_s_per_frame"], "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16 network / f32 FISTA",
+            "data": "synthetic (seeded 300-800 tracers and rods per hologram)",
+            "config": {"workload": f"stream of {n_frames} holograms 1024x1024x128 (C2 optics), learned start vs plain"},
+            "summary": s}
+
+
+def c4_bench(pkg, steps: int, warmup: int, crop=(64, 128, 128), n_pairs: int = 16):
+    """Training throughput of the warm-start network at the 256^2 x 64 training workload's
+    optics: pairs from the plain solver (200 iterations), Adam steps on crops; the dominant
+    cost is the BF16 MFMA convolutions (forward, input and weight gradients), whose FLOPs per
+    step are counted from the architecture."""
+    import torch
+    ws = entry._load_submodule("warmstart")
+    xs, ys = ws.make_pairs("c4", 256, n_pairs, 4000001, 200, log=None)
+    i_s, o_s, frac = ws.scales(xs, ys)
+    D, H, W = crop
+    V = D * H * W
+    conv_fwd = 0.0
+    for lv, c0, c1, co in ((1, 16, 0, 32), (2, 32, 0, 64), (3, 64, 0, 128), (2, 64, 128, 64), (1, 32, 64, 32),
+                           (0, 16, 32, 16)):
+        conv_fwd += 2.0 * 27 * (c0 + c1) * co * V / 8 ** lv
+    first = 2.0 * 27 * 2 * 16 * V  # the input layer's two real channels
+    flops_step = 3 * conv_fwd + 2 * first  # forward + input gradients + weight gradients
+    with pkg.WarmStartNet(D, H, W, seed=1) as net:
+        p, _, _ = net.get_params()
+        net.set_params(p, i_s, o_s)
+        net.set_loss_weight((1 - frac) / (2 * max(frac, 1e-9)))
+        rng = np.random.default_rng(0)
+        bx = torch.empty((D, H, W), dtype=torch.complex64, device="cuda")
+        by = torch.empty_like(bx)
+
+        def step():
+            i = int(rng.integers(0, n_pairs))
+            z0 = int(rng.integers(0, xs.shape[1] - D + 1))
+            y0 = int(rng.integers(0, 256 - H + 1))
+            x0 = int(rng.integers(0, 256 - W + 1))
+            bx.copy_(xs[i, z0:z0 + D, y0:y0 + H, x0:x0 + W])
+            by.copy_(ys[i, z0:z0 + D, y0:y0 + H, x0:x0 + W])
+            return net.train_step(bx, by, 1e-3)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            loss = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+    achieved = flops_step / dt / 1e12
+    return {"metric": "warm-start training steps/s (Adam, BF16 MFMA convolutions), 256^2x64 optics",
+            "value": 1.0 / dt, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16 (fp32 accumulation and master weights)",
+            "data": f"synthetic ({n_pairs} pairs from the plain solver, 200 iterations, C4 optics)",
+            "config": {"workload": f"one Adam step on a {D}x{H}x{W} crop", "last_loss": loss},
+            "roofline": {"bound": "mfma", "kernel": "train step (convolutions)", "achieved": achieved,
+                         "peak": 2500.0, "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
+                         "algorithmic_work_per_launch": flops_step}}
 
 
 def background_bench(pkg, steps: int, warmup: int, n_frames: int = 200, size: int = 1024):
@@ -298,7 +405,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10,
                     help="timed steps; a step of the c2/c1 workloads is one whole solve")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c1", "background", "segmentation", "preprocess"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c1", "c4", "c5", "background", "segmentation", "preprocess"],
+                    default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--frames", type=int, default=3,
                     help="frames of the s/frame-to-tolerance leg (0 skips it)")
@@ -321,6 +429,12 @@ def main():
         print(json.dumps(preprocess_bench(pkg, max(1, min(args.steps, 20)), min(args.warmup, 3))), flush=True)
         return
     scenes = entry._load_submodule("scenes")
+    if args.workload == "c5":
+        print(json.dumps(c5_bench(pkg, scenes, max(1, args.steps))), flush=True)
+        return
+    if args.workload == "c4":
+        print(json.dumps(c4_bench(pkg, max(1, args.steps * 10), max(1, args.warmup))), flush=True)
+        return
     if args.workload == "segmentation":
         print(json.dumps(segmentation_bench(pkg, scenes, max(1, min(args.steps, 20)), min(args.warmup, 3),
                                             cpu=not args.no_cpu)), flush=True)

@@ -47,11 +47,12 @@ def config(optics: str, size: int, iterations: int = 100) -> dict:
 
 
 def scene(optics: str, cfg: dict, seed: int):
-    """A seeded hologram of tracers and rods (3:1) at the workload's object density."""
+    """A seeded hologram of tracers and rods at the workload's object density, the rods a
+    fraction drawn uniformly from 2 % to 30 % of the objects."""
     rng = np.random.default_rng(seed)
     lo, hi, area = DENSITY[optics]
     n = int(round(rng.uniform(lo, hi) * cfg["width"] * cfg["height"] / area))
-    n_rods = max(1, n // 4)
+    n_rods = max(1, int(round(n * rng.uniform(0.02, 0.30))))
     o = scenes.optics_of(cfg)
     if optics == "c4":
         holo, _ = scenes.tracers_rods_scene(o, n - n_rods, n_rods, seed, 0.02, tracer_diam=(4.0, 12.0))
