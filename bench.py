@@ -156,7 +156,7 @@ def frames_leg(pkg, scenes, cfg, n_frames: int, tol: float = 1e-3, seed: int = 2
     it_cold = sum(f["iterations"] for f in cold[1:])
     it_warm = sum(f["iterations"] for f in warm[1:])
     lrn = leg(learned, t_learned)
-    lrn.update(fallback=[f["fallback"] for f in learned],
+    lrn.update(fallback=[f["fallback"] for f in learned], alpha=[round(f["alpha"], 4) for f in learned],
                ms_network=[f["ms_network"] for f in learned], ms_adjoint=[f["ms_adjoint"] for f in learned],
                ms_solve=[f["ms_solve"] for f in learned])
     return {
@@ -196,6 +196,105 @@ def c5_bench(pkg, scenes, n_frames: int, tol: float = 1e-3, seed: int = 910001):
             "data": "synthetic (seeded 300-800 tracers and rods per hologram)",
             "config": {"workload": f"stream of {n_frames} holograms 1024x1024x128 (C2 optics), learned start vs plain"},
             "summary": s}
+
+
+C3_SEQ = dict(width=512, height=512, num_planes=100, start_plane=1000.0, plane_stepsize=25.0, wavelength=0.633,
+              resolution=2.0, num_inverse_iterations=50, regularization_param=0.01, regularization_TV_param=0.01,
+              num_tv_iterations=5, mode=7)
+
+
+def c3_mixed_specs(n: int, seed: int = 930001):
+    """The mixed batch: n holograms, each of side 384, 512 or 768 padded by 32 or 64 zeros on each
+    side, Nz 40, 64 or 100, L1 weight log-uniform in [0.001, 0.01], TV 0.01, at the sequence's
+    optics (dz 25 um, dx 2 um, lambda 0.633 um), with 1000 tracers per 512^2 of field and 10 rods."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        side = int(rng.choice([384, 512, 768]))
+        pad = int(rng.choice([32, 64]))
+        nz = int(rng.choice([40, 64, 100]))
+        reg = float(np.exp(rng.uniform(np.log(0.001), np.log(0.01))))
+        c = dict(C3_SEQ, width=side + 2 * pad, height=side + 2 * pad, num_planes=nz, regularization_param=reg,
+                 zero_padding=pad)
+        out.append((c, side, pad, int(round(1000 * side * side / 512 ** 2)), 10, seed + 1 + k))
+    return out
+
+
+def c3_bench(pkg, scenes, n_seq: int = 64, n_mixed: int = 48):
+    """The batched workload: (1) a 64-frame 512^2 x 100 sequence (1000 tracers advected by the
+    seeded 16-mode divergence-free flow, 3 px/frame rms) solved frame after frame on one context
+    (rihvr_reconstruct_batch, 50 FISTA x 5 TV each, cold starts); (2) a mixed batch of 48 holograms
+    of uneven padded sizes, Nz and weights in one rihvr_reconstruct_multi call.  Each part runs with
+    the default schedule and with the schedule rihvr_autotune picks per geometry (tuned once on the
+    geometry's first hologram, tuning time reported apart); the queued iterations are
+    graph-captured in both.  Holograms are generated on the GPU (torch, complex128) and are
+    device-resident before the timed calls."""
+    import torch
+    prop = scenes.propagate_sum_torch
+    out = {}
+    # (1) the sequence
+    seq = scenes.sequence(scenes.optics_of(C3_SEQ), n_seq, n_tracers=1000, n_rods=0, seed=920001, sigma=0.02,
+                          propagate=prop)
+    hs = torch.from_numpy(seq).cuda()
+    p = pkg.make_params(**{k: v for k, v in C3_SEQ.items()})
+    t0 = time.perf_counter()
+    best, tried = pkg.autotune(p, hs[0].contiguous(), iterations=10)
+    tune_s = time.perf_counter() - t0
+    legs = {}
+    for name, sch in (("default", None), ("autotuned", best)):
+        with pkg.SparseCompressiveHolo(p, sch) as rec:
+            rec.reconstructBatch(hs[:1])  # warm-up: first launches, graph captures
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fs = rec.reconstructBatch(hs)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            legs[name] = {"s_total": dt, "frames_per_s": n_seq / dt,
+                          "iter_per_s": sum(f["iterations"] for f in fs) / dt, "schedule": rec.schedule()}
+    out["sequence"] = {"workload": f"{n_seq} frames 512x512x100, 1000 tracers, 50 FISTA x 5 TV each",
+                       "tune_s": tune_s, "candidates": len(tried), **legs,
+                       "autotuned_speedup": legs["default"]["s_total"] / legs["autotuned"]["s_total"]}
+    del hs
+    # (2) the mixed batch
+    specs = c3_mixed_specs(n_mixed)
+    holos = []
+    for c, side, pad, n_tr, n_rods, seed in specs:
+        inner = dict(c, width=side, height=side)
+        h, _ = scenes.tracers_rods_scene(scenes.optics_of(inner), n_tr, n_rods, seed, 0.02, propagate=prop)
+        holos.append(torch.from_numpy(np.pad(h, pad)).cuda())
+    tuned = {}
+    t0 = time.perf_counter()
+    for (c, *_), h in zip(specs, holos):
+        key = (c["width"], c["num_planes"])
+        if key not in tuned:
+            tuned[key] = pkg.autotune(pkg.make_params(**{k: v for k, v in c.items()}), h, iterations=10)[0]
+    tune_s = time.perf_counter() - t0
+    total_it = sum(c["num_inverse_iterations"] for c, *_ in specs)
+    voxels = sum(c["width"] * c["height"] * c["num_planes"] * c["num_inverse_iterations"] for c, *_ in specs)
+    legs = {}
+    for name, mode in (("default", 0), ("autotuned", 2)):
+        items = [dict(params=pkg.make_params(**{k: v for k, v in c.items()}), holo=h,
+                      schedule=tuned[(c["width"], c["num_planes"])]) for (c, *_), h in zip(specs, holos)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = pkg.reconstruct_multi(items, autotune=mode)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        legs[name] = {"s_total": dt, "holograms_per_s": n_mixed / dt, "iter_per_s": total_it / dt,
+                      "voxel_iterations_per_s": voxels / dt, "all_reached": all(r["reached"] for r in res)}
+    out["mixed"] = {"workload": (f"{n_mixed} holograms: sides 384/512/768 + 2 x (32 or 64) zero padding, Nz 40/64/100, "
+                                 "L1 0.001-0.01, TV 0.01, 50 FISTA x 5 TV each"),
+                    "groups": len(tuned), "tune_s": tune_s, **legs,
+                    "schedules": {f"{w}x{w}x{nz}": s.as_dict() for (w, nz), s in sorted(tuned.items())},
+                    "autotuned_speedup": legs["default"]["s_total"] / legs["autotuned"]["s_total"]}
+    total = out["sequence"]["autotuned"]["s_total"] + out["mixed"]["autotuned"]["s_total"]
+    return {"metric": "holograms/s, batched sequence + mixed-size batch, autotuned schedules, FP32",
+            "value": (n_seq + n_mixed) / total, "unit": "holograms/s", "n_gpus": 1, "steps": n_seq + n_mixed,
+            "warmup": 1, "ms_per_step": 1e3 * total / (n_seq + n_mixed), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded tracers and rods, angular-spectrum forward model, sigma 0.02)",
+            "config": {"workload": f"{n_seq}-frame 512^2x100 sequence + {n_mixed} mixed-size holograms"},
+            "summary": out}
 
 
 def c4_bench(pkg, steps: int, warmup: int, crop=(64, 128, 128), n_pairs: int = 16):
@@ -405,7 +504,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10,
                     help="timed steps; a step of the c2/c1 workloads is one whole solve")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c1", "c4", "c5", "background", "segmentation", "preprocess"],
+    ap.add_argument("--workload", choices=["c2", "c1", "c3", "c4", "c5", "background", "segmentation", "preprocess"],
                     default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--frames", type=int, default=3,
@@ -429,6 +528,9 @@ def main():
         print(json.dumps(preprocess_bench(pkg, max(1, min(args.steps, 20)), min(args.warmup, 3))), flush=True)
         return
     scenes = entry._load_submodule("scenes")
+    if args.workload == "c3":
+        print(json.dumps(c3_bench(pkg, scenes)), flush=True)
+        return
     if args.workload == "c5":
         print(json.dumps(c5_bench(pkg, scenes, max(1, args.steps))), flush=True)
         return
@@ -496,10 +598,11 @@ def main():
     rec.getVolume(out=vol)
     f_line = float((vol != 0).reshape(Nz, H, W // 16, 16).any(-1).float().mean().item())
     del vol
+    schedule = rec.schedule()
     rec.destroy()
 
     ips = args.steps * n_it / elapsed
-    zchunk = min(Nz, int(os.environ.get("RIHVR_ZCHUNK", "32")))  # planes per partial spectrum (library default 32)
+    zchunk = schedule["zchunk_fwd"]  # planes per partial spectrum
     nparts = (Nz + zchunk - 1) // zchunk
     dom_launches, dom_ms = timing[dom_name]
     dom_avg_ms = dom_ms / max(1, dom_launches)
@@ -547,6 +650,7 @@ def main():
                         f"TV {cfg['regularization_TV_param']}, {cfg['num_tv_iterations']} TV iterations",
             "width": W, "height": H, "num_planes": Nz,
         },
+        "schedule": schedule,
         "roofline": {
             "bound": bound,
             "kernel": dom_name,

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define RIHVR_ABI_VERSION 3
+#define RIHVR_ABI_VERSION 4
 
 /* Status codes.  1..12 are the reference's umnholo::HoloError values
  * (src/core/umnholo.h:132-146), thrown by the reference where this ABI returns. */
@@ -164,6 +164,73 @@ rihvr_status rihvr_reconstruct_batch(rihvr_ctx* ctx, const float* holos, int hol
                                      const float* warm_start, int warm_loc, int32_t warm_from_previous,
                                      const double* f_target, int32_t max_iterations, float* volumes,
                                      int vol_loc, rihvr_frame_stats* frame_stats);
+
+/* ---- Schedules and the batched call (no counterpart in the reference, which runs one
+ * hologram per process, src/QTGUI/rihvr.cpp:2414-2906).  A schedule holds the launch choices
+ * that change speed but not the mathematics (the sums' order changes with them, so results
+ * agree to rounding across schedules and bit for bit within one). */
+typedef struct rihvr_schedule {
+    int32_t fft_path;    /* 0: the fixed-length FFT kernels where the size has them, 1: run-time plans */
+    int32_t max_radix;   /* largest radix of the run-time plans: 16, 8 or 4 */
+    int32_t zchunk_fwd;  /* planes per partial spectrum of A */
+    int32_t zchunk_adj;  /* planes per workgroup of A^T's first pass */
+    int32_t prox_rows;   /* output rows per wave of the fused mode-7 prox (0: the device rule) */
+    int32_t graphs;      /* 1: the queued iterations run as captured HIP graphs */
+    double  ms_per_iteration;  /* measured by rihvr_autotune (else 0) */
+} rihvr_schedule;
+
+/* rihvr_create with a schedule (NULL: the defaults; environment variables RIHVR_ZCHUNK,
+ * RIHVR_ZCHUNK_ADJ, RIHVR_PROX_ROWS, RIHVR_GRAPHS apply only without one).  Errors as rihvr_create,
+ * and RIHVR_E_INVALID_ARGUMENT for a schedule field out of range. */
+rihvr_status rihvr_create_scheduled(const rihvr_params* params, const rihvr_schedule* schedule, rihvr_ctx** out);
+/* The schedule a context runs with. */
+rihvr_status rihvr_get_schedule(const rihvr_ctx* ctx, rihvr_schedule* out);
+/* Measures schedules for params by coordinate search: the FFT path and the run-time plans'
+ * largest radix, then zchunk_fwd, zchunk_adj, prox_rows (mode 7 with n_tv <= 10) and graphs,
+ * each kept when it is faster.  Each candidate is a fresh context solving holo ([H][W] complex64
+ * at holo_loc; NULL: A x for a pseudo-random x with about 1 voxel in 1000 non-zero) from the zero
+ * start: 12 untimed iterations, then `iterations` timed ones.  The iterations' cost follows the
+ * iterates' sparsity (the occupancy maps skip empty lines), so a hologram of the workload times
+ * the workload best.  best receives the fastest; tried (capacity entries, may be NULL) the
+ * candidates in the order measured, each with its time; *n_tried how many were measured (those
+ * beyond capacity are not stored).  Runs on the current device, one context at a time. */
+rihvr_status rihvr_autotune(const rihvr_params* params, const float* holo, int holo_loc, int32_t iterations,
+                            rihvr_schedule* best, rihvr_schedule* tried, int32_t capacity, int32_t* n_tried);
+
+/* One hologram of a batch: its own parameter set (geometry, optics, weights, iterations and
+ * mode), its input and output, and its result. */
+typedef struct rihvr_multi_item {
+    rihvr_params params;
+    const float* holo;         /* [H][W] complex64 at the call's holo_loc */
+    float* volume;             /* [Nz][H][W] complex64 at the call's vol_loc, or NULL */
+    double f_target;           /* > 0: stop at f <= f_target (momentum point), else run
+                                  num_inverse_iterations */
+    rihvr_frame_stats stats;   /* out */
+    rihvr_schedule schedule;   /* in (autotune == 2, read from a group's first item): the group's
+                                  schedule; out: the schedule the item ran with */
+} rihvr_multi_item;
+
+/* Reconstructs a batch of holograms of any mix of sizes and parameter sets in one call: items
+ * sharing geometry and optics (W, H, Nz, z0, dz, lambda, dx, n_tv, mode) share one context, built
+ * once per group and destroyed after it.  The group's schedule: autotune 0, the defaults; 1, the
+ * one rihvr_autotune picks on the group's first hologram (10 timed iterations); 2, the schedule
+ * field of the group's first item (schedules tuned once and kept by the caller).  Each item is
+ * the reference's inverseReconstruct from the zero start with its own
+ * weights and iteration count.  Items keep their order within a group; groups run in order of
+ * their first item.  Each result equals the single-hologram solve with the same schedule bit
+ * for bit. */
+rihvr_status rihvr_reconstruct_multi(rihvr_multi_item* items, int32_t n_items, int holo_loc, int vol_loc,
+                                     int32_t autotune);
+
+/* SparseCompressiveHolo::estimateLipschitz (src/core/sparse_compressive_holo.cu:2205-2464): the
+ * Lipschitz constant of the data term's gradient, ||A^T A||.  The reference's secant estimate
+ * ||grad f(x2) - grad f(x1)|| / ||x2 - x1|| over two random volumes is ||A^T A d|| / ||d|| for
+ * d = x2 - x1; here d is a pseudo-random volume from seed and each of `iterations` (>= 1) steps
+ * applies A^T A to the normalised last result (power iteration), so iterations = 1 is the
+ * reference's one secant and more converge to ||A||^2 (= Nz for real holograms, A A^T = Nz I).
+ * At least 1e-6.  The reference discards its estimate (the step is 1/Nz, :247); so does the
+ * solver here: the call informs and changes no solve.  Invalidates the context's result. */
+rihvr_status rihvr_estimate_lipschitz(rihvr_ctx* ctx, int32_t seed, int32_t iterations, double* lipschitz);
 
 /* Stepwise form of rihvr_reconstruct, split where the reference's
  * inverseReconstruct splits (sparse_compressive_holo.cu:173-633):
@@ -579,7 +646,7 @@ rihvr_status rihvr_conv3d_bf16_wgrad(const uint16_t* in0, int32_t c0, const uint
 /* Result of rihvr_reconstruct_learned. */
 typedef struct rihvr_learned_stats {
     int32_t iterations;     /* FISTA iterations run from the chosen start */
-    int32_t fallback;       /* 1: the network's start was rejected and the zero start used */
+    int32_t fallback;       /* 0: the network's start x_net, 2: alpha x_net, 1: the zero start */
     int32_t reached;        /* 1: the objective target was reached */
     int32_t total_backtracks;
     double  resid_net;      /* 0.5 ||A x_net - b||^2 of the network's start */
@@ -593,12 +660,15 @@ typedef struct rihvr_learned_stats {
     double  ms_adjoint;     /* wall time of A^T b */
     double  ms_network;     /* wall time of the network's prediction and the residual test */
     double  ms_solve;       /* wall time of FISTA from the chosen start */
+    double  alpha;          /* the objective along the ray alpha x_net (alpha >= 0) is least here */
+    double  f_scaled;       /* objective at alpha x_net */
 } rihvr_learned_stats;
 
-/* Reconstructs one hologram with the network's start: A^T b -> network -> x_net; when the
- * objective at x_net exceeds the objective at the A^T b start (t A^T b, t = 1/Nz) or at the
- * zero start, the zero start of the reference is used instead (fallback); then FISTA (the
- * context's mode and parameters)
+/* Reconstructs one hologram with the network's start: A^T b -> network -> x_net.  x_net is the
+ * start when its objective is not above the objectives at the A^T b start (t A^T b, t = 1/Nz)
+ * and at the zero start; else alpha x_net, with alpha >= 0 the minimiser of the objective along
+ * the ray (in closed form), when alpha > 0 and its objective is not above both; else the zero
+ * start of the reference (fallback).  Then FISTA (the context's mode and parameters)
  * until the objective at the momentum point is <= f_target (f_target <= 0: no target) or
  * max_iterations (<= 0: num_inverse_iterations).  The network must have been created for the
  * context's Nz x H x W.  The result is read as after rihvr_reconstruct. */

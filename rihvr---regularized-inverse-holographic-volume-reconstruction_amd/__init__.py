@@ -51,7 +51,9 @@ EXPORTED = ("rihvr_abi_version", "rihvr_status_string", "rihvr_create", "rihvr_d
             "rihvr_net_init", "rihvr_net_set_params", "rihvr_net_get_params", "rihvr_net_predict",
             "rihvr_net_loss_grad", "rihvr_net_adam_step", "rihvr_net_train_step", "rihvr_net_sync",
             "rihvr_conv3d_bf16_forward", "rihvr_conv3d_bf16_dgrad", "rihvr_conv3d_bf16_wgrad",
-            "rihvr_reconstruct_learned", "rihvr_net_set_loss_weight", "rihvr_background_correlation")
+            "rihvr_reconstruct_learned", "rihvr_net_set_loss_weight", "rihvr_background_correlation",
+            "rihvr_create_scheduled", "rihvr_get_schedule", "rihvr_autotune", "rihvr_reconstruct_multi",
+            "rihvr_estimate_lipschitz")
 PATH_MAX = 4096
 
 
@@ -124,6 +126,27 @@ class FrameStats(ctypes.Structure):
         ("f_final", ctypes.c_double),
         ("nnz", ctypes.c_int64),
     ]
+
+
+class Schedule(ctypes.Structure):
+    """Mirror of ``rihvr_schedule`` (include/rihvr.h): launch choices of a context."""
+
+    _fields_ = [
+        ("fft_path", ctypes.c_int32),
+        ("max_radix", ctypes.c_int32),
+        ("zchunk_fwd", ctypes.c_int32),
+        ("zchunk_adj", ctypes.c_int32),
+        ("prox_rows", ctypes.c_int32),
+        ("graphs", ctypes.c_int32),
+        ("ms_per_iteration", ctypes.c_double),
+    ]
+
+    @classmethod
+    def make(cls, fft_path=0, max_radix=16, zchunk_fwd=0, zchunk_adj=0, prox_rows=0, graphs=1):
+        return cls(int(fft_path), int(max_radix), int(zchunk_fwd), int(zchunk_adj), int(prox_rows), int(graphs), 0.0)
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class BackgroundStats(ctypes.Structure):
@@ -203,10 +226,25 @@ class LearnedStats(ctypes.Structure):
         ("ms_adjoint", ctypes.c_double),
         ("ms_network", ctypes.c_double),
         ("ms_solve", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("f_scaled", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MultiItem(ctypes.Structure):
+    """Mirror of ``rihvr_multi_item`` (include/rihvr.h)."""
+
+    _fields_ = [
+        ("params", Params),
+        ("holo", ctypes.c_void_p),
+        ("volume", ctypes.c_void_p),
+        ("f_target", ctypes.c_double),
+        ("stats", FrameStats),
+        ("schedule", Schedule),
+    ]
 
 
 def make_params(width, height, num_planes, start_plane, plane_stepsize, wavelength, resolution,
@@ -317,6 +355,17 @@ def lib():
         L.rihvr_reconstruct_batch.argtypes = [vp, vp, ip, ctypes.c_int32, vp, ip, ctypes.c_int32,
                                               _P(ctypes.c_double), ctypes.c_int32, vp, ip, _P(FrameStats)]
         L.rihvr_reconstruct_batch.restype = st
+        L.rihvr_create_scheduled.argtypes = [_P(Params), _P(Schedule), _P(vp)]
+        L.rihvr_create_scheduled.restype = st
+        L.rihvr_get_schedule.argtypes = [vp, _P(Schedule)]
+        L.rihvr_get_schedule.restype = st
+        L.rihvr_autotune.argtypes = [_P(Params), vp, ip, ctypes.c_int32, _P(Schedule), _P(Schedule), ctypes.c_int32,
+                                     _P(ctypes.c_int32)]
+        L.rihvr_autotune.restype = st
+        L.rihvr_reconstruct_multi.argtypes = [_P(MultiItem), ctypes.c_int32, ip, ip, ctypes.c_int32]
+        L.rihvr_reconstruct_multi.restype = st
+        L.rihvr_estimate_lipschitz.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_double)]
+        L.rihvr_estimate_lipschitz.restype = st
         L.rihvr_write_png8.argtypes = [cp, vp, ctypes.c_int32, ctypes.c_int32]
         L.rihvr_write_png8.restype = st
         L.rihvr_background_mean.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ip, vp,
@@ -623,13 +672,30 @@ def _arg(a, shape, writable=False):
 class SparseCompressiveHolo:
     """Mirror of umnholo::SparseCompressiveHolo over the C ABI."""
 
-    def __init__(self, params: Params):
+    def __init__(self, params: Params, schedule: "Schedule" = None):
         self.params = params
         self.shape_holo = (params.height, params.width)
         self.shape_vol = (params.num_planes, params.height, params.width)
         self._ctx = ctypes.c_void_p()
-        _check(lib().rihvr_create(ctypes.byref(params), ctypes.byref(self._ctx)), "rihvr_create")
+        if schedule is None:
+            _check(lib().rihvr_create(ctypes.byref(params), ctypes.byref(self._ctx)), "rihvr_create")
+        else:
+            _check(lib().rihvr_create_scheduled(ctypes.byref(params), ctypes.byref(schedule), ctypes.byref(self._ctx)),
+                   "rihvr_create_scheduled")
         self.last_stats = None
+
+    def estimateLipschitz(self, seed=0, iterations=1) -> float:
+        """rihvr_estimate_lipschitz: ||A^T A|| by `iterations` power steps from a seeded volume."""
+        L = ctypes.c_double(0.0)
+        _check(lib().rihvr_estimate_lipschitz(self._ctx, int(seed), int(iterations), ctypes.byref(L)),
+               "rihvr_estimate_lipschitz")
+        return L.value
+
+    def schedule(self) -> dict:
+        """rihvr_get_schedule: the launch choices this context runs with."""
+        s = Schedule()
+        _check(lib().rihvr_get_schedule(self._ctx, ctypes.byref(s)), "rihvr_get_schedule")
+        return s.as_dict()
 
     # -- lifecycle --------------------------------------------------------
     def destroy(self):
@@ -800,6 +866,57 @@ class SparseCompressiveHolo:
 
 # -- segmentation and centroids (the segmentation executable) ---------------
 # -- learned warm start ------------------------------------------------------------
+def autotune(params: Params, holo=None, iterations=20, capacity=64):
+    """rihvr_autotune: (best Schedule, [every measured candidate as a dict])."""
+    hp, hl, _k = _arg(holo, (params.height, params.width))
+    best = Schedule()
+    tried = (Schedule * capacity)()
+    n = ctypes.c_int32(0)
+    _check(lib().rihvr_autotune(ctypes.byref(params), hp, hl, int(iterations), ctypes.byref(best), tried, capacity,
+                                ctypes.byref(n)), "rihvr_autotune")
+    return best, [t.as_dict() for t in tried[:min(n.value, capacity)]]
+
+
+def reconstruct_multi(items, autotune=0):
+    """rihvr_reconstruct_multi over items: dicts with "params" (Params), "holo" ([H][W] complex64),
+    optional "volume" (a writable [Nz][H][W] complex64 output), "f_target" and "schedule" (a
+    Schedule, read with autotune=2).  autotune: 0 (or False) the defaults, 1 (or True) tune each
+    group, 2 the groups' given schedules.  Inputs and outputs must all be host arrays or all CUDA
+    tensors.  Returns one dict per item (the frame statistics and the schedule it ran with)."""
+    n = len(items)
+    arr = (MultiItem * max(1, n))()
+    keep = []
+    hloc = vloc = None
+    for k, it in enumerate(items):
+        p = it["params"]
+        hp, hl, kh = _arg(it["holo"], (p.height, p.width))
+        keep.append(kh)
+        if hloc is not None and hl != hloc:
+            raise ValueError("holograms must all be host arrays or all device tensors")
+        hloc = hl
+        arr[k].params = p
+        arr[k].holo = hp
+        vol = it.get("volume")
+        if vol is not None:
+            vp_, vl, kv = _arg(vol, (p.num_planes, p.height, p.width), writable=True)
+            keep.append(kv)
+            if vloc is not None and vl != vloc:
+                raise ValueError("volumes must all be host arrays or all device tensors")
+            vloc = vl
+            arr[k].volume = vp_
+        arr[k].f_target = float(it.get("f_target", 0.0))
+        if it.get("schedule") is not None:
+            arr[k].schedule = it["schedule"]
+    _check(lib().rihvr_reconstruct_multi(arr, n, HOST if hloc is None else hloc, HOST if vloc is None else vloc,
+                                         int(autotune)), "rihvr_reconstruct_multi")
+    out = []
+    for a in arr[:n]:
+        f = a.stats
+        out.append(dict(iterations=f.iterations, total_backtracks=f.total_backtracks, reached=bool(f.reached),
+                        f_initial=f.f_initial, f_final=f.f_final, nnz=f.nnz, schedule=a.schedule.as_dict()))
+    return out
+
+
 def net_param_count() -> int:
     n = ctypes.c_int64()
     _check(lib().rihvr_net_param_count(ctypes.byref(n)), "rihvr_net_param_count")

@@ -29,7 +29,9 @@ struct GenPlan {
     int rad[kGenMaxRadices];
 };
 
-bool make_plan(int L, GenPlan* p) {
+// radices of L: as many of the largest power of two up to max_radix (16, 8 or 4) as divide it,
+// the rest of its power of two, then its odd prime factors in increasing order
+bool make_plan(int L, GenPlan* p, int max_radix = 16) {
     if (L < 2 || L > kGenMaxL) return false;
     p->L = L;
     p->nr = 0;
@@ -40,10 +42,12 @@ bool make_plan(int L, GenPlan* p) {
         m /= r;
         return true;
     };
-    while (m % 16 == 0)
-        if (!push(16)) return false;
-    while (m % 8 == 0)
-        if (!push(8)) return false;
+    if (max_radix >= 16)
+        while (m % 16 == 0)
+            if (!push(16)) return false;
+    if (max_radix >= 8)
+        while (m % 8 == 0)
+            if (!push(8)) return false;
     while (m % 4 == 0)
         if (!push(4)) return false;
     while (m % 2 == 0)
@@ -303,34 +307,35 @@ bool gen_fft_supported(int L) {
     return make_plan(L, &p);
 }
 
-hipError_t gen_rows_plain(int L, int sign, const float2* in, float2* out, long nrows, const float2* tw, hipStream_t s) {
+hipError_t gen_rows_plain(int L, int sign, const float2* in, float2* out, long nrows, const float2* tw, hipStream_t s,
+                          int max_radix) {
     GenRowArgs a{};
-    if (!make_plan(L, &a.plan) || nrows > 0x7fffffffL) return hipErrorInvalidValue;
+    if (!make_plan(L, &a.plan, max_radix) || nrows > 0x7fffffffL) return hipErrorInvalidValue;
     a.in = in; a.out = out; a.tw = tw; a.nrows = (int)nrows;
     return sign < 0 ? launch_rows<-1, GR_PLAIN>(a, (unsigned)nrows, 1, s) : launch_rows<1, GR_PLAIN>(a, (unsigned)nrows, 1, s);
 }
 
 hipError_t gen_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out, int nrows,
-                             const float2* tw_inv, hipStream_t s) {
+                             const float2* tw_inv, hipStream_t s, int max_radix) {
     GenRowArgs a{};
-    if (!make_plan(L, &a.plan)) return hipErrorInvalidValue;
+    if (!make_plan(L, &a.plan, max_radix)) return hipErrorInvalidValue;
     a.in = parts; a.out = out; a.tw = tw_inv; a.nrows = nrows; a.nparts = nparts; a.part_stride = part_stride;
     return launch_rows<1, GR_SUMPART>(a, (unsigned)nrows, 1, s);
 }
 
 hipError_t gen_rows_adj(int L, const float2* rhat, const double* kappa, const float* zlist, float2* out, int nrows,
-                        int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s) {
+                        int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s, int max_radix) {
     GenRowArgs a{};
-    if (!make_plan(L, &a.plan) || zchunk < 1) return hipErrorInvalidValue;
+    if (!make_plan(L, &a.plan, max_radix) || zchunk < 1) return hipErrorInvalidValue;
     a.out = out; a.tw = tw_inv; a.nrows = nrows; a.rhat = rhat; a.kappa = kappa; a.zlist = zlist; a.nz = nz;
     a.zchunk = zchunk; a.invN = invN;
     return launch_rows<1, GR_ADJ>(a, (unsigned)nrows, (unsigned)((nz + zchunk - 1) / zchunk), s);
 }
 
 hipError_t gen_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts, int nrows,
-                           int nz, int zchunk, const float2* tw_fwd, hipStream_t s) {
+                           int nz, int zchunk, const float2* tw_fwd, hipStream_t s, int max_radix) {
     GenRowArgs a{};
-    if (!make_plan(L, &a.plan) || zchunk < 1) return hipErrorInvalidValue;
+    if (!make_plan(L, &a.plan, max_radix) || zchunk < 1) return hipErrorInvalidValue;
     a.in = in; a.out = parts; a.tw = tw_fwd; a.nrows = nrows; a.kappa = kappa; a.zlist = zlist; a.nz = nz;
     a.zchunk = zchunk; a.part_stride = (long)nrows * L;
     return launch_rows<-1, GR_FWDACC>(a, (unsigned)nrows, (unsigned)((nz + zchunk - 1) / zchunk), s);

@@ -55,13 +55,15 @@ hipError_t fft_cols_real(int L, const float2* in, float2* est, const float2* mea
 // Row transforms of length L with a plan chosen at run time; transforms along y run between
 // tiled transposes ([z][R][C] -> [z][C][R]).  Same roles as the fixed-length launchers above.
 bool gen_fft_supported(int L);
-hipError_t gen_rows_plain(int L, int sign, const float2* in, float2* out, long nrows, const float2* tw, hipStream_t s);
+// max_radix: the largest radix of the plan (16, 8 or 4; an autotuner choice)
+hipError_t gen_rows_plain(int L, int sign, const float2* in, float2* out, long nrows, const float2* tw, hipStream_t s,
+                          int max_radix = 16);
 hipError_t gen_rows_sumparts(int L, const float2* parts, int nparts, long part_stride, float2* out, int nrows,
-                             const float2* tw_inv, hipStream_t s);
+                             const float2* tw_inv, hipStream_t s, int max_radix = 16);
 hipError_t gen_rows_adj(int L, const float2* rhat, const double* kappa, const float* zlist, float2* out, int nrows,
-                        int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s);
+                        int nz, int zchunk, float invN, const float2* tw_inv, hipStream_t s, int max_radix = 16);
 hipError_t gen_rows_fwdacc(int L, const float2* in, const double* kappa, const float* zlist, float2* parts, int nrows,
-                           int nz, int zchunk, const float2* tw_fwd, hipStream_t s);
+                           int nz, int zchunk, const float2* tw_fwd, hipStream_t s, int max_radix = 16);
 // out[z][c][r] = in[z][r][c] (times phase[z], conjugated with conj; phase may be NULL)
 hipError_t gen_transpose(const float2* in, float2* out, int R, int C, int nz, const float2* phase, bool conj,
                          hipStream_t s);
@@ -91,6 +93,7 @@ struct ProxLaunch {
     float2* Y;          // output y_k
     int W, H, nz;
     float s_mom;        // x0 = Y0 + s_mom * (Y0 - YM)
+    const float* s_mom_dev = nullptr;  // when set, s_mom is read from device memory
     double t;           // step size
     double reg;         // L1 weight
     double reg_tv;      // TV weight
@@ -142,9 +145,18 @@ hipError_t momentum_out(const float2* Y, const float2* Y0, float s_mom, float2* 
 // est = cur + s (cur - prev); resid = est - meas; partial[block] = sum |resid|^2.
 // restart_sum (device, may be NULL): when > 0 the step is 0 (a restarted iteration).
 int holo_num_blocks(long n);
+// step_dev (device, may be NULL): when set, the step is read from it instead of s_mom.
 hipError_t holo_momentum(const float2* cur, const float2* prev, double s_mom, const float2* meas,
                          float2* est, float2* resid, long n, double* partial, hipStream_t s,
-                         const double* restart_sum = nullptr);
+                         const double* restart_sum = nullptr, const double* step_dev = nullptr);
+// p[0..n) = v (a kernel, so that it can be captured in a graph)
+hipError_t fill_u64(uint64_t* p, long n, uint64_t v, hipStream_t s);
+// partial[b] (b < nblocks) = part of sum |v_i|^2 over n values
+hipError_t norm2_partials(const float2* v, long n, int nblocks, double* partial, hipStream_t s);
+// a sparse pseudo-random volume (about 1 / period of its voxels non-zero)
+hipError_t synth_volume(float2* v, long n, unsigned period, unsigned seed, hipStream_t s);
+// *f = fv and *d = dv (either may be NULL), in stream order
+hipError_t set_scalars(float* f, float fv, double* d, double dv, hipStream_t s);
 // resid = est - meas with partial sum |resid|^2
 hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, long n, double* partial,
                          hipStream_t s);

@@ -67,6 +67,7 @@ struct ProxDev {
     float2* Y;
     int W, H, nz;
     float s_mom, t, mu, sc, thr;
+    const float* s_mom_dev;  // when set, the momentum step is read from it (replays of captured graphs)
     int do_soft;
     int lsh;  // a map line is 2^lsh voxels of a row: 16 up to W = 1024, then 32, 64 (64 lines per row)
     ProxGeom geo;
@@ -136,6 +137,7 @@ template <int NTV> struct ProxWave {
     uint64_t wlines = ~0ull, outlines = ~0ull;
     uint64_t plane_bits = 0;  // lines this wave set in the new map (its plane word's share)
     float acc_ls1 = 0.f, acc_ls2 = 0.f, acc_rs = 0.f, acc_l1 = 0.f, acc_tv = 0.f, acc_nnz = 0.f;
+    float smom = 0.f;  // the momentum step s of x0 = y0 + s (y0 - ym)
 
     static __device__ __forceinline__ int wrap_row(int tau, int H) {
         const int row = tau % H;
@@ -205,7 +207,7 @@ template <int NTV> struct ProxWave {
             if ((xrows >> 1) & 1u) {              // row tau (its flags were shifted in 2 rows ago)
                 const unsigned off = (unsigned)row_cur * rowb;
                 const float2 y0 = at(y0p, off, xoff), ym = at(ymp, off, xoff);
-                x0 = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
+                x0 = make_float2(fmaf(smom, y0.x - ym.x, y0.x), fmaf(smom, y0.y - ym.y, y0.y));
             }
             row_cur = next_row(row_cur, a.H);
             hist[0][SLOT] = make_float2(fmaf(-a.t, g.x, x0.x), fmaf(-a.t, g.y, x0.y));
@@ -335,11 +337,11 @@ template <int NTV> struct ProxWave {
             float2 x0o = make_float2(0.0f, 0.0f), x0u = make_float2(0.0f, 0.0f);
             if ((xrows >> (NTV + 3)) & 1u) {
                 const float2 ym = at(ymp, ro, xoff);
-                x0o = make_float2(fmaf(a.s_mom, y0.x - ym.x, y0.x), fmaf(a.s_mom, y0.y - ym.y, y0.y));
+                x0o = make_float2(fmaf(smom, y0.x - ym.x, y0.x), fmaf(smom, y0.y - ym.y, y0.y));
             }
             if ((xrows >> (NTV + 4)) & 1u) {
                 const float2 y0u = at(y0p, ru, xoff), ymu = at(ymp, ru, xoff);
-                x0u = make_float2(fmaf(a.s_mom, y0u.x - ymu.x, y0u.x), fmaf(a.s_mom, y0u.y - ymu.y, y0u.y));
+                x0u = make_float2(fmaf(smom, y0u.x - ymu.x, y0u.x), fmaf(smom, y0u.y - ymu.y, y0u.y));
             }
             const float x0lr = lane_from_left(x0o.x);
             const float x0li = lane_from_left(x0o.y);
@@ -414,6 +416,7 @@ __global__ void __launch_bounds__(64 * kProxWaves)
     const int zrow = z * a.H;  // first row of plane z in the occupancy maps
 
     PW st;
+    st.smom = a.s_mom_dev ? *a.s_mom_dev : a.s_mom;
     // the wave's lines: the periodic wrap can add a line at the other end of the row
     st.wlines = 0;
     st.outlines = 0;
@@ -482,6 +485,7 @@ static ProxDev make_dev(const ProxLaunch& a) {
     d.Y0 = a.Y0; d.YM = a.YM; d.G = a.G; d.Y = a.Y;
     d.W = a.W; d.H = a.H; d.nz = a.nz;
     d.s_mom = a.s_mom;
+    d.s_mom_dev = a.s_mom_dev;
     d.t = (float)a.t;
     d.mu = (float)(a.t * a.reg_tv);
     d.sc = a.reg_tv != 0.0 ? (float)(1 / (4 * 3 * a.t * a.reg_tv)) : 0.0f;
@@ -597,7 +601,7 @@ __global__ void __launch_bounds__(kEwThreads) k_prox_l1(ProxDev a) {
     const long N = (long)a.W * a.H;
     const long n = N * a.nz;
     double acc[PS_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const float s = a.s_mom;
+    const float s = a.s_mom_dev ? *a.s_mom_dev : a.s_mom;
     for (int it = 0; it < kEwItems; ++it) {
         long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
         if (i >= n) break;
@@ -653,7 +657,7 @@ __global__ void __launch_bounds__(kEwThreads) k_fl3d_pre(ProxDev a) {
     const long N = (long)W * H;
     const long n = N * a.nz;
     double l1 = 0, tv = 0, nnz = 0;
-    const float s = a.s_mom;
+    const float s = a.s_mom_dev ? *a.s_mom_dev : a.s_mom;
     auto mom = [&](long i) {
         const float2 y0 = a.Y0[i], ym = a.YM[i];
         return make_float2(fmaf(s, y0.x - ym.x, y0.x), fmaf(s, y0.y - ym.y, y0.y));
@@ -723,7 +727,7 @@ __global__ void __launch_bounds__(kEwThreads) k_fl3d_final(ProxDev a, const floa
     const long N = (long)a.W * a.H;
     const long n = N * a.nz;
     double ls1 = 0, ls2 = 0, rs = 0, nnzy = 0;
-    const float s = a.s_mom;
+    const float s = a.s_mom_dev ? *a.s_mom_dev : a.s_mom;
     for (int it = 0; it < kEwItems; ++it) {
         long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
         if (i >= n) break;
@@ -803,7 +807,7 @@ __global__ void __launch_bounds__(kEwThreads) k_tv2_pre(ProxDev a, Tv2Buf b, dou
     const long N = (long)W * H;
     const long n = N * a.nz;
     double l1 = 0, tv = 0, nnz = 0;
-    const float s = a.s_mom;
+    const float s = a.s_mom_dev ? *a.s_mom_dev : a.s_mom;
     auto mom = [&](long i) {
         const float2 y0 = a.Y0[i], ym = a.YM[i];
         return make_float2(fmaf(s, y0.x - ym.x, y0.x), fmaf(s, y0.y - ym.y, y0.y));
@@ -879,7 +883,7 @@ __global__ void __launch_bounds__(kEwThreads) k_tv2_final(ProxDev a, const float
     const long N = (long)a.W * a.H;
     const long n = N * a.nz;
     double ls1 = 0, ls2 = 0, rs = 0, nnzy = 0;
-    const float s = a.s_mom;
+    const float s = a.s_mom_dev ? *a.s_mom_dev : a.s_mom;
     for (int it = 0; it < kEwItems; ++it) {
         long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
         if (i >= n) break;
@@ -1002,11 +1006,13 @@ int holo_num_blocks(long n) { return (int)((n + (long)kEwThreads * kHoloItems - 
 
 __global__ void __launch_bounds__(kEwThreads)
 k_holo_momentum(const float2* __restrict__ cur, const float2* __restrict__ prev, double s_mom,
+                const double* __restrict__ step_dev,
                 const float2* __restrict__ meas, float2* est, float2* resid, long n, double* partial,
                 const double* __restrict__ restart_sum) {
     __shared__ double red[kEwThreads / 64];
     // with restart_sum: the step is 0 when the iteration restarts (restart_sum > 0), as the
     // host's alpha update gives, so the step can be chosen before the host has the sum
+    if (step_dev) s_mom = *step_dev;
     if (restart_sum && *restart_sum > 0) s_mom = 0.0;
     double acc = 0;
     for (int it = 0; it < kHoloItems; ++it) {
@@ -1026,9 +1032,49 @@ k_holo_momentum(const float2* __restrict__ cur, const float2* __restrict__ prev,
 }
 
 hipError_t holo_momentum(const float2* cur, const float2* prev, double s_mom, const float2* meas, float2* est,
-                         float2* resid, long n, double* partial, hipStream_t s, const double* restart_sum) {
-    hipLaunchKernelGGL(k_holo_momentum, dim3(holo_num_blocks(n)), dim3(kEwThreads), 0, s, cur, prev, s_mom,
+                         float2* resid, long n, double* partial, hipStream_t s, const double* restart_sum,
+                         const double* step_dev) {
+    hipLaunchKernelGGL(k_holo_momentum, dim3(holo_num_blocks(n)), dim3(kEwThreads), 0, s, cur, prev, s_mom, step_dev,
                        meas, est, resid, n, partial, restart_sum);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u64(uint64_t* __restrict__ p, long n, uint64_t v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+hipError_t fill_u64(uint64_t* p, long n, uint64_t v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// A sparse synthetic volume: voxel i is non-zero (re, im in [-0.5, 0.5)) when a hash of i falls
+// in 1 / period of its range, else +0.
+__global__ void k_synth_volume(float2* __restrict__ v, long n, unsigned period, unsigned seed) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned h = (unsigned)i * 0x9E3779B1u ^ seed;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    float2 val = make_float2(0.0f, 0.0f);
+    if (h % period == 0) val = make_float2((float)((h >> 8) & 0xffff) / 65536.0f - 0.5f, (float)(h & 0xff) / 256.0f - 0.5f);
+    v[i] = val;
+}
+hipError_t synth_volume(float2* v, long n, unsigned period, unsigned seed, hipStream_t s) {
+    hipLaunchKernelGGL(k_synth_volume, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v, n, period, seed);
+    return hipGetLastError();
+}
+
+__global__ void k_set_scalars(float* __restrict__ f, float fv, double* __restrict__ d, double dv) {
+    if (f) *f = fv;
+    if (d) *d = dv;
+}
+hipError_t set_scalars(float* f, float fv, double* d, double dv, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, s, f, fv, d, dv);
     return hipGetLastError();
 }
 
@@ -1047,6 +1093,22 @@ k_holo_residual(const float2* __restrict__ est, const float2* __restrict__ meas,
     }
     double v = block_sum(acc, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// sum |v|^2 over n complex values: nblocks partial sums (grid-stride), partial[b]
+__global__ void __launch_bounds__(kEwThreads) k_norm2(const float2* __restrict__ v, long n, double* partial) {
+    __shared__ double red[kEwThreads / 64];
+    double acc = 0;
+    for (long i = (long)blockIdx.x * kEwThreads + threadIdx.x; i < n; i += (long)gridDim.x * kEwThreads) {
+        const float2 e = v[i];
+        acc += (double)e.x * e.x + (double)e.y * e.y;
+    }
+    double r = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+hipError_t norm2_partials(const float2* v, long n, int nblocks, double* partial, hipStream_t s) {
+    hipLaunchKernelGGL(k_norm2, dim3(nblocks), dim3(kEwThreads), 0, s, v, n, partial);
+    return hipGetLastError();
 }
 
 hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, long n, double* partial,

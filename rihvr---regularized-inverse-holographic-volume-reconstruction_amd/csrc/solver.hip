@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -109,6 +110,7 @@ struct rihvr_ctx {
     // FFTs of lengths outside the fixed-length kernels' set: rows with run-time plans between
     // transposes (fft_generic.hip); T2 holds the transposed planes
     bool gen = false;
+    int gen_radix = 16;  // the run-time plans' largest radix
     float2* T2 = nullptr;
     float2* tvbuf = nullptr;                      // 4 volumes for mode 7 with n_tv above the fused prox's limit
     float2* out = nullptr;                        // points at the returned volume
@@ -141,6 +143,19 @@ struct rihvr_ctx {
     int n_groups = 1;
     bool groups_one_stream = false;  // run A^T, prox and A group by group on the main stream
     int prox_rows = 0;    // output rows per prox wave (0: default)
+    // Captured iterations: the two halves of a queued-ahead iteration (prox + A, then the
+    // hologram momentum + the next A^T) as HIP graphs, one per pointer and step combination
+    // (the volume ring, the maps and the spectra rotate through at most 12), replayed with the
+    // momentum step read from device memory
+    bool graphs = true;
+    float* d_smom = nullptr;
+    double* d_step = nullptr;
+    struct CachedGraph {
+        std::vector<const void*> key;
+        double t;
+        hipGraphExec_t exec;
+    };
+    std::vector<CachedGraph> gcache[2];
     bool have_result = false;
     bool force_step = false;
     double forced_step = 0;
@@ -313,12 +328,12 @@ rihvr_status forward_group(rihvr_ctx* c, const float2* vol, int g0, int gz, hipS
         float2* Tt = c->T2 + (size_t)g0 * c->N;
         RIHVR_TRY(timed_on(c, K_COLS_FWD, st, [&] {
             hipError_t e = gen_transpose(vol + (size_t)g0 * c->N, Tt, c->H, c->W, gz, c->phase_fwd + g0, true, st);
-            if (e == hipSuccess) e = gen_rows_plain(c->H, -1, Tt, Tt, (long)gz * c->W, c->twfH, st);
+            if (e == hipSuccess) e = gen_rows_plain(c->H, -1, Tt, Tt, (long)gz * c->W, c->twfH, st, c->gen_radix);
             if (e == hipSuccess) e = gen_transpose(Tt, Tg, c->W, c->H, gz, nullptr, false, st);
             return e;
         }));
         return timed_on(c, K_ROWS_FWDACC, st, [&] {
-            return gen_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, c->twfW, st);
+            return gen_rows_fwdacc(c->W, Tg, c->kappa, c->zlist + g0, spg, c->H, gz, c->zchunk_fwd, c->twfW, st, c->gen_radix);
         });
     }
     uint8_t* smg = c->smap + (size_t)g0 * (c->W / 16);
@@ -354,11 +369,11 @@ rihvr_status forward_finish(rihvr_ctx* c, float2* est) {
 rihvr_status spatial_estimate(rihvr_ctx* c, const float2* est_spec) {
     if (c->gen) {
         RIHVR_TRY(timed(c, K_ROWS_SUMPARTS, [&] {
-            return gen_rows_sumparts(c->W, est_spec, 1, c->N, c->htmp, c->H, c->twiW, c->stream);
+            return gen_rows_sumparts(c->W, est_spec, 1, c->N, c->htmp, c->H, c->twiW, c->stream, c->gen_radix);
         }));
         return timed(c, K_COLS_REAL, [&] {
             hipError_t e = gen_transpose(c->htmp, c->T2, c->H, c->W, 1, nullptr, false, c->stream);
-            if (e == hipSuccess) e = gen_rows_plain(c->H, 1, c->T2, c->T2, c->W, c->twiH, c->stream);
+            if (e == hipSuccess) e = gen_rows_plain(c->H, 1, c->T2, c->T2, c->W, c->twiH, c->stream, c->gen_radix);
             if (e == hipSuccess)
                 e = gen_transpose_real(c->T2, c->hspat, c->meas, c->resid, c->partial_holo, c->H, c->W, c->stream);
             return e;
@@ -380,10 +395,10 @@ rihvr_status apply_forward(rihvr_ctx* c, const float2* vol, float2* est) {
 // into out; ev_rhat marks rhat
 rihvr_status spectrum(rihvr_ctx* c, const float2* r, float2* out) {
     if (c->gen) {
-        RIHVR_TRY(timed(c, K_ROWS_PLAIN, [&] { return gen_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream); }));
+        RIHVR_TRY(timed(c, K_ROWS_PLAIN, [&] { return gen_rows_plain(c->W, -1, r, c->htmp, c->H, c->twfW, c->stream, c->gen_radix); }));
         RIHVR_TRY(timed(c, K_COLS_PLAIN, [&] {
             hipError_t e = gen_transpose(c->htmp, c->T2, c->H, c->W, 1, nullptr, false, c->stream);
-            if (e == hipSuccess) e = gen_rows_plain(c->H, -1, c->T2, c->T2, c->W, c->twfH, c->stream);
+            if (e == hipSuccess) e = gen_rows_plain(c->H, -1, c->T2, c->T2, c->W, c->twfH, c->stream, c->gen_radix);
             if (e == hipSuccess) e = gen_transpose(c->T2, out, c->W, c->H, 1, nullptr, false, c->stream);
             return e;
         }));
@@ -418,11 +433,11 @@ rihvr_status adjoint_group(rihvr_ctx* c, float2* G, int g0, int gz, hipStream_t 
         float2* Tt = c->T2 + (size_t)g0 * c->N;
         RIHVR_TRY(timed_on(c, K_ROWS_ADJ, st, [&] {
             return gen_rows_adj(c->W, c->rhat, c->kappa, c->zlist + g0, Tg, c->H, gz, std::min(c->zchunk_adj, gz), invN,
-                                c->twiW, st);
+                                c->twiW, st, c->gen_radix);
         }));
         return timed_on(c, K_COLS_ADJ, st, [&] {
             hipError_t e = gen_transpose(Tg, Tt, c->H, c->W, gz, nullptr, false, st);
-            if (e == hipSuccess) e = gen_rows_plain(c->H, 1, Tt, Tt, (long)gz * c->W, c->twiH, st);
+            if (e == hipSuccess) e = gen_rows_plain(c->H, 1, Tt, Tt, (long)gz * c->W, c->twiH, st, c->gen_radix);
             if (e == hipSuccess) e = gen_transpose(Tt, G + (size_t)g0 * c->N, c->W, c->H, gz, c->phase + g0, false, st);
             return e;
         });
@@ -454,12 +469,16 @@ int effective_mode(const rihvr_ctx* c) {
 bool tv3d(const rihvr_ctx* c) { return c->ss.mode == RIHVR_MODE_FASTA_FUSED_LASSO; }
 
 // y = prox(x0 - t g), x0 = Y0 + s (Y0 - YM); reductions into scal[SL_PROX..]
+// s_mom_dev (may be NULL): the momentum step in device memory (a captured graph reads it there);
+// enqueue = false performs only the host-side bookkeeping of the maps (a replayed graph holds
+// the launches)
 rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* YM, float s_mom, const float2* G,
-                        double t, float2* Y) {
+                        double t, float2* Y, const float* s_mom_dev = nullptr, bool enqueue = true) {
     ProxLaunch a;
     a.Y0 = Y0; a.YM = YM; a.G = G; a.Y = Y;
     a.W = c->W; a.H = c->H; a.nz = c->Nz;
     a.s_mom = s_mom; a.t = t;
+    a.s_mom_dev = s_mom_dev;
     a.reg = (double)c->p.regularization_param;
     a.reg_tv = (double)c->p.regularization_TV_param;
     a.n_tv = c->p.num_tv_iterations;
@@ -469,7 +488,7 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
     const int ky = ring_index(c, Y);
     if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && c->tvbuf) {
         // more TV iterations than the fused kernel holds: the multi-pass form (no maps)
-        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_tv2_general(a, c->tvbuf, c->stream); }));
+        if (enqueue) RIHVR_TRY(timed(c, K_PROX, [&] { return prox_tv2_general(a, c->tvbuf, c->stream); }));
         rows = prox_tv2_num_blocks(c->W, c->H, c->Nz);
         if (ky >= 0) c->occ_valid[ky] = false;
     } else if (mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D) {
@@ -482,8 +501,10 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
         a.occ_old = (ky >= 0 && c->occ_valid[ky]) ? c->occ[ky] : c->occ_ones;
         a.occ_new = c->occ_tmp;
         a.occ_new_planes = c->occ_tmp + (size_t)c->Nz * c->H;
-        RIHVR_HIP(hipMemsetAsync(c->occ_tmp, 0, map_words(c) * sizeof(uint64_t), c->stream));
-        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
+        if (enqueue) {
+            RIHVR_HIP(fill_u64(c->occ_tmp, (long)map_words(c), 0ull, c->stream));
+            RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl2d(a, c->stream); }));
+        }
         rows = prox_fl2d_num_waves(c->W, c->H, c->Nz, a.n_tv, c->prox_rows);
         if (ky >= 0) {
             std::swap(c->occ[ky], c->occ_tmp);
@@ -491,14 +512,15 @@ rihvr_status apply_prox(rihvr_ctx* c, int mode, const float2* Y0, const float2* 
         }
     } else if (mode == RIHVR_MODE_FASTA_FUSED_LASSO) {
         // the FFT scratch volume is free between A^T and A
-        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl3d(a, c->T, c->stream); }));
+        if (enqueue) RIHVR_TRY(timed(c, K_PROX, [&] { return prox_fl3d(a, c->T, c->stream); }));
         rows = prox_fl3d_num_blocks(c->W, c->H, c->Nz);
     } else {
-        RIHVR_TRY(timed(c, K_PROX, [&] { return prox_l1(a, c->stream); }));
+        if (enqueue) RIHVR_TRY(timed(c, K_PROX, [&] { return prox_l1(a, c->stream); }));
         rows = prox_l1_num_blocks(c->W, c->H, c->Nz);
     }
     if ((mode != RIHVR_MODE_FASTA_FUSED_LASSO_2D || c->tvbuf) && ky >= 0) c->occ_valid[ky] = false;
-    RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial, rows, PS_COUNT, c->red_work, c->scal + SL_PROX, c->stream); }));
+    if (enqueue)
+        RIHVR_TRY(timed(c, K_REDUCE, [&] { return reduce_partials(c->partial, rows, PS_COUNT, c->red_work, c->scal + SL_PROX, c->stream); }));
     return RIHVR_OK;
 }
 
@@ -584,12 +606,26 @@ void release(rihvr_ctx* c) {
     for (hipEvent_t e : c->ev_adj) (void)hipEventDestroy(e);
     if (c->ev_rhat) (void)hipEventDestroy(c->ev_rhat);
     if (c->ev_scal) (void)hipEventDestroy(c->ev_scal);
+    for (auto& gc : c->gcache)
+        for (auto& g : gc) (void)hipGraphExecDestroy(g.exec);
+    (void)hipFree(c->d_smom);
+    (void)hipFree(c->d_step);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
-rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
+rihvr_status check_schedule(const rihvr_schedule* s) {
+    if (!s) return RIHVR_OK;
+    const bool ok = (s->fft_path == 0 || s->fft_path == 1) &&
+                    (s->max_radix == 0 || s->max_radix == 4 || s->max_radix == 8 || s->max_radix == 16) &&
+                    s->zchunk_fwd >= 0 && s->zchunk_adj >= 0 && s->prox_rows >= 0 && (s->graphs == 0 || s->graphs == 1);
+    return ok ? RIHVR_OK : RIHVR_E_INVALID_ARGUMENT;
+}
+
+// sch: the launch choices (NULL: the defaults, with the RIHVR_* environment overrides)
+rihvr_status create_impl(const rihvr_params* params, const rihvr_schedule* sch, rihvr_ctx** out) {
     RIHVR_TRY(validate(params));
+    RIHVR_TRY(check_schedule(sch));
     rihvr_ctx* c = new rihvr_ctx();
     *out = c;
     c->p = *params;
@@ -603,16 +639,26 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
     RIHVR_HIP(hipEventCreateWithFlags(&c->ev_rhat, hipEventDisableTiming));
     RIHVR_HIP(hipEventCreateWithFlags(&c->ev_scal, hipEventDisableTiming));
-    int zc = 32;
-    if (const char* e = std::getenv("RIHVR_ZCHUNK")) zc = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("RIHVR_PROX_ROWS"))
-        c->prox_rows = std::max(0, std::atoi(e));
+    int zc = 32, za = 16, prow = -1;
+    bool force_gen = false;
+    if (sch) {
+        if (sch->zchunk_fwd > 0) zc = sch->zchunk_fwd;
+        if (sch->zchunk_adj > 0) za = sch->zchunk_adj;
+        if (sch->prox_rows > 0) prow = sch->prox_rows;
+        if (sch->max_radix > 0) c->gen_radix = sch->max_radix;
+        force_gen = sch->fft_path == 1;
+        c->graphs = sch->graphs != 0;
+    } else {
+        if (const char* e = std::getenv("RIHVR_ZCHUNK")) zc = std::max(1, std::atoi(e));
+        if (const char* e = std::getenv("RIHVR_ZCHUNK_ADJ")) za = std::max(1, std::atoi(e));
+        if (const char* e = std::getenv("RIHVR_PROX_ROWS")) prow = std::max(0, std::atoi(e));
+        }
+    if (prow >= 0)
+        c->prox_rows = std::min(prow, c->H);
     else if (params->num_tv_iterations >= 0 && params->num_tv_iterations <= 10)
         c->prox_rows = prox_fl2d_auto_rows(c->W, c->H, c->Nz, params->num_tv_iterations);
     c->zchunk_fwd = std::min(c->Nz, zc);
     c->nparts = (c->Nz + c->zchunk_fwd - 1) / c->zchunk_fwd;
-    int za = 16;
-    if (const char* e = std::getenv("RIHVR_ZCHUNK_ADJ")) za = std::max(1, std::atoi(e));
     c->zchunk_adj = std::min(c->Nz, za);
     // pipeline groups: one group (everything on the main stream) unless RIHVR_PIPE_PLANES
     // asks for groups of that many planes (rounded up to a multiple of 16).  Measured at
@@ -624,7 +670,7 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     pp = ((pp + 15) / 16) * 16;
     c->pipe_planes = std::min(c->Nz, std::max(16, pp));
     if (c->pipe_planes % c->zchunk_fwd || c->pipe_planes % c->zchunk_adj) c->pipe_planes = c->Nz;
-    c->gen = !(fft_length_supported(c->W) && fft_length_supported(c->H));
+    c->gen = force_gen || !(fft_length_supported(c->W) && fft_length_supported(c->H));
     const bool tv_general = params->mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D &&
                             !prox_fl2d_supported(params->num_tv_iterations);
     if (c->gen || tv_general) c->pipe_planes = c->Nz;  // the plane-group schedule needs the fused kernels
@@ -668,6 +714,8 @@ rihvr_status create_impl(const rihvr_params* params, rihvr_ctx** out) {
     RIHVR_TRY(dalloc(&c->scal, SL_COUNT));
     RIHVR_TRY(dalloc(&c->red_work, (size_t)kReduceBlocks * PS_COUNT));
     RIHVR_TRY(dalloc(&c->dflag, 4));
+    RIHVR_TRY(dalloc(&c->d_smom, 1));
+    RIHVR_TRY(dalloc(&c->d_step, 1));
     RIHVR_HIP(hipHostMalloc((void**)&c->h_scal, SL_COUNT * sizeof(double), hipHostMallocDefault));
 
     // kappa table, plane list and per-plane phase factors, computed on the host
@@ -809,6 +857,43 @@ rihvr_status momentum_objective(rihvr_ctx* c, double summag, double* f) {
     return RIHVR_OK;
 }
 
+// The prox's weights as one word (they are baked into a captured prox's arguments).
+uint64_t weight_bits(const rihvr_ctx* c) {
+    uint32_t a = 0, b = 0;
+    std::memcpy(&a, &c->p.regularization_param, 4);
+    std::memcpy(&b, &c->p.regularization_TV_param, 4);
+    return ((uint64_t)a << 32) | b;
+}
+
+// Replays the graph cached for (key, t) on the context's stream, or captures `enqueue`'s launches
+// into one first.  `bookkeep` performs the host-side state changes of the launches (it runs after
+// a replay; a capture runs `enqueue`, which performs them itself).
+template <class F, class B>
+rihvr_status graph_run(rihvr_ctx* c, int which, const std::vector<const void*>& key, double t, F&& enqueue,
+                       B&& bookkeep) {
+    for (auto& g : c->gcache[which])
+        if (g.t == t && g.key == key) {
+            RIHVR_HIP(hipGraphLaunch(g.exec, c->stream));
+            return bookkeep();
+        }
+    RIHVR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const rihvr_status st = enqueue();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (st != RIHVR_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    RIHVR_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    RIHVR_HIP(ie);
+    c->gcache[which].push_back({key, t, exec});
+    RIHVR_HIP(hipGraphLaunch(exec, c->stream));
+    return RIHVR_OK;
+}
+
 // f_stop > 0: stop once the objective at the momentum point of a completed iteration
 // is <= f_stop.  That objective comes with the next iteration's first prox (its x0
 // reductions), so the test runs there and a stop abandons that iteration: the ring
@@ -837,14 +922,30 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
         float2* YM = c->Yb[c->iym];
         bool queued = false;  // the accepted-iteration work is queued
         for (;;) {
+            const bool use_graph = ahead && c->graphs && !c->tm.on;
             if (!s.g_ready && mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && c->n_groups > 1) {
                 RIHVR_TRY(pipelined_pass(c, Y0, YM, s.s_mom, s.t, Yn));
             } else {
                 if (!s.g_ready)
                     for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
                         RIHVR_TRY(adjoint_group(c, c->G, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
-                RIHVR_TRY(apply_prox(c, mode, Y0, YM, s.s_mom, c->G, s.t, Yn));
-                RIHVR_TRY(apply_forward(c, Yn, c->hX));
+                if (use_graph) {
+                    RIHVR_HIP(set_scalars(c->d_smom, s.s_mom, nullptr, 0.0, c->stream));
+                    const std::vector<const void*> key = {
+                        Y0, YM, Yn, c->G, c->hX, c->occ[0], c->occ[1], c->occ[2], c->occ_tmp,
+                        (const void*)(intptr_t)(c->occ_valid[0] | (c->occ_valid[1] << 1) | (c->occ_valid[2] << 2)),
+                        (const void*)(intptr_t)mode, (const void*)(uintptr_t)weight_bits(c)};
+                    RIHVR_TRY(graph_run(
+                        c, 0, key, s.t,
+                        [&] {
+                            RIHVR_TRY(apply_prox(c, mode, Y0, YM, s.s_mom, c->G, s.t, Yn, c->d_smom, true));
+                            return apply_forward(c, Yn, c->hX);
+                        },
+                        [&] { return apply_prox(c, mode, Y0, YM, s.s_mom, c->G, s.t, Yn, c->d_smom, false); }));
+                } else {
+                    RIHVR_TRY(apply_prox(c, mode, Y0, YM, s.s_mom, c->G, s.t, Yn));
+                    RIHVR_TRY(apply_forward(c, Yn, c->hX));
+                }
             }
             s.g_ready = true;
             if (ahead) {
@@ -852,18 +953,41 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
                 const double a1 = (1.0 + std::sqrt(1.0 + 4.0 * alpha0 * alpha0)) / 2.0;
                 std::swap(c->hP, c->hC);
                 std::swap(c->hC, c->hX);
-                RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
-                    return holo_momentum(c->hC, c->hP, (alpha0 - 1) / a1, c->bspec, c->hE, c->rhat, c->N,
-                                         c->partial_holo, c->stream, c->scal + SL_PROX + PS_RESTART);
-                }));
-                RIHVR_TRY(timed(c, K_REDUCE, [&] {
-                    return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work,
-                                           c->scal + SL_RESID_M, c->stream);
-                }));
-                RIHVR_TRY(rhat_written(c));
-                if (i + 1 < n_it)
-                    for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
-                        RIHVR_TRY(adjoint_group(c, c->G_alt, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
+                if (use_graph) {
+                    RIHVR_HIP(set_scalars(nullptr, 0.0f, c->d_step, (alpha0 - 1) / a1, c->stream));
+                    c->rhat_scaled = true;  // the graph's hologram momentum writes the residual spectrum
+                    const bool next = i + 1 < n_it;
+                    const std::vector<const void*> key = {c->hC, c->hP, c->hE, c->rhat, c->G_alt,
+                                                          (const void*)(intptr_t)next};
+                    RIHVR_TRY(graph_run(
+                        c, 1, key, 0.0,
+                        [&] {
+                            RIHVR_HIP(holo_momentum(c->hC, c->hP, 0.0, c->bspec, c->hE, c->rhat, c->N, c->partial_holo,
+                                                    c->stream, c->scal + SL_PROX + PS_RESTART, c->d_step));
+                            RIHVR_HIP(reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work,
+                                                      c->scal + SL_RESID_M, c->stream));
+                            if (next)
+                                for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
+                                    RIHVR_TRY(adjoint_group(c, c->G_alt, g0, std::min(c->pipe_planes, c->Nz - g0),
+                                                            c->stream));
+                            return RIHVR_OK;
+                        },
+                        [&] { return RIHVR_OK; }));
+                    RIHVR_HIP(hipEventRecord(c->ev_rhat, c->stream));
+                } else {
+                    RIHVR_TRY(timed(c, K_HOLO_MOM, [&] {
+                        return holo_momentum(c->hC, c->hP, (alpha0 - 1) / a1, c->bspec, c->hE, c->rhat, c->N,
+                                             c->partial_holo, c->stream, c->scal + SL_PROX + PS_RESTART);
+                    }));
+                    RIHVR_TRY(timed(c, K_REDUCE, [&] {
+                        return reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work,
+                                               c->scal + SL_RESID_M, c->stream);
+                    }));
+                    RIHVR_TRY(rhat_written(c));
+                    if (i + 1 < n_it)
+                        for (int g0 = 0; g0 < c->Nz; g0 += c->pipe_planes)
+                            RIHVR_TRY(adjoint_group(c, c->G_alt, g0, std::min(c->pipe_planes, c->Nz - g0), c->stream));
+                }
                 queued = true;
                 RIHVR_TRY(wait_scalars(c));
             } else {
@@ -1074,7 +1198,9 @@ rihvr_status batch_impl(rihvr_ctx* c, const float* holos, int holo_loc, int n_fr
 // r = b) -> network -> x_net in a ring slot.  The objective (calcObjectiveFunction) at x_net is
 // compared with the objectives at the A^T b start t A^T b (t = 1/Nz; its data residual is zero
 // for a real hologram since A A^T = Nz I there) and at the reference's zero start (0.5 ||b||^2):
-// x_net is used only when it is below both, else the zero start.  Then FISTA to the target.
+// x_net is used when it is below both; else alpha x_net, alpha >= 0 the minimiser of the
+// objective along the ray (closed form: the data term is quadratic in alpha, the regularisers
+// linear), when alpha > 0 and it is below both; else the zero start.  Then FISTA to the target.
 rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int holo_loc, double f_target,
                           int max_iterations, rihvr_learned_stats* out) {
     if (!holo || !rihvr::net::net_matches(net, c->Nz, c->H, c->W)) return RIHVR_E_INVALID_ARGUMENT;
@@ -1123,12 +1249,33 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
         return RIHVR_OK;
     };
     RIHVR_TRY(volume_objective(xw, &st.resid_net, &st.f_net));
+    // ||A x_net||^2 from the spectrum the objective left in hE (hC is still zero)
+    RIHVR_HIP(holo_residual(c->hE, c->hC, c->htmp, c->N, c->partial_holo, c->stream));
+    RIHVR_TRY(reduce_partials_on(c, c->partial_holo, holo_num_blocks(c->N), 1, c->scal + SL_RESID_M));
+    RIHVR_TRY(fetch_scalars(c));
+    const double aa = c->h_scal[SL_RESID_M];
+    // The objective along the ray alpha x_net (alpha >= 0; the L1 and TV terms are positively
+    // homogeneous): f(alpha) = 0.5 (alpha^2 aa - 2 alpha p + bb) + alpha R, with p = <A x_net, b>
+    // = (aa + bb - ||A x_net - b||^2) / 2 and R = f_net - 0.5 ||A x_net - b||^2, is least at
+    // alpha = max(0, (p - R) / aa); f(alpha) <= f(0) = f_zero and <= f(1) = f_net.
+    const double bb = 2.0 * st.f_zero, rr = 2.0 * st.resid_net;
+    const double pp = 0.5 * (aa + bb - rr), R = st.f_net - st.resid_net;
+    st.alpha = aa > 0 ? std::max(0.0, (pp - R) / aa) : 0.0;
+    st.f_scaled = 0.5 * (st.alpha * st.alpha * aa - 2.0 * st.alpha * pp + bb) + st.alpha * R;
     RIHVR_HIP(holo_scale(c->G, c->G_alt, c->V, (float)(1.0 / (double)c->Nz), c->stream));
     RIHVR_TRY(volume_objective(c->G_alt, &st.resid_adjoint, &st.f_adjoint));
-    st.fallback = (st.f_net <= st.f_adjoint && st.f_net <= st.f_zero) ? 0 : 1;
+    // x_net as predicted when it is below both other starts (shrinking a good prediction costs
+    // iterations later), else alpha x_net when that is, else the zero start
+    if (st.f_net <= st.f_adjoint && st.f_net <= st.f_zero)
+        st.fallback = 0;
+    else if (st.alpha > 0 && st.f_scaled <= st.f_adjoint && st.f_scaled <= st.f_zero)
+        st.fallback = 2;
+    else
+        st.fallback = 1;
+    if (st.fallback == 2) RIHVR_HIP(holo_scale(xw, xw, c->V, (float)st.alpha, c->stream));
     const auto t2 = clk::now();
     rihvr_frame_stats fs{};
-    RIHVR_TRY(solve_frame(c, holo, holo_loc, st.fallback ? nullptr : (const float*)xw, RIHVR_DEVICE, f_target,
+    RIHVR_TRY(solve_frame(c, holo, holo_loc, st.fallback == 1 ? nullptr : (const float*)xw, RIHVR_DEVICE, f_target,
                           max_it, &fs));
     const auto t3 = clk::now();
     st.iterations = fs.iterations;
@@ -1150,6 +1297,234 @@ rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, con
     RIHVR_TRY(iterate_impl(c, n_it, st, true));
     RIHVR_TRY(finish_impl(c, st, n_it - 1));
     if (st) st->iterations = n_it;
+    return RIHVR_OK;
+}
+
+// sum |v|^2 of a volume into *out (host), through the prox partials' buffer
+rihvr_status volume_norm2(rihvr_ctx* c, const float2* v, double* out) {
+    const int nb = (int)std::max<long>(1, std::min<long>(256, c->partial_rows * PS_COUNT));
+    RIHVR_HIP(norm2_partials(v, c->V, nb, c->partial, c->stream));
+    RIHVR_TRY(reduce_partials_on(c, c->partial, nb, 1, c->scal + SL_RESID_M));
+    RIHVR_TRY(fetch_scalars(c));
+    *out = c->h_scal[SL_RESID_M];
+    return RIHVR_OK;
+}
+
+// The Lipschitz constant of the data term's gradient, ||A^T A||, by power iteration from a
+// seeded pseudo-random volume.  The reference (estimateLipschitz, sparse_compressive_holo.cu:
+// 2205-2464) takes the secant ||grad f(x2) - grad f(x1)|| / ||x2 - x1|| of two random volumes,
+// which is ||A^T A d|| / ||d|| for d = x2 - x1 (the measurement cancels): the first ratio here.
+// Each further iteration applies A^T A to the normalised last result.  Floor 1e-6 (:2460).
+rihvr_status lipschitz_impl(rihvr_ctx* c, int seed, int iterations, double* L) {
+    if (!L || iterations < 1) return RIHVR_E_INVALID_ARGUMENT;
+    c->have_result = false;
+    c->ss.active = false;
+    float2* v = c->Yb[0];
+    c->occ_valid[0] = false;
+    RIHVR_HIP(synth_volume(v, c->V, 1u, (unsigned)seed, c->stream));  // every voxel in [-0.5, 0.5)^2
+    RIHVR_HIP(hipMemsetAsync(c->meas, 0, c->N * sizeof(float2), c->stream));
+    RIHVR_HIP(hipMemsetAsync(c->bspec, 0, c->N * sizeof(float2), c->stream));
+    double nv = 0, ratio = 0;
+    RIHVR_TRY(volume_norm2(c, v, &nv));
+    for (int k = 0; k < iterations; ++k) {
+        RIHVR_TRY(apply_forward(c, v, c->hE));
+        RIHVR_TRY(spatial_estimate(c, c->hE));
+        RIHVR_TRY(apply_adjoint(c, c->hspat, c->G));
+        double ng = 0;
+        RIHVR_TRY(volume_norm2(c, c->G, &ng));
+        ratio = nv > 0 ? std::sqrt(ng / nv) : 0.0;
+        if (k + 1 < iterations && ng > 0) {
+            RIHVR_HIP(holo_scale(c->G, v, c->V, (float)(1.0 / std::sqrt(ng)), c->stream));
+            RIHVR_TRY(volume_norm2(c, v, &nv));
+        }
+    }
+    *L = std::max(ratio, 1e-6);
+    return RIHVR_OK;
+}
+
+rihvr_schedule schedule_of(const rihvr_ctx* c) {
+    rihvr_schedule s{};
+    s.fft_path = c->gen ? 1 : 0;
+    s.max_radix = c->gen_radix;
+    s.zchunk_fwd = c->zchunk_fwd;
+    s.zchunk_adj = c->zchunk_adj;
+    s.prox_rows = c->prox_rows;
+    s.graphs = c->graphs ? 1 : 0;
+    return s;
+}
+
+// Owns a context made by create_impl (also a half-made one) until the scope ends.
+struct CtxGuard {
+    rihvr_ctx* c = nullptr;
+    ~CtxGuard() {
+        if (!c) return;
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->aux) (void)hipStreamSynchronize(c->aux);
+        release(c);
+        delete c;
+    }
+};
+
+// The schedule search: each candidate is a fresh context solving `holo` (device memory) from the
+// zero start, `warm` untimed iterations (enough to capture every graph of the ring's pointer
+// combinations) then `iterations` timed ones between stream synchronisations.
+rihvr_status autotune_impl(const rihvr_params* p, const float* holo, int holo_loc, int iterations, rihvr_schedule* best,
+                           rihvr_schedule* tried, int capacity, int32_t* n_tried) {
+    RIHVR_TRY(validate(p));
+    if (!best || iterations < 1 || capacity < 0 || (capacity > 0 && !tried)) return RIHVR_E_INVALID_ARGUMENT;
+    const size_t N = (size_t)p->width * p->height;
+    float2* hol = nullptr;
+    RIHVR_HIP(hipMalloc((void**)&hol, N * sizeof(float2)));
+    struct Free {
+        float2* q;
+        ~Free() { (void)hipFree(q); }
+    } free_hol{hol};
+    if (holo) {
+        RIHVR_HIP(hipMemcpy(hol, holo, N * sizeof(float2),
+                            holo_loc == RIHVR_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    } else {
+        // A x of a sparse pseudo-random x (about 1 voxel in 1000 non-zero)
+        CtxGuard g;
+        RIHVR_TRY(create_impl(p, nullptr, &g.c));
+        rihvr_ctx* c = g.c;
+        RIHVR_HIP(synth_volume(c->Yb[0], c->V, 1000u, 0x5eedu, c->stream));
+        c->occ_valid[0] = false;
+        RIHVR_HIP(hipMemsetAsync(c->meas, 0, c->N * sizeof(float2), c->stream));
+        RIHVR_HIP(hipMemsetAsync(c->bspec, 0, c->N * sizeof(float2), c->stream));
+        RIHVR_TRY(apply_forward(c, c->Yb[0], c->hE));
+        RIHVR_TRY(spatial_estimate(c, c->hE));
+        RIHVR_HIP(hipMemcpyAsync(hol, c->hspat, N * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
+        RIHVR_HIP(hipStreamSynchronize(c->stream));
+    }
+    const int warm = 12;
+    int count = 0;
+    auto measure = [&](rihvr_schedule& s) -> rihvr_status {
+        CtxGuard g;
+        RIHVR_TRY(create_impl(p, &s, &g.c));
+        rihvr_ctx* c = g.c;
+        RIHVR_TRY(begin_impl(c, (const float*)hol, RIHVR_DEVICE, nullptr, RIHVR_DEVICE, nullptr));
+        RIHVR_TRY(iterate_impl(c, warm, nullptr, false));
+        RIHVR_HIP(hipStreamSynchronize(c->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        RIHVR_TRY(iterate_impl(c, iterations, nullptr, false));
+        RIHVR_HIP(hipStreamSynchronize(c->stream));
+        const auto t1 = std::chrono::steady_clock::now();
+        const rihvr_schedule eff = schedule_of(c);
+        s.fft_path = eff.fft_path;
+        s.zchunk_fwd = eff.zchunk_fwd;
+        s.zchunk_adj = eff.zchunk_adj;
+        s.ms_per_iteration = std::chrono::duration<double, std::milli>(t1 - t0).count() / iterations;
+        if (count < capacity) tried[count] = s;
+        ++count;
+        return RIHVR_OK;
+    };
+    auto same = [](const rihvr_schedule& a, const rihvr_schedule& b) {
+        return a.fft_path == b.fft_path && a.max_radix == b.max_radix && a.zchunk_fwd == b.zchunk_fwd &&
+               a.zchunk_adj == b.zchunk_adj && a.prox_rows == b.prox_rows && a.graphs == b.graphs;
+    };
+    // the starting point: the defaults
+    rihvr_schedule cur{};
+    cur.max_radix = 16;
+    cur.zchunk_fwd = std::min(32, p->num_planes);
+    cur.zchunk_adj = std::min(16, p->num_planes);
+    cur.graphs = 1;
+    const bool fixed = fft_length_supported(p->width) && fft_length_supported(p->height);
+    if (!fixed) cur.fft_path = 1;
+    RIHVR_TRY(measure(cur));
+    auto try_set = [&](std::vector<rihvr_schedule> cands) -> rihvr_status {
+        for (rihvr_schedule s : cands) {
+            if (same(s, cur)) continue;
+            RIHVR_TRY(measure(s));
+            if (s.ms_per_iteration < cur.ms_per_iteration) cur = s;
+        }
+        return RIHVR_OK;
+    };
+    std::vector<rihvr_schedule> v;
+    for (int r : {16, 8, 4}) {
+        rihvr_schedule s = cur;
+        s.fft_path = 1;
+        s.max_radix = r;
+        v.push_back(s);
+    }
+    RIHVR_TRY(try_set(v));
+    const rihvr_schedule base = cur;
+    v.clear();
+    for (int z : {8, 16, 32, 64}) {
+        rihvr_schedule s = base;
+        s.zchunk_fwd = std::min(z, p->num_planes);
+        v.push_back(s);
+    }
+    RIHVR_TRY(try_set(v));
+    v.clear();
+    for (int z : {4, 8, 16, 32}) {
+        rihvr_schedule s = cur;
+        s.zchunk_adj = std::min(z, p->num_planes);
+        v.push_back(s);
+    }
+    RIHVR_TRY(try_set(v));
+    if (p->mode == RIHVR_MODE_FASTA_FUSED_LASSO_2D && prox_fl2d_supported(p->num_tv_iterations)) {
+        v.clear();
+        for (int r : {64, 128, 256}) {
+            rihvr_schedule s = cur;
+            s.prox_rows = std::min(r, p->height);
+            v.push_back(s);
+        }
+        RIHVR_TRY(try_set(v));
+    }
+    v.clear();
+    {
+        rihvr_schedule s = cur;
+        s.graphs = 1 - cur.graphs;
+        v.push_back(s);
+    }
+    RIHVR_TRY(try_set(v));
+    *best = cur;
+    if (n_tried) *n_tried = count;
+    return RIHVR_OK;
+}
+
+// The parameters that fix a context's tables and buffers; items that agree on them share one.
+bool same_geometry(const rihvr_params& a, const rihvr_params& b) {
+    return a.width == b.width && a.height == b.height && a.num_planes == b.num_planes &&
+           std::memcmp(&a.start_plane, &b.start_plane, 4) == 0 &&
+           std::memcmp(&a.plane_stepsize, &b.plane_stepsize, 4) == 0 &&
+           std::memcmp(&a.wavelength, &b.wavelength, 4) == 0 && std::memcmp(&a.resolution, &b.resolution, 4) == 0 &&
+           a.num_tv_iterations == b.num_tv_iterations && a.mode == b.mode;
+}
+
+rihvr_status multi_impl(rihvr_multi_item* items, int n, int holo_loc, int vol_loc, int autotune) {
+    if (n < 0 || (n > 0 && !items) || autotune < 0 || autotune > 2) return RIHVR_E_INVALID_ARGUMENT;
+    for (int i = 0; i < n; ++i) {
+        RIHVR_TRY(validate(&items[i].params));
+        if (!items[i].holo) return RIHVR_E_INVALID_ARGUMENT;
+    }
+    std::vector<char> done(n, 0);
+    for (int i = 0; i < n; ++i) {
+        if (done[i]) continue;
+        rihvr_schedule sch{};
+        if (autotune == 1)
+            RIHVR_TRY(autotune_impl(&items[i].params, items[i].holo, holo_loc, 10, &sch, nullptr, 0, nullptr));
+        else if (autotune == 2)
+            sch = items[i].schedule;
+        CtxGuard g;
+        RIHVR_TRY(create_impl(&items[i].params, autotune ? &sch : nullptr, &g.c));
+        rihvr_ctx* c = g.c;
+        for (int j = i; j < n; ++j) {
+            if (done[j] || !same_geometry(items[i].params, items[j].params)) continue;
+            rihvr_multi_item& it = items[j];
+            // the weights and the iteration count are read per solve
+            c->p.regularization_param = it.params.regularization_param;
+            c->p.regularization_TV_param = it.params.regularization_TV_param;
+            c->p.num_inverse_iterations = it.params.num_inverse_iterations;
+            c->p.zero_padding = it.params.zero_padding;
+            RIHVR_TRY(solve_frame(c, it.holo, holo_loc, nullptr, RIHVR_DEVICE, it.f_target,
+                                  it.params.num_inverse_iterations, &it.stats));
+            if (it.volume) RIHVR_TRY(copy_out(c, it.volume, c->out, vol_loc, c->V));
+            it.schedule = schedule_of(c);
+            it.schedule.ms_per_iteration = autotune ? sch.ms_per_iteration : 0.0;
+            done[j] = 1;
+        }
+    }
     return RIHVR_OK;
 }
 
@@ -1183,7 +1558,7 @@ rihvr_status rihvr_create(const rihvr_params* params, rihvr_ctx** out) {
     if (!out) return RIHVR_E_INVALID_ARGUMENT;
     *out = nullptr;
     rihvr_ctx* c = nullptr;
-    rihvr_status s = create_impl(params, &c);
+    rihvr_status s = create_impl(params, nullptr, &c);
     if (s != RIHVR_OK) {
         if (c) {
             release(c);
@@ -1193,6 +1568,38 @@ rihvr_status rihvr_create(const rihvr_params* params, rihvr_ctx** out) {
     }
     *out = c;
     return RIHVR_OK;
+}
+
+rihvr_status rihvr_estimate_lipschitz(rihvr_ctx* ctx, int32_t seed, int32_t iterations, double* lipschitz) {
+    if (!ctx) return RIHVR_E_INVALID_ARGUMENT;
+    return lipschitz_impl(ctx, seed, iterations, lipschitz);
+}
+
+rihvr_status rihvr_create_scheduled(const rihvr_params* params, const rihvr_schedule* schedule, rihvr_ctx** out) {
+    if (!out) return RIHVR_E_INVALID_ARGUMENT;
+    *out = nullptr;
+    CtxGuard g;
+    RIHVR_TRY(create_impl(params, schedule, &g.c));
+    *out = g.c;
+    g.c = nullptr;
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_get_schedule(const rihvr_ctx* ctx, rihvr_schedule* out) {
+    if (!ctx || !out) return RIHVR_E_INVALID_ARGUMENT;
+    *out = schedule_of(ctx);
+    return RIHVR_OK;
+}
+
+rihvr_status rihvr_autotune(const rihvr_params* params, const float* holo, int holo_loc, int32_t iterations,
+                            rihvr_schedule* best, rihvr_schedule* tried, int32_t capacity, int32_t* n_tried) {
+    if (n_tried) *n_tried = 0;
+    return autotune_impl(params, holo, holo_loc, iterations, best, tried, capacity, n_tried);
+}
+
+rihvr_status rihvr_reconstruct_multi(rihvr_multi_item* items, int32_t n_items, int holo_loc, int vol_loc,
+                                     int32_t autotune) {
+    return multi_impl(items, n_items, holo_loc, vol_loc, autotune);
 }
 
 rihvr_status rihvr_destroy(rihvr_ctx* ctx) {

@@ -75,8 +75,28 @@ def propagate_sum(masks, optics: Optics):
     return _ifft2(acc)
 
 
-def hologram_from_masks(masks, optics: Optics, sigma: float, rng):
-    U = 1.0 + propagate_sum(masks, optics)
+def propagate_sum_torch(masks, optics: Optics):
+    """propagate_sum evaluated with torch on the GPU in complex128 (the same sum; for large
+    synthetic batches, where the host FFTs take seconds per hologram)."""
+    import torch
+    H, W = optics.height, optics.width
+    lam, dx = float(np.float32(optics.wavelength)), float(np.float32(optics.resolution))
+    fx = torch.fft.fftfreq(W, dtype=torch.float64, device="cuda")[None, :]
+    fy = torch.fft.fftfreq(H, dtype=torch.float64, device="cuda")[:, None]
+    root = torch.sqrt(torch.clamp(1.0 - (fx * fx + fy * fy) * (lam / dx) ** 2, min=0.0))
+    occupied = np.flatnonzero(masks.reshape(masks.shape[0], -1).any(1))
+    zs = torch.from_numpy(optics.planes()[occupied]).cuda()
+    acc = torch.zeros((H, W), dtype=torch.complex128, device="cuda")
+    for k0 in range(0, len(occupied), 16):
+        idx = occupied[k0:k0 + 16]
+        m = torch.from_numpy(np.ascontiguousarray(masks[idx])).cuda().to(torch.float64)
+        ph = (2 * np.pi / lam) * zs[k0:k0 + 16, None, None] * (root - 1.0)[None]
+        acc += (torch.fft.fft2(-m) * torch.polar(torch.ones_like(ph), ph)).sum(0)
+    return torch.fft.ifft2(acc).cpu().numpy()
+
+
+def hologram_from_masks(masks, optics: Optics, sigma: float, rng, propagate=None):
+    U = 1.0 + (propagate or propagate_sum)(masks, optics)
     inten = np.abs(U) ** 2 + rng.normal(0.0, sigma, size=U.shape)
     b = inten / inten.mean() - 1.0
     return b.astype(np.float32).astype(np.complex64)
@@ -95,7 +115,7 @@ def disks_scene(optics: Optics, n_disks: int, rmin: float, rmax: float, seed: in
 
 
 def tracers_rods_scene(optics: Optics, n_tracers: int, n_rods: int, seed: int, sigma: float,
-                       rod_len=(20.0, 80.0), rod_width=3.0, tracer_diam=(1.0, 2.0)):
+                       rod_len=(20.0, 80.0), rod_width=3.0, tracer_diam=(1.0, 2.0), propagate=None):
     """C2-style scene: point tracers plus rods of random 3D orientation, the rods
     rasterised into per-plane occluding slices (discs of the rod width sampled
     every half pixel along the axis; z in plane units = pixels * dx / dz)."""
@@ -118,7 +138,7 @@ def tracers_rods_scene(optics: Optics, n_tracers: int, n_rods: int, seed: int, s
             zi = int(np.floor(cz + s * v[2] * z_per_px))
             if 0 <= zi < Nz:
                 _draw_disc(masks[zi], x, y, rod_width / 2)
-    return hologram_from_masks(masks, optics, sigma, rng), masks
+    return hologram_from_masks(masks, optics, sigma, rng, propagate), masks
 
 
 class FlowField:
@@ -152,7 +172,8 @@ class FlowField:
 
 
 def sequence(optics: Optics, n_frames: int, n_tracers: int, n_rods: int, seed: int, sigma: float,
-             rms=3.0, return_tracks=False, tracer_diam=(1.0, 2.0), rod_len=(20.0, 80.0), rod_width=3.0):
+             rms=3.0, return_tracks=False, tracer_diam=(1.0, 2.0), rod_len=(20.0, 80.0), rod_width=3.0,
+             propagate=None):
     """Time sequence of holograms: tracers and rods (fixed depth and orientation)
     advected between frames by a FlowField.  Returns holograms [n][H][W] complex64
     (and the tracer tracks [n][n_tracers][3] = x, y, plane with return_tracks)."""
@@ -178,7 +199,7 @@ def sequence(optics: Optics, n_frames: int, n_tracers: int, n_rods: int, seed: i
                 zi = int(np.floor(rz[i] + s_ * rv[i, 2] * z_per_px))
                 if 0 <= zi < Nz:
                     _draw_disc(masks[zi], rx[i] + s_ * rv[i, 0], ry[i] + s_ * rv[i, 1], rod_width / 2)
-        holos[f] = hologram_from_masks(masks, optics, sigma, rng)
+        holos[f] = hologram_from_masks(masks, optics, sigma, rng, propagate)
         tracks[f] = np.stack([tx, ty, tz.astype(np.float64)], 1)
         u, v = flow(tx, ty)
         tx, ty = (tx + u) % W, (ty + v) % H

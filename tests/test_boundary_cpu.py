@@ -24,7 +24,7 @@ def test_library_exports_every_declared_function(pkg):
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, missing
     assert set(declared_functions()) == set(pkg.EXPORTED)
-    assert L.rihvr_abi_version() == 3
+    assert L.rihvr_abi_version() == 4
 
 
 def test_status_strings_cover_holoerror_codes(pkg):
@@ -38,12 +38,55 @@ def test_status_strings_cover_holoerror_codes(pkg):
         assert re.search(rf"{name}\s*=\s*{val}\b", HEADER)
 
 
+def test_struct_layouts_match_the_c_compiler(tmp_path, pkg):
+    """sizeof and every field's offsetof, as gcc lays the header's structs out, against the
+    ctypes mirrors."""
+    import subprocess
+    mirrors = {"rihvr_params": pkg.Params, "rihvr_stats": pkg.Stats, "rihvr_frame_stats": pkg.FrameStats,
+               "rihvr_schedule": pkg.Schedule, "rihvr_multi_item": pkg.MultiItem,
+               "rihvr_learned_stats": pkg.LearnedStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rihvr.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = pathlib.Path(__file__).resolve().parents[1] / "include"
+    subprocess.run(["gcc", "-std=c11", "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (ln.split() for ln in out if ln)}
+    for cname, py in mirrors.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
+
+
 def test_null_arguments_are_rejected_without_a_gpu(pkg):
     L = pkg.lib()
     out = ctypes.c_void_p()
     assert L.rihvr_create(None, ctypes.byref(out)) == 7
     assert L.rihvr_destroy(None) == 0
     assert L.rihvr_iterate(None, 1, None) == 7
+    # schedules out of range, the autotuner's and the batched call's arguments (host checks)
+    p = pkg.make_params(64, 64, 4, 1000.0, 50.0, 0.633, 2.0, 5, 0.01, 0.01, 5, 7)
+    for bad in (dict(fft_path=2), dict(max_radix=32), dict(max_radix=2), dict(zchunk_fwd=-1), dict(graphs=3),
+                dict(prox_rows=-5)):
+        s = pkg.Schedule.make(**bad)
+        assert L.rihvr_create_scheduled(ctypes.byref(p), ctypes.byref(s), ctypes.byref(out)) == 7, bad
+        assert not out
+    best = pkg.Schedule()
+    assert L.rihvr_autotune(ctypes.byref(p), None, 0, 0, ctypes.byref(best), None, 0, None) == 7
+    assert L.rihvr_autotune(ctypes.byref(p), None, 0, 5, None, None, 0, None) == 7
+    assert L.rihvr_autotune(ctypes.byref(p), None, 0, 5, ctypes.byref(best), None, 4, None) == 7
+    assert L.rihvr_reconstruct_multi(None, 1, 0, 0, 0) == 7
+    assert L.rihvr_reconstruct_multi(None, 0, 0, 0, 3) == 7
+    assert L.rihvr_estimate_lipschitz(None, 0, 1, None) == 7
+    item = (pkg.MultiItem * 1)()
+    item[0].params = p
+    assert L.rihvr_reconstruct_multi(item, 1, 0, 0, 0) == 7  # no hologram
 
 
 def _struct_fields(name):

@@ -104,3 +104,19 @@ def test_backtracks_with_work_queued_ahead(pkg, frames, gpu_available, monkeypat
     with pkg.SparseCompressiveHolo(pkg.make_params(**c12)) as rec:
         plain = rec.inverseReconstruct(holos[0])
     assert plain["total_backtracks"] == 0
+
+
+def test_scene_propagation_on_the_gpu_matches_the_host(scenes, gpu_available):
+    """The batched workloads' holograms come from propagate_sum_torch (complex128 on the GPU):
+    the same sum as the host's propagate_sum, to 1e-12 relative, and the same seeded holograms."""
+    c = cfg(96, 80, 10)
+    o = scenes.optics_of(c)
+    rng = np.random.default_rng(5)
+    masks = (rng.random((10, 80, 96)) < 0.01).astype(np.float32)
+    masks[3] = 0  # an empty plane is skipped by both
+    a = scenes.propagate_sum(masks, o)
+    b = scenes.propagate_sum_torch(masks, o)
+    assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(a)
+    h0, _ = scenes.tracers_rods_scene(o, 30, 2, 17, 0.02)
+    h1, _ = scenes.tracers_rods_scene(o, 30, 2, 17, 0.02, propagate=scenes.propagate_sum_torch)
+    assert np.max(np.abs(h0 - h1)) <= 1e-6

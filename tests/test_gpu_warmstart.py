@@ -190,8 +190,8 @@ def _small_cfg():
 def test_zero_network_start_equals_the_plain_solve(pkg, scenes, gpu_available):
     """A network whose output is zero everywhere (all parameters 0) proposes the zero start: its
     objective is the zero start's 0.5 ||b||^2, the A^T b start has zero data residual (A A^T =
-    Nz I on a real hologram), the decision follows the rule, and either way the solve equals the
-    plain solver's solve to the same target bit for bit."""
+    Nz I on a real hologram), the ray alpha x_net has no direction (alpha = 0), so the zero start
+    is taken, and the solve equals the plain solver's solve to the same target bit for bit."""
     import torch
     c = _small_cfg()
     holo, _ = scenes.disks_scene(scenes.optics_of(c), 8, 2.0, 4.0, 11, 0.01)
@@ -209,20 +209,30 @@ def test_zero_network_start_equals_the_plain_solve(pkg, scenes, gpu_available):
     assert abs(st["f_zero"] - b2) <= 1e-6 * b2
     assert abs(st["f_net"] - b2) <= 1e-6 * b2 and abs(st["resid_net"] - b2) <= 1e-6 * b2
     assert st["resid_adjoint"] <= 1e-9 * b2
+    assert st["alpha"] == 0.0 and st["f_scaled"] == st["f_zero"]
     assert st["fallback"] == (0 if st["f_net"] <= min(st["f_adjoint"], st["f_zero"]) else 1)
     assert st["iterations"] == ref[0]["iterations"]
     assert np.array_equal(vol, vref)
 
 
 def test_learned_start_decision_follows_the_residual_rule(pkg, scenes, gpu_available):
-    """With a randomly initialised network the returned decision is the rule's (the network's
-    start is kept when its residual is not above the A^T b start's), and without a target the
-    solve runs max_iterations."""
+    """With a randomly initialised network the returned decision is the rule's (x_net when its
+    objective is not above the A^T b start's nor the zero start's, else alpha x_net when alpha > 0
+    and its objective is not above them, else the zero start); the closed-form objective at
+    alpha x_net is not above the objectives at 0 and at x_net; the objective the solver itself
+    computes at the chosen start (a forward pass of it) equals (1e-4) the one the decision used,
+    which pins the closed form when alpha x_net is chosen; without a target the solve runs
+    max_iterations."""
     import torch
     c = _small_cfg()
     holo, _ = scenes.disks_scene(scenes.optics_of(c), 8, 2.0, 4.0, 12, 0.01)
     with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec, pkg.WarmStartNet(16, 64, 64, seed=3) as net:
         st = rec.reconstructLearned(net, torch.from_numpy(holo).cuda(), f_target=0.0, max_iterations=5)
     print(st, flush=True)
-    assert st["fallback"] == (0 if st["f_net"] <= min(st["f_adjoint"], st["f_zero"]) else 1)
+    lim = min(st["f_adjoint"], st["f_zero"])
+    rule = 0 if st["f_net"] <= lim else (2 if st["alpha"] > 0 and st["f_scaled"] <= lim else 1)
+    assert st["fallback"] == rule
+    assert st["f_scaled"] <= st["f_zero"] * (1 + 1e-9) and st["f_scaled"] <= st["f_net"] * (1 + 1e-9)
+    f_chosen = {0: st["f_net"], 1: st["f_zero"], 2: st["f_scaled"]}[st["fallback"]]
+    assert abs(st["f_start"] - f_chosen) <= 1e-4 * f_chosen
     assert st["iterations"] == 5
