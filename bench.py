@@ -156,7 +156,8 @@ def frames_leg(pkg, scenes, cfg, n_frames: int, tol: float = 1e-3, seed: int = 2
     it_cold = sum(f["iterations"] for f in cold[1:])
     it_warm = sum(f["iterations"] for f in warm[1:])
     lrn = leg(learned, t_learned)
-    lrn.update(fallback=[f["fallback"] for f in learned], alpha=[round(f["alpha"], 4) for f in learned],
+    lrn.update(fallback=[f["fallback"] for f in learned], tau=[f["tau"] for f in learned],
+               f_thresholded=[f["f_thresholded"] for f in learned], f_net=[f["f_net"] for f in learned],
                ms_network=[f["ms_network"] for f in learned], ms_adjoint=[f["ms_adjoint"] for f in learned],
                ms_solve=[f["ms_solve"] for f in learned])
     return {

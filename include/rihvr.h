@@ -646,28 +646,31 @@ rihvr_status rihvr_conv3d_bf16_wgrad(const uint16_t* in0, int32_t c0, const uint
 /* Result of rihvr_reconstruct_learned. */
 typedef struct rihvr_learned_stats {
     int32_t iterations;     /* FISTA iterations run from the chosen start */
-    int32_t fallback;       /* 0: the network's start x_net, 2: alpha x_net, 1: the zero start */
+    int32_t fallback;       /* 0: the network's start soft(x_net, tau), 1: the zero start */
     int32_t reached;        /* 1: the objective target was reached */
     int32_t total_backtracks;
     double  resid_net;      /* 0.5 ||A x_net - b||^2 of the network's start */
     double  resid_adjoint;  /* 0.5 ||A (t A^T b) - b||^2 of the A^T b start, t = 1/Nz (zero for a
                                real hologram: A A^T = Nz I on real holograms) */
-    double  f_net;          /* objective (calcObjectiveFunction) at the network's start */
+    double  f_net;          /* objective (calcObjectiveFunction) at the network's output x_net */
     double  f_adjoint;      /* objective at the A^T b start */
     double  f_zero;         /* objective at the zero start, 0.5 ||b||^2 */
     double  f_start;        /* objective at the chosen start */
     double  f_final;        /* objective at the returned volume */
     double  ms_adjoint;     /* wall time of A^T b */
-    double  ms_network;     /* wall time of the network's prediction and the residual test */
+    double  ms_network;     /* wall time of the network's prediction, the threshold choice and the
+                               start test */
     double  ms_solve;       /* wall time of FISTA from the chosen start */
-    double  alpha;          /* the objective along the ray alpha x_net (alpha >= 0) is least here */
-    double  f_scaled;       /* objective at alpha x_net */
+    double  tau;            /* the soft threshold applied to x_net: q max|x_net|, q in {0.01, 0.02,
+                               0.05}, the one whose start has the least objective */
+    double  f_thresholded;  /* objective at soft(x_net, tau) */
 } rihvr_learned_stats;
 
-/* Reconstructs one hologram with the network's start: A^T b -> network -> x_net.  x_net is the
- * start when its objective is not above the objectives at the A^T b start (t A^T b, t = 1/Nz)
- * and at the zero start; else alpha x_net, with alpha >= 0 the minimiser of the objective along
- * the ray (in closed form), when alpha > 0 and its objective is not above both; else the zero
+/* Reconstructs one hologram with the network's start: A^T b -> network -> x_net ->
+ * soft(x_net, tau) (complex soft threshold; tau = q max|x_net| for the q in {0.01, 0.02, 0.05}
+ * whose start has the least objective: the network's output holds small values at every voxel,
+ * which FISTA's first prox would remove).  That start is used when its objective is not above
+ * the objectives at the A^T b start (t A^T b, t = 1/Nz) and at the zero start; else the zero
  * start of the reference (fallback).  Then FISTA (the context's mode and parameters)
  * until the objective at the momentum point is <= f_target (f_target <= 0: no target) or
  * max_iterations (<= 0: num_inverse_iterations).  The network must have been created for the

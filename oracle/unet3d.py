@@ -111,6 +111,50 @@ def conv_wgrad(x, g):
     return dw, gv.sum(0)
 
 
+# Nearest upsampling followed by a 3x3x3 convolution, per output parity class: along one axis an
+# output at parity p reads the low-resolution voxels (v >> 1) + p + o - 1, o in {0, 1}, with the
+# taps S[(p, o)] summed (p = 0: o = 0 takes tap -1, o = 1 taps 0 and +1; p = 1: o = 0 taps -1 and
+# 0, o = 1 tap +1): 8 taps on the low-resolution grid instead of 27 on the fine one.
+PARITY_TAPS = {(0, 0): (-1,), (0, 1): (0, 1), (1, 0): (-1, 0), (1, 1): (1,)}
+
+
+def parity_weights(w, c0, rounding=True):
+    """Combined weights of the upsampled channels (c0: ...) of w [27][Cout][Cin], summed in
+    float64 from w's values and rounded to BF16 (rounding): [pz][py][px][oz][oy][ox][Cout][Cin - c0]."""
+    w = np.asarray(w, np.float64)
+    out = np.zeros((2, 2, 2, 2, 2, 2, w.shape[1], w.shape[2] - c0))
+    for pz in range(2):
+        for py in range(2):
+            for px in range(2):
+                for oz in range(2):
+                    for oy in range(2):
+                        for ox in range(2):
+                            acc = np.zeros(out.shape[6:])
+                            for tz in PARITY_TAPS[(pz, oz)]:
+                                for ty in PARITY_TAPS[(py, oy)]:
+                                    for tx in PARITY_TAPS[(px, ox)]:
+                                        acc += w[(tz + 1) * 9 + (ty + 1) * 3 + (tx + 1), :, c0:]
+                            out[pz, py, px, oz, oy, ox] = acc
+    return bf16(out) if rounding else out
+
+
+def conv_up_parity(xl, wp):
+    """conv(up(xl), w) restricted to the upsampled channels, with parity_weights wp (xl: the
+    low-resolution [D/2][H/2][W/2][C1])."""
+    Dl, Hl, Wl, _ = xl.shape
+    out = np.zeros((2 * Dl, 2 * Hl, 2 * Wl, wp.shape[6]))
+    for pz in range(2):
+        for py in range(2):
+            for px in range(2):
+                acc = np.zeros((Dl, Hl, Wl, wp.shape[6]))
+                for oz in range(2):
+                    for oy in range(2):
+                        for ox in range(2):
+                            acc += shift(xl, pz + oz - 1, py + oy - 1, px + ox - 1) @ wp[pz, py, px, oz, oy, ox].T
+                out[pz::2, py::2, px::2] = acc
+    return out
+
+
 def param_count():
     n = 0
     for _, c0, c1, co in LAYERS:
@@ -137,8 +181,10 @@ def pack(layer_grads, head_grads):
     return np.concatenate(parts)
 
 
-def forward(params, vol, in_scale, out_scale, rounding=True):
-    """Network output (complex [D][H][W]) and the activations, for volume vol [D][H][W] complex."""
+def forward(params, vol, in_scale, out_scale, rounding=True, parity=False):
+    """Network output (complex [D][H][W]) and the activations, for volume vol [D][H][W] complex.
+    parity: the decoder convolutions take their upsampled channels through parity_weights (the
+    inference path), else through the 27 BF16 taps on the upsampled grid (the training path)."""
     r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
     layers, (wo, bo) = unpack(params)
     wl = [(r(w), b) for w, b in layers]
@@ -161,9 +207,17 @@ def forward(params, vol, in_scale, out_scale, rounding=True):
     for k in range(1, LEVELS):
         p.append(r(pool(e[k - 1])))
         e.append(layer(k, p[k]))
-    d2 = layer(4, np.concatenate([e[2], up(e[3])], axis=3))
-    d1 = layer(5, np.concatenate([e[1], up(d2)], axis=3))
-    d0 = layer(6, np.concatenate([e[0], up(d1)], axis=3))
+    def dec(i, skip, low):
+        if not parity:
+            return layer(i, np.concatenate([skip, up(low)], axis=3))
+        w, b = wl[i]
+        c0 = skip.shape[3]
+        wp = parity_weights(layers[i][0], c0, rounding)
+        return r(np.maximum(conv(skip, w[:, :, :c0], b) + conv_up_parity(low, wp), 0.0))
+
+    d2 = dec(4, e[2], e[3])
+    d1 = dec(5, e[1], d2)
+    d0 = dec(6, e[0], d1)
     o = d0 @ wo.T + bo
     act.update(e=e, p=p, d=[d0, d1, d2], o=o)
     return out_scale * (o[..., 0] + 1j * o[..., 1]), act

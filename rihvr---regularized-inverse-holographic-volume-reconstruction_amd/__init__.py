@@ -226,8 +226,8 @@ class LearnedStats(ctypes.Structure):
         ("ms_adjoint", ctypes.c_double),
         ("ms_network", ctypes.c_double),
         ("ms_solve", ctypes.c_double),
-        ("alpha", ctypes.c_double),
-        ("f_scaled", ctypes.c_double),
+        ("tau", ctypes.c_double),
+        ("f_thresholded", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -1111,6 +1111,48 @@ hipError_t norm2_partials(const float2* v, long n, int nblocks, double* partial,
     return hipGetLastError();
 }
 
+// max |v| over n complex values: nblocks partial maxima (grid-stride), partial[b]
+__global__ void __launch_bounds__(kEwThreads) k_maxabs(const float2* __restrict__ v, long n, double* partial) {
+    __shared__ float red[kEwThreads / 64];
+    float m = 0.0f;
+    for (long i = (long)blockIdx.x * kEwThreads + threadIdx.x; i < n; i += (long)gridDim.x * kEwThreads) {
+        const float2 e = v[i];
+        m = fmaxf(m, e.x * e.x + e.y * e.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = red[0];
+        for (int w = 1; w < kEwThreads / 64; ++w) r = fmaxf(r, red[w]);
+        partial[blockIdx.x] = sqrt((double)r);
+    }
+}
+hipError_t maxabs_partials(const float2* v, long n, int nblocks, double* partial, hipStream_t s) {
+    hipLaunchKernelGGL(k_maxabs, dim3(nblocks), dim3(kEwThreads), 0, s, v, n, partial);
+    return hipGetLastError();
+}
+
+// complex soft threshold: out = in * max(0, 1 - tau / |in|), +0 where |in| <= tau
+__global__ void __launch_bounds__(kEwThreads) k_soft(const float2* __restrict__ in, float2* __restrict__ out, long n,
+                                                    float tau) {
+    const long i = (long)blockIdx.x * kEwThreads + threadIdx.x;
+    if (i >= n) return;
+    const float2 e = in[i];
+    const float m = sqrtf(e.x * e.x + e.y * e.y);
+    float2 r = make_float2(0.0f, 0.0f);
+    if (m > tau) {
+        const float sc = (m - tau) / m;
+        r = make_float2(e.x * sc, e.y * sc);
+    }
+    out[i] = r;
+}
+hipError_t soft_threshold(const float2* in, float2* out, long n, float tau, hipStream_t s) {
+    hipLaunchKernelGGL(k_soft, dim3((unsigned)((n + kEwThreads - 1) / kEwThreads)), dim3(kEwThreads), 0, s, in, out,
+                       n, tau);
+    return hipGetLastError();
+}
+
 hipError_t holo_residual(const float2* est, const float2* meas, float2* resid, long n, double* partial,
                          hipStream_t s) {
     hipLaunchKernelGGL(k_holo_residual, dim3(holo_num_blocks(n)), dim3(kEwThreads), 0, s, est, meas, resid, n,

@@ -1195,12 +1195,14 @@ rihvr_status batch_impl(rihvr_ctx* c, const float* holos, int holo_loc, int n_fr
 }
 
 // The learned start: A^T b (the adjoint of the measurement, SparseGradient::reconstruct with
-// r = b) -> network -> x_net in a ring slot.  The objective (calcObjectiveFunction) at x_net is
-// compared with the objectives at the A^T b start t A^T b (t = 1/Nz; its data residual is zero
-// for a real hologram since A A^T = Nz I there) and at the reference's zero start (0.5 ||b||^2):
-// x_net is used when it is below both; else alpha x_net, alpha >= 0 the minimiser of the
-// objective along the ray (closed form: the data term is quadratic in alpha, the regularisers
-// linear), when alpha > 0 and it is below both; else the zero start.  Then FISTA to the target.
+// r = b) -> network -> x_net in a ring slot -> soft(x_net, tau) in another, tau = q max|x_net|
+// for the q of kSoftFractions whose start has the least objective (calcObjectiveFunction).
+// That objective is compared with the objectives at the A^T b start t A^T b (t = 1/Nz; its
+// data residual is zero for a real hologram since A A^T = Nz I there) and at the reference's
+// zero start (0.5 ||b||^2): the thresholded start is used when it is below both, else the zero
+// start.  Then FISTA to the target.
+constexpr double kSoftFractions[] = {0.01, 0.02, 0.05};
+
 rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int holo_loc, double f_target,
                           int max_iterations, rihvr_learned_stats* out) {
     if (!holo || !rihvr::net::net_matches(net, c->Nz, c->H, c->W)) return RIHVR_E_INVALID_ARGUMENT;
@@ -1249,33 +1251,36 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
         return RIHVR_OK;
     };
     RIHVR_TRY(volume_objective(xw, &st.resid_net, &st.f_net));
-    // ||A x_net||^2 from the spectrum the objective left in hE (hC is still zero)
-    RIHVR_HIP(holo_residual(c->hE, c->hC, c->htmp, c->N, c->partial_holo, c->stream));
-    RIHVR_TRY(reduce_partials_on(c, c->partial_holo, holo_num_blocks(c->N), 1, c->scal + SL_RESID_M));
-    RIHVR_TRY(fetch_scalars(c));
-    const double aa = c->h_scal[SL_RESID_M];
-    // The objective along the ray alpha x_net (alpha >= 0; the L1 and TV terms are positively
-    // homogeneous): f(alpha) = 0.5 (alpha^2 aa - 2 alpha p + bb) + alpha R, with p = <A x_net, b>
-    // = (aa + bb - ||A x_net - b||^2) / 2 and R = f_net - 0.5 ||A x_net - b||^2, is least at
-    // alpha = max(0, (p - R) / aa); f(alpha) <= f(0) = f_zero and <= f(1) = f_net.
-    const double bb = 2.0 * st.f_zero, rr = 2.0 * st.resid_net;
-    const double pp = 0.5 * (aa + bb - rr), R = st.f_net - st.resid_net;
-    st.alpha = aa > 0 ? std::max(0.0, (pp - R) / aa) : 0.0;
-    st.f_scaled = 0.5 * (st.alpha * st.alpha * aa - 2.0 * st.alpha * pp + bb) + st.alpha * R;
+    // The network's output is dense (every voxel holds a small value), which the objective's L1
+    // and TV terms price although FISTA's first prox removes it.  The candidate starts are
+    // soft(x_net, q max|x_net|), q in kSoftFractions: the one of least objective is the
+    // network's start.
+    const int nb = (int)std::max<long>(1, std::min<long>(256, c->partial_rows * PS_COUNT));
+    RIHVR_HIP(maxabs_partials(xw, c->V, nb, c->partial, c->stream));
+    std::vector<double> mx(nb);
+    RIHVR_HIP(hipMemcpyAsync(mx.data(), c->partial, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RIHVR_HIP(hipStreamSynchronize(c->stream));
+    const double xmax = *std::max_element(mx.begin(), mx.end());
+    float2* xs = c->Yb[2];
+    invalidate_occ(c, xs);
+    st.f_thresholded = HUGE_VAL;
+    for (double q : kSoftFractions) {
+        const float tau = (float)(q * xmax);
+        RIHVR_HIP(soft_threshold(xw, xs, c->V, tau, c->stream));
+        double r = 0, f = 0;
+        RIHVR_TRY(volume_objective(xs, &r, &f));
+        if (f < st.f_thresholded) {
+            st.f_thresholded = f;
+            st.tau = tau;
+        }
+    }
     RIHVR_HIP(holo_scale(c->G, c->G_alt, c->V, (float)(1.0 / (double)c->Nz), c->stream));
     RIHVR_TRY(volume_objective(c->G_alt, &st.resid_adjoint, &st.f_adjoint));
-    // x_net as predicted when it is below both other starts (shrinking a good prediction costs
-    // iterations later), else alpha x_net when that is, else the zero start
-    if (st.f_net <= st.f_adjoint && st.f_net <= st.f_zero)
-        st.fallback = 0;
-    else if (st.alpha > 0 && st.f_scaled <= st.f_adjoint && st.f_scaled <= st.f_zero)
-        st.fallback = 2;
-    else
-        st.fallback = 1;
-    if (st.fallback == 2) RIHVR_HIP(holo_scale(xw, xw, c->V, (float)st.alpha, c->stream));
+    st.fallback = (st.f_thresholded <= st.f_adjoint && st.f_thresholded <= st.f_zero) ? 0 : 1;
+    if (!st.fallback) RIHVR_HIP(soft_threshold(xw, xs, c->V, (float)st.tau, c->stream));
     const auto t2 = clk::now();
     rihvr_frame_stats fs{};
-    RIHVR_TRY(solve_frame(c, holo, holo_loc, st.fallback == 1 ? nullptr : (const float*)xw, RIHVR_DEVICE, f_target,
+    RIHVR_TRY(solve_frame(c, holo, holo_loc, st.fallback ? nullptr : (const float*)xs, RIHVR_DEVICE, f_target,
                           max_it, &fs));
     const auto t3 = clk::now();
     st.iterations = fs.iterations;

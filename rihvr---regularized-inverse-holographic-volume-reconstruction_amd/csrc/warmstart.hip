@@ -83,18 +83,30 @@ __device__ __forceinline__ s4 load_in4(const unsigned short* __restrict__ in0, c
 // and B fragments from LDS.  ROWS = 2 (for CB = 16, whose stages hold little MFMA work) doubles
 // the work per stage.  Grid: (ceil(W/64) * Cout/CB, ceil(H/(4 ROWS)), D).
 // OUTF32: raw FP32 sums (input gradient), else BF16 with bias and optional ReLU.
+// PAR (inference of the decoder layers): in1's channels go through the parity-class form of
+// the upsampled convolution.  An output voxel at parity (pz, py, px) reads in1 at the half-
+// resolution voxels (v >> 1) + p + o - 1, o in {0, 1}^3, with wpar's combined weights (the
+// taps that land on one half-resolution voxel summed: 8 taps instead of 27).  So that an
+// M-tile's 16 voxels share their x parity, its voxels are x0 + 32 (m&3 >> 1) + 2 r + (m & 1)
+// (r = 0..15) in every phase; per (chunk, oz) the (TY/2 + 2) x 34 half-resolution halo of
+// plane (z >> 1) + pz + oz - 1 is staged with the 16 weight slabs (py, px, oy, ox) of that oz.
 constexpr int kHX = 66;  // halo columns: 64 + 2
-template <bool OUTF32, int ROWS>
+constexpr int kHX2 = 34;  // half-resolution halo columns: 32 + 2
+template <bool OUTF32, int ROWS, bool PAR>
 __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* __restrict__ in0,
                                                           const unsigned short* __restrict__ in1, ConvGeo g,
                                                           const unsigned short* __restrict__ w,
+                                                          const unsigned short* __restrict__ wpar,
                                                           const float* __restrict__ bias, int relu,
                                                           void* __restrict__ out) {
     constexpr int TY = 4 * ROWS, HY = TY + 2, HALO = HY * kHX;
+    constexpr int HY2 = TY / 2 + 2, HALO2 = HY2 * kHX2;
+    static_assert(HALO2 <= HALO, "the half-resolution halo reuses the halo buffer");
     constexpr int MT = 4 * ROWS;                       // M-tiles per wave
     constexpr int NTMAX = ROWS == 1 ? 4 : 1;           // N-tiles per wave at most
+    constexpr int SLABS = PAR ? 16 : 9;
     __shared__ uint4 halo[HALO * 2];     // [row][x][16 ch] as two 16-byte halves per voxel
-    __shared__ s4 wl[9 * kMaxCB * 4];    // [tap of the slab][co][4 groups of 4 ci]
+    __shared__ s4 wl[SLABS * kMaxCB * 4];  // [tap of the slab][co][4 groups of 4 ci]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = lane >> 4, r16 = lane & 15;
     const int nxt = (g.W + 63) >> 6;
@@ -111,8 +123,71 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
         for (int n = 0; n < NTMAX; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     const int nchunks = Cin >> 4;
     const unsigned short* hs = (const unsigned short*)halo;
+    // column of M-tile m's row r inside the 64-wide tile
+    auto xcol = [](int m, int r) { return PAR ? 32 * ((m & 3) >> 1) + 2 * r + (m & 1) : 16 * (m & 3) + r; };
     for (int c = 0; c < nchunks; ++c) {
         const bool hi = c * 16 >= g.C0;
+        if (PAR && hi) {
+            const int pz = z & 1, Dl = g.D >> 1;
+            const int yl0 = (y0 >> 1) - 1, xl0 = (x0 >> 1) - 1;
+            const int ci0 = c * 16 - g.C0;
+            for (int oz = 0; oz < 2; ++oz) {
+                const int zl = (z >> 1) + pz + oz - 1;
+                if (zl < 0 || zl >= Dl) continue;  // zeros (uniform)
+                __syncthreads();
+                constexpr int kPer2 = (HALO2 * 2 + kFwdThreads - 1) / kFwdThreads;
+                uint4 v[kPer2];
+#pragma unroll
+                for (int k = 0; k < kPer2; ++k) {
+                    const int e = threadIdx.x + k * kFwdThreads;
+                    const int half = e & 1, vox = e >> 1;
+                    const int rx = vox % kHX2, ry = vox / kHX2;
+                    const int yy = yl0 + ry, xx = xl0 + rx;
+                    v[k] = make_uint4(0, 0, 0, 0);
+                    if (e < HALO2 * 2 && yy >= 0 && yy < H2 && xx >= 0 && xx < W2)
+                        v[k] = *(const uint4*)(in1 + (((size_t)zl * H2 + yy) * W2 + xx) * g.C1 + ci0 + half * 8);
+                }
+#pragma unroll
+                for (int k = 0; k < kPer2; ++k) {
+                    const int e = threadIdx.x + k * kFwdThreads;
+                    if (e < HALO2 * 2) halo[e] = v[k];
+                }
+                // slab sl = (py, px, oy, ox) of this (pz, oz): wpar[pz][py][px][oz][oy][ox][co][ci]
+                for (int e = threadIdx.x; e < 16 * g.CB * 4; e += kFwdThreads) {
+                    const int qq = e & 3, co = (e >> 2) % g.CB, sl = (e >> 2) / g.CB;
+                    const int cls = (pz * 2 + (sl >> 3)) * 2 + ((sl >> 2) & 1);
+                    const int tap = oz * 4 + (sl & 3);
+                    wl[(sl * kMaxCB + co) * 4 + qq] =
+                        *(const s4*)(wpar + ((size_t)(cls * 8 + tap) * g.Cout + co0 + co) * g.C1 + ci0 + qq * 4);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int oo = 0; oo < 4; ++oo) {
+                    const int oy = oo >> 1, ox = oo & 1;
+                    s4 a[MT];
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) {
+                        const int yr = wave * ROWS + (m >> 2);  // y0 is even, so y's parity is yr's
+                        const int hrow = (yr >> 1) + (yr & 1) + oy;
+                        const int hcol = 16 * ((m & 3) >> 1) + r16 + (m & 1) + ox;
+                        a[m] = *(const s4*)(hs + (size_t)(hrow * kHX2 + hcol) * 16 + q * 4);
+                    }
+#pragma unroll
+                    for (int n = 0; n < NTMAX; ++n) {
+                        if (n < NT) {
+#pragma unroll
+                            for (int m = 0; m < MT; ++m) {
+                                const int yr = wave * ROWS + (m >> 2);
+                                const int sl = ((yr & 1) * 2 + (m & 1)) * 4 + oo;
+                                const s4 b = wl[(sl * kMaxCB + n * 16 + r16) * 4 + q];
+                                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n], 0, 0, 0);
+                            }
+                        }
+                    }
+                }
+            }
+            continue;
+        }
         for (int dz = -1; dz <= 1; ++dz) {
             const int zz = z + dz;
             if (zz < 0 || zz >= g.D) continue;  // a plane outside the grid adds zeros (uniform)
@@ -153,7 +228,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
                     const int row = (wave * ROWS + (m >> 2) + dy + 1) * kHX;
-                    a[m] = *(const s4*)(hs + (size_t)(row + (m & 3) * 16 + r16 + dx + 1) * 16 + q * 4);
+                    a[m] = *(const s4*)(hs + (size_t)(row + xcol(m, r16) + dx + 1) * 16 + q * 4);
                 }
 #pragma unroll
                 for (int n = 0; n < NTMAX; ++n) {
@@ -167,7 +242,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
             }
         }
     }
-    // D[i][j]: lane (q, r16) item i holds voxel x0 + 16 (m & 3) + 4q + i of row y0 + wave ROWS + m / 4,
+    // D[i][j]: lane (q, r16) item i holds voxel x0 + xcol(m, 4q + i) of row y0 + wave ROWS + m / 4,
     // output channel co0 + 16n + r16
 #pragma unroll
     for (int n = 0; n < NTMAX; ++n) {
@@ -180,7 +255,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
             if (y >= g.H) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int x = x0 + (m & 3) * 16 + q * 4 + i;
+                const int x = x0 + xcol(m, q * 4 + i);
                 if (x >= g.W) continue;
                 const size_t o = (((size_t)z * g.H + y) * g.W + x) * g.Cout + co;
                 float val = acc[m][n][i] + b;
@@ -584,6 +659,32 @@ __global__ void k_prep_weights(const float* __restrict__ p, int Cout, int Cin, u
     wT[((size_t)(kTaps - 1 - t) * Cin + ci) * Cout + co] = b;
 }
 
+// Combined BF16 weights of the upsampled channels for the parity-class form (PAR above):
+// wpar[pz][py][px][oz][oy][ox][co][ci] = bf16(sum of w[t][co][C0 + ci] over the taps t whose
+// offset along each axis lies in S(p, o): S(0, 0) = {-1}, S(0, 1) = {0, 1}, S(1, 0) = {-1, 0},
+// S(1, 1) = {+1}), summed in double.
+__global__ void k_prep_parity(const float* __restrict__ p, int Cout, int C0, int C1, unsigned short* __restrict__ wpar) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 64L * Cout * C1) return;
+    const int ci = (int)(i % C1), co = (int)((i / C1) % Cout), t8 = (int)((i / ((long)C1 * Cout)) % 8);
+    const int cls = (int)(i / (8L * C1 * Cout));
+    const int par[3] = {cls >> 2, (cls >> 1) & 1, cls & 1}, off[3] = {t8 >> 2, (t8 >> 1) & 1, t8 & 1};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = (par[d] == 0) ? (off[d] == 0 ? -1 : 0) : (off[d] == 0 ? -1 : 1);
+        hi[d] = (par[d] == 0) ? (off[d] == 0 ? -1 : 1) : (off[d] == 0 ? 0 : 1);
+    }
+    const int Cin = C0 + C1;
+    double acc = 0.0;
+    for (int tz = lo[0]; tz <= hi[0]; ++tz)
+        for (int ty = lo[1]; ty <= hi[1]; ++ty)
+            for (int tx = lo[2]; tx <= hi[2]; ++tx) {
+                const int t = (tz + 1) * 9 + (ty + 1) * 3 + (tx + 1);
+                acc += (double)p[((size_t)t * Cout + co) * Cin + C0 + ci];
+            }
+    wpar[i] = f2bf((float)acc);
+}
+
 // ----------------------------------------------------------------------------
 // Host launchers
 // ----------------------------------------------------------------------------
@@ -595,7 +696,7 @@ int pick_cb(int Cout) {
 
 hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short* in1, int C1, int D, int H, int W,
                         const unsigned short* w, const float* bias, int Cout, int relu, void* out, bool out_f32,
-                        hipStream_t s) {
+                        hipStream_t s, const unsigned short* wpar = nullptr) {
     if (C0 <= 0 || C0 % 16 || C1 < 0 || C1 % 16 || Cout <= 0 || Cout % 16 || D <= 0 || H <= 0 || W <= 0)
         return hipErrorInvalidValue;
     if (C1 > 0 && (D % 2 || H % 2 || W % 2 || !in1)) return hipErrorInvalidValue;
@@ -608,16 +709,22 @@ hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short*
     const int ty = 4 * rows;
     if (gx > 0x7fffffffL || (H + ty - 1) / ty > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)gx, (unsigned)((H + ty - 1) / ty), (unsigned)D);
-    if (out_f32) {
+    const dim3 blk(kFwdThreads);
+    if (wpar && C1 > 0 && !out_f32) {
         if (rows == 2)
-            hipLaunchKernelGGL((k_conv_fwd<true, 2>), grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+            hipLaunchKernelGGL((k_conv_fwd<false, 2, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out);
         else
-            hipLaunchKernelGGL((k_conv_fwd<true, 1>), grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+            hipLaunchKernelGGL((k_conv_fwd<false, 1, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out);
+    } else if (out_f32) {
+        if (rows == 2)
+            hipLaunchKernelGGL((k_conv_fwd<true, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
+        else
+            hipLaunchKernelGGL((k_conv_fwd<true, 1, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
     } else {
         if (rows == 2)
-            hipLaunchKernelGGL((k_conv_fwd<false, 2>), grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+            hipLaunchKernelGGL((k_conv_fwd<false, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
         else
-            hipLaunchKernelGGL((k_conv_fwd<false, 1>), grid, dim3(kFwdThreads), 0, s, in0, in1, g, w, bias, relu, out);
+            hipLaunchKernelGGL((k_conv_fwd<false, 1, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
     }
     return hipGetLastError();
 }
@@ -667,6 +774,7 @@ struct Layer {
     int level, C0, C1, Cout;  // C0: in0 channels, C1: in1 (half resolution) channels
     size_t w_off, b_off;       // offsets in the flat parameter array
     unsigned short *w = nullptr, *wT = nullptr;
+    unsigned short* wpar = nullptr;  // parity-class weights of the in1 channels (layers with C1 > 0)
 };
 
 #define NET_HIP(call)                                                                              \
@@ -765,6 +873,7 @@ void net_release(rihvr_net* n) {
     for (auto& l : n->L) {
         ptrs.push_back(l.w);
         ptrs.push_back(l.wT);
+        ptrs.push_back(l.wpar);
     }
     for (void* q : ptrs) (void)hipFree(q);
     if (n->h_lpart) (void)hipHostFree(n->h_lpart);
@@ -777,6 +886,9 @@ rihvr_status prep_weights(rihvr_net* n) {
         const long cnt = (long)kTaps * l.Cout * Cin;
         hipLaunchKernelGGL(k_prep_weights, dim3(blocks_of(cnt)), dim3(kEw), 0, n->stream, n->params + l.w_off, l.Cout,
                            Cin, l.w, l.wT);
+        if (l.C1 > 0)
+            hipLaunchKernelGGL(k_prep_parity, dim3(blocks_of(64L * l.Cout * l.C1)), dim3(kEw), 0, n->stream,
+                               n->params + l.w_off, l.Cout, l.C0, l.C1, l.wpar);
     }
     hipLaunchKernelGGL(k_pack_first, dim3(4), dim3(256), 0, n->stream, n->params + n->L[0].w_off, n->w0p);
     NET_HIP(hipGetLastError());
@@ -784,13 +896,14 @@ rihvr_status prep_weights(rihvr_net* n) {
     return RIHVR_OK;
 }
 
-// the convolution of layer i on its inputs
+// the convolution of layer i on its inputs; parity: in1's channels through the parity-class
+// weights (inference), else through the 27 taps on the upsampled grid (training)
 rihvr_status layer_forward(rihvr_net* n, int i, const unsigned short* in0, const unsigned short* in1,
-                           unsigned short* out) {
+                           unsigned short* out, bool parity = false) {
     const Layer& l = n->L[i];
     const int k = l.level;
     NET_HIP(conv_forward(in0, l.C0, in1, l.C1, n->Dk[k], n->Hk[k], n->Wk[k], l.w, n->params + l.b_off, l.Cout, 1, out,
-                         false, n->stream));
+                         false, n->stream, parity ? l.wpar : nullptr));
     return RIHVR_OK;
 }
 
@@ -821,9 +934,10 @@ rihvr_status forward_body(rihvr_net* n, const float* in, bool with_x16) {
         if ((st = pool(n, k - 1, n->e[k - 1], n->p[k]))) return st;
         if ((st = layer_forward(n, k, n->p[k], nullptr, n->e[k]))) return st;
     }
-    if ((st = layer_forward(n, 4, n->e[2], n->e[3], n->d[2]))) return st;
-    if ((st = layer_forward(n, 5, n->e[1], n->d[2], n->d[1]))) return st;
-    if ((st = layer_forward(n, 6, n->e[0], n->d[1], n->d[0]))) return st;
+    const bool parity = !with_x16;  // prediction: the decoder's upsampled channels in parity-class form
+    if ((st = layer_forward(n, 4, n->e[2], n->e[3], n->d[2], parity))) return st;
+    if ((st = layer_forward(n, 5, n->e[1], n->d[2], n->d[1], parity))) return st;
+    if ((st = layer_forward(n, 6, n->e[0], n->d[1], n->d[0], parity))) return st;
     return RIHVR_OK;
 }
 
@@ -1013,6 +1127,7 @@ rihvr_status rihvr_net_create(int32_t num_planes, int32_t height, int32_t width,
     for (auto& l : n->L) {
         const size_t cnt = (size_t)kTaps * l.Cout * (l.C0 + l.C1);
         if ((st = nalloc(&l.w, cnt)) || (st = nalloc(&l.wT, cnt))) return fail(st);
+        if (l.C1 > 0 && (st = nalloc(&l.wpar, (size_t)64 * l.Cout * l.C1))) return fail(st);
     }
     if ((st = nalloc(&n->w0p, 4 * 16 * 16))) return fail(st);
     for (int k = 0; k < kLevels; ++k)

@@ -148,10 +148,10 @@ def evaluate(net, optics: str, size: int, n_frames: int, seed: int, iterations: 
             t_learn = time.perf_counter() - t0
             rows.append(dict(f_plain=plain["f_final"], cold_iterations=cold["iterations"], cold_s=t_cold,
                              learned_iterations=lr_["iterations"], learned_s=t_learn,
-                             fallback=int(lr_["fallback"] == 1), scaled=int(lr_["fallback"] == 2),
+                             fallback=lr_["fallback"],
                              reached=lr_["reached"], f_learned=lr_["f_final"], f_net=lr_["f_net"],
                              f_adjoint=lr_["f_adjoint"], f_zero=lr_["f_zero"], resid_net=lr_["resid_net"],
-                             alpha=lr_["alpha"], f_scaled=lr_["f_scaled"],
+                             tau=lr_["tau"], f_thresholded=lr_["f_thresholded"],
                              ms_network=lr_["ms_network"], ms_adjoint=lr_["ms_adjoint"], ms_solve=lr_["ms_solve"]))
             if log:
                 log(json.dumps(rows[-1]))
@@ -161,7 +161,6 @@ def evaluate(net, optics: str, size: int, n_frames: int, seed: int, iterations: 
              cold_iterations=[r["cold_iterations"] for r in rows],
              learned_iterations=[r["learned_iterations"] for r in rows],
              fallback_rate=float(np.mean([r["fallback"] for r in rows])),
-             scaled_rate=float(np.mean([r["scaled"] for r in rows])),
              reached=[r["reached"] for r in rows], rows=rows)
     s["speedup"] = s["cold_s_per_frame"] / s["learned_s_per_frame"]
     return s

@@ -152,15 +152,22 @@ def test_weighted_loss_gradients_match_oracle(pkg, gpu_available):
     assert _rel(g, eg) <= 3e-2
 
 
-def test_network_prediction_matches_oracle(pkg, gpu_available):
+@pytest.mark.parametrize("shape", [(8, 32, 16), (8, 24, 136), (16, 16, 64)])
+def test_network_prediction_matches_oracle(pkg, gpu_available, shape):
     import torch
-    vol, _, p = _net_problem(4, shape=(8, 32, 16))
+    vol, _, p = _net_problem(4, shape=shape)
     with pkg.WarmStartNet(*vol.shape) as net:
         net.set_params(p, 0.8, 1.5)
         got = net.predict(torch.from_numpy(vol).cuda()).cpu().numpy()
         a, i, o = net.get_params()
-    exp, _ = U.forward(p.astype(np.float64), vol.astype(np.complex128), 0.8, 1.5)
+    # the prediction takes the decoder's upsampled channels in parity-class form (8 combined
+    # BF16 taps on the half-resolution grid); the training form rounds the 27 taps separately
+    exp, _ = U.forward(p.astype(np.float64), vol.astype(np.complex128), 0.8, 1.5, parity=True)
+    plain, _ = U.forward(p.astype(np.float64), vol.astype(np.complex128), 0.8, 1.5)
+    print(f"prediction vs oracle: {_rel(got, exp):.3e} (parity form), {_rel(got, plain):.3e} (training form)",
+          flush=True)
     assert _rel(got, exp) <= 1e-2
+    assert _rel(got, plain) <= 3e-2
     assert np.array_equal(a, p) and i == np.float32(0.8) and o == np.float32(1.5)
 
 
@@ -189,9 +196,9 @@ def _small_cfg():
 
 def test_zero_network_start_equals_the_plain_solve(pkg, scenes, gpu_available):
     """A network whose output is zero everywhere (all parameters 0) proposes the zero start: its
-    objective is the zero start's 0.5 ||b||^2, the A^T b start has zero data residual (A A^T =
-    Nz I on a real hologram), the ray alpha x_net has no direction (alpha = 0), so the zero start
-    is taken, and the solve equals the plain solver's solve to the same target bit for bit."""
+    objective is the zero start's 0.5 ||b||^2 (every threshold leaves it zero), the A^T b start
+    has zero data residual (A A^T = Nz I on a real hologram), the decision follows the rule, and
+    either way the solve equals the plain solver's solve to the same target bit for bit."""
     import torch
     c = _small_cfg()
     holo, _ = scenes.disks_scene(scenes.optics_of(c), 8, 2.0, 4.0, 11, 0.01)
@@ -208,31 +215,43 @@ def test_zero_network_start_equals_the_plain_solve(pkg, scenes, gpu_available):
     b2 = 0.5 * float(np.sum(np.abs(holo.astype(np.complex128)) ** 2))
     assert abs(st["f_zero"] - b2) <= 1e-6 * b2
     assert abs(st["f_net"] - b2) <= 1e-6 * b2 and abs(st["resid_net"] - b2) <= 1e-6 * b2
+    assert st["tau"] == 0.0 and abs(st["f_thresholded"] - b2) <= 1e-6 * b2
     assert st["resid_adjoint"] <= 1e-9 * b2
-    assert st["alpha"] == 0.0 and st["f_scaled"] == st["f_zero"]
-    assert st["fallback"] == (0 if st["f_net"] <= min(st["f_adjoint"], st["f_zero"]) else 1)
+    assert st["fallback"] == (0 if st["f_thresholded"] <= min(st["f_adjoint"], st["f_zero"]) else 1)
     assert st["iterations"] == ref[0]["iterations"]
     assert np.array_equal(vol, vref)
 
 
 def test_learned_start_decision_follows_the_residual_rule(pkg, scenes, gpu_available):
-    """With a randomly initialised network the returned decision is the rule's (x_net when its
-    objective is not above the A^T b start's nor the zero start's, else alpha x_net when alpha > 0
-    and its objective is not above them, else the zero start); the closed-form objective at
-    alpha x_net is not above the objectives at 0 and at x_net; the objective the solver itself
-    computes at the chosen start (a forward pass of it) equals (1e-4) the one the decision used,
-    which pins the closed form when alpha x_net is chosen; without a target the solve runs
-    max_iterations."""
+    """With a randomly initialised network: tau is q max|x_net| for the q in {0.01, 0.02, 0.05}
+    whose thresholded start has the least objective (each candidate's objective recomputed here
+    by rihvr_begin on a soft threshold formed by torch from net.predict(A^T b)); the decision is
+    the rule's (the thresholded start when its objective is not above the A^T b start's nor the
+    zero start's, else the zero start); the objective the solver computes at the chosen start
+    equals the one the decision used; without a target the solve runs max_iterations."""
     import torch
     c = _small_cfg()
     holo, _ = scenes.disks_scene(scenes.optics_of(c), 8, 2.0, 4.0, 12, 0.01)
+    h = torch.from_numpy(holo).cuda()
+    shape = (c["num_planes"], c["height"], c["width"])
     with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec, pkg.WarmStartNet(16, 64, 64, seed=3) as net:
-        st = rec.reconstructLearned(net, torch.from_numpy(holo).cuda(), f_target=0.0, max_iterations=5)
-    print(st, flush=True)
-    lim = min(st["f_adjoint"], st["f_zero"])
-    rule = 0 if st["f_net"] <= lim else (2 if st["alpha"] > 0 and st["f_scaled"] <= lim else 1)
-    assert st["fallback"] == rule
-    assert st["f_scaled"] <= st["f_zero"] * (1 + 1e-9) and st["f_scaled"] <= st["f_net"] * (1 + 1e-9)
-    f_chosen = {0: st["f_net"], 1: st["f_zero"], 2: st["f_scaled"]}[st["fallback"]]
+        st = rec.reconstructLearned(net, h, f_target=0.0, max_iterations=5)
+        g = torch.empty(shape, dtype=torch.complex64, device="cuda")
+        rec.adjoint(h, out=g)
+        x = net.predict(g)
+        torch.cuda.synchronize()
+        xmax = float(torch.abs(x).max())
+        fq = {}
+        for q in (0.01, 0.02, 0.05):
+            tau = np.float32(q * xmax)
+            m = torch.abs(x)
+            xs = x * torch.clamp((m - float(tau)) / torch.clamp(m, min=1e-30), min=0)
+            fq[q] = (float(tau), rec.begin(h, warm_start=xs.contiguous()))
+    print(st, fq, flush=True)
+    best = min(fq.values(), key=lambda v: v[1])
+    assert abs(st["f_thresholded"] - best[1]) <= 1e-5 * best[1]
+    assert any(abs(st["tau"] - t) <= 1e-6 * t and abs(f - best[1]) <= 1e-5 * best[1] for t, f in fq.values())
+    assert st["fallback"] == (0 if st["f_thresholded"] <= min(st["f_adjoint"], st["f_zero"]) else 1)
+    f_chosen = st["f_zero"] if st["fallback"] else st["f_thresholded"]
     assert abs(st["f_start"] - f_chosen) <= 1e-4 * f_chosen
     assert st["iterations"] == 5

@@ -161,3 +161,36 @@ def test_param_count_matches_layout():
     layers, head = U.unpack(p)
     assert [w.shape for w, _ in layers][0] == (27, 16, 16)
     assert np.all(layers[0][0][:, :, 2:] == 0)
+
+
+def test_parity_class_upsampled_conv_equals_conv_of_the_upsampled_input():
+    """conv(up(x), w) over the upsampled channels equals the 8-tap parity-class form on the low
+    grid with the taps summed per class (exact in float64, borders included); with BF16-rounded
+    combined weights it differs by the weights' rounding only."""
+    rng = np.random.default_rng(3)
+    xl = rng.standard_normal((3, 4, 5, 6))
+    w = rng.standard_normal((27, 5, 2 + 6))
+    ref = U.conv(U.up(xl), w[:, :, 2:])
+    got = U.conv_up_parity(xl, U.parity_weights(w, 2, rounding=False))
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+    got_r = U.conv_up_parity(xl, U.parity_weights(w, 2))
+    assert np.max(np.abs(got_r - ref)) <= 2 ** -8 * 27 * np.max(np.abs(w)) * np.max(np.abs(xl)) * 6
+    # a class's weights: parity (0, 0, 0) offset (1, 1, 1) holds taps {0, +1}^3 summed
+    pw = U.parity_weights(w, 2, rounding=False)
+    taps = [(tz + 1) * 9 + (ty + 1) * 3 + (tx + 1) for tz in (0, 1) for ty in (0, 1) for tx in (0, 1)]
+    assert np.allclose(pw[0, 0, 0, 1, 1, 1], w[taps, :, 2:].sum(0))
+    assert np.allclose(pw[1, 0, 1, 0, 1, 1], w[[(tz + 1) * 9 + (ty + 1) * 3 + 2 for tz in (-1, 0) for ty in (0, 1)], :, 2:].sum(0))
+
+
+def test_parity_forward_is_the_network_within_bf16():
+    """The inference form of the network (parity-class decoder convolutions) against the training
+    form on the same parameters: within BF16 rounding of the outputs."""
+    rng = np.random.default_rng(4)
+    p = U.init_params(seed=5)
+    vol = (rng.standard_normal((8, 16, 16)) + 1j * rng.standard_normal((8, 16, 16))) * 0.3
+    a, _ = U.forward(p, vol, 0.9, 1.3)
+    b, _ = U.forward(p, vol, 0.9, 1.3, parity=True)
+    assert np.linalg.norm(a - b) <= 2e-2 * np.linalg.norm(a)
+    c, _ = U.forward(p, vol, 0.9, 1.3, rounding=False)
+    d, _ = U.forward(p, vol, 0.9, 1.3, rounding=False, parity=True)
+    assert np.linalg.norm(c - d) <= 1e-12 * np.linalg.norm(c)
