@@ -661,13 +661,13 @@ typedef struct rihvr_learned_stats {
     double  ms_network;     /* wall time of the network's prediction, the threshold choice and the
                                start test */
     double  ms_solve;       /* wall time of FISTA from the chosen start */
-    double  tau;            /* the soft threshold applied to x_net: q max|x_net|, q in {0.01, 0.02,
-                               0.05}, the one whose start has the least objective */
+    double  tau;            /* the soft threshold applied to x_net: q max|x_net|, q in {0.02, 0.05},
+                               the one whose start has the least objective */
     double  f_thresholded;  /* objective at soft(x_net, tau) */
 } rihvr_learned_stats;
 
 /* Reconstructs one hologram with the network's start: A^T b -> network -> x_net ->
- * soft(x_net, tau) (complex soft threshold; tau = q max|x_net| for the q in {0.01, 0.02, 0.05}
+ * soft(x_net, tau) (complex soft threshold; tau = q max|x_net| for the q in {0.02, 0.05}
  * whose start has the least objective: the network's output holds small values at every voxel,
  * which FISTA's first prox would remove).  That start is used when its objective is not above
  * the objectives at the A^T b start (t A^T b, t = 1/Nz) and at the zero start; else the zero

@@ -88,8 +88,10 @@ __device__ __forceinline__ s4 load_in4(const unsigned short* __restrict__ in0, c
 // resolution voxels (v >> 1) + p + o - 1, o in {0, 1}^3, with wpar's combined weights (the
 // taps that land on one half-resolution voxel summed: 8 taps instead of 27).  So that an
 // M-tile's 16 voxels share their x parity, its voxels are x0 + 32 (m&3 >> 1) + 2 r + (m & 1)
-// (r = 0..15) in every phase; per (chunk, oz) the (TY/2 + 2) x 34 half-resolution halo of
-// plane (z >> 1) + pz + oz - 1 is staged with the 16 weight slabs (py, px, oy, ox) of that oz.
+// (r = 0..15) in every phase (the full-resolution halo's columns are stored even ones first,
+// then odd ones, so such a tile reads contiguous LDS); per (chunk, oz) the (TY/2 + 2) x 34
+// half-resolution halo of plane (z >> 1) + pz + oz - 1 is staged with the 16 weight slabs
+// (py, px, oy, ox) of that oz.
 constexpr int kHX = 66;  // halo columns: 64 + 2
 constexpr int kHX2 = 34;  // half-resolution halo columns: 32 + 2
 template <bool OUTF32, int ROWS, bool PAR>
@@ -106,7 +108,8 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
     constexpr int NTMAX = ROWS == 1 ? 4 : 1;           // N-tiles per wave at most
     constexpr int SLABS = PAR ? 16 : 9;
     __shared__ uint4 halo[HALO * 2];     // [row][x][16 ch] as two 16-byte halves per voxel
-    __shared__ s4 wl[SLABS * kMaxCB * 4];  // [tap of the slab][co][4 groups of 4 ci]
+    constexpr int CBM = ROWS == 2 ? 16 : kMaxCB;  // ROWS = 2 runs CB = 16 only (conv_forward)
+    __shared__ s4 wl[SLABS * CBM * 4];     // [tap of the slab][co][4 groups of 4 ci]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = lane >> 4, r16 = lane & 15;
     const int nxt = (g.W + 63) >> 6;
@@ -157,7 +160,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
                     const int qq = e & 3, co = (e >> 2) % g.CB, sl = (e >> 2) / g.CB;
                     const int cls = (pz * 2 + (sl >> 3)) * 2 + ((sl >> 2) & 1);
                     const int tap = oz * 4 + (sl & 3);
-                    wl[(sl * kMaxCB + co) * 4 + qq] =
+                    wl[(sl * CBM + co) * 4 + qq] =
                         *(const s4*)(wpar + ((size_t)(cls * 8 + tap) * g.Cout + co0 + co) * g.C1 + ci0 + qq * 4);
                 }
                 __syncthreads();
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
                             for (int m = 0; m < MT; ++m) {
                                 const int yr = wave * ROWS + (m >> 2);
                                 const int sl = ((yr & 1) * 2 + (m & 1)) * 4 + oo;
-                                const s4 b = wl[(sl * kMaxCB + n * 16 + r16) * 4 + q];
+                                const s4 b = wl[(sl * CBM + n * 16 + r16) * 4 + q];
                                 acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n], 0, 0, 0);
                             }
                         }
@@ -212,12 +215,19 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int e = threadIdx.x + k * kFwdThreads;
-                if (e < HALO * 2) halo[e] = v[k];
+                if (e < HALO * 2) {
+                    int dst = e;
+                    if (PAR) {  // x-parity-split columns: an M-tile's voxels (one parity) read contiguously
+                        const int vox = e >> 1, rx = vox % kHX, ry = vox / kHX;
+                        dst = ((ry * kHX + (rx & 1) * (kHX / 2) + (rx >> 1)) << 1) | (e & 1);
+                    }
+                    halo[dst] = v[k];
+                }
             }
             const int t0 = (dz + 1) * 9;
             for (int e = threadIdx.x; e < 9 * g.CB * 4; e += kFwdThreads) {
                 const int qq = e & 3, co = (e >> 2) % g.CB, tt = (e >> 2) / g.CB;
-                wl[(tt * kMaxCB + co) * 4 + qq] =
+                wl[(tt * CBM + co) * 4 + qq] =
                     *(const s4*)(w + ((size_t)(t0 + tt) * g.Cout + co0 + co) * Cin + c * 16 + qq * 4);
             }
             __syncthreads();
@@ -228,12 +238,14 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
                     const int row = (wave * ROWS + (m >> 2) + dy + 1) * kHX;
-                    a[m] = *(const s4*)(hs + (size_t)(row + xcol(m, r16) + dx + 1) * 16 + q * 4);
+                    const int hc = xcol(m, r16) + dx + 1;
+                    const int col = PAR ? (hc & 1) * (kHX / 2) + (hc >> 1) : hc;
+                    a[m] = *(const s4*)(hs + (size_t)(row + col) * 16 + q * 4);
                 }
 #pragma unroll
                 for (int n = 0; n < NTMAX; ++n) {
                     if (n < NT) {
-                        const s4 b = wl[(tt * kMaxCB + n * 16 + r16) * 4 + q];
+                        const s4 b = wl[(tt * CBM + n * 16 + r16) * 4 + q];
 #pragma unroll
                         for (int m = 0; m < MT; ++m)
                             acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n], 0, 0, 0);
@@ -710,11 +722,12 @@ hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short*
     if (gx > 0x7fffffffL || (H + ty - 1) / ty > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)gx, (unsigned)((H + ty - 1) / ty), (unsigned)D);
     const dim3 blk(kFwdThreads);
-    if (wpar && C1 > 0 && !out_f32) {
-        if (rows == 2)
-            hipLaunchKernelGGL((k_conv_fwd<false, 2, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out);
-        else
-            hipLaunchKernelGGL((k_conv_fwd<false, 1, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out);
+    // The parity-class form for 16 output channels (two rows per wave) only: with 32 or 64
+    // channels per workgroup (one row per wave) its 16 weight slabs and per-M-tile B operands
+    // measured slower than the 27 taps (1024^2 x 128: 8.05 against 7.33 ms at 32 channels,
+    // 3.44 against 2.87 ms at 64), at 16 channels faster (13.8 against 20.5 ms).
+    if (wpar && C1 > 0 && !out_f32 && rows == 2) {
+        hipLaunchKernelGGL((k_conv_fwd<false, 2, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out);
     } else if (out_f32) {
         if (rows == 2)
             hipLaunchKernelGGL((k_conv_fwd<true, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
@@ -774,7 +787,7 @@ struct Layer {
     int level, C0, C1, Cout;  // C0: in0 channels, C1: in1 (half resolution) channels
     size_t w_off, b_off;       // offsets in the flat parameter array
     unsigned short *w = nullptr, *wT = nullptr;
-    unsigned short* wpar = nullptr;  // parity-class weights of the in1 channels (layers with C1 > 0)
+    unsigned short* wpar = nullptr;  // parity-class weights of the in1 channels (decoder layers of 16 channels)
 };
 
 #define NET_HIP(call)                                                                              \
@@ -886,7 +899,7 @@ rihvr_status prep_weights(rihvr_net* n) {
         const long cnt = (long)kTaps * l.Cout * Cin;
         hipLaunchKernelGGL(k_prep_weights, dim3(blocks_of(cnt)), dim3(kEw), 0, n->stream, n->params + l.w_off, l.Cout,
                            Cin, l.w, l.wT);
-        if (l.C1 > 0)
+        if (l.wpar)
             hipLaunchKernelGGL(k_prep_parity, dim3(blocks_of(64L * l.Cout * l.C1)), dim3(kEw), 0, n->stream,
                                n->params + l.w_off, l.Cout, l.C0, l.C1, l.wpar);
     }
@@ -1127,7 +1140,7 @@ rihvr_status rihvr_net_create(int32_t num_planes, int32_t height, int32_t width,
     for (auto& l : n->L) {
         const size_t cnt = (size_t)kTaps * l.Cout * (l.C0 + l.C1);
         if ((st = nalloc(&l.w, cnt)) || (st = nalloc(&l.wT, cnt))) return fail(st);
-        if (l.C1 > 0 && (st = nalloc(&l.wpar, (size_t)64 * l.Cout * l.C1))) return fail(st);
+        if (l.C1 > 0 && pick_cb(l.Cout) == 16 && (st = nalloc(&l.wpar, (size_t)64 * l.Cout * l.C1))) return fail(st);
     }
     if ((st = nalloc(&n->w0p, 4 * 16 * 16))) return fail(st);
     for (int k = 0; k < kLevels; ++k)

@@ -223,7 +223,7 @@ def test_zero_network_start_equals_the_plain_solve(pkg, scenes, gpu_available):
 
 
 def test_learned_start_decision_follows_the_residual_rule(pkg, scenes, gpu_available):
-    """With a randomly initialised network: tau is q max|x_net| for the q in {0.01, 0.02, 0.05}
+    """With a randomly initialised network: tau is q max|x_net| for the q in {0.02, 0.05}
     whose thresholded start has the least objective (each candidate's objective recomputed here
     by rihvr_begin on a soft threshold formed by torch from net.predict(A^T b)); the decision is
     the rule's (the thresholded start when its objective is not above the A^T b start's nor the
@@ -242,7 +242,7 @@ def test_learned_start_decision_follows_the_residual_rule(pkg, scenes, gpu_avail
         torch.cuda.synchronize()
         xmax = float(torch.abs(x).max())
         fq = {}
-        for q in (0.01, 0.02, 0.05):
+        for q in (0.02, 0.05):
             tau = np.float32(q * xmax)
             m = torch.abs(x)
             xs = x * torch.clamp((m - float(tau)) / torch.clamp(m, min=1e-30), min=0)
