@@ -183,9 +183,9 @@ def pack(layer_grads, head_grads):
 
 def forward(params, vol, in_scale, out_scale, rounding=True, parity=False):
     """Network output (complex [D][H][W]) and the activations, for volume vol [D][H][W] complex.
-    parity: the decoder convolutions with 16 output channels take their upsampled channels
+    parity: the decoder convolutions of up to 32 output channels take their upsampled channels
     through parity_weights (the inference path), else through the 27 BF16 taps on the upsampled
-    grid (the training path, and the inference path of the wider decoder layers)."""
+    grid (the training path, and the inference path of the 64-channel decoder layer)."""
     r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
     layers, (wo, bo) = unpack(params)
     wl = [(r(w), b) for w, b in layers]
@@ -209,7 +209,7 @@ def forward(params, vol, in_scale, out_scale, rounding=True, parity=False):
         p.append(r(pool(e[k - 1])))
         e.append(layer(k, p[k]))
     def dec(i, skip, low):
-        if not parity or wl[i][0].shape[1] != 16:
+        if not parity or wl[i][0].shape[1] > 32:
             return layer(i, np.concatenate([skip, up(low)], axis=3))
         w, b = wl[i]
         c0 = skip.shape[3]

@@ -75,6 +75,7 @@ struct SolveState {
     double t = 0;             // step size
     double alpha1 = 1.0;      // FISTA alpha
     float s_mom = 0.0f;       // momentum step of the last update
+    double step = 0.0;        // the same step in double, as the hologram momentum used it
     double prev_summag = 0;   // sum |r|^2 at the current momentum point
     double f_initial = 0;
     int total_bt = 0;
@@ -907,12 +908,13 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
     if (n_it > 0) c->have_result = false;  // a snapshot lives in the ring slot the prox writes
     SolveState& s = c->ss;
     const int mode = s.mode;
-    // Without an objective target, the work of an accepted iteration (hologram momentum,
-    // R-hat and the next gradient into G_alt) is queued before the host has read the
-    // scalars of the prox and A, so the device does not idle while the host decides; a
-    // backtrack (rare) discards it.  The step it uses is the host's, chosen on the device
-    // by the restart test (a restart makes it 0).
-    const bool ahead = f_stop <= 0 && c->n_groups == 1;
+    // The work of an accepted iteration (hologram momentum, R-hat and the next gradient into
+    // G_alt) is queued before the host has read the scalars of the prox and A, so the device
+    // does not idle while the host decides; a backtrack (rare) discards it, and so does a stop
+    // at the objective target, which then restores the accepted point's hologram momentum.
+    // The step it uses is the host's, chosen on the device by the restart test (a restart
+    // makes it 0).
+    const bool ahead = c->n_groups == 1;
     for (int i = 0; i < n_it; ++i) {
         double alpha0 = s.alpha1;
         int bt = 0;
@@ -1006,8 +1008,21 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
                     if (st->f) st->f[i - 1] = f_prev;
                 }
                 if (f_stop > 0 && f_prev <= f_stop) {
-                    // keep the accepted point: its estimate is still in hE (the forward pass
-                    // above wrote hX)
+                    // keep the accepted point.  Unqueued, its estimate is still in hE (the
+                    // forward pass above wrote hX).  Queued, the hologram ring is put back and
+                    // the accepted point's momentum (its estimate, R-hat and residual norm) is
+                    // formed again from the same spectra and step, which gives the same bits.
+                    if (queued) {
+                        std::swap(c->hC, c->hX);
+                        std::swap(c->hP, c->hC);
+                        queued = false;
+                        RIHVR_HIP(holo_momentum(c->hC, c->hP, s.step, c->bspec, c->hE, c->rhat, c->N,
+                                                c->partial_holo, c->stream));
+                        RIHVR_HIP(reduce_partials(c->partial_holo, holo_num_blocks(c->N), 1, c->red_work,
+                                                  c->scal + SL_RESID_M, c->stream));
+                        RIHVR_TRY(rhat_written(c));
+                        s.g_ready = false;
+                    }
                     if (done) *done = i;
                     n_it = i;
                     break;
@@ -1064,6 +1079,7 @@ rihvr_status iterate_impl(rihvr_ctx* c, int n_it, rihvr_stats* st, bool last_f_b
         c->iy0 = c->iy;
         c->iy = old_m;
         s.s_mom = (float)step;
+        s.step = step;
         s.iters++;
         if (st) {
             if (st->ratio) st->ratio[i] = ratio;

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -83,33 +84,21 @@ __device__ __forceinline__ s4 load_in4(const unsigned short* __restrict__ in0, c
 // and B fragments from LDS.  ROWS = 2 (for CB = 16, whose stages hold little MFMA work) doubles
 // the work per stage.  Grid: (ceil(W/64) * Cout/CB, ceil(H/(4 ROWS)), D).
 // OUTF32: raw FP32 sums (input gradient), else BF16 with bias and optional ReLU.
-// PAR (inference of the decoder layers): in1's channels go through the parity-class form of
-// the upsampled convolution.  An output voxel at parity (pz, py, px) reads in1 at the half-
-// resolution voxels (v >> 1) + p + o - 1, o in {0, 1}^3, with wpar's combined weights (the
-// taps that land on one half-resolution voxel summed: 8 taps instead of 27).  So that an
-// M-tile's 16 voxels share their x parity, its voxels are x0 + 32 (m&3 >> 1) + 2 r + (m & 1)
-// (r = 0..15) in every phase (the full-resolution halo's columns are stored even ones first,
-// then odd ones, so such a tile reads contiguous LDS); per (chunk, oz) the (TY/2 + 2) x 34
-// half-resolution halo of plane (z >> 1) + pz + oz - 1 is staged with the 16 weight slabs
-// (py, px, oy, ox) of that oz.
+// (conv_forward runs this kernel with ROWS = 1; two rows per wave run k_conv_dma below.)
 constexpr int kHX = 66;  // halo columns: 64 + 2
 constexpr int kHX2 = 34;  // half-resolution halo columns: 32 + 2
-template <bool OUTF32, int ROWS, bool PAR>
+template <bool OUTF32, int ROWS>
 __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* __restrict__ in0,
                                                           const unsigned short* __restrict__ in1, ConvGeo g,
                                                           const unsigned short* __restrict__ w,
-                                                          const unsigned short* __restrict__ wpar,
                                                           const float* __restrict__ bias, int relu,
                                                           void* __restrict__ out) {
     constexpr int TY = 4 * ROWS, HY = TY + 2, HALO = HY * kHX;
-    constexpr int HY2 = TY / 2 + 2, HALO2 = HY2 * kHX2;
-    static_assert(HALO2 <= HALO, "the half-resolution halo reuses the halo buffer");
     constexpr int MT = 4 * ROWS;                       // M-tiles per wave
     constexpr int NTMAX = ROWS == 1 ? 4 : 1;           // N-tiles per wave at most
-    constexpr int SLABS = PAR ? 16 : 9;
     __shared__ uint4 halo[HALO * 2];     // [row][x][16 ch] as two 16-byte halves per voxel
     constexpr int CBM = ROWS == 2 ? 16 : kMaxCB;  // ROWS = 2 runs CB = 16 only (conv_forward)
-    __shared__ s4 wl[SLABS * CBM * 4];     // [tap of the slab][co][4 groups of 4 ci]
+    __shared__ s4 wl[9 * CBM * 4];       // [tap of the slab][co][4 groups of 4 ci]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = lane >> 4, r16 = lane & 15;
     const int nxt = (g.W + 63) >> 6;
@@ -126,71 +115,8 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
         for (int n = 0; n < NTMAX; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     const int nchunks = Cin >> 4;
     const unsigned short* hs = (const unsigned short*)halo;
-    // column of M-tile m's row r inside the 64-wide tile
-    auto xcol = [](int m, int r) { return PAR ? 32 * ((m & 3) >> 1) + 2 * r + (m & 1) : 16 * (m & 3) + r; };
     for (int c = 0; c < nchunks; ++c) {
         const bool hi = c * 16 >= g.C0;
-        if (PAR && hi) {
-            const int pz = z & 1, Dl = g.D >> 1;
-            const int yl0 = (y0 >> 1) - 1, xl0 = (x0 >> 1) - 1;
-            const int ci0 = c * 16 - g.C0;
-            for (int oz = 0; oz < 2; ++oz) {
-                const int zl = (z >> 1) + pz + oz - 1;
-                if (zl < 0 || zl >= Dl) continue;  // zeros (uniform)
-                __syncthreads();
-                constexpr int kPer2 = (HALO2 * 2 + kFwdThreads - 1) / kFwdThreads;
-                uint4 v[kPer2];
-#pragma unroll
-                for (int k = 0; k < kPer2; ++k) {
-                    const int e = threadIdx.x + k * kFwdThreads;
-                    const int half = e & 1, vox = e >> 1;
-                    const int rx = vox % kHX2, ry = vox / kHX2;
-                    const int yy = yl0 + ry, xx = xl0 + rx;
-                    v[k] = make_uint4(0, 0, 0, 0);
-                    if (e < HALO2 * 2 && yy >= 0 && yy < H2 && xx >= 0 && xx < W2)
-                        v[k] = *(const uint4*)(in1 + (((size_t)zl * H2 + yy) * W2 + xx) * g.C1 + ci0 + half * 8);
-                }
-#pragma unroll
-                for (int k = 0; k < kPer2; ++k) {
-                    const int e = threadIdx.x + k * kFwdThreads;
-                    if (e < HALO2 * 2) halo[e] = v[k];
-                }
-                // slab sl = (py, px, oy, ox) of this (pz, oz): wpar[pz][py][px][oz][oy][ox][co][ci]
-                for (int e = threadIdx.x; e < 16 * g.CB * 4; e += kFwdThreads) {
-                    const int qq = e & 3, co = (e >> 2) % g.CB, sl = (e >> 2) / g.CB;
-                    const int cls = (pz * 2 + (sl >> 3)) * 2 + ((sl >> 2) & 1);
-                    const int tap = oz * 4 + (sl & 3);
-                    wl[(sl * CBM + co) * 4 + qq] =
-                        *(const s4*)(wpar + ((size_t)(cls * 8 + tap) * g.Cout + co0 + co) * g.C1 + ci0 + qq * 4);
-                }
-                __syncthreads();
-#pragma unroll
-                for (int oo = 0; oo < 4; ++oo) {
-                    const int oy = oo >> 1, ox = oo & 1;
-                    s4 a[MT];
-#pragma unroll
-                    for (int m = 0; m < MT; ++m) {
-                        const int yr = wave * ROWS + (m >> 2);  // y0 is even, so y's parity is yr's
-                        const int hrow = (yr >> 1) + (yr & 1) + oy;
-                        const int hcol = 16 * ((m & 3) >> 1) + r16 + (m & 1) + ox;
-                        a[m] = *(const s4*)(hs + (size_t)(hrow * kHX2 + hcol) * 16 + q * 4);
-                    }
-#pragma unroll
-                    for (int n = 0; n < NTMAX; ++n) {
-                        if (n < NT) {
-#pragma unroll
-                            for (int m = 0; m < MT; ++m) {
-                                const int yr = wave * ROWS + (m >> 2);
-                                const int sl = ((yr & 1) * 2 + (m & 1)) * 4 + oo;
-                                const s4 b = wl[(sl * CBM + n * 16 + r16) * 4 + q];
-                                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n], 0, 0, 0);
-                            }
-                        }
-                    }
-                }
-            }
-            continue;
-        }
         for (int dz = -1; dz <= 1; ++dz) {
             const int zz = z + dz;
             if (zz < 0 || zz >= g.D) continue;  // a plane outside the grid adds zeros (uniform)
@@ -215,14 +141,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int e = threadIdx.x + k * kFwdThreads;
-                if (e < HALO * 2) {
-                    int dst = e;
-                    if (PAR) {  // x-parity-split columns: an M-tile's voxels (one parity) read contiguously
-                        const int vox = e >> 1, rx = vox % kHX, ry = vox / kHX;
-                        dst = ((ry * kHX + (rx & 1) * (kHX / 2) + (rx >> 1)) << 1) | (e & 1);
-                    }
-                    halo[dst] = v[k];
-                }
+                if (e < HALO * 2) halo[e] = v[k];
             }
             const int t0 = (dz + 1) * 9;
             for (int e = threadIdx.x; e < 9 * g.CB * 4; e += kFwdThreads) {
@@ -238,9 +157,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
                     const int row = (wave * ROWS + (m >> 2) + dy + 1) * kHX;
-                    const int hc = xcol(m, r16) + dx + 1;
-                    const int col = PAR ? (hc & 1) * (kHX / 2) + (hc >> 1) : hc;
-                    a[m] = *(const s4*)(hs + (size_t)(row + col) * 16 + q * 4);
+                    a[m] = *(const s4*)(hs + (size_t)(row + (m & 3) * 16 + r16 + dx + 1) * 16 + q * 4);
                 }
 #pragma unroll
                 for (int n = 0; n < NTMAX; ++n) {
@@ -254,7 +171,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
             }
         }
     }
-    // D[i][j]: lane (q, r16) item i holds voxel x0 + xcol(m, 4q + i) of row y0 + wave ROWS + m / 4,
+    // D[i][j]: lane (q, r16) item i holds voxel x0 + 16 (m & 3) + 4q + i of row y0 + wave ROWS + m / 4,
     // output channel co0 + 16n + r16
 #pragma unroll
     for (int n = 0; n < NTMAX; ++n) {
@@ -267,10 +184,240 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
             if (y >= g.H) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int x = x0 + xcol(m, q * 4 + i);
+                const int x = x0 + (m & 3) * 16 + q * 4 + i;
                 if (x >= g.W) continue;
                 const size_t o = (((size_t)z * g.H + y) * g.W + x) * g.Cout + co;
                 float val = acc[m][n][i] + b;
+                if (relu) val = fmaxf(val, 0.0f);
+                if constexpr (OUTF32)
+                    ((float*)out)[o] = val;
+                else
+                    ((unsigned short*)out)[o] = f2bf(val);
+            }
+        }
+    }
+}
+
+// PAR (inference of the 16-channel decoder layer): in1's channels go through the parity-class
+// form of the upsampled convolution.  An output voxel at parity (pz, py, px) reads in1 at the
+// half-resolution voxels (v >> 1) + p + o - 1, o in {0, 1}^3, with wpar's combined weights
+// (the taps that land on one half-resolution voxel summed: 8 taps instead of 27).  So that an
+// M-tile's 16 voxels share their x parity, its voxels are x0 + 32 (m&3 >> 1) + 2 r + (m & 1)
+// (r = 0..15) in every phase (the full-resolution halo's columns are stored even ones first,
+// then odd ones, so such a tile reads contiguous LDS); per (chunk, oz) the (TY/2 + 2) x 34
+// half-resolution halo of plane (z >> 1) + pz + oz - 1 is staged with the 16 weight slabs
+// (py, px, oy, ox) of that oz.
+//
+// k_conv_fwd with its stages double-buffered: the halo and weight slab of stage s + 1 travel
+// from global memory straight into LDS (global_load_lds_dwordx4, no registers) while stage s
+// runs its MFMAs from the other buffer.  A stage is (16-channel chunk, dz) or, for PAR's
+// half-resolution chunks, (chunk, oz); planes outside the grid are skipped (uniform per
+// workgroup).  Each lane's 16 bytes land at the wave's LDS base + 16 lane, so every staged
+// buffer is filled in slot order: a slot's source is computed per lane (elements outside the
+// grid read a 16-byte zero in global memory).  Same arithmetic and order as k_conv_fwd.
+__device__ const uint4 g_zero16 = {0u, 0u, 0u, 0u};
+
+__device__ __forceinline__ void dma16(const void* src, uint4* lds_wave_base) {
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
+}
+
+template <bool OUTF32, int ROWS, bool PAR>
+__global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* __restrict__ in0,
+                                                          const unsigned short* __restrict__ in1, ConvGeo g,
+                                                          const unsigned short* __restrict__ w,
+                                                          const unsigned short* __restrict__ wpar,
+                                                          const float* __restrict__ bias, int relu,
+                                                          void* __restrict__ out) {
+    constexpr int TY = 4 * ROWS, HY = TY + 2, HALO = HY * kHX;
+    constexpr int HY2 = TY / 2 + 2, HALO2 = HY2 * kHX2;
+    static_assert(HALO2 <= HALO, "the half-resolution halo reuses the halo buffer");
+    constexpr int MT = 4 * ROWS;
+    constexpr int NTMAX = ROWS == 1 ? 4 : 1;
+    constexpr int SLABS = PAR ? 16 : 9;
+    constexpr int CBM = ROWS == 2 ? 16 : kMaxCB;
+    // two buffers as separate arrays, used in a loop unrolled by two, so that the compiler sees
+    // that the MFMA stage's LDS reads and the copies in flight touch different objects and
+    // does not wait for the copies before the reads
+    __shared__ uint4 haloA[HALO * 2], haloB[HALO * 2];        // [row][x][2 halves of 8 ch]
+    __shared__ uint4 wlsA[SLABS * CBM * 2], wlsB[SLABS * CBM * 2];  // [slab][co][2 halves of 8 ci] (co stride CB)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = lane >> 4, r16 = lane & 15;
+    const int nxt = (g.W + 63) >> 6;
+    const int xt = blockIdx.x % nxt, cb = blockIdx.x / nxt;
+    const int x0 = xt * 64, y0 = blockIdx.y * TY, z = blockIdx.z;
+    const int co0 = cb * g.CB;
+    const int NT = g.CB >> 4;
+    const int Cin = g.C0 + g.C1;
+    const int H2 = g.H >> 1, W2 = g.W >> 1, D2 = g.D >> 1;
+    const int pz = z & 1;
+    f4 acc[MT][NTMAX];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NTMAX; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+    auto xcol = [](int m, int r) { return PAR ? 32 * ((m & 3) >> 1) + 2 * r + (m & 1) : 16 * (m & 3) + r; };
+    const int S = 3 * (Cin >> 4);
+    // stage s: chunk s / 3, d = s % 3 (dz = d - 1, or oz = d for PAR's half-resolution chunks)
+    auto valid = [&](int st) {
+        const int c = st / 3, d = st - 3 * (st / 3);
+        if (PAR && c * 16 >= g.C0) {
+            const int zl = (z >> 1) + pz + d - 1;
+            return d < 2 && zl >= 0 && zl < D2;
+        }
+        const int zz = z + d - 1;
+        return zz >= 0 && zz < g.D;
+    };
+    auto issue = [&](int st, uint4* hb, uint4* wb) {
+        const int c = st / 3, d = st - 3 * (st / 3);
+        const bool hi = c * 16 >= g.C0;
+        if (PAR && hi) {
+            const int zl = (z >> 1) + pz + d - 1, ci0 = c * 16 - g.C0;
+            const int yl0 = (y0 >> 1) - 1, xl0 = (x0 >> 1) - 1;
+            for (int e0 = wave * 64; e0 < HALO2 * 2; e0 += kFwdThreads) {
+                const int e = e0 + lane;
+                if (e < HALO2 * 2) {
+                    const int vox = e >> 1, rx = vox % kHX2, ry = vox / kHX2;
+                    const int yy = yl0 + ry, xx = xl0 + rx;
+                    const void* src = &g_zero16;
+                    if (yy >= 0 && yy < H2 && xx >= 0 && xx < W2)
+                        src = in1 + (((size_t)zl * H2 + yy) * W2 + xx) * g.C1 + ci0 + (e & 1) * 8;
+                    dma16(src, hb + e0);
+                }
+            }
+            const int nw = 16 * g.CB * 2;
+            for (int e0 = wave * 64; e0 < nw; e0 += kFwdThreads) {
+                const int e = e0 + lane;
+                if (e < nw) {
+                    const int sl = e / (g.CB * 2), co = (e >> 1) % g.CB;
+                    const int cls = (pz * 2 + (sl >> 3)) * 2 + ((sl >> 2) & 1), tap = d * 4 + (sl & 3);
+                    dma16(wpar + ((size_t)(cls * 8 + tap) * g.Cout + co0 + co) * g.C1 + ci0 + (e & 1) * 8, wb + e0);
+                }
+            }
+            return;
+        }
+        const int zz = z + d - 1;
+        for (int e0 = wave * 64; e0 < HALO * 2; e0 += kFwdThreads) {
+            const int e = e0 + lane;
+            if (e < HALO * 2) {
+                const int vox = e >> 1;
+                const int ry = vox / kHX, col = vox - ry * kHX;
+                // PAR stores the columns x-parity-split (even ones, then odd ones)
+                const int rx = PAR ? (col < kHX / 2 ? 2 * col : 2 * (col - kHX / 2) + 1) : col;
+                const int yy = y0 + ry - 1, xx = x0 + rx - 1;
+                const void* src = &g_zero16;
+                if (yy >= 0 && yy < g.H && xx >= 0 && xx < g.W) {
+                    if (!hi)
+                        src = in0 + (((size_t)zz * g.H + yy) * g.W + xx) * g.C0 + c * 16 + (e & 1) * 8;
+                    else
+                        src = in1 + (((size_t)(zz >> 1) * H2 + (yy >> 1)) * W2 + (xx >> 1)) * g.C1 + (c * 16 - g.C0) +
+                              (e & 1) * 8;
+                }
+                dma16(src, hb + e0);
+            }
+        }
+        const int nw = 9 * g.CB * 2;
+        for (int e0 = wave * 64; e0 < nw; e0 += kFwdThreads) {
+            const int e = e0 + lane;
+            if (e < nw) {
+                const int tt = e / (g.CB * 2), co = (e >> 1) % g.CB;
+                dma16(w + ((size_t)(d * 9 + tt) * g.Cout + co0 + co) * Cin + c * 16 + (e & 1) * 8, wb + e0);
+            }
+        }
+    };
+    auto compute = [&](int st, const uint4* hb, const uint4* wb) {
+        const int c = st / 3, d = st - 3 * (st / 3);
+        const bool hi = c * 16 >= g.C0;
+        const unsigned short* hs = (const unsigned short*)hb;
+        const s4* wq = (const s4*)wb;
+        if (PAR && hi) {
+#pragma unroll
+            for (int oo = 0; oo < 4; ++oo) {
+                const int oy = oo >> 1, ox = oo & 1;
+                s4 a[MT];
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    const int yr = wave * ROWS + (m >> 2);
+                    const int hrow = (yr >> 1) + (yr & 1) + oy;
+                    const int hcol = 16 * ((m & 3) >> 1) + r16 + (m & 1) + ox;
+                    a[m] = *(const s4*)(hs + (size_t)(hrow * kHX2 + hcol) * 16 + q * 4);
+                }
+#pragma unroll
+                for (int n = 0; n < NTMAX; ++n) {
+                    if (n < NT) {
+#pragma unroll
+                        for (int m = 0; m < MT; ++m) {
+                            const int yr = wave * ROWS + (m >> 2);
+                            const int sl = ((yr & 1) * 2 + (m & 1)) * 4 + oo;
+                            const s4 bb = wq[(sl * g.CB + n * 16 + r16) * 4 + q];
+                            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bb, acc[m][n], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            return;
+        }
+        (void)d;
+#pragma unroll
+        for (int tt = 0; tt < 9; ++tt) {
+            const int dy = tt / 3 - 1, dx = tt % 3 - 1;
+            s4 a[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                const int row = (wave * ROWS + (m >> 2) + dy + 1) * kHX;
+                const int hc = xcol(m, r16) + dx + 1;
+                const int col = PAR ? (hc & 1) * (kHX / 2) + (hc >> 1) : hc;
+                a[m] = *(const s4*)(hs + (size_t)(row + col) * 16 + q * 4);
+            }
+#pragma unroll
+            for (int n = 0; n < NTMAX; ++n) {
+                if (n < NT) {
+                    const s4 bb = wq[(tt * g.CB + n * 16 + r16) * 4 + q];
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+                        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bb, acc[m][n], 0, 0, 0);
+                }
+            }
+        }
+    };
+    auto next = [&](int st) {
+        ++st;
+        while (st < S && !valid(st)) ++st;
+        return st;
+    };
+    int st = next(-1);
+    if (st < S) issue(st, haloA, wlsA);
+    for (;;) {
+        // stage st is in flight to buffer A; the next goes to B
+        if (st >= S) break;
+        int sn = next(st);
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies of stage st are in LDS
+        __syncthreads();                     // every wave's are, and the last stage's reads of B are done
+        if (sn < S) issue(sn, haloB, wlsB);
+        compute(st, haloA, wlsA);
+        st = sn;
+        if (st >= S) break;
+        sn = next(st);
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        if (sn < S) issue(sn, haloA, wlsA);
+        compute(st, haloB, wlsB);
+        st = sn;
+    }
+#pragma unroll
+    for (int n = 0; n < NTMAX; ++n) {
+        if (n >= NT) continue;
+        const int co = co0 + n * 16 + r16;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int y = y0 + wave * ROWS + (m >> 2);
+            if (y >= g.H) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = x0 + xcol(m, q * 4 + i);
+                if (x >= g.W) continue;
+                const size_t o = (((size_t)z * g.H + y) * g.W + x) * g.Cout + co;
+                float val = acc[m][n][i] + bv;
                 if (relu) val = fmaxf(val, 0.0f);
                 if constexpr (OUTF32)
                     ((float*)out)[o] = val;
@@ -713,7 +860,10 @@ hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short*
         return hipErrorInvalidValue;
     if (C1 > 0 && (D % 2 || H % 2 || W % 2 || !in1)) return hipErrorInvalidValue;
     if (D > 65535) return hipErrorInvalidValue;
-    ConvGeo g{D, H, W, C0, C1, Cout, pick_cb(Cout)};
+    // the parity-class form (wpar, inference of decoder layers of up to 32 channels) runs 16
+    // output channels per workgroup, two rows per wave
+    const bool par = wpar && C1 > 0 && !out_f32 && Cout <= 32;
+    ConvGeo g{D, H, W, C0, C1, Cout, par ? 16 : pick_cb(Cout)};
     const long plane = (long)H * W;
     if (plane > (1L << 30)) return hipErrorInvalidValue;
     const long gx = (long)((W + 63) / 64) * (Cout / g.CB);
@@ -722,22 +872,23 @@ hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short*
     if (gx > 0x7fffffffL || (H + ty - 1) / ty > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)gx, (unsigned)((H + ty - 1) / ty), (unsigned)D);
     const dim3 blk(kFwdThreads);
-    // The parity-class form for 16 output channels (two rows per wave) only: with 32 or 64
-    // channels per workgroup (one row per wave) its 16 weight slabs and per-M-tile B operands
-    // measured slower than the 27 taps (1024^2 x 128: 8.05 against 7.33 ms at 32 channels,
-    // 3.44 against 2.87 ms at 64), at 16 channels faster (13.8 against 20.5 ms).
-    if (wpar && C1 > 0 && !out_f32 && rows == 2) {
-        hipLaunchKernelGGL((k_conv_fwd<false, 2, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out);
+    // Two rows per wave (16 output channels) run the double-buffered kernel, the parity-class
+    // form with wpar; one row per wave the register-staged one.  At 1024^2 x 128 the 16-channel
+    // decoder layer took 20.5 ms register-staged with 27 taps, 13.8 with the parity form and
+    // 11.4 double-buffered; the layers of 32 and 64 channels were slower double-buffered (e1
+    // 2.16 against 1.70 ms, d1 8.96 against 7.33: its two buffers hold the workgroup to two per
+    // CU) and slower in parity form (d1 8.05, d2 3.44 against 2.87 ms).
+    if (rows == 2) {
+        if (par)
+            hipLaunchKernelGGL((k_conv_dma<false, 2, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out);
+        else if (out_f32)
+            hipLaunchKernelGGL((k_conv_dma<true, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
+        else
+            hipLaunchKernelGGL((k_conv_dma<false, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
     } else if (out_f32) {
-        if (rows == 2)
-            hipLaunchKernelGGL((k_conv_fwd<true, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
-        else
-            hipLaunchKernelGGL((k_conv_fwd<true, 1, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
+        hipLaunchKernelGGL((k_conv_fwd<true, 1>), grid, blk, 0, s, in0, in1, g, w, bias, relu, out);
     } else {
-        if (rows == 2)
-            hipLaunchKernelGGL((k_conv_fwd<false, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
-        else
-            hipLaunchKernelGGL((k_conv_fwd<false, 1, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
+        hipLaunchKernelGGL((k_conv_fwd<false, 1>), grid, blk, 0, s, in0, in1, g, w, bias, relu, out);
     }
     return hipGetLastError();
 }
@@ -787,7 +938,7 @@ struct Layer {
     int level, C0, C1, Cout;  // C0: in0 channels, C1: in1 (half resolution) channels
     size_t w_off, b_off;       // offsets in the flat parameter array
     unsigned short *w = nullptr, *wT = nullptr;
-    unsigned short* wpar = nullptr;  // parity-class weights of the in1 channels (decoder layers of 16 channels)
+    unsigned short* wpar = nullptr;  // parity-class weights of the in1 channels (decoder layers of <= 32 channels)
 };
 
 #define NET_HIP(call)                                                                              \
@@ -1140,7 +1291,7 @@ rihvr_status rihvr_net_create(int32_t num_planes, int32_t height, int32_t width,
     for (auto& l : n->L) {
         const size_t cnt = (size_t)kTaps * l.Cout * (l.C0 + l.C1);
         if ((st = nalloc(&l.w, cnt)) || (st = nalloc(&l.wT, cnt))) return fail(st);
-        if (l.C1 > 0 && pick_cb(l.Cout) == 16 && (st = nalloc(&l.wpar, (size_t)64 * l.Cout * l.C1))) return fail(st);
+        if (l.C1 > 0 && l.Cout <= 32 && (st = nalloc(&l.wpar, (size_t)64 * l.Cout * l.C1))) return fail(st);
     }
     if ((st = nalloc(&n->w0p, 4 * 16 * 16))) return fail(st);
     for (int k = 0; k < kLevels; ++k)
