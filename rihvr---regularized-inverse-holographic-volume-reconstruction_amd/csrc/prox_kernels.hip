@@ -950,8 +950,12 @@ k_momentum_out(const float2* __restrict__ Y, const float2* __restrict__ Y0, floa
     const long N = (long)W * H;
     const long n = N * nz;
     double l1 = 0, tv = 0, nnz = 0;
+    // s = 0 (no momentum: a start volume's objective, or the last update restarted) reads Y only
+    const bool still = s == 0.0f;
     auto mom = [&](long i) {
-        const float2 y = Y[i], y0 = Y0[i];
+        const float2 y = Y[i];
+        if (still) return y;
+        const float2 y0 = Y0[i];
         return make_float2(fmaf(s, y.x - y0.x, y.x), fmaf(s, y.y - y0.y, y.y));
     };
     for (int it = 0; it < kEwItems; ++it) {

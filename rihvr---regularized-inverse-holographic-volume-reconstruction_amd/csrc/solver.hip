@@ -1273,7 +1273,7 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
     // and TV terms price although FISTA's first prox removes it.  The candidate starts are
     // soft(x_net, q max|x_net|), q in kSoftFractions: the one of least objective is the
     // network's start.
-    const int nb = (int)std::max<long>(1, std::min<long>(256, c->partial_rows * PS_COUNT));
+    const int nb = (int)std::max<long>(1, std::min<long>(4096, c->partial_rows * PS_COUNT));
     RIHVR_HIP(maxabs_partials(xw, c->V, nb, c->partial, c->stream));
     std::vector<double> mx(nb);
     RIHVR_HIP(hipMemcpyAsync(mx.data(), c->partial, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1328,7 +1328,7 @@ rihvr_status reconstruct_impl(rihvr_ctx* c, const float* holo, int holo_loc, con
 
 // sum |v|^2 of a volume into *out (host), through the prox partials' buffer
 rihvr_status volume_norm2(rihvr_ctx* c, const float2* v, double* out) {
-    const int nb = (int)std::max<long>(1, std::min<long>(256, c->partial_rows * PS_COUNT));
+    const int nb = (int)std::max<long>(1, std::min<long>(4096, c->partial_rows * PS_COUNT));
     RIHVR_HIP(norm2_partials(v, c->V, nb, c->partial, c->stream));
     RIHVR_TRY(reduce_partials_on(c, c->partial, nb, 1, c->scal + SL_RESID_M));
     RIHVR_TRY(fetch_scalars(c));

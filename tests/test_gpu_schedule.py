@@ -142,3 +142,21 @@ def test_multi_equals_single_solves(pkg, scenes, gpu_available, autotune, device
         assert r["iterations"] == fs["iterations"] and r["f_final"] == fs["f_final"] and r["nnz"] == fs["nnz"], k
         assert r["reached"]
     assert res[4]["iterations"] <= 6
+
+
+def test_multi_items_match_the_oracle_per_size_class(pkg, orc, scenes, gpu_available):
+    """One item per size class of a mixed batch (two fixed-plan sizes, a run-time-plan size, three
+    depths) against the FP32 oracle: 1e-4 rel-L2 with the oracle's decisions."""
+    specs = [cfg(128, 96, 12, reg=0.01, its=8), cfg(272, 208, 5, reg=0.02, its=6), cfg(64, 64, 8, reg=0.004, its=7)]
+    holos = [scenes.disks_scene(scenes.optics_of(c), 6, 2.0, 4.0, 400 + k, 0.01)[0] for k, c in enumerate(specs)]
+    items = [dict(params=pkg.make_params(**c), holo=h, volume=np.empty((c["num_planes"], c["height"], c["width"]),
+                                                                         np.complex64))
+             for c, h in zip(specs, holos)]
+    res = pkg.reconstruct_multi(items, autotune=1)
+    for c, h, it, r in zip(specs, holos, items, res):
+        ref, _, _, rst = orc.solve(orc.make_params(**c), h, None, "f32")
+        print(f"multi {c['width']}x{c['height']}x{c['num_planes']}: rel {rel(it['volume'], ref):.2e}, "
+              f"schedule {r['schedule']}", flush=True)
+        assert rel(it["volume"], ref) < 1e-4
+        assert r["iterations"] == c["num_inverse_iterations"]
+        assert sum(rst["backtracks"]) == r["total_backtracks"]
