@@ -217,17 +217,26 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
 // grid read a 16-byte zero in global memory).  Same arithmetic and order as k_conv_fwd.
 __device__ const uint4 g_zero16 = {0u, 0u, 0u, 0u};
 
+// The 1x1x1 head fused into the last decoder layer's epilogue (HEAD): the layer's BF16 outputs
+// are not stored; out[v] = out_scale (bo + sum_co wo[.][co] d0[v][co]) is formed from them.
+struct HeadArgs {
+    const float* wo;  // [2][16]
+    const float* bo;  // [2]
+    float out_scale;
+    float2* out;
+};
+
 __device__ __forceinline__ void dma16(const void* src, uint4* lds_wave_base) {
     __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
 }
 
-template <bool OUTF32, int ROWS, bool PAR>
+template <bool OUTF32, int ROWS, bool PAR, bool HEAD = false>
 __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* __restrict__ in0,
                                                           const unsigned short* __restrict__ in1, ConvGeo g,
                                                           const unsigned short* __restrict__ w,
                                                           const unsigned short* __restrict__ wpar,
                                                           const float* __restrict__ bias, int relu,
-                                                          void* __restrict__ out) {
+                                                          void* __restrict__ out, HeadArgs head) {
     constexpr int TY = 4 * ROWS, HY = TY + 2, HALO = HY * kHX;
     constexpr int HY2 = TY / 2 + 2, HALO2 = HY2 * kHX2;
     static_assert(HALO2 <= HALO, "the half-resolution halo reuses the halo buffer");
@@ -403,6 +412,72 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
         compute(st, haloB, wlsB);
         st = sn;
     }
+    if constexpr (HEAD) {
+        // 16 output channels (NT = 1): the BF16 outputs go through LDS (the halo buffer, free
+        // after the barrier), then a thread per voxel forms the head's two sums in channel order
+        // (as k_head_fwd does) and stores its complex output
+        __syncthreads();
+        unsigned short* ot = (unsigned short*)haloA;  // [row][x][16 ch]
+        const float bv = bias[r16];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int row = wave * ROWS + (m >> 2);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                ot[(row * 64 + xcol(m, q * 4 + i)) * 16 + r16] = f2bf(fmaxf(acc[m][0][i] + bv, 0.0f));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TY * 64 / kFwdThreads; ++k) {
+            const int vox = threadIdx.x + k * kFwdThreads, row = vox >> 6, x = vox & 63;
+            const int y = y0 + row, xx = x0 + x;
+            if (y >= g.H || xx >= g.W) continue;
+            float o0 = head.bo[0], o1 = head.bo[1];
+#pragma unroll
+            for (int hf = 0; hf < 2; ++hf) {
+                const uint4 u = haloA[vox * 2 + hf];
+                const unsigned wd[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int co = hf * 8 + j;
+                    const float h = bf2f((unsigned short)((wd[j >> 1] >> ((j & 1) * 16)) & 0xffffu));
+                    o0 += head.wo[co] * h;
+                    o1 += head.wo[16 + co] * h;
+                }
+            }
+            head.out[((size_t)z * g.H + y) * g.W + xx] = make_float2(o0 * head.out_scale, o1 * head.out_scale);
+        }
+        return;
+    }
+    if constexpr (!OUTF32 && ROWS == 2) {
+        // 16 output channels (CB = 16): through LDS (the halo buffer, free after the barrier),
+        // so that each voxel's 32 bytes leave as two 16-byte stores of coalesced rows
+        __syncthreads();
+        unsigned short* ot = (unsigned short*)haloA;  // [row][x][16 ch]
+        const int co = co0 + r16;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int row = wave * ROWS + (m >> 2);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float val = acc[m][0][i] + bv;
+                if (relu) val = fmaxf(val, 0.0f);
+                ot[(row * 64 + xcol(m, q * 4 + i)) * 16 + r16] = f2bf(val);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TY * 64 * 2 / kFwdThreads; ++k) {
+            const int e = threadIdx.x + k * kFwdThreads;  // uint4 of the tile
+            const int vox = e >> 1, row = vox >> 6, x = vox & 63;
+            const int y = y0 + row, xx = x0 + x;
+            if (y < g.H && xx < g.W)
+                *(uint4*)((unsigned short*)out + (((size_t)z * g.H + y) * g.W + xx) * g.Cout + co0 + (e & 1) * 8) =
+                    haloA[e];
+        }
+        return;
+    }
 #pragma unroll
     for (int n = 0; n < NTMAX; ++n) {
         if (n >= NT) continue;
@@ -532,21 +607,28 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_first(const float2* __rest
                                                             float scale, const unsigned short* __restrict__ wp,
                                                             const float* __restrict__ bias,
                                                             unsigned short* __restrict__ out) {
-    constexpr int HY = 10, HX = 66, HALO = 3 * HY * HX;
-    __shared__ unsigned halo[HALO];  // (Re, Im) BF16 pair per voxel
+    constexpr int HY = 10, HX = 66, HROWS = 3 * HY;
+    __shared__ unsigned halo[HROWS * HX];   // (Re, Im) BF16 pair per voxel, [dz][row][x]
+    __shared__ uint4 otile[8 * 64 * 2];      // the tile's outputs, [row][x][16 ch] BF16
     __shared__ s4 wls[4 * 16 * 4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = lane >> 4, r16 = lane & 15;
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 8, z = blockIdx.z;
-    for (int e = threadIdx.x; e < HALO; e += kFwdThreads) {
-        const int rx = e % HX, ry = (e / HX) % HY, dzi = e / (HX * HY);
-        const int zz = z + dzi - 1, yy = y0 + ry - 1, xx = x0 + rx - 1;
-        unsigned v = 0;
-        if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W) {
-            const float2 a = in[((size_t)zz * H + yy) * W + xx];
-            v = (unsigned)f2bf(a.x * scale) | ((unsigned)f2bf(a.y * scale) << 16);
+    // a wave per halo row: 64 + 2 voxels, one (two for lanes 0, 1) coalesced 8-byte load per lane
+    for (int r = wave; r < HROWS; r += 4) {
+        const int dzi = r / HY, ry = r - dzi * HY;
+        const int zz = z + dzi - 1, yy = y0 + ry - 1;
+        const bool row_in = zz >= 0 && zz < D && yy >= 0 && yy < H;
+        const float2* src = in + ((size_t)(row_in ? zz : 0) * H + (row_in ? yy : 0)) * W;
+        for (int rx = lane; rx < HX; rx += 64) {
+            const int xx = x0 + rx - 1;
+            unsigned v = 0;
+            if (row_in && xx >= 0 && xx < W) {
+                const float2 a = src[xx];
+                v = (unsigned)f2bf(a.x * scale) | ((unsigned)f2bf(a.y * scale) << 16);
+            }
+            halo[r * HX + rx] = v;
         }
-        halo[e] = v;
     }
     for (int e = threadIdx.x; e < 4 * 16 * 4; e += kFwdThreads) wls[e] = *(const s4*)(wp + e * 4);
     __syncthreads();
@@ -578,17 +660,25 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_first(const float2* __rest
             acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[m], 0, 0, 0);
         }
     }
+    // through LDS, so that the tile's rows (64 voxels x 32 bytes each) leave as 16-byte stores
     const float bv = bias[r16];
+    unsigned short* ot = (unsigned short*)otile;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int y = y0 + wave * 2 + (m >> 2);
-        if (y >= H) continue;
+        const int row = wave * 2 + (m >> 2);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int x = x0 + (m & 3) * 16 + q * 4 + i;
-            if (x >= W) continue;
-            out[(((size_t)z * H + y) * W + x) * 16 + r16] = f2bf(fmaxf(acc[m][i] + bv, 0.0f));
+            const int x = (m & 3) * 16 + q * 4 + i;
+            ot[(row * 64 + x) * 16 + r16] = f2bf(fmaxf(acc[m][i] + bv, 0.0f));
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int e = threadIdx.x + k * kFwdThreads;  // uint4 of the tile
+        const int vox = e >> 1, row = vox >> 6, x = vox & 63;
+        const int y = y0 + row, xx = x0 + x;
+        if (y < H && xx < W) *(uint4*)(out + (((size_t)z * H + y) * W + xx) * 16 + (e & 1) * 8) = otile[e];
     }
 }
 
@@ -855,7 +945,7 @@ int pick_cb(int Cout) {
 
 hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short* in1, int C1, int D, int H, int W,
                         const unsigned short* w, const float* bias, int Cout, int relu, void* out, bool out_f32,
-                        hipStream_t s, const unsigned short* wpar = nullptr) {
+                        hipStream_t s, const unsigned short* wpar = nullptr, const HeadArgs* head = nullptr) {
     if (C0 <= 0 || C0 % 16 || C1 < 0 || C1 % 16 || Cout <= 0 || Cout % 16 || D <= 0 || H <= 0 || W <= 0)
         return hipErrorInvalidValue;
     if (C1 > 0 && (D % 2 || H % 2 || W % 2 || !in1)) return hipErrorInvalidValue;
@@ -878,13 +968,21 @@ hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short*
     // 11.4 double-buffered; the layers of 32 and 64 channels were slower double-buffered (e1
     // 2.16 against 1.70 ms, d1 8.96 against 7.33: its two buffers hold the workgroup to two per
     // CU) and slower in parity form (d1 8.05, d2 3.44 against 2.87 ms).
+    const HeadArgs nohead{nullptr, nullptr, 0.0f, nullptr};
+    if (head && !(par && Cout == 16 && relu)) return hipErrorInvalidValue;
     if (rows == 2) {
-        if (par)
-            hipLaunchKernelGGL((k_conv_dma<false, 2, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out);
+        if (par && head)
+            hipLaunchKernelGGL((k_conv_dma<false, 2, true, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu,
+                               out, *head);
+        else if (par)
+            hipLaunchKernelGGL((k_conv_dma<false, 2, true>), grid, blk, 0, s, in0, in1, g, w, wpar, bias, relu, out,
+                               nohead);
         else if (out_f32)
-            hipLaunchKernelGGL((k_conv_dma<true, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
+            hipLaunchKernelGGL((k_conv_dma<true, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out,
+                               nohead);
         else
-            hipLaunchKernelGGL((k_conv_dma<false, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out);
+            hipLaunchKernelGGL((k_conv_dma<false, 2, false>), grid, blk, 0, s, in0, in1, g, w, nullptr, bias, relu, out,
+                               nohead);
     } else if (out_f32) {
         hipLaunchKernelGGL((k_conv_fwd<true, 1>), grid, blk, 0, s, in0, in1, g, w, bias, relu, out);
     } else {
@@ -1063,11 +1161,11 @@ rihvr_status prep_weights(rihvr_net* n) {
 // the convolution of layer i on its inputs; parity: in1's channels through the parity-class
 // weights (inference), else through the 27 taps on the upsampled grid (training)
 rihvr_status layer_forward(rihvr_net* n, int i, const unsigned short* in0, const unsigned short* in1,
-                           unsigned short* out, bool parity = false) {
+                           unsigned short* out, bool parity = false, const HeadArgs* head = nullptr) {
     const Layer& l = n->L[i];
     const int k = l.level;
     NET_HIP(conv_forward(in0, l.C0, in1, l.C1, n->Dk[k], n->Hk[k], n->Wk[k], l.w, n->params + l.b_off, l.Cout, 1, out,
-                         false, n->stream, parity ? l.wpar : nullptr));
+                         false, n->stream, parity ? l.wpar : nullptr, head));
     return RIHVR_OK;
 }
 
@@ -1081,7 +1179,9 @@ rihvr_status pool(rihvr_net* n, int k, const unsigned short* e, unsigned short* 
 
 // forward pass up to d0 (the head is applied by the caller); with_x16 also writes the 16-channel
 // input the input layer's weight gradient reads
-rihvr_status forward_body(rihvr_net* n, const float* in, bool with_x16) {
+// with_x16: the training form (x16 kept for the backward pass, 27 taps everywhere); else the
+// prediction, whose head is fused into the last decoder layer and writes pred (complex volume)
+rihvr_status forward_body(rihvr_net* n, const float* in, bool with_x16, float* pred = nullptr) {
     if (!n->weights_ready) {
         const rihvr_status st = prep_weights(n);
         if (st) return st;
@@ -1101,6 +1201,10 @@ rihvr_status forward_body(rihvr_net* n, const float* in, bool with_x16) {
     const bool parity = !with_x16;  // prediction: the decoder's upsampled channels in parity-class form
     if ((st = layer_forward(n, 4, n->e[2], n->e[3], n->d[2], parity))) return st;
     if ((st = layer_forward(n, 5, n->e[1], n->d[2], n->d[1], parity))) return st;
+    if (parity && pred) {
+        const HeadArgs h{n->params + n->head_off, n->params + n->head_off + 32, n->out_scale, (float2*)pred};
+        return layer_forward(n, 6, n->e[0], n->d[1], n->d[0], true, &h);
+    }
     if ((st = layer_forward(n, 6, n->e[0], n->d[1], n->d[0], parity))) return st;
     return RIHVR_OK;
 }
@@ -1235,11 +1339,7 @@ rihvr_status predict(rihvr_net* n, const float* in_dev, float* out_dev, hipStrea
         NET_HIP(hipEventDestroy(ev));
     }
     rihvr_status st;
-    if ((st = forward_body(n, in_dev, false))) return st;
-    const float* wo = n->params + n->head_off;
-    hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)n->head_blocks), dim3(kEw), 0, n->stream, n->d[0], n->Vk[0], wo,
-                       wo + 32, n->out_scale, (float2*)out_dev, (const float2*)nullptr, 0.0f, (double*)nullptr);
-    NET_HIP(hipGetLastError());
+    if ((st = forward_body(n, in_dev, false, out_dev))) return st;
     NET_HIP(hipStreamSynchronize(n->stream));
     return RIHVR_OK;
 }
