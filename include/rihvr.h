@@ -649,10 +649,11 @@ typedef struct rihvr_learned_stats {
     int32_t fallback;       /* 0: the network's start soft(x_net, tau), 1: the zero start */
     int32_t reached;        /* 1: the objective target was reached */
     int32_t total_backtracks;
-    double  resid_net;      /* 0.5 ||A x_net - b||^2 of the network's start */
+    double  resid_net;      /* NaN: 0.5 ||A x_net - b||^2 of the raw prediction is not evaluated */
     double  resid_adjoint;  /* 0.5 ||A (t A^T b) - b||^2 of the A^T b start, t = 1/Nz (zero for a
                                real hologram: A A^T = Nz I on real holograms) */
-    double  f_net;          /* objective (calcObjectiveFunction) at the network's output x_net */
+    double  f_net;          /* NaN: the raw prediction's objective is not evaluated (the start test
+                               needs only the thresholded candidates') */
     double  f_adjoint;      /* objective at the A^T b start */
     double  f_zero;         /* objective at the zero start, 0.5 ||b||^2 */
     double  f_start;        /* objective at the chosen start */

@@ -1268,11 +1268,11 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
         *f = objective(c, mode, c->h_scal[SL_RESID_A], c->h_scal[SL_PROX + PS_L1X0], c->h_scal[SL_PROX + PS_TVX0]);
         return RIHVR_OK;
     };
-    RIHVR_TRY(volume_objective(xw, &st.resid_net, &st.f_net));
     // The network's output is dense (every voxel holds a small value), which the objective's L1
     // and TV terms price although FISTA's first prox removes it.  The candidate starts are
     // soft(x_net, q max|x_net|), q in kSoftFractions: the one of least objective is the
-    // network's start.
+    // network's start (x_net's own objective is not needed, and not evaluated).
+    st.resid_net = st.f_net = std::nan("");
     const int nb = (int)std::max<long>(1, std::min<long>(4096, c->partial_rows * PS_COUNT));
     RIHVR_HIP(maxabs_partials(xw, c->V, nb, c->partial, c->stream));
     std::vector<double> mx(nb);

@@ -149,8 +149,8 @@ def evaluate(net, optics: str, size: int, n_frames: int, seed: int, iterations: 
             rows.append(dict(f_plain=plain["f_final"], cold_iterations=cold["iterations"], cold_s=t_cold,
                              learned_iterations=lr_["iterations"], learned_s=t_learn,
                              fallback=lr_["fallback"],
-                             reached=lr_["reached"], f_learned=lr_["f_final"], f_net=lr_["f_net"],
-                             f_adjoint=lr_["f_adjoint"], f_zero=lr_["f_zero"], resid_net=lr_["resid_net"],
+                             reached=lr_["reached"], f_learned=lr_["f_final"],
+                             f_adjoint=lr_["f_adjoint"], f_zero=lr_["f_zero"],
                              tau=lr_["tau"], f_thresholded=lr_["f_thresholded"],
                              ms_network=lr_["ms_network"], ms_adjoint=lr_["ms_adjoint"], ms_solve=lr_["ms_solve"]))
             if log:
