@@ -575,13 +575,41 @@ __global__ void __launch_bounds__(64) k_conv_wgrad(const unsigned short* __restr
     }
 }
 
-// out[i] = sum over chunks (in order) of part[chunk][i]
-__global__ void k_sum_chunks(const float* __restrict__ part, long n, int nchunks, float* __restrict__ out) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// out[i] = sum over chunks of part[chunk][i]: a workgroup takes 16 entries, its 16 lanes per
+// entry sum chunks j, j + 16, ... in order, then a fixed tree adds the 16 lane sums
+constexpr int kSumLanes = 16;
+__global__ void __launch_bounds__(256) k_sum_chunks(const float* __restrict__ part, long n, int nchunks,
+                                                    float* __restrict__ out) {
+    __shared__ float red[kSumLanes][16 + 1];
+    const int e = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const long i = (long)blockIdx.x * 16 + e;
     float s = 0.0f;
-    for (int c = 0; c < nchunks; ++c) s += part[(size_t)c * n + i];
-    out[i] = s;
+    if (i < n)
+        for (int c = j; c < nchunks; c += kSumLanes) s += part[(size_t)c * n + i];
+    red[j][e] = s;
+    __syncthreads();
+    for (int h = kSumLanes / 2; h > 0; h >>= 1) {
+        if (j < h) red[j][e] += red[j + h][e];
+        __syncthreads();
+    }
+    if (j == 0 && i < n) out[i] = red[0][e];
+}
+
+// out[i] = sum over chunks of part[chunk][i] for few entries and many chunks: a workgroup per
+// entry, thread t sums chunks t, t + 256, ... in order, then a fixed tree over the threads
+__global__ void __launch_bounds__(256) k_sum_chunks_wide(const float* __restrict__ part, long n, int nchunks,
+                                                         float* __restrict__ out) {
+    __shared__ float red[256];
+    const long i = blockIdx.x;
+    float s = 0.0f;
+    for (int c = threadIdx.x; c < nchunks; c += 256) s += part[(size_t)c * n + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = red[0];
 }
 
 // x16[v] = (in_scale Re, in_scale Im, 0 x 14) in BF16
@@ -992,12 +1020,16 @@ hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short*
 }
 
 // rows per chunk so that about 8192 waves run and the partials stay within cap floats
+// Rows per chunk so that about 4096 waves run and the partials stay within 16 M floats (and
+// cap): the chunk sum reads nchunks x per floats, one sequential sum per weight, so many small
+// chunks cost more in the sum than they gain in the MFMA pass (at 8,192 waves a 64 x 128 x 128
+// training step spent 4.4 ms summing partials against 1.5 ms computing them).
 int wgrad_chunks(long rows, int Cout, int Cin, long cap) {
     const long tiles = (long)(Cout / 16) * (Cin / 16);
     const long per = (long)kTaps * Cout * Cin + Cout;
-    long n = std::max(1L, (8192 + tiles - 1) / tiles);
+    long n = std::max(1L, (4096 + tiles - 1) / tiles);
     n = std::min(n, rows);
-    n = std::min(n, std::max(1L, cap / per));
+    n = std::min(n, std::max(1L, std::min(cap, 16L << 20) / per));
     return (int)n;
 }
 
@@ -1013,7 +1045,7 @@ hipError_t conv_wgrad(const unsigned short* in0, int C0, const unsigned short* i
     if ((long)nch * per > part_cap) return hipErrorInvalidValue;
     ConvGeo g{D, H, W, C0, C1, Cout, 16};
     hipLaunchKernelGGL(k_conv_wgrad, dim3(Cout / 16, Cin / 16, nch), dim3(64), 0, s, in0, in1, g, gout, nch, part);
-    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, s, (const float*)part, per,
+    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((per + 15) / 16)), dim3(256), 0, s, (const float*)part, per,
                        nch, dw_db);
     return hipGetLastError();
 }
@@ -1269,7 +1301,7 @@ rihvr_status loss_grad(rihvr_net* n, const float* in, const float* target, doubl
                        n->out_scale, (float2*)nullptr, (const float2*)target, n->pos_weight, n->lpart);
     hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)n->head_blocks), dim3(kEw), 0, n->stream, n->d[0], n->Vk[0], wo, bo,
                        n->out_scale, (const float2*)target, n->pos_weight, n->gd[0], n->hpart);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(64), 0, n->stream, (const float*)n->hpart, 34L,
+    hipLaunchKernelGGL(k_sum_chunks_wide, dim3(34), dim3(256), 0, n->stream, (const float*)n->hpart, 34L,
                        (int)n->head_blocks, n->grads + n->head_off);
     NET_HIP(hipGetLastError());
     // decoder
