@@ -508,13 +508,16 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
 // (NW = 27 Cout Cin; the bias sum is written by the ci-tile 0 workgroups).  One wave per
 // workgroup; grid (Cout/16, Cin/16, nchunks).  Per 16-voxel row segment the gradient tile is
 // staged transposed ([co][x]) and the input window of the 9 (dz, dy) rows over x0-1 .. x0+16
-// as loaded ([dzdy][x][ci]); MFMA: M = co, N = ci, K = the 16 voxels of the segment.
+// transposed too ([dzdy][ci][x]), so that a lane's B fragment (4 consecutive voxels of one
+// channel) for the three dx of a row comes from two 8-byte reads; MFMA: M = co, N = ci, K = the
+// 16 voxels of the segment.
+constexpr int kWX = 24;  // the weight-gradient window's row: 18 voxels padded to 24 (48 bytes)
 __global__ void __launch_bounds__(64) k_conv_wgrad(const unsigned short* __restrict__ in0,
                                                    const unsigned short* __restrict__ in1, ConvGeo g,
                                                    const unsigned short* __restrict__ gout, int nchunks,
                                                    float* __restrict__ part) {
     __shared__ unsigned short ga[16 * 16];         // [co][x]
-    __shared__ unsigned short win[9 * 18 * 16];    // [dz dy][x + 1][ci]
+    __shared__ __align__(16) unsigned short win[9 * 16 * kWX];  // [dz dy][ci][x + 1], rows padded to kWX
     const int lane = threadIdx.x;
     const int q = lane >> 4, r16 = lane & 15;
     const int co0 = blockIdx.x * 16, ci0 = blockIdx.y * 16;
@@ -537,11 +540,23 @@ __global__ void __launch_bounds__(64) k_conv_wgrad(const unsigned short* __restr
 #pragma unroll
                 for (int j = 0; j < 4; ++j) ga[(cg + j) * 16 + xi] = (unsigned short)v[j];
             }
-            for (int e = lane; e < 9 * 18 * 4; e += 64) {
-                const int combo = e / 72, rem = e - combo * 72;
-                const int xi = rem >> 2, cg = (rem & 3) * 4;
-                const s4 v = load_in4(in0, in1, g, z + combo / 3 - 1, y + combo % 3 - 1, x0 - 1 + xi, ci0 + cg);
-                *(s4*)(win + (combo * 18 + xi) * 16 + cg) = v;
+            // a task: 4 voxels (x + 1 = 4 xg .. 4 xg + 3) x 4 channels of one (dz, dy) row, loaded
+            // as 4 voxel vectors and written transposed as 4 channel rows of 4 voxels
+            for (int e = lane; e < 9 * 5 * 4; e += 64) {
+                const int combo = e / 20, rem = e - combo * 20;
+                const int xg = rem >> 2, cg = (rem & 3) * 4;
+                s4 v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int xi = xg * 4 + k;
+                    v[k] = xi < 18 ? load_in4(in0, in1, g, z + combo / 3 - 1, y + combo % 3 - 1, x0 - 1 + xi, ci0 + cg)
+                                   : s4{0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const s4 row = {v[0][j], v[1][j], v[2][j], v[3][j]};
+                    *(s4*)(win + (combo * 16 + cg + j) * kWX + xg * 4) = row;
+                }
             }
             __syncthreads();
             const s4 a = *(const s4*)(ga + r16 * 16 + q * 4);
@@ -551,12 +566,16 @@ __global__ void __launch_bounds__(64) k_conv_wgrad(const unsigned short* __restr
             }
 #pragma unroll
             for (int combo = 0; combo < 9; ++combo) {
+                // B for the 3 dx of this (dz, dy): voxels 4q + j + dx + 1 of channel r16, from the
+                // 8 voxels 4q .. 4q + 7 (two 8-byte reads) by byte shifts
+                const unsigned short* rowp = win + (combo * 16 + r16) * kWX + q * 4;
+                const unsigned long long lo = *(const unsigned long long*)rowp;
+                const unsigned long long hi = *(const unsigned long long*)(rowp + 4);
+                const unsigned long long bw[3] = {lo, (lo >> 16) | (hi << 48), (lo >> 32) | (hi << 32)};
 #pragma unroll
-                for (int dx = -1; dx <= 1; ++dx) {
-                    s4 b;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) b[j] = (short)win[(combo * 18 + q * 4 + j + dx + 1) * 16 + r16];
-                    const int t = combo * 3 + dx + 1;
+                for (int d = 0; d < 3; ++d) {
+                    const s4 b = __builtin_bit_cast(s4, bw[d]);
+                    const int t = combo * 3 + d;
                     acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
                 }
             }
