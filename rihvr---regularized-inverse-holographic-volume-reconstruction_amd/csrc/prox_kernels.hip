@@ -1151,6 +1151,56 @@ __global__ void __launch_bounds__(kEwThreads) k_soft(const float2* __restrict__ 
     }
     out[i] = r;
 }
+// k_soft that also builds the output's line-occupancy map (cleared beforehand): bit x >> lsh of
+// word z H + y for every non-zero voxel, and the plane words after the Nz H row words.  A wave
+// inside one row ORs its lanes' bits and sets them with one atomic per word; a wave across rows
+// sets each non-zero lane's bit itself.
+__global__ void __launch_bounds__(kEwThreads) k_soft_map(const float2* __restrict__ in, float2* __restrict__ out,
+                                                         int W, int H, int nz, float tau, int lsh, uint64_t* occ) {
+    const long n = (long)W * H * nz;
+    const long i = (long)blockIdx.x * kEwThreads + threadIdx.x;
+    bool nzv = false;
+    long row = 0;
+    int x = 0;
+    if (i < n) {
+        const float2 e = in[i];
+        const float m = sqrtf(e.x * e.x + e.y * e.y);
+        float2 r = make_float2(0.0f, 0.0f);
+        if (m > tau) {
+            const float sc = (m - tau) / m;
+            r = make_float2(e.x * sc, e.y * sc);
+            nzv = r.x != 0.0f || r.y != 0.0f;
+        }
+        out[i] = r;
+        row = i / W;
+        x = (int)(i - row * W);
+    }
+    uint64_t* planes = occ + (size_t)nz * H;
+    const long wave0 = i - (threadIdx.x & 63);
+    const long last = std::min(wave0 + 63, n - 1);
+    if (wave0 / W == last / W) {  // uniform: the wave lies in one row
+        uint64_t bits = nzv ? (1ull << (x >> lsh)) : 0ull;
+        for (int o = 32; o > 0; o >>= 1) bits |= __shfl_xor(bits, o);
+        if ((threadIdx.x & 63) == 0 && bits && i < n) {
+            atomicOr((unsigned long long*)&occ[row], (unsigned long long)bits);
+            atomicOr((unsigned long long*)&planes[row / H], (unsigned long long)bits);
+        }
+    } else if (nzv) {
+        const unsigned long long b = 1ull << (x >> lsh);
+        atomicOr((unsigned long long*)&occ[row], b);
+        atomicOr((unsigned long long*)&planes[row / H], b);
+    }
+}
+hipError_t soft_threshold_map(const float2* in, float2* out, int W, int H, int nz, float tau, uint64_t* occ,
+                              hipStream_t s) {
+    int lsh = 4;
+    while (((W - 1) >> lsh) >= 64) ++lsh;
+    const long n = (long)W * H * nz;
+    hipLaunchKernelGGL(k_soft_map, dim3((unsigned)((n + kEwThreads - 1) / kEwThreads)), dim3(kEwThreads), 0, s, in,
+                       out, W, H, nz, tau, lsh, occ);
+    return hipGetLastError();
+}
+
 hipError_t soft_threshold(const float2* in, float2* out, long n, float tau, hipStream_t s) {
     hipLaunchKernelGGL(k_soft, dim3((unsigned)((n + kEwThreads - 1) / kEwThreads)), dim3(kEwThreads), 0, s, in, out,
                        n, tau);

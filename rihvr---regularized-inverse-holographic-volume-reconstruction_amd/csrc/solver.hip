@@ -798,10 +798,18 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     if (warm && in_ring < 0) RIHVR_TRY(copy_in(c, Y0, warm, warm_loc, c->V));
     else if (!warm) RIHVR_HIP(hipMemsetAsync(Y0, 0, c->V * sizeof(float2), c->stream));
     RIHVR_HIP(hipMemcpyAsync(YM, Y0, c->V * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
-    // a zero start has the empty map; a warm start none (every line is read once)
-    for (int k : {c->iy0, c->iym}) {
-        c->occ_valid[k] = !warm;
-        if (!warm) RIHVR_HIP(hipMemsetAsync(c->occ[k], 0, map_words(c) * sizeof(uint64_t), c->stream));
+    // a zero start has the empty map; a warm start in a ring slot keeps the map it has (YM gets
+    // a copy); another warm start none (every line is read once)
+    const bool keep_map = warm && in_ring >= 0 && c->occ_valid[in_ring];
+    if (keep_map) {
+        RIHVR_HIP(hipMemcpyAsync(c->occ[c->iym], c->occ[in_ring], map_words(c) * sizeof(uint64_t),
+                                 hipMemcpyDeviceToDevice, c->stream));
+        c->occ_valid[c->iym] = true;
+    } else {
+        for (int k : {c->iy0, c->iym}) {
+            c->occ_valid[k] = !warm;
+            if (!warm) RIHVR_HIP(hipMemsetAsync(c->occ[k], 0, map_words(c) * sizeof(uint64_t), c->stream));
+        }
     }
     RIHVR_HIP(hipMemsetAsync(c->hC, 0, c->N * sizeof(float2), c->stream));
     RIHVR_HIP(hipMemsetAsync(c->hP, 0, c->N * sizeof(float2), c->stream));
@@ -1285,9 +1293,12 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
     st.f_thresholded = HUGE_VAL;
     for (int k = 0; k < (int)(sizeof(kSoftFractions) / sizeof(kSoftFractions[0])); ++k) {
         float2* b = cand[k & 1] == xs ? cand[(k + 1) & 1] : cand[k & 1];
-        invalidate_occ(c, b);
+        const int kb = ring_index(c, b);
         const float tau = (float)(kSoftFractions[k] * xmax);
-        RIHVR_HIP(soft_threshold(xw, b, c->V, tau, c->stream));
+        // the candidate is sparse: with its occupancy map, A and the first proxes skip its empty lines
+        RIHVR_HIP(fill_u64(c->occ[kb], (long)map_words(c), 0ull, c->stream));
+        RIHVR_HIP(soft_threshold_map(xw, b, c->W, c->H, c->Nz, tau, c->occ[kb], c->stream));
+        c->occ_valid[kb] = true;
         double r = 0, f = 0;
         RIHVR_TRY(volume_objective(b, &r, &f));
         if (f < st.f_thresholded) {
@@ -1457,10 +1468,14 @@ rihvr_status autotune_impl(const rihvr_params* p, const float* holo, int holo_lo
     const bool fixed = fft_length_supported(p->width) && fft_length_supported(p->height);
     if (!fixed) cur.fft_path = 1;
     RIHVR_TRY(measure(cur));
+    // a candidate that does not fit in memory (the run-time plans' transposed planes take one more
+    // volume) is skipped; any other error ends the search
     auto try_set = [&](std::vector<rihvr_schedule> cands) -> rihvr_status {
         for (rihvr_schedule s : cands) {
             if (same(s, cur)) continue;
-            RIHVR_TRY(measure(s));
+            const rihvr_status e = measure(s);
+            if (e == RIHVR_E_OUT_OF_MEMORY) continue;
+            RIHVR_TRY(e);
             if (s.ms_per_iteration < cur.ms_per_iteration) cur = s;
         }
         return RIHVR_OK;

@@ -237,6 +237,7 @@ def test_learned_start_decision_follows_the_residual_rule(pkg, scenes, gpu_avail
     shape = (c["num_planes"], c["height"], c["width"])
     with pkg.SparseCompressiveHolo(pkg.make_params(**c)) as rec, pkg.WarmStartNet(16, 64, 64, seed=3) as net:
         st = rec.reconstructLearned(net, h, f_target=0.0, max_iterations=5)
+        vol_learned = rec.getVolume()
         g = torch.empty(shape, dtype=torch.complex64, device="cuda")
         rec.adjoint(h, out=g)
         x = net.predict(g)
@@ -247,8 +248,18 @@ def test_learned_start_decision_follows_the_residual_rule(pkg, scenes, gpu_avail
             tau = np.float32(q * xmax)
             m = torch.abs(x)
             xs = x * torch.clamp((m - float(tau)) / torch.clamp(m, min=1e-30), min=0)
-            fq[q] = (float(tau), rec.begin(h, warm_start=xs.contiguous()))
+            fq[q] = (float(tau), rec.begin(h, warm_start=xs.contiguous()), xs.contiguous())
+        # the learned solve started from its candidate in a ring slot with the candidate's line
+        # map; the same start passed from outside (no map: every line read) gives the same solve
+        best_q = min(fq, key=lambda k: fq[k][1])
+        ref = None
+        if st["fallback"] == 0:
+            rec.reconstructBatch(h[None], warm_start=fq[best_q][2], max_iterations=5)
+            ref = rec.getVolume()
+    fq = {k: v[:2] for k, v in fq.items()}
     print(st, fq, flush=True)
+    if ref is not None:
+        assert np.linalg.norm(vol_learned - ref) <= 1e-5 * np.linalg.norm(ref)
     best = min(fq.values(), key=lambda v: v[1])
     assert abs(st["f_thresholded"] - best[1]) <= 1e-5 * best[1]
     assert any(abs(st["tau"] - t) <= 1e-6 * t and abs(f - best[1]) <= 1e-5 * best[1] for t, f in fq.values())

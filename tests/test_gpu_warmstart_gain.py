@@ -33,3 +33,36 @@ def test_trained_network_reaches_the_plain_objective_in_fewer_iterations(pkg, gp
     # the learned solves end at the plain solver's objective (within the tolerance)
     for r in s["rows"]:
         assert r["f_learned"] <= (1 + 1e-3) * r["f_plain"] * (1 + 1e-9)
+
+
+def test_learned_start_with_line_maps_equals_the_unmapped_start(pkg, gpu_available):
+    """The accepted learned start stays in its ring slot with the occupancy map its threshold
+    kernel built, so A and the first proxes skip its empty lines; the same start passed in from
+    outside (no map: every line read) must give the same solve.  With the committed network on a
+    held-out 256^2 x 128 hologram, 6 iterations: 1e-5 rel-L2 (the start differs only by the
+    rounding of the soft threshold's |x|, formed here by torch)."""
+    import torch
+    import __graft_entry__ as entry
+    ws = entry._load_submodule("warmstart")
+    with np.load(WEIGHTS, allow_pickle=False) as z:
+        params, i_s, o_s = z["params"], float(z["in_scale"]), float(z["out_scale"])
+    cfg = ws.config("c2", 256, 100)
+    shape = (cfg["num_planes"], 256, 256)
+    h = torch.from_numpy(ws.scene("c2", cfg, 700101)).cuda()
+    with pkg.SparseCompressiveHolo(pkg.make_params(**cfg)) as rec, pkg.WarmStartNet(*shape) as net:
+        net.set_params(params, i_s, o_s)
+        st = rec.reconstructLearned(net, h, f_target=0.0, max_iterations=6)
+        vol = rec.getVolume()
+        assert st["fallback"] == 0, st
+        g = torch.empty(shape, dtype=torch.complex64, device="cuda")
+        rec.adjoint(h, out=g)
+        x = net.predict(g)
+        m = torch.abs(x)
+        xs = (x * torch.clamp((m - float(st["tau"])) / torch.clamp(m, min=1e-30), min=0)).contiguous()
+        f0 = rec.begin(h, warm_start=xs)
+        assert abs(f0 - st["f_start"]) <= 1e-5 * abs(f0)
+        rec.reconstructBatch(h[None], warm_start=xs, max_iterations=6)
+        ref = rec.getVolume()
+    r = float(np.linalg.norm(vol - ref) / np.linalg.norm(ref))
+    print(f"mapped vs unmapped learned start: rel {r:.2e}, f_start {st['f_start']:.6g} vs {f0:.6g}", flush=True)
+    assert r <= 1e-5
