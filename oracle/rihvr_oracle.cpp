@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -816,7 +818,14 @@ template <class T> struct Solver {
         adjoint_from_spectrum(Rh.data(), G.data());
         int total_bt = 0;
         if (st) { st->f_initial = f; st->iterations = 0; }
+        // RIHVR_ORACLE_PROGRESS=k: a line on stderr every k iterations (long solves)
+        const char* pe = std::getenv("RIHVR_ORACLE_PROGRESS");
+        const int progress = pe ? std::atoi(pe) : 0;
         for (int it = 0; it < p.num_inverse_iterations; ++it) {
+            if (progress > 0 && it > 0 && it % progress == 0) {
+                std::fprintf(stderr, "oracle solve: %d of %d iterations done\n", it, p.num_inverse_iterations);
+                std::fflush(stderr);
+            }
             x0 = x1;
             double alpha0 = alpha1;
             prox(mode, x0.data(), G.data(), t, x1.data());

@@ -224,15 +224,22 @@ template <int R, int SIGN, int L> struct Dft {
 // Pass recursion.  src(i, r, j) returns element j of the transform for the
 // first pass; dst(i, r, j, v) receives output element j of the last pass;
 // lds(j) is this transform's LDS slot j.  tw: W_L^m table in LDS, m < L.
+// TOFF < 0: every twiddle is read from tw.  TOFF >= 0: the pass's twiddles are
+// tc[TOFF ...] (registers filled once by TwCache::load, for kernels that run the
+// same transform positions over many planes: no LDS reads or index arithmetic
+// per plane, the same values).
 // ---------------------------------------------------------------------------
-template <int L, int TPT, int SIGN, int NS, int R, int... REST> struct FftPasses {
+template <int TOFF, int L, int TPT, int SIGN, int NS, int R, int... REST> struct FftPasses {
     static constexpr int NBF = L / R;
     static constexpr int NB = (NBF + TPT - 1) / TPT;
     static constexpr bool EVEN = (NBF % TPT) == 0;
+    // twiddles this pass takes from tc, and the next pass's offset
+    static constexpr int NTW = NS > 1 ? NB * (R - 1) : 0;
+    static constexpr int NEXT = TOFF < 0 ? -1 : TOFF + NTW;
 
     template <class Src, class Dst, class Lds>
-    __device__ static __forceinline__ void run(const float2* __restrict__ tw, int t, Src&& src, Dst&& dst,
-                                               Lds&& lds) {
+    __device__ static __forceinline__ void run(const float2* __restrict__ tw, const float2* tc, int t, Src&& src,
+                                               Dst&& dst, Lds&& lds) {
         float2 v[NB][R];
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
@@ -250,7 +257,12 @@ template <int L, int TPT, int SIGN, int NS, int R, int... REST> struct FftPasses
                 const int k = b % NS;
                 if constexpr (NS > 1) {
 #pragma unroll
-                    for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[(r * k) * (L / (NS * R))]);
+                    for (int r = 1; r < R; ++r) {
+                        if constexpr (TOFF >= 0)
+                            v[i][r] = cmul(v[i][r], tc[TOFF + i * (R - 1) + (r - 1)]);
+                        else
+                            v[i][r] = cmul(v[i][r], tw[(r * k) * (L / (NS * R))]);
+                    }
                 }
                 Dft<R, SIGN, L>::run(v[i], tw);
                 const int base = (b / NS) * NS * R + k;
@@ -265,17 +277,54 @@ template <int L, int TPT, int SIGN, int NS, int R, int... REST> struct FftPasses
         }
         if constexpr (sizeof...(REST) > 0) {
             __syncthreads();
-            FftPasses<L, TPT, SIGN, NS * R, REST...>::run(
-                tw, t, [&](int, int, int j) -> float2 { return lds(j); }, dst, lds);
+            FftPasses<NEXT, L, TPT, SIGN, NS * R, REST...>::run(
+                tw, tc, t, [&](int, int, int j) -> float2 { return lds(j); }, dst, lds);
         }
     }
+
+    // the twiddles run() takes from tc at this pass and the later ones, in its order
+    __device__ static __forceinline__ void load(const float2* __restrict__ tw, int t, float2* tc) {
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const int b = t + i * TPT;
+                const int k = (EVEN || b < NBF) ? b % NS : 0;
+#pragma unroll
+                for (int r = 1; r < R; ++r) tc[TOFF + i * (R - 1) + (r - 1)] = tw[(r * k) * (L / (NS * R))];
+            }
+        }
+        if constexpr (sizeof...(REST) > 0) FftPasses<NEXT, L, TPT, SIGN, NS * R, REST...>::load(tw, t, tc);
+    }
+};
+
+template <int L, int TPT, int NS, int... R> struct TwCount;
+template <int L, int TPT, int NS> struct TwCount<L, TPT, NS> { static constexpr int value = 0; };
+template <int L, int TPT, int NS, int R, int... REST> struct TwCount<L, TPT, NS, R, REST...> {
+    static constexpr int NB = (L / R + TPT - 1) / TPT;
+    static constexpr int value = (NS > 1 ? NB * (R - 1) : 0) + TwCount<L, TPT, NS * R, REST...>::value;
 };
 
 template <int L, int SIGN, class Rad, int TPT = FftPlan<L>::TPT> struct FftRun;
 template <int L, int SIGN, int... R, int TPT> struct FftRun<L, SIGN, Radices<R...>, TPT> {
     template <class Src, class Dst, class Lds>
     __device__ static __forceinline__ void run(const float2* tw, int t, Src&& src, Dst&& dst, Lds&& lds) {
-        FftPasses<L, TPT, SIGN, 1, R...>::run(tw, t, src, dst, lds);
+        FftPasses<-1, L, TPT, SIGN, 1, R...>::run(tw, nullptr, t, src, dst, lds);
+    }
+};
+
+// A thread's twiddles of one transform position held in registers (N complex values), for
+// kernels that loop over planes with the same positions: load() once after the table is in
+// LDS, then run() per plane.
+template <int L, int SIGN, class Rad, int TPT> struct TwCache;
+template <int L, int SIGN, int... R, int TPT> struct TwCache<L, SIGN, Radices<R...>, TPT> {
+    static constexpr int N = TwCount<L, TPT, 1, R...>::value;
+    float2 w[N > 0 ? N : 1];
+    __device__ __forceinline__ void load(const float2* tw, int t) {
+        FftPasses<0, L, TPT, SIGN, 1, R...>::load(tw, t, w);
+    }
+    template <class Src, class Dst, class Lds>
+    __device__ __forceinline__ void run(const float2* tw, int t, Src&& src, Dst&& dst, Lds&& lds) const {
+        FftPasses<0, L, TPT, SIGN, 1, R...>::run(tw, w, t, src, dst, lds);
     }
 };
 

@@ -120,26 +120,59 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
         const int z1 = min(a.nz, z0 + a.zchunk);
         constexpr int NS = L / 16;  // strips per row (<= 64)
         const int lane = threadIdx.x & 63;
-        for (int z = z0; z < z1; ++z) {
-            // the plane's strips as a wave-uniform mask (every wave loads the same bytes); a
-            // plane without any strip adds zeros to the accumulators (never -0: they start
-            // at +0), so it is skipped; absent strips are read as zeros
+        TwCache<L, SIGN, typename Plan::radices, Plan::TPT> twc;
+        twc.load(tw_s, t);
+        // The plane's strips as a wave-uniform mask (every wave loads the same bytes); a plane
+        // without any strip adds zeros to the accumulators (never -0: they start at +0), so it
+        // is skipped; absent strips are read as zeros.  The same for every wave: no barrier is
+        // skipped by part of the workgroup.
+        auto plane_mask = [&](int z) -> uint64_t {
             const uint8_t* sm = a.smap + (size_t)z * NS;
-            const uint64_t smask = __ballot(lane < NS && sm[lane < NS ? lane : 0] != 0);
-            if (!smask) continue;  // the same for every wave: no barrier is skipped by part
-            const double mz = -(double)a.zlist[z];
+            return __ballot(lane < NS && sm[lane < NS ? lane : 0] != 0);
+        };
+        // The first pass's elements of the next plane with data are loaded while the current
+        // plane is transformed (element b + r L/R0 of butterfly b = t + i TPT).
+        constexpr int R0 = Plan::first;
+        constexpr int NBF0 = L / R0;
+        constexpr int NB0 = (NBF0 + Plan::TPT - 1) / Plan::TPT;
+        float2 pre[NB0][R0];
+        auto prefetch = [&](int z, uint64_t m) {
             const float2* src_row = a.in + ((size_t)z * a.nrows + rrow) * L;
-            FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
-                tw_s, t,
-                [&](int, int, int j) {
-                    return ((smask >> (j >> 4)) & 1ull) ? src_row[j] : make_float2(0.0f, 0.0f);
-                },
+#pragma unroll
+            for (int i = 0; i < NB0; ++i) {
+                const int b = t + i * Plan::TPT;
+                const int bb = b < NBF0 ? b : NBF0 - 1;
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    const int j = bb + r * NBF0;
+                    pre[i][r] = ((m >> (j >> 4)) & 1ull) ? src_row[j] : make_float2(0.0f, 0.0f);
+                }
+            }
+        };
+        int z = z0;
+        uint64_t smask = 0;
+        while (z < z1 && !(smask = plane_mask(z))) ++z;
+        if (z < z1) prefetch(z, smask);
+        while (z < z1) {
+            int zn = z + 1;
+            uint64_t nmask = 0;
+            while (zn < z1 && !(nmask = plane_mask(zn))) ++zn;
+            float2 cur[NB0][R0];
+#pragma unroll
+            for (int i = 0; i < NB0; ++i)
+#pragma unroll
+                for (int r = 0; r < R0; ++r) cur[i][r] = pre[i][r];
+            if (zn < z1) prefetch(zn, nmask);
+            const double mz = -(double)a.zlist[z];
+            twc.run(
+                tw_s, t, [&](int i, int r, int) { return cur[i][r]; },
                 [&](int i, int r, int, float2 v) {
                     float2 h = transfer(mz, kp[i][r]);
                     acc[i][r] = cadd(acc[i][r], cmul(h, v));  // 1/N is in the first pass's input
                 },
                 lds);
             __syncthreads();
+            z = zn;
         }
         if (valid) {
             float2* dst_row = a.out + (size_t)blockIdx.y * a.part_stride + (size_t)rrow * L;
@@ -171,10 +204,12 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
         }
         const int z0 = blockIdx.y * a.zchunk;
         const int z1 = min(a.nz, z0 + a.zchunk);
+        TwCache<L, SIGN, typename Plan::radices, Plan::TPT> twc;
+        twc.load(tw_s, t);
         for (int z = z0; z < z1; ++z) {
             const double zz = (double)a.zlist[z];
             float2* dst_row = a.out + ((size_t)z * a.nrows + rrow) * L;
-            FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
+            twc.run(
                 tw_s, t,
                 [&](int i, int r, int) {
                     float2 h = transfer(zz, kp[i][r]);

@@ -50,7 +50,7 @@ def sparse_to_dense(fix, shape, dtype):
 
 
 @pytest.mark.timeout(1200)
-def test_solve_c2_matches_oracle(pkg, orc, scenes, gpu_available):
+def test_solve_c2_matches_oracle(pkg, orc, scenes, gpu_available, capfd, monkeypatch):
     import torch
     c = dict(scenes.C2, mode=7)
     holo = scenes.c2_hologram()
@@ -61,7 +61,10 @@ def test_solve_c2_matches_oracle(pkg, orc, scenes, gpu_available):
         vol = rec.getVolume()
     print(f"GPU solve done: nnz {st['nnz']} f_final {st['f_final']:.9g}", flush=True)
     t0 = time.perf_counter()
-    ref, _, _, rst = orc.solve(orc.make_params(**c), holo, None, "f32")
+    # the oracle's solve takes minutes on the host cores: its progress lines go to the terminal
+    monkeypatch.setenv("RIHVR_ORACLE_PROGRESS", "10")
+    with capfd.disabled():
+        ref, _, _, rst = orc.solve(orc.make_params(**c), holo, None, "f32")
     print(f"FP32 oracle solve on {orc.get_threads()} host threads: {time.perf_counter() - t0:.1f} s", flush=True)
     thr = float(np.float32(c["regularization_param"])) / Nz  # t * lambda_1 at t = 1/Nz
     r = rel(vol, ref)
