@@ -185,7 +185,12 @@ def c5_bench(pkg, scenes, n_frames: int, tol: float = 1e-3, seed: int = 910001):
         params, i_s, o_s = z["params"], float(z["in_scale"]), float(z["out_scale"])
     with pkg.WarmStartNet(128, 1024, 1024) as net:
         net.set_params(params, i_s, o_s)
-        s = ws.evaluate(net, "c2", 1024, n_frames, seed, log=None)
+        def progress(k):
+            if (k + 1) % 16 == 0 or k + 1 == n_frames:
+                print(f"c5: {k + 1} of {n_frames} holograms done", file=sys.stderr, flush=True)
+
+        s = ws.evaluate(net, "c2", 1024, n_frames, seed, log=None, propagate=scenes.propagate_sum_torch,
+                        progress=progress)
     rows = s.pop("rows")
     s["iterations_cold_mean"] = float(np.mean(s["cold_iterations"]))
     s["iterations_learned_mean"] = float(np.mean(s["learned_iterations"]))

@@ -46,18 +46,20 @@ def config(optics: str, size: int, iterations: int = 100) -> dict:
     return dict(OPTICS[optics], width=size, height=size, num_inverse_iterations=iterations, mode=7)
 
 
-def scene(optics: str, cfg: dict, seed: int):
+def scene(optics: str, cfg: dict, seed: int, propagate=None):
     """A seeded hologram of tracers and rods at the workload's object density, the rods a
-    fraction drawn uniformly from 2 % to 30 % of the objects."""
+    fraction drawn uniformly from 2 % to 30 % of the objects (propagate: the forward model's
+    plane sum, numpy by default; scenes.propagate_sum_torch runs it on the GPU)."""
     rng = np.random.default_rng(seed)
     lo, hi, area = DENSITY[optics]
     n = int(round(rng.uniform(lo, hi) * cfg["width"] * cfg["height"] / area))
     n_rods = max(1, int(round(n * rng.uniform(0.02, 0.30))))
     o = scenes.optics_of(cfg)
+    kw = {} if propagate is None else {"propagate": propagate}
     if optics == "c4":
-        holo, _ = scenes.tracers_rods_scene(o, n - n_rods, n_rods, seed, 0.02, tracer_diam=(4.0, 12.0))
+        holo, _ = scenes.tracers_rods_scene(o, n - n_rods, n_rods, seed, 0.02, tracer_diam=(4.0, 12.0), **kw)
     else:
-        holo, _ = scenes.tracers_rods_scene(o, n - n_rods, n_rods, seed, 0.02)
+        holo, _ = scenes.tracers_rods_scene(o, n - n_rods, n_rods, seed, 0.02, **kw)
     return holo
 
 
@@ -125,16 +127,16 @@ def train(net, xs, ys, epochs: int, lr: float, crop, seed: int = 0, log=print, l
 
 
 def evaluate(net, optics: str, size: int, n_frames: int, seed: int, iterations: int = 100, tol: float = 1e-3,
-             log=print):
+             log=print, propagate=None, progress=None):
     """Per frame: the plain solve's f(iterations) sets the target f <= (1 + tol) f_plain; then
     plain cold start and the learned start each run to it (capped at iterations).  Returns a
-    summary with s/frame and iteration counts."""
+    summary with s/frame and iteration counts.  progress(k) is called after frame k."""
     import torch
     cfg = config(optics, size, iterations)
     rows = []
     with pkg.SparseCompressiveHolo(pkg.make_params(**cfg)) as rec:
         for k in range(n_frames):
-            h = torch.from_numpy(scene(optics, cfg, seed + k)).cuda()
+            h = torch.from_numpy(scene(optics, cfg, seed + k, propagate)).cuda()
             plain = rec.reconstructBatch(h[None])[0]
             target = (1.0 + tol) * plain["f_final"]
             torch.cuda.synchronize()
@@ -155,6 +157,8 @@ def evaluate(net, optics: str, size: int, n_frames: int, seed: int, iterations: 
                              ms_network=lr_["ms_network"], ms_adjoint=lr_["ms_adjoint"], ms_solve=lr_["ms_solve"]))
             if log:
                 log(json.dumps(rows[-1]))
+            if progress:
+                progress(k)
     s = dict(frames=n_frames, size=size, optics=optics, tolerance=tol,
              cold_s_per_frame=float(np.mean([r["cold_s"] for r in rows])),
              learned_s_per_frame=float(np.mean([r["learned_s"] for r in rows])),
