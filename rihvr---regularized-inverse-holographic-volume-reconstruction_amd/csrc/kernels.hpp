@@ -140,6 +140,11 @@ hipError_t prox_fl3d(const ProxLaunch& a, float2* scratch, hipStream_t s);
 int momentum_out_num_blocks(int W, int H, int nz);
 hipError_t momentum_out(const float2* Y, const float2* Y0, float s_mom, float2* out, int W, int H,
                         int nz, bool tv3d, double* partial, hipStream_t s);
+// momentum_out(X, X, 0, NULL, ...) for a volume with a valid line-occupancy map occ ([nz][H]
+// row words; a clear line holds +0): the same partials, bit for bit, reading only the lines
+// whose terms can be non-zero.
+hipError_t volume_terms_map(const float2* X, const uint64_t* occ, int W, int H, int nz, bool tv3d,
+                            double* partial, hipStream_t s);
 
 // Hologram momentum by linearity (sparse_compressive_holo.cu:494-498):
 // est = cur + s (cur - prev); resid = est - meas; partial[block] = sum |resid|^2.

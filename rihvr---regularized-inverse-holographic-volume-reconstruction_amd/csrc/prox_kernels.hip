@@ -940,15 +940,31 @@ hipError_t prox_tv2_general(const ProxLaunch& a, float2* work, hipStream_t s) {
 
 // ----------------------------------------------------------------------------
 // Returned volume: x1 = y + s (y - y0) with its L1 / TV / nnz.
+// A wave takes whole rows (row r of plane z, z and r from one division per row; the lanes
+// stride along x), so no element index is divided; a workgroup of kVtWaves waves writes one
+// row of partials.  MAP (s = 0, no output): the volume's line-occupancy map ([nz][H] row
+// words) selects the 64-column chunks whose terms can be non-zero -- a voxel's L1 and nnz
+// need its line set, its TV term its own line, the line of its left neighbour, the line above
+// (and the previous plane's with tv3d) -- and the others are not read.  The terms skipped are
+// exactly +0, which leaves every sum unchanged, so both forms give the same bits.
 // ----------------------------------------------------------------------------
-int momentum_out_num_blocks(int W, int H, int nz) { return prox_l1_num_blocks(W, H, nz); }
+constexpr int kVtWaves = 4;
+inline int vt_rows_per_wave(int W) { return W >= 512 ? 1 : (512 + W - 1) / W; }
+int momentum_out_num_blocks(int W, int H, int nz) {
+    const long rows = (long)H * nz;
+    const long per = (long)kVtWaves * vt_rows_per_wave(W);
+    return (int)((rows + per - 1) / per);
+}
 
-__global__ void __launch_bounds__(kEwThreads)
-k_momentum_out(const float2* __restrict__ Y, const float2* __restrict__ Y0, float s, float2* out, int W, int H,
-               int nz, int tv3d, double* partial) {
-    __shared__ double red[kEwThreads / 64];
+template <bool MAP>
+__global__ void __launch_bounds__(64 * kVtWaves)
+k_volume_terms(const float2* __restrict__ Y, const float2* __restrict__ Y0, float s, float2* out, int W, int H,
+               int nz, int tv3d, int rpw, const uint64_t* __restrict__ occ, int lsh, double* partial) {
+    __shared__ double red[kVtWaves];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     const long N = (long)W * H;
-    const long n = N * nz;
+    const long rows = (long)H * nz;
     double l1 = 0, tv = 0, nnz = 0;
     // s = 0 (no momentum: a start volume's objective, or the last update restarted) reads Y only
     const bool still = s == 0.0f;
@@ -958,38 +974,60 @@ k_momentum_out(const float2* __restrict__ Y, const float2* __restrict__ Y0, floa
         const float2 y0 = Y0[i];
         return make_float2(fmaf(s, y.x - y0.x, y.x), fmaf(s, y.y - y0.y, y.y));
     };
-    for (int it = 0; it < kEwItems; ++it) {
-        long i = ((long)blockIdx.x * kEwItems + it) * kEwThreads + threadIdx.x;
-        if (i >= n) break;
-        const long zo = (i / N) * N;
-        const int p = (int)(i - zo);
-        const int y = p / W, x = p - y * W;
-        const float2 v = mom(i);
-        if (out) out[i] = v;
-        if (v.x != 0.0f || v.y != 0.0f) {
-            l1 += (double)sqrtf(v.x * v.x + v.y * v.y);
-            nnz += 1.0;
+    const int nl = ((W - 1) >> lsh) + 1;  // map lines per row (<= 64)
+    for (int r = 0; r < rpw; ++r) {
+        const long gr = ((long)blockIdx.x * kVtWaves + wv) * rpw + r;
+        if (gr >= rows) break;
+        const int z = (int)(gr / H);
+        const int y = (int)(gr - (long)z * H);
+        const int yu = y > 0 ? y - 1 : H - 1;
+        const int zp = z > 0 ? z - 1 : nz - 1;
+        const long ro = (long)z * N + (long)y * W;
+        const long ru = (long)z * N + (long)yu * W;
+        const long rp = (long)zp * N + (long)y * W;
+        uint64_t mask = ~0ull;
+        if constexpr (MAP) {
+            const uint64_t w = occ[gr];
+            // line 0's left neighbour is the row's last line (periodic x)
+            mask = w | occ[(long)z * H + yu] | (w << 1) | ((w >> (nl - 1)) & 1ull);
+            if (tv3d) mask |= occ[(long)zp * H + y];
+            if (!mask) continue;
         }
-        const int xm = x > 0 ? x - 1 : W - 1, ym = y > 0 ? y - 1 : H - 1;
-        const float2 a = mom(zo + (long)y * W + xm), b = mom(zo + (long)ym * W + x);
-        const float gxx = a.x - v.x, gyx = b.x - v.x, gxy = a.y - v.y, gyy = b.y - v.y;
-        float n2 = gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy;
-        if (tv3d) {  // calcTVNorm: against the previous plane, z-periodic (sparse_volume.cu:236-265)
-            const float2 c = mom((zo > 0 ? zo - N : (long)(nz - 1) * N) + p);
-            const float gzx = c.x - v.x, gzy = c.y - v.y;
-            n2 = n2 + gzx * gzx + gzy * gzy;
+        for (int x0 = 0; x0 < W; x0 += 64) {
+            if constexpr (MAP) {
+                const int la = x0 >> lsh, lb = min(x0 + 63, W - 1) >> lsh;
+                const uint64_t cm = (lb >= 63 ? ~0ull : ((1ull << (lb + 1)) - 1ull)) & ~((1ull << la) - 1ull);
+                if (!(mask & cm)) continue;
+            }
+            const int x = x0 + lane;
+            if (x >= W) continue;
+            const float2 v = mom(ro + x);
+            if (out) out[ro + x] = v;
+            if (v.x != 0.0f || v.y != 0.0f) {
+                l1 += (double)sqrtf(v.x * v.x + v.y * v.y);
+                nnz += 1.0;
+            }
+            const int xm = x > 0 ? x - 1 : W - 1;
+            const float2 a = mom(ro + xm), b = mom(ru + x);
+            const float gxx = a.x - v.x, gyx = b.x - v.x, gxy = a.y - v.y, gyy = b.y - v.y;
+            float n2 = gxx * gxx + gxy * gxy + gyx * gyx + gyy * gyy;
+            if (tv3d) {  // calcTVNorm: against the previous plane, z-periodic (sparse_volume.cu:236-265)
+                const float2 c = mom(rp + x);
+                const float gzx = c.x - v.x, gzy = c.y - v.y;
+                n2 = n2 + gzx * gzx + gzy * gzy;
+            }
+            tv += (double)sqrtf(n2);
         }
-        tv += (double)sqrtf(n2);
     }
     double* pp = partial + (size_t)blockIdx.x * PS_COUNT;
-    double r;
-    r = block_sum(l1, red);
-    if (threadIdx.x == 0) pp[PS_L1X0] = r;
-    r = block_sum(tv, red);
-    if (threadIdx.x == 0) pp[PS_TVX0] = r;
-    r = block_sum(nnz, red);
+    double rsum;
+    rsum = block_sum(l1, red);
+    if (threadIdx.x == 0) pp[PS_L1X0] = rsum;
+    rsum = block_sum(tv, red);
+    if (threadIdx.x == 0) pp[PS_TVX0] = rsum;
+    rsum = block_sum(nnz, red);
     if (threadIdx.x == 0) {
-        pp[PS_NNZX0] = r;
+        pp[PS_NNZX0] = rsum;
         pp[PS_LS1] = pp[PS_LS2] = pp[PS_RESTART] = pp[PS_NNZY] = pp[7] = 0.0;
     }
 }
@@ -997,8 +1035,18 @@ k_momentum_out(const float2* __restrict__ Y, const float2* __restrict__ Y0, floa
 hipError_t momentum_out(const float2* Y, const float2* Y0, float s_mom, float2* out, int W, int H, int nz,
                         bool tv3d, double* partial, hipStream_t s) {
     dim3 grid(momentum_out_num_blocks(W, H, nz));
-    hipLaunchKernelGGL(k_momentum_out, grid, dim3(kEwThreads), 0, s, Y, Y0, s_mom, out, W, H, nz, tv3d ? 1 : 0,
-                       partial);
+    hipLaunchKernelGGL(k_volume_terms<false>, grid, dim3(64 * kVtWaves), 0, s, Y, Y0, s_mom, out, W, H, nz,
+                       tv3d ? 1 : 0, vt_rows_per_wave(W), (const uint64_t*)nullptr, 4, partial);
+    return hipGetLastError();
+}
+
+hipError_t volume_terms_map(const float2* X, const uint64_t* occ, int W, int H, int nz, bool tv3d,
+                            double* partial, hipStream_t s) {
+    int lsh = 4;
+    while (((W - 1) >> lsh) >= 64) ++lsh;
+    dim3 grid(momentum_out_num_blocks(W, H, nz));
+    hipLaunchKernelGGL(k_volume_terms<true>, grid, dim3(64 * kVtWaves), 0, s, X, X, 0.0f, (float2*)nullptr, W, H,
+                       nz, tv3d ? 1 : 0, vt_rows_per_wave(W), occ, lsh, partial);
     return hipGetLastError();
 }
 

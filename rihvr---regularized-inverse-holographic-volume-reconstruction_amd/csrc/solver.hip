@@ -818,8 +818,10 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     RIHVR_TRY(apply_forward(c, Y0, c->hE));
     double l1 = 0, tv = 0;
     if (warm) {
+        // a start with its map reads only the lines whose terms can be non-zero (the same sums)
         RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
-            return momentum_out(Y0, Y0, 0.0f, nullptr, c->W, c->H, c->Nz, tv3d(c), c->partial, c->stream);
+            return keep_map ? volume_terms_map(Y0, c->occ[c->iy0], c->W, c->H, c->Nz, tv3d(c), c->partial, c->stream)
+                            : momentum_out(Y0, Y0, 0.0f, nullptr, c->W, c->H, c->Nz, tv3d(c), c->partial, c->stream);
         }));
         RIHVR_TRY(timed(c, K_REDUCE, [&] {
             return reduce_partials(c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
@@ -1264,10 +1266,12 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
     RIHVR_TRY(fetch_scalars(c));
     st.f_zero = 0.5 * c->h_scal[SL_RESID_M];
     // objective and data residual of a start volume X
-    auto volume_objective = [&](const float2* X, double* resid, double* f) -> rihvr_status {
+    // (map: X's valid line-occupancy map, or NULL)
+    auto volume_objective = [&](const float2* X, const uint64_t* map, double* resid, double* f) -> rihvr_status {
         RIHVR_TRY(apply_forward(c, X, c->hE));
         RIHVR_TRY(timed(c, K_MOM_OUT, [&] {
-            return momentum_out(X, X, 0.0f, nullptr, c->W, c->H, c->Nz, tv3d(c), c->partial, c->stream);
+            return map ? volume_terms_map(X, map, c->W, c->H, c->Nz, tv3d(c), c->partial, c->stream)
+                       : momentum_out(X, X, 0.0f, nullptr, c->W, c->H, c->Nz, tv3d(c), c->partial, c->stream);
         }));
         RIHVR_TRY(reduce_partials_on(c, c->partial, momentum_out_num_blocks(c->W, c->H, c->Nz), PS_COUNT,
                                      c->scal + SL_PROX));
@@ -1300,7 +1304,7 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
         RIHVR_HIP(soft_threshold_map(xw, b, c->W, c->H, c->Nz, tau, c->occ[kb], c->stream));
         c->occ_valid[kb] = true;
         double r = 0, f = 0;
-        RIHVR_TRY(volume_objective(b, &r, &f));
+        RIHVR_TRY(volume_objective(b, c->occ[kb], &r, &f));
         if (f < st.f_thresholded) {
             st.f_thresholded = f;
             st.tau = tau;
@@ -1308,7 +1312,7 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
         }
     }
     RIHVR_HIP(holo_scale(c->G, c->G_alt, c->V, (float)(1.0 / (double)c->Nz), c->stream));
-    RIHVR_TRY(volume_objective(c->G_alt, &st.resid_adjoint, &st.f_adjoint));
+    RIHVR_TRY(volume_objective(c->G_alt, nullptr, &st.resid_adjoint, &st.f_adjoint));
     st.fallback = (xs && st.f_thresholded <= st.f_adjoint && st.f_thresholded <= st.f_zero) ? 0 : 1;
     const auto t2 = clk::now();
     rihvr_frame_stats fs{};

@@ -66,3 +66,36 @@ def test_learned_start_with_line_maps_equals_the_unmapped_start(pkg, gpu_availab
     r = float(np.linalg.norm(vol - ref) / np.linalg.norm(ref))
     print(f"mapped vs unmapped learned start: rel {r:.2e}, f_start {st['f_start']:.6g} vs {f0:.6g}", flush=True)
     assert r <= 1e-5
+
+
+MAPPED_CASES = [(256, 256, 128, 7), (1040, 64, 128, 7), (2048, 48, 32, 7), (256, 256, 64, 6)]
+
+
+@pytest.mark.parametrize("case", MAPPED_CASES)
+def test_mapped_start_objective_equals_the_dense_one(pkg, gpu_available, case):
+    """The learned start's objective is summed over the lines its occupancy map marks (its own
+    line, the line of each voxel's left neighbour, the line above, the previous plane's with 3D
+    TV); the same start passed in from outside is summed over every voxel.  The terms skipped
+    are exactly +0, so the two objectives must be equal bit for bit: with 16-voxel lines
+    (256 wide), 32-voxel lines with a partial last line (1040), 32-voxel lines over 64 per row
+    (2048) and mode 6.  The start is returned unchanged by a target no start misses (no
+    iteration runs)."""
+    import torch
+    import __graft_entry__ as entry
+    ws = entry._load_submodule("warmstart")
+    W, H, Nz, mode = case
+    with np.load(WEIGHTS, allow_pickle=False) as z:
+        params, i_s, o_s = z["params"], float(z["in_scale"]), float(z["out_scale"])
+    cfg = dict(ws.config("c2", 256, 100), width=W, height=H, num_planes=Nz, mode=mode)
+    h = torch.from_numpy(ws.scene("c2", cfg, 700201 + W)).cuda()
+    with pkg.SparseCompressiveHolo(pkg.make_params(**cfg)) as rec, pkg.WarmStartNet(Nz, H, W) as net:
+        net.set_params(params, i_s, o_s)
+        st = rec.reconstructLearned(net, h, f_target=1e30, max_iterations=5)
+        print(f"{W}x{H}x{Nz} mode {mode}: fallback {st['fallback']} tau {st['tau']:.4g} "
+              f"f_start {st['f_start']!r}", flush=True)
+        assert st["iterations"] == 0
+        assert st["fallback"] == 0, st
+        start = torch.from_numpy(rec.getVolume()).cuda()
+        f0 = rec.begin(h, warm_start=start)
+    print(f"dense objective {f0!r}", flush=True)
+    assert f0 == st["f_start"]
