@@ -1199,53 +1199,57 @@ __global__ void __launch_bounds__(kEwThreads) k_soft(const float2* __restrict__ 
     }
     out[i] = r;
 }
-// k_soft that also builds the output's line-occupancy map (cleared beforehand): bit x >> lsh of
-// word z H + y for every non-zero voxel, and the plane words after the Nz H row words.  A wave
-// inside one row ORs its lanes' bits and sets them with one atomic per word; a wave across rows
-// sets each non-zero lane's bit itself.
-__global__ void __launch_bounds__(kEwThreads) k_soft_map(const float2* __restrict__ in, float2* __restrict__ out,
-                                                         int W, int H, int nz, float tau, int lsh, uint64_t* occ) {
-    const long n = (long)W * H * nz;
-    const long i = (long)blockIdx.x * kEwThreads + threadIdx.x;
-    bool nzv = false;
-    long row = 0;
-    int x = 0;
-    if (i < n) {
-        const float2 e = in[i];
-        const float m = sqrtf(e.x * e.x + e.y * e.y);
+// k_soft into two outputs at once (thresholds tau0, tau1: one read of the input), each with its
+// line-occupancy map: a wave takes whole rows (as k_volume_terms), ORs its lanes' line bits over
+// the row, stores the row word (the row is the wave's alone) and ORs it into the plane word
+// after the Nz H row words (cleared beforehand).
+__global__ void __launch_bounds__(64 * kVtWaves)
+k_soft_map2(const float2* __restrict__ in, int W, int H, int nz, int rpw, int lsh, float tau0, float2* __restrict__ out0,
+            uint64_t* occ0, float tau1, float2* __restrict__ out1, uint64_t* occ1) {
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const long rows = (long)H * nz;
+    auto soft = [](float2 e, float m, float tau) {
         float2 r = make_float2(0.0f, 0.0f);
         if (m > tau) {
             const float sc = (m - tau) / m;
             r = make_float2(e.x * sc, e.y * sc);
-            nzv = r.x != 0.0f || r.y != 0.0f;
         }
-        out[i] = r;
-        row = i / W;
-        x = (int)(i - row * W);
-    }
-    uint64_t* planes = occ + (size_t)nz * H;
-    const long wave0 = i - (threadIdx.x & 63);
-    const long last = std::min(wave0 + 63, n - 1);
-    if (wave0 / W == last / W) {  // uniform: the wave lies in one row
-        uint64_t bits = nzv ? (1ull << (x >> lsh)) : 0ull;
-        for (int o = 32; o > 0; o >>= 1) bits |= __shfl_xor(bits, o);
-        if ((threadIdx.x & 63) == 0 && bits && i < n) {
-            atomicOr((unsigned long long*)&occ[row], (unsigned long long)bits);
-            atomicOr((unsigned long long*)&planes[row / H], (unsigned long long)bits);
+        return r;
+    };
+    for (int rr = 0; rr < rpw; ++rr) {
+        const long gr = ((long)blockIdx.x * kVtWaves + wv) * rpw + rr;
+        if (gr >= rows) break;
+        const long ro = gr * W;
+        uint64_t b0 = 0, b1 = 0;
+        for (int x = lane; x < W; x += 64) {
+            const float2 e = in[ro + x];
+            const float m = sqrtf(e.x * e.x + e.y * e.y);
+            const float2 r0 = soft(e, m, tau0), r1 = soft(e, m, tau1);
+            out0[ro + x] = r0;
+            out1[ro + x] = r1;
+            if (r0.x != 0.0f || r0.y != 0.0f) b0 |= 1ull << (x >> lsh);
+            if (r1.x != 0.0f || r1.y != 0.0f) b1 |= 1ull << (x >> lsh);
         }
-    } else if (nzv) {
-        const unsigned long long b = 1ull << (x >> lsh);
-        atomicOr((unsigned long long*)&occ[row], b);
-        atomicOr((unsigned long long*)&planes[row / H], b);
+        for (int o = 32; o > 0; o >>= 1) {
+            b0 |= __shfl_xor(b0, o);
+            b1 |= __shfl_xor(b1, o);
+        }
+        if (lane == 0) {
+            const long z = gr / H;
+            occ0[gr] = b0;
+            occ1[gr] = b1;
+            if (b0) atomicOr((unsigned long long*)&occ0[rows + z], (unsigned long long)b0);
+            if (b1) atomicOr((unsigned long long*)&occ1[rows + z], (unsigned long long)b1);
+        }
     }
 }
-hipError_t soft_threshold_map(const float2* in, float2* out, int W, int H, int nz, float tau, uint64_t* occ,
-                              hipStream_t s) {
+hipError_t soft_threshold_map2(const float2* in, int W, int H, int nz, float tau0, float2* out0, uint64_t* occ0,
+                               float tau1, float2* out1, uint64_t* occ1, hipStream_t s) {
     int lsh = 4;
     while (((W - 1) >> lsh) >= 64) ++lsh;
-    const long n = (long)W * H * nz;
-    hipLaunchKernelGGL(k_soft_map, dim3((unsigned)((n + kEwThreads - 1) / kEwThreads)), dim3(kEwThreads), 0, s, in,
-                       out, W, H, nz, tau, lsh, occ);
+    hipLaunchKernelGGL(k_soft_map2, dim3(momentum_out_num_blocks(W, H, nz)), dim3(64 * kVtWaves), 0, s, in, W, H, nz,
+                       vt_rows_per_wave(W), lsh, tau0, out0, occ0, tau1, out1, occ1);
     return hipGetLastError();
 }
 

@@ -788,8 +788,14 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     int in_ring = -1;
     for (int k = 0; k < 3; ++k)
         if (warm && warm_loc == RIHVR_DEVICE && warm == (const float*)c->Yb[k]) in_ring = k;
+    // a warm start in a ring slot with its occupancy map keeps it (a zero start has the empty
+    // map; another warm start none: every line is read once)
+    const bool keep_map = warm && in_ring >= 0 && c->occ_valid[in_ring];
     if (in_ring >= 0) {
         c->iy0 = in_ring; c->iy = (in_ring + 1) % 3; c->iym = (in_ring + 2) % 3;
+        // with its map, y_{k-2} is a slot whose map is valid when one is (the ring's order is
+        // a permutation, not a cycle)
+        if (keep_map && !c->occ_valid[c->iym] && c->occ_valid[c->iy]) std::swap(c->iy, c->iym);
     } else {
         c->iy = 0; c->iy0 = 1; c->iym = 2;
     }
@@ -797,18 +803,23 @@ rihvr_status begin_impl(rihvr_ctx* c, const float* holo, int holo_loc, const flo
     float2* YM = c->Yb[c->iym];
     if (warm && in_ring < 0) RIHVR_TRY(copy_in(c, Y0, warm, warm_loc, c->V));
     else if (!warm) RIHVR_HIP(hipMemsetAsync(Y0, 0, c->V * sizeof(float2), c->stream));
-    RIHVR_HIP(hipMemcpyAsync(YM, Y0, c->V * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
-    // a zero start has the empty map; a warm start in a ring slot keeps the map it has (YM gets
-    // a copy); another warm start none (every line is read once)
-    const bool keep_map = warm && in_ring >= 0 && c->occ_valid[in_ring];
-    if (keep_map) {
-        RIHVR_HIP(hipMemcpyAsync(c->occ[c->iym], c->occ[in_ring], map_words(c) * sizeof(uint64_t),
-                                 hipMemcpyDeviceToDevice, c->stream));
-        c->occ_valid[c->iym] = true;
+    if (keep_map && c->occ_valid[c->iym]) {
+        // y_{k-2} keeps what it holds (with its map): the first prox runs at step s = 0, where
+        // x0 = y0 + 0 (y0 - ym) is y0 for any finite ym, and after it the slot is the next
+        // output, written over the lines its map marks.  Its content is finite here: the ring
+        // slots of a warm start with a map (the learned start) hold the network's output and
+        // the threshold candidates, and no candidate is chosen when they hold a NaN.
     } else {
-        for (int k : {c->iy0, c->iym}) {
-            c->occ_valid[k] = !warm;
-            if (!warm) RIHVR_HIP(hipMemsetAsync(c->occ[k], 0, map_words(c) * sizeof(uint64_t), c->stream));
+        RIHVR_HIP(hipMemcpyAsync(YM, Y0, c->V * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
+        if (keep_map) {
+            RIHVR_HIP(hipMemcpyAsync(c->occ[c->iym], c->occ[in_ring], map_words(c) * sizeof(uint64_t),
+                                     hipMemcpyDeviceToDevice, c->stream));
+            c->occ_valid[c->iym] = true;
+        } else {
+            for (int k : {c->iy0, c->iym}) {
+                c->occ_valid[k] = !warm;
+                if (!warm) RIHVR_HIP(hipMemsetAsync(c->occ[k], 0, map_words(c) * sizeof(uint64_t), c->stream));
+            }
         }
     }
     RIHVR_HIP(hipMemsetAsync(c->hC, 0, c->N * sizeof(float2), c->stream));
@@ -1291,24 +1302,29 @@ rihvr_status learned_impl(rihvr_ctx* c, rihvr_net* net, const float* holo, int h
     RIHVR_HIP(hipMemcpyAsync(mx.data(), c->partial, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     RIHVR_HIP(hipStreamSynchronize(c->stream));
     const double xmax = *std::max_element(mx.begin(), mx.end());
-    // the candidates go to the two other ring slots, the best one is kept where it is
+    // the candidates go to the two other ring slots (both formed in one pass over x_net, each with
+    // its occupancy map, so A and the first proxes skip its empty lines); the best one is kept
+    // where it is
+    static_assert(sizeof(kSoftFractions) / sizeof(kSoftFractions[0]) == 2, "two candidates per pass");
     float2* cand[2] = {c->Yb[2], c->Yb[0]};
+    float tau[2];
+    for (int k = 0; k < 2; ++k) {
+        const int kb = ring_index(c, cand[k]);
+        tau[k] = (float)(kSoftFractions[k] * xmax);
+        RIHVR_HIP(fill_u64(c->occ[kb] + (size_t)c->Nz * c->H, (long)c->Nz, 0ull, c->stream));
+        c->occ_valid[kb] = true;
+    }
+    RIHVR_HIP(soft_threshold_map2(xw, c->W, c->H, c->Nz, tau[0], cand[0], c->occ[ring_index(c, cand[0])], tau[1],
+                                  cand[1], c->occ[ring_index(c, cand[1])], c->stream));
     float2* xs = nullptr;
     st.f_thresholded = HUGE_VAL;
-    for (int k = 0; k < (int)(sizeof(kSoftFractions) / sizeof(kSoftFractions[0])); ++k) {
-        float2* b = cand[k & 1] == xs ? cand[(k + 1) & 1] : cand[k & 1];
-        const int kb = ring_index(c, b);
-        const float tau = (float)(kSoftFractions[k] * xmax);
-        // the candidate is sparse: with its occupancy map, A and the first proxes skip its empty lines
-        RIHVR_HIP(fill_u64(c->occ[kb], (long)map_words(c), 0ull, c->stream));
-        RIHVR_HIP(soft_threshold_map(xw, b, c->W, c->H, c->Nz, tau, c->occ[kb], c->stream));
-        c->occ_valid[kb] = true;
+    for (int k = 0; k < 2; ++k) {
         double r = 0, f = 0;
-        RIHVR_TRY(volume_objective(b, c->occ[kb], &r, &f));
+        RIHVR_TRY(volume_objective(cand[k], c->occ[ring_index(c, cand[k])], &r, &f));
         if (f < st.f_thresholded) {
             st.f_thresholded = f;
-            st.tau = tau;
-            xs = b;
+            st.tau = tau[k];
+            xs = cand[k];
         }
     }
     RIHVR_HIP(holo_scale(c->G, c->G_alt, c->V, (float)(1.0 / (double)c->Nz), c->stream));
