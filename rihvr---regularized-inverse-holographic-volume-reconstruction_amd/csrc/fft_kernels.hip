@@ -64,6 +64,12 @@ struct RowArgs {
     const uint8_t* smap;
 };
 
+// The row passes' twiddles in registers across planes (TwCache) and the forward pass's
+// next-plane prefetch: at L = 1024 (8.8.8.2 on 128 threads), where they were measured; at the
+// other lengths, whose plans hold more elements per thread, the extra registers cost waves per
+// SIMD (576, 768, 832: 2 -> 1), so those keep the plain passes.
+template <int L> constexpr bool row_regs_fast() { return L == 1024; }
+
 template <int L, int SIGN, int MODE>
 __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
     using C = RowCfg<L>;
@@ -120,8 +126,6 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
         const int z1 = min(a.nz, z0 + a.zchunk);
         constexpr int NS = L / 16;  // strips per row (<= 64)
         const int lane = threadIdx.x & 63;
-        TwCache<L, SIGN, typename Plan::radices, Plan::TPT> twc;
-        twc.load(tw_s, t);
         // The plane's strips as a wave-uniform mask (every wave loads the same bytes); a plane
         // without any strip adds zeros to the accumulators (never -0: they start at +0), so it
         // is skipped; absent strips are read as zeros.  The same for every wave: no barrier is
@@ -130,49 +134,71 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
             const uint8_t* sm = a.smap + (size_t)z * NS;
             return __ballot(lane < NS && sm[lane < NS ? lane : 0] != 0);
         };
-        // The first pass's elements of the next plane with data are loaded while the current
-        // plane is transformed (element b + r L/R0 of butterfly b = t + i TPT).
-        constexpr int R0 = Plan::first;
-        constexpr int NBF0 = L / R0;
-        constexpr int NB0 = (NBF0 + Plan::TPT - 1) / Plan::TPT;
-        float2 pre[NB0][R0];
-        auto prefetch = [&](int z, uint64_t m) {
-            const float2* src_row = a.in + ((size_t)z * a.nrows + rrow) * L;
-#pragma unroll
-            for (int i = 0; i < NB0; ++i) {
-                const int b = t + i * Plan::TPT;
-                const int bb = b < NBF0 ? b : NBF0 - 1;
-#pragma unroll
-                for (int r = 0; r < R0; ++r) {
-                    const int j = bb + r * NBF0;
-                    pre[i][r] = ((m >> (j >> 4)) & 1ull) ? src_row[j] : make_float2(0.0f, 0.0f);
-                }
+        if constexpr (!row_regs_fast<L>()) {
+            for (int z = z0; z < z1; ++z) {
+                const uint64_t smask = plane_mask(z);
+                if (!smask) continue;
+                const double mz = -(double)a.zlist[z];
+                const float2* src_row = a.in + ((size_t)z * a.nrows + rrow) * L;
+                FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(
+                    tw_s, t,
+                    [&](int, int, int j) {
+                        return ((smask >> (j >> 4)) & 1ull) ? src_row[j] : make_float2(0.0f, 0.0f);
+                    },
+                    [&](int i, int r, int, float2 v) {
+                        float2 h = transfer(mz, kp[i][r]);
+                        acc[i][r] = cadd(acc[i][r], cmul(h, v));  // 1/N is in the first pass's input
+                    },
+                    lds);
+                __syncthreads();
             }
-        };
-        int z = z0;
-        uint64_t smask = 0;
-        while (z < z1 && !(smask = plane_mask(z))) ++z;
-        if (z < z1) prefetch(z, smask);
-        while (z < z1) {
-            int zn = z + 1;
-            uint64_t nmask = 0;
-            while (zn < z1 && !(nmask = plane_mask(zn))) ++zn;
-            float2 cur[NB0][R0];
+        } else {
+            TwCache<L, SIGN, typename Plan::radices, Plan::TPT> twc;
+            twc.load(tw_s, t);
+            // The first pass's elements of the next plane with data are loaded while the current
+            // plane is transformed (element b + r L/R0 of butterfly b = t + i TPT).
+            constexpr int R0 = Plan::first;
+            constexpr int NBF0 = L / R0;
+            constexpr int NB0 = (NBF0 + Plan::TPT - 1) / Plan::TPT;
+            float2 pre[NB0][R0];
+            auto prefetch = [&](int z, uint64_t m) {
+                const float2* src_row = a.in + ((size_t)z * a.nrows + rrow) * L;
 #pragma unroll
-            for (int i = 0; i < NB0; ++i)
+                for (int i = 0; i < NB0; ++i) {
+                    const int b = t + i * Plan::TPT;
+                    const int bb = b < NBF0 ? b : NBF0 - 1;
 #pragma unroll
-                for (int r = 0; r < R0; ++r) cur[i][r] = pre[i][r];
-            if (zn < z1) prefetch(zn, nmask);
-            const double mz = -(double)a.zlist[z];
-            twc.run(
-                tw_s, t, [&](int i, int r, int) { return cur[i][r]; },
-                [&](int i, int r, int, float2 v) {
-                    float2 h = transfer(mz, kp[i][r]);
-                    acc[i][r] = cadd(acc[i][r], cmul(h, v));  // 1/N is in the first pass's input
-                },
-                lds);
-            __syncthreads();
-            z = zn;
+                    for (int r = 0; r < R0; ++r) {
+                        const int j = bb + r * NBF0;
+                        pre[i][r] = ((m >> (j >> 4)) & 1ull) ? src_row[j] : make_float2(0.0f, 0.0f);
+                    }
+                }
+            };
+            int z = z0;
+            uint64_t smask = 0;
+            while (z < z1 && !(smask = plane_mask(z))) ++z;
+            if (z < z1) prefetch(z, smask);
+            while (z < z1) {
+                int zn = z + 1;
+                uint64_t nmask = 0;
+                while (zn < z1 && !(nmask = plane_mask(zn))) ++zn;
+                float2 cur[NB0][R0];
+#pragma unroll
+                for (int i = 0; i < NB0; ++i)
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) cur[i][r] = pre[i][r];
+                if (zn < z1) prefetch(zn, nmask);
+                const double mz = -(double)a.zlist[z];
+                twc.run(
+                    tw_s, t, [&](int i, int r, int) { return cur[i][r]; },
+                    [&](int i, int r, int, float2 v) {
+                        const float2 h = transfer(mz, kp[i][r]);
+                        acc[i][r] = cadd(acc[i][r], cmul(h, v));  // 1/N is in the first pass's input
+                    },
+                    lds);
+                __syncthreads();
+                z = zn;
+            }
         }
         if (valid) {
             float2* dst_row = a.out + (size_t)blockIdx.y * a.part_stride + (size_t)rrow * L;
@@ -204,19 +230,26 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_rows(RowArgs a) {
         }
         const int z0 = blockIdx.y * a.zchunk;
         const int z1 = min(a.nz, z0 + a.zchunk);
-        TwCache<L, SIGN, typename Plan::radices, Plan::TPT> twc;
-        twc.load(tw_s, t);
-        for (int z = z0; z < z1; ++z) {
+        auto plane = [&](int z, auto&& fft) {
             const double zz = (double)a.zlist[z];
             float2* dst_row = a.out + ((size_t)z * a.nrows + rrow) * L;
-            twc.run(
-                tw_s, t,
-                [&](int i, int r, int) {
+            fft([&](int i, int r, int) {
                     float2 h = transfer(zz, kp[i][r]);
                     return cscale(cmul(h, rh[i][r]), a.invN);
                 },
-                [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; }, lds);
+                [&](int, int, int j, float2 v) { if (valid) dst_row[j] = v; });
             __syncthreads();
+        };
+        if constexpr (row_regs_fast<L>()) {
+            TwCache<L, SIGN, typename Plan::radices, Plan::TPT> twc;
+            twc.load(tw_s, t);
+            for (int z = z0; z < z1; ++z)
+                plane(z, [&](auto&& src, auto&& dst) { twc.run(tw_s, t, src, dst, lds); });
+        } else {
+            for (int z = z0; z < z1; ++z)
+                plane(z, [&](auto&& src, auto&& dst) {
+                    FftRun<L, SIGN, typename Plan::radices, Plan::TPT>::run(tw_s, t, src, dst, lds);
+                });
         }
     }
 }
