@@ -175,6 +175,26 @@ def frames_leg(pkg, scenes, cfg, n_frames: int, tol: float = 1e-3, seed: int = 2
     }
 
 
+C2_F64 = os.path.join(ROOT, "tests", "golden", "c2_oracle_f64.npz")
+
+
+def fp64_error(vol, holo):
+    """Relative L2 error of the returned volume (device, complex64) against the FP64 oracle's
+    solution of the same hologram: the committed fixture written by tests/golden/make_c2_golden.py
+    (its non-zero voxels).  None when the fixture is absent or belongs to another input."""
+    import torch
+    if not os.path.exists(C2_F64):
+        return None
+    with np.load(C2_F64, allow_pickle=False) as z:
+        if complex(z["holo_sum"]) != complex(np.sum(holo.astype(np.complex128))):
+            return None
+        idx, val = z["idx"].astype(np.int64), z["val"]
+    d = vol.reshape(-1).to(torch.complex128)
+    ref = torch.from_numpy(val).to(d.device)
+    d[torch.from_numpy(idx).to(d.device)] -= ref
+    return float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))
+
+
 def c5_bench(pkg, scenes, n_frames: int, tol: float = 1e-3, seed: int = 910001):
     """A stream of holograms at the 1024^2 x 128 workload's optics with 300-800 tracers and
     rods each (independent scenes, not a sequence): plain cold start against the learned
@@ -605,6 +625,7 @@ def main():
     vol = torch.empty((Nz, H, W), dtype=torch.complex64, device="cuda")
     rec.getVolume(out=vol)
     f_line = float((vol != 0).reshape(Nz, H, W // 16, 16).any(-1).float().mean().item())
+    err64 = fp64_error(vol, holo) if args.workload == "c2" else None
     del vol
     schedule = rec.schedule()
     rec.destroy()
@@ -692,7 +713,10 @@ def main():
         "gpu_ms_per_iteration_after": gpu_ms_after / n_after,
         "solver": {"iterations_per_solve": n_it, "solves_total": args.warmup + args.steps + 1,
                    "backtracks_timed": bt_timed, "restarts_timed": rs_timed,
-                   "stepsize": st["stepsize"], "f_final": fin["f_final"], "sparsity": fin["sparsity"]},
+                   "stepsize": st["stepsize"], "f_final": fin["f_final"], "sparsity": fin["sparsity"],
+                   "error_vs_fp64_oracle": err64,
+                   "error_vs_fp64_oracle_source": "relative L2 of the returned volume against the FP64 oracle's "
+                                                  "solution of the same hologram (tests/golden/c2_oracle_f64.npz)"},
     }
     if args.frames > 0:
         out["frames"] = frames_leg(pkg, scenes, cfg, args.frames)
