@@ -158,13 +158,11 @@ hipError_t holo_momentum(const float2* cur, const float2* prev, double s_mom, co
 hipError_t fill_u64(uint64_t* p, long n, uint64_t v, hipStream_t s);
 // partial[b] (b < nblocks) = max |v_i| over block b's share of n values
 hipError_t maxabs_partials(const float2* v, long n, int nblocks, double* partial, hipStream_t s);
-// the same at two thresholds in one pass, out_k = in * max(0, 1 - tau_k / |in|), each with its
-// line-occupancy map ([nz][H] row words, written; then nz plane words, cleared by the caller)
-// set for its non-zero lines
+// the complex soft threshold at two thresholds in one pass, out_k = in * max(0, 1 - tau_k / |in|)
+// (+0 where |in| <= tau_k), each with its line-occupancy map ([nz][H] row words, written; then
+// nz plane words, cleared by the caller) set for its non-zero lines
 hipError_t soft_threshold_map2(const float2* in, int W, int H, int nz, float tau0, float2* out0, uint64_t* occ0,
                                float tau1, float2* out1, uint64_t* occ1, hipStream_t s);
-// out = in * max(0, 1 - tau / |in|) (+0 where |in| <= tau); in and out may alias
-hipError_t soft_threshold(const float2* in, float2* out, long n, float tau, hipStream_t s);
 // partial[b] (b < nblocks) = part of sum |v_i|^2 over n values
 hipError_t norm2_partials(const float2* v, long n, int nblocks, double* partial, hipStream_t s);
 // a sparse pseudo-random volume (about 1 / period of its voxels non-zero)

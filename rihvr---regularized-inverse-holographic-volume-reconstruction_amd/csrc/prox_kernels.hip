@@ -1185,22 +1185,9 @@ hipError_t maxabs_partials(const float2* v, long n, int nblocks, double* partial
     return hipGetLastError();
 }
 
-// complex soft threshold: out = in * max(0, 1 - tau / |in|), +0 where |in| <= tau
-__global__ void __launch_bounds__(kEwThreads) k_soft(const float2* __restrict__ in, float2* __restrict__ out, long n,
-                                                    float tau) {
-    const long i = (long)blockIdx.x * kEwThreads + threadIdx.x;
-    if (i >= n) return;
-    const float2 e = in[i];
-    const float m = sqrtf(e.x * e.x + e.y * e.y);
-    float2 r = make_float2(0.0f, 0.0f);
-    if (m > tau) {
-        const float sc = (m - tau) / m;
-        r = make_float2(e.x * sc, e.y * sc);
-    }
-    out[i] = r;
-}
-// k_soft into two outputs at once (thresholds tau0, tau1: one read of the input), each with its
-// line-occupancy map: a wave takes whole rows (as k_volume_terms), ORs its lanes' line bits over
+// The complex soft threshold out = in * max(0, 1 - tau / |in|) (+0 where |in| <= tau) into two
+// outputs at once (thresholds tau0, tau1: one read of the input), each with its line-occupancy
+// map: a wave takes whole rows (as k_volume_terms), ORs its lanes' line bits over
 // the row, stores the row word (the row is the wave's alone) and ORs it into the plane word
 // after the Nz H row words (cleared beforehand).
 __global__ void __launch_bounds__(64 * kVtWaves)
@@ -1250,12 +1237,6 @@ hipError_t soft_threshold_map2(const float2* in, int W, int H, int nz, float tau
     while (((W - 1) >> lsh) >= 64) ++lsh;
     hipLaunchKernelGGL(k_soft_map2, dim3(momentum_out_num_blocks(W, H, nz)), dim3(64 * kVtWaves), 0, s, in, W, H, nz,
                        vt_rows_per_wave(W), lsh, tau0, out0, occ0, tau1, out1, occ1);
-    return hipGetLastError();
-}
-
-hipError_t soft_threshold(const float2* in, float2* out, long n, float tau, hipStream_t s) {
-    hipLaunchKernelGGL(k_soft, dim3((unsigned)((n + kEwThreads - 1) / kEwThreads)), dim3(kEwThreads), 0, s, in, out,
-                       n, tau);
     return hipGetLastError();
 }
 
