@@ -115,40 +115,61 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
         for (int n = 0; n < NTMAX; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     const int nchunks = Cin >> 4;
     const unsigned short* hs = (const unsigned short*)halo;
+    // The halo and weight slots this thread stages are the same in every stage: their source
+    // offsets (halo: within a plane and chunk of in0, or of in1 at half resolution, -1 outside
+    // the grid, -2 no slot; weights: within a dz slab and chunk, -1 no slot) are computed once,
+    // and a stage adds its base.  The weight slab is stored [tap][co][4 groups of 4 ci] with co
+    // stride CB, so slot e of the slab is wl[e].  (Offsets are 32-bit: conv_forward checks that
+    // they fit.)
+    constexpr int kPer = (HALO * 2 + kFwdThreads - 1) / kFwdThreads;
+    constexpr int kPerW = (9 * CBM * 4 + kFwdThreads - 1) / kFwdThreads;
+    int hoff0[kPer], hoff1[kPer], woff[kPerW];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int e = threadIdx.x + k * kFwdThreads;
+        hoff0[k] = hoff1[k] = -2;
+        if (e < HALO * 2) {
+            const int half = e & 1, vox = e >> 1;
+            const int rx = vox % kHX, ry = vox / kHX;
+            const int yy = y0 + ry - 1, xx = x0 + rx - 1;
+            const bool inb = yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+            hoff0[k] = inb ? (yy * g.W + xx) * g.C0 + half * 8 : -1;
+            hoff1[k] = inb ? ((yy >> 1) * W2 + (xx >> 1)) * g.C1 + half * 8 : -1;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kPerW; ++k) {
+        const int e = threadIdx.x + k * kFwdThreads;
+        woff[k] = -1;
+        if (e < 9 * g.CB * 4) {
+            const int qq = e & 3, co = (e >> 2) % g.CB, tt = (e >> 2) / g.CB;
+            woff[k] = (tt * g.Cout + co0 + co) * Cin + qq * 4;
+        }
+    }
     for (int c = 0; c < nchunks; ++c) {
         const bool hi = c * 16 >= g.C0;
         for (int dz = -1; dz <= 1; ++dz) {
             const int zz = z + dz;
             if (zz < 0 || zz >= g.D) continue;  // a plane outside the grid adds zeros (uniform)
             __syncthreads();                    // the previous stage's LDS reads are done
-            constexpr int kPer = (HALO * 2 + kFwdThreads - 1) / kFwdThreads;
+            const unsigned short* base = hi ? in1 + (size_t)(zz >> 1) * H2 * W2 * g.C1 + (c * 16 - g.C0)
+                                            : in0 + (size_t)zz * g.H * g.W * g.C0 + c * 16;
             uint4 v[kPer];
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
-                const int e = threadIdx.x + k * kFwdThreads;
-                const int half = e & 1, vox = e >> 1;
-                const int rx = vox % kHX, ry = vox / kHX;
-                const int yy = y0 + ry - 1, xx = x0 + rx - 1;
+                const int o = hi ? hoff1[k] : hoff0[k];
                 v[k] = make_uint4(0, 0, 0, 0);
-                if (e < HALO * 2 && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W) {
-                    if (!hi)
-                        v[k] = *(const uint4*)(in0 + (((size_t)zz * g.H + yy) * g.W + xx) * g.C0 + c * 16 + half * 8);
-                    else
-                        v[k] = *(const uint4*)(in1 + (((size_t)(zz >> 1) * H2 + (yy >> 1)) * W2 + (xx >> 1)) * g.C1 +
-                                               (c * 16 - g.C0) + half * 8);
-                }
+                if (o >= 0) v[k] = *(const uint4*)(base + o);
             }
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int e = threadIdx.x + k * kFwdThreads;
                 if (e < HALO * 2) halo[e] = v[k];
             }
-            const int t0 = (dz + 1) * 9;
-            for (int e = threadIdx.x; e < 9 * g.CB * 4; e += kFwdThreads) {
-                const int qq = e & 3, co = (e >> 2) % g.CB, tt = (e >> 2) / g.CB;
-                wl[(tt * CBM + co) * 4 + qq] =
-                    *(const s4*)(w + ((size_t)(t0 + tt) * g.Cout + co0 + co) * Cin + c * 16 + qq * 4);
-            }
+            const unsigned short* wbase = w + (size_t)((dz + 1) * 9) * g.Cout * Cin + c * 16;
+#pragma unroll
+            for (int k = 0; k < kPerW; ++k)
+                if (woff[k] >= 0) wl[threadIdx.x + k * kFwdThreads] = *(const s4*)(wbase + woff[k]);
             __syncthreads();
 #pragma unroll
             for (int tt = 0; tt < 9; ++tt) {
@@ -162,7 +183,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_fwd(const unsigned short* 
 #pragma unroll
                 for (int n = 0; n < NTMAX; ++n) {
                     if (n < NT) {
-                        const s4 b = wl[(tt * CBM + n * 16 + r16) * 4 + q];
+                        const s4 b = wl[(tt * g.CB + n * 16 + r16) * 4 + q];
 #pragma unroll
                         for (int m = 0; m < MT; ++m)
                             acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n], 0, 0, 0);
@@ -281,8 +302,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
     // -2: no slot) are computed once and a stage adds its plane's and chunk's base (the
     // prediction at 1024^2 x 128 22.7 -> 20.9 ms; the kernels without PAR, which also take the
     // half-resolution channels at full-resolution slots, keep the per-stage arithmetic: the
-    // registers would spill there).  Offsets are 32-bit: conv_forward runs this kernel when a
-    // plane's elements fit.
+    // registers would spill there).  Offsets are 32-bit (conv_forward checks that they fit).
     constexpr int NHS = PAR ? (HALO * 2 + kFwdThreads - 1) / kFwdThreads : 1;
     constexpr int NHS2 = PAR ? (HALO2 * 2 + kFwdThreads - 1) / kFwdThreads : 1;
     int hoff0[NHS], hoff2[NHS2];
@@ -1036,14 +1056,15 @@ hipError_t conv_forward(const unsigned short* in0, int C0, const unsigned short*
     if (D > 65535) return hipErrorInvalidValue;
     // the parity-class form (wpar, inference of decoder layers of up to 32 channels) runs 16
     // output channels per workgroup, two rows per wave
-    // (the double-buffered kernel addresses a plane's elements with 32-bit offsets)
-    const bool dma_ok = (long)H * W * std::max(std::max(C0, C1), 1) < (1L << 31);
-    const bool par = dma_ok && wpar && C1 > 0 && !out_f32 && Cout <= 32;
+    // the kernels address a plane's elements (and a weight slab's) with 32-bit offsets
+    if ((long)H * W * std::max(C0, C1) >= (1L << 31) || 27L * Cout * (C0 + C1) >= (1L << 31))
+        return hipErrorInvalidValue;
+    const bool par = wpar && C1 > 0 && !out_f32 && Cout <= 32;
     ConvGeo g{D, H, W, C0, C1, Cout, par ? 16 : pick_cb(Cout)};
     const long plane = (long)H * W;
     if (plane > (1L << 30)) return hipErrorInvalidValue;
     const long gx = (long)((W + 63) / 64) * (Cout / g.CB);
-    const int rows = (g.CB == 16 && dma_ok) ? 2 : 1;  // rows per wave
+    const int rows = g.CB == 16 ? 2 : 1;  // rows per wave
     const int ty = 4 * rows;
     if (gx > 0x7fffffffL || (H + ty - 1) / ty > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)gx, (unsigned)((H + ty - 1) / ty), (unsigned)D);
