@@ -42,6 +42,7 @@ constexpr int kNumConv = 7;  // e0 e1 e2 e3 d2 d1 d0
 constexpr int kTaps = 27;
 constexpr int kFwdThreads = 256;   // 4 waves, one output row of 64 voxels each
 constexpr int kMaxCB = 64;         // output channels per workgroup
+constexpr int kFirstZ = 8;         // planes per workgroup of the input layer
 constexpr int kEw = 256;           // elementwise block size
 constexpr float kB1 = 0.9f, kB2 = 0.999f, kEps = 1e-8f;
 
@@ -704,85 +705,102 @@ __global__ void k_input16(const float2* __restrict__ in, long V, float scale, un
 // The input layer e0 = relu(conv(x) + b) straight from the complex64 volume: its two real input
 // channels (in_scale Re, in_scale Im in BF16) make K = 27 taps x 2 channels, packed as K = 64
 // (taps 27..31 zero) into 4 MFMA K-steps: step s, lane group q holds taps 8s + 2q and 8s + 2q + 1.
-// A workgroup computes 8 rows x 64 columns of one plane (wave w: rows 2w, 2w + 1, 4 M-tiles each)
-// from the 3 x 10 x 66 halo of (Re, Im) pairs staged in LDS.  wp: [4][16 co][16 k] BF16 with
-// k = 2 (tap - 8s) + channel.  Grid: (ceil(W/64), ceil(H/8), D).
+// A workgroup computes 8 rows x 64 columns of kFirstZ consecutive planes (wave w: rows 2w, 2w + 1,
+// 4 M-tiles each) from the 3 x 10 x 66 halo of (Re, Im) pairs staged in LDS (at one plane per
+// workgroup, each loaded its three halo planes for 32 MFMAs: 2.6 ms at 1024^2 x 128, now 2.2; held
+// to 6 waves per SIMD instead of 4 it spills).  wp:
+// [4][16 co][16 k] BF16 with k = 2 (tap - 8s) + channel.  Grid: (ceil(W/64), ceil(H/8),
+// ceil(D/kFirstZ)).
 __global__ void __launch_bounds__(kFwdThreads) k_conv_first(const float2* __restrict__ in, int D, int H, int W,
                                                             float scale, const unsigned short* __restrict__ wp,
                                                             const float* __restrict__ bias,
                                                             unsigned short* __restrict__ out) {
-    constexpr int HY = 10, HX = 66, HROWS = 3 * HY;
-    __shared__ unsigned halo[HROWS * HX];   // (Re, Im) BF16 pair per voxel, [dz][row][x]
+    // A workgroup walks kFirstZ planes of its 8 x 64 tile: a ring of three halo planes in LDS
+    // (plane p in slot p mod 3), one new plane loaded per step.
+    constexpr int HY = 10, HX = 66;
+    __shared__ unsigned halo[3 * HY * HX];   // (Re, Im) BF16 pair per voxel, [slot][row][x]
     __shared__ uint4 otile[8 * 64 * 2];      // the tile's outputs, [row][x][16 ch] BF16
     __shared__ s4 wls[4 * 16 * 4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = lane >> 4, r16 = lane & 15;
-    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 8, z = blockIdx.z;
-    // a wave per halo row: 64 + 2 voxels, one (two for lanes 0, 1) coalesced 8-byte load per lane
-    for (int r = wave; r < HROWS; r += 4) {
-        const int dzi = r / HY, ry = r - dzi * HY;
-        const int zz = z + dzi - 1, yy = y0 + ry - 1;
-        const bool row_in = zz >= 0 && zz < D && yy >= 0 && yy < H;
-        const float2* src = in + ((size_t)(row_in ? zz : 0) * H + (row_in ? yy : 0)) * W;
-        for (int rx = lane; rx < HX; rx += 64) {
-            const int xx = x0 + rx - 1;
-            unsigned v = 0;
-            if (row_in && xx >= 0 && xx < W) {
-                const float2 a = src[xx];
-                v = (unsigned)f2bf(a.x * scale) | ((unsigned)f2bf(a.y * scale) << 16);
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 8;
+    const int z0 = blockIdx.z * kFirstZ, z1 = min(D, z0 + kFirstZ);
+    // plane zz's 10 halo rows into its ring slot: a wave per row, 64 + 2 voxels, one (two for
+    // lanes 0, 1) coalesced 8-byte load per lane; zeros outside the grid
+    auto load_plane = [&](int zz) {
+        const int slot = (zz + 3) % 3;
+        for (int ry = wave; ry < HY; ry += 4) {
+            const int yy = y0 + ry - 1;
+            const bool row_in = zz >= 0 && zz < D && yy >= 0 && yy < H;
+            const float2* src = in + ((size_t)(row_in ? zz : 0) * H + (row_in ? yy : 0)) * W;
+            for (int rx = lane; rx < HX; rx += 64) {
+                const int xx = x0 + rx - 1;
+                unsigned v = 0;
+                if (row_in && xx >= 0 && xx < W) {
+                    const float2 a = src[xx];
+                    v = (unsigned)f2bf(a.x * scale) | ((unsigned)f2bf(a.y * scale) << 16);
+                }
+                halo[(slot * HY + ry) * HX + rx] = v;
             }
-            halo[r * HX + rx] = v;
         }
-    }
+    };
     for (int e = threadIdx.x; e < 4 * 16 * 4; e += kFwdThreads) wls[e] = *(const s4*)(wp + e * 4);
-    __syncthreads();
-    f4 acc[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int st = 0; st < 4; ++st) {
-        const s4 b = wls[(st * 16 + r16) * 4 + q];
-        int off[2];
-        bool ok[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int t = 8 * st + 2 * q + h;
-            ok[h] = t < 27;
-            const int tc = ok[h] ? t : 0;
-            off[h] = ((tc / 9) * HY + (tc / 3) % 3) * HX + (tc % 3);  // halo offset of the tap at (row 0, x 0)
-        }
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const int base = (wave * 2 + (m >> 2)) * HX + (m & 3) * 16 + r16;
-            const unsigned lo = ok[0] ? halo[base + off[0]] : 0u;
-            const unsigned hi = ok[1] ? halo[base + off[1]] : 0u;
-            s4 a;
-            a[0] = (short)(lo & 0xffffu);
-            a[1] = (short)(lo >> 16);
-            a[2] = (short)(hi & 0xffffu);
-            a[3] = (short)(hi >> 16);
-            acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[m], 0, 0, 0);
-        }
-    }
-    // through LDS, so that the tile's rows (64 voxels x 32 bytes each) leave as 16-byte stores
+    load_plane(z0 - 1);
+    load_plane(z0);
     const float bv = bias[r16];
     unsigned short* ot = (unsigned short*)otile;
+    for (int z = z0; z < z1; ++z) {
+        // the slot of plane z + 1 held plane z - 2, read by the last step (which ended at a
+        // barrier); otile's last reads (the last step's stores) precede this step's barrier
+        load_plane(z + 1);
+        __syncthreads();
+        f4 acc[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        const int row = wave * 2 + (m >> 2);
+        for (int m = 0; m < 8; ++m) acc[m] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = (m & 3) * 16 + q * 4 + i;
-            ot[(row * 64 + x) * 16 + r16] = f2bf(fmaxf(acc[m][i] + bv, 0.0f));
+        for (int st = 0; st < 4; ++st) {
+            const s4 b = wls[(st * 16 + r16) * 4 + q];
+            int off[2];
+            bool ok[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int t = 8 * st + 2 * q + h;
+                ok[h] = t < 27;
+                const int tc = ok[h] ? t : 0;
+                // the tap's plane z + tc / 9 - 1 in its ring slot, at (row 0, x 0)
+                off[h] = (((z + tc / 9 + 2) % 3) * HY + (tc / 3) % 3) * HX + (tc % 3);
+            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int base = (wave * 2 + (m >> 2)) * HX + (m & 3) * 16 + r16;
+                const unsigned lo = ok[0] ? halo[base + off[0]] : 0u;
+                const unsigned hi = ok[1] ? halo[base + off[1]] : 0u;
+                s4 a;
+                a[0] = (short)(lo & 0xffffu);
+                a[1] = (short)(lo >> 16);
+                a[2] = (short)(hi & 0xffffu);
+                a[3] = (short)(hi >> 16);
+                acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[m], 0, 0, 0);
+            }
         }
-    }
-    __syncthreads();
+        // through LDS, so that the tile's rows (64 voxels x 32 bytes each) leave as 16-byte stores
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int e = threadIdx.x + k * kFwdThreads;  // uint4 of the tile
-        const int vox = e >> 1, row = vox >> 6, x = vox & 63;
-        const int y = y0 + row, xx = x0 + x;
-        if (y < H && xx < W) *(uint4*)(out + (((size_t)z * H + y) * W + xx) * 16 + (e & 1) * 8) = otile[e];
+        for (int m = 0; m < 8; ++m) {
+            const int row = wave * 2 + (m >> 2);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = (m & 3) * 16 + q * 4 + i;
+                ot[(row * 64 + x) * 16 + r16] = f2bf(fmaxf(acc[m][i] + bv, 0.0f));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = threadIdx.x + k * kFwdThreads;  // uint4 of the tile
+            const int vox = e >> 1, row = vox >> 6, x = vox & 63;
+            const int y = y0 + row, xx = x0 + x;
+            if (y < H && xx < W) *(uint4*)(out + (((size_t)z * H + y) * W + xx) * 16 + (e & 1) * 8) = otile[e];
+        }
     }
 }
 
@@ -1300,7 +1318,8 @@ rihvr_status forward_body(rihvr_net* n, const float* in, bool with_x16, float* p
     if (with_x16)
         hipLaunchKernelGGL(k_input16, dim3(blocks_of(n->Vk[0])), dim3(kEw), 0, n->stream, (const float2*)in,
                            n->Vk[0], n->in_scale, n->x16);
-    hipLaunchKernelGGL(k_conv_first, dim3((unsigned)((n->W + 63) / 64), (unsigned)((n->H + 7) / 8), (unsigned)n->D),
+    hipLaunchKernelGGL(k_conv_first, dim3((unsigned)((n->W + 63) / 64), (unsigned)((n->H + 7) / 8),
+                                          (unsigned)((n->D + kFirstZ - 1) / kFirstZ)),
                        dim3(kFwdThreads), 0, n->stream, (const float2*)in, n->D, n->H, n->W, n->in_scale, n->w0p,
                        n->params + n->L[0].b_off, n->e[0]);
     NET_HIP(hipGetLastError());
