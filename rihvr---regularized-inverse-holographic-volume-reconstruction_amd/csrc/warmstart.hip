@@ -714,7 +714,8 @@ __global__ void k_input16(const float2* __restrict__ in, long V, float scale, un
 __global__ void __launch_bounds__(kFwdThreads) k_conv_first(const float2* __restrict__ in, int D, int H, int W,
                                                             float scale, const unsigned short* __restrict__ wp,
                                                             const float* __restrict__ bias,
-                                                            unsigned short* __restrict__ out) {
+                                                            unsigned short* __restrict__ out,
+                                                            unsigned short* __restrict__ pooled) {
     // A workgroup walks kFirstZ planes of its 8 x 64 tile: a ring of three halo planes in LDS
     // (plane p in slot p mod 3), one new plane loaded per step.
     constexpr int HY = 10, HX = 66;
@@ -749,6 +750,12 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_first(const float2* __rest
     load_plane(z0);
     const float bv = bias[r16];
     unsigned short* ot = (unsigned short*)otile;
+    // the 2 x 2 x 2 average pool of the outputs (the next level's input), formed as k_pool does:
+    // a thread holds both rows of a y pair (M-tiles m and m + 4), both voxels of an x pair (items
+    // 2j, 2j + 1) and sees the z pair in consecutive steps; FP32 sums of the BF16 outputs in
+    // k_pool's order (z, then y, then x), the same bits
+    float psum[4][2];
+    const int H2 = H >> 1, W2 = W >> 1;
     for (int z = z0; z < z1; ++z) {
         // the slot of plane z + 1 held plane z - 2, read by the last step (which ended at a
         // barrier); otile's last reads (the last step's stores) precede this step's barrier
@@ -784,13 +791,42 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_first(const float2* __rest
             }
         }
         // through LDS, so that the tile's rows (64 voxels x 32 bytes each) leave as 16-byte stores
+        unsigned short vb[8][4];
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             const int row = wave * 2 + (m >> 2);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int x = (m & 3) * 16 + q * 4 + i;
-                ot[(row * 64 + x) * 16 + r16] = f2bf(fmaxf(acc[m][i] + bv, 0.0f));
+                vb[m][i] = f2bf(fmaxf(acc[m][i] + bv, 0.0f));
+                ot[(row * 64 + x) * 16 + r16] = vb[m][i];
+            }
+        }
+        if (pooled) {
+            if (((z - z0) & 1) == 0) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) psum[t][j] = 0.0f;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int d = 0; d < 2; ++d) psum[t][j] += bf2f(vb[t + 4 * b][2 * j + d]);
+            if (((z - z0) & 1) == 1) {
+                const int uy = (y0 >> 1) + wave;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        const int ux = (x0 >> 1) + t * 8 + q * 2 + j;
+                        if (uy < H2 && ux < W2)
+                            pooled[(((size_t)(z >> 1) * H2 + uy) * W2 + ux) * 16 + r16] = f2bf(psum[t][j] * 0.125f);
+                    }
             }
         }
         __syncthreads();
@@ -1321,11 +1357,11 @@ rihvr_status forward_body(rihvr_net* n, const float* in, bool with_x16, float* p
     hipLaunchKernelGGL(k_conv_first, dim3((unsigned)((n->W + 63) / 64), (unsigned)((n->H + 7) / 8),
                                           (unsigned)((n->D + kFirstZ - 1) / kFirstZ)),
                        dim3(kFwdThreads), 0, n->stream, (const float2*)in, n->D, n->H, n->W, n->in_scale, n->w0p,
-                       n->params + n->L[0].b_off, n->e[0]);
+                       n->params + n->L[0].b_off, n->e[0], n->p[1]);  // (its pool fused: p[1])
     NET_HIP(hipGetLastError());
     rihvr_status st;
     for (int k = 1; k < kLevels; ++k) {
-        if ((st = pool(n, k - 1, n->e[k - 1], n->p[k]))) return st;
+        if (k > 1 && (st = pool(n, k - 1, n->e[k - 1], n->p[k]))) return st;
         if ((st = layer_forward(n, k, n->p[k], nullptr, n->e[k]))) return st;
     }
     const bool parity = !with_x16;  // prediction: the decoder's upsampled channels in parity-class form
