@@ -253,7 +253,10 @@ __device__ __forceinline__ void dma16(const void* src, uint4* lds_wave_base) {
 }
 
 template <bool OUTF32, int ROWS, bool PAR, bool HEAD = false>
-__global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* __restrict__ in0,
+// (two waves per SIMD, which the two LDS buffers allow: without the hint the parity form's
+// split-half addressing took 256 VGPRs and one wave)
+__global__ void __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_conv_dma(const unsigned short* __restrict__ in0,
                                                           const unsigned short* __restrict__ in1, ConvGeo g,
                                                           const unsigned short* __restrict__ w,
                                                           const unsigned short* __restrict__ wpar,
@@ -313,13 +316,16 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
             const int e = wave * 64 + i * kFwdThreads + lane;
             hoff0[i] = -2;
             if (e < HALO * 2) {
-                const int vox = e >> 1;
+                // slot e: channel half e / HALO of voxel e % HALO (the halves in two arrays, so
+                // that 16 lanes reading one 8-byte chunk of 16 voxels meet 2-way bank conflicts,
+                // not the 4-way of 32-byte voxel records)
+                const int vox = e % HALO, half = e / HALO;
                 const int ry = vox / kHX, col = vox - ry * kHX;
                 // the columns x-parity-split (even ones, then odd ones)
                 const int rx = col < kHX / 2 ? 2 * col : 2 * (col - kHX / 2) + 1;
                 const int yy = y0 + ry - 1, xx = x0 + rx - 1;
                 const bool inb = yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
-                hoff0[i] = inb ? (yy * g.W + xx) * g.C0 + (e & 1) * 8 : -1;
+                hoff0[i] = inb ? (yy * g.W + xx) * g.C0 + half * 8 : -1;
             }
         }
 #pragma unroll
@@ -327,9 +333,9 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
             const int e = wave * 64 + i * kFwdThreads + lane;
             hoff2[i] = -2;
             if (e < HALO2 * 2) {
-                const int vox = e >> 1, rx = vox % kHX2, ry = vox / kHX2;
+                const int vox = e % HALO2, half = e / HALO2, rx = vox % kHX2, ry = vox / kHX2;
                 const int yy = (y0 >> 1) - 1 + ry, xx = (x0 >> 1) - 1 + rx;
-                hoff2[i] = (yy >= 0 && yy < H2 && xx >= 0 && xx < W2) ? (yy * W2 + xx) * g.C1 + (e & 1) * 8 : -1;
+                hoff2[i] = (yy >= 0 && yy < H2 && xx >= 0 && xx < W2) ? (yy * W2 + xx) * g.C1 + half * 8 : -1;
             }
         }
     }
@@ -348,9 +354,11 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
             for (int e0 = wave * 64; e0 < nw; e0 += kFwdThreads) {
                 const int e = e0 + lane;
                 if (e < nw) {
-                    const int sl = e / (g.CB * 2), co = (e >> 1) % g.CB;
+                    // [half][slab][co], as the halo
+                    const int half = e / (16 * g.CB), rem = e - half * 16 * g.CB;
+                    const int sl = rem / g.CB, co = rem - sl * g.CB;
                     const int cls = (pz * 2 + (sl >> 3)) * 2 + ((sl >> 2) & 1), tap = d * 4 + (sl & 3);
-                    dma16(wpar + ((size_t)(cls * 8 + tap) * g.Cout + co0 + co) * g.C1 + ci0 + (e & 1) * 8, wb + e0);
+                    dma16(wpar + ((size_t)(cls * 8 + tap) * g.Cout + co0 + co) * g.C1 + ci0 + half * 8, wb + e0);
                 }
             }
             return;
@@ -386,8 +394,18 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
         for (int e0 = wave * 64; e0 < nw; e0 += kFwdThreads) {
             const int e = e0 + lane;
             if (e < nw) {
-                const int tt = e / (g.CB * 2), co = (e >> 1) % g.CB;
-                dma16(w + ((size_t)(d * 9 + tt) * g.Cout + co0 + co) * Cin + c * 16 + (e & 1) * 8, wb + e0);
+                int tt, co, half;
+                if constexpr (PAR) {  // [half][tap][co], as the halo
+                    half = e / (9 * g.CB);
+                    const int rem = e - half * 9 * g.CB;
+                    tt = rem / g.CB;
+                    co = rem - tt * g.CB;
+                } else {
+                    tt = e / (g.CB * 2);
+                    co = (e >> 1) % g.CB;
+                    half = e & 1;
+                }
+                dma16(w + ((size_t)(d * 9 + tt) * g.Cout + co0 + co) * Cin + c * 16 + half * 8, wb + e0);
             }
         }
     };
@@ -406,7 +424,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
                     const int yr = wave * ROWS + (m >> 2);
                     const int hrow = (yr >> 1) + (yr & 1) + oy;
                     const int hcol = 16 * ((m & 3) >> 1) + r16 + (m & 1) + ox;
-                    a[m] = *(const s4*)(hs + (size_t)(hrow * kHX2 + hcol) * 16 + q * 4);
+                    a[m] = *(const s4*)(hs + ((size_t)(q >> 1) * HALO2 + hrow * kHX2 + hcol) * 8 + (q & 1) * 4);
                 }
 #pragma unroll
                 for (int n = 0; n < NTMAX; ++n) {
@@ -415,7 +433,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
                         for (int m = 0; m < MT; ++m) {
                             const int yr = wave * ROWS + (m >> 2);
                             const int sl = ((yr & 1) * 2 + (m & 1)) * 4 + oo;
-                            const s4 bb = wq[(sl * g.CB + n * 16 + r16) * 4 + q];
+                            const s4 bb = wq[((q >> 1) * 16 * g.CB + sl * g.CB + n * 16 + r16) * 2 + (q & 1)];
                             acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bb, acc[m][n], 0, 0, 0);
                         }
                     }
@@ -433,12 +451,14 @@ __global__ void __launch_bounds__(kFwdThreads) k_conv_dma(const unsigned short* 
                 const int row = (wave * ROWS + (m >> 2) + dy + 1) * kHX;
                 const int hc = xcol(m, r16) + dx + 1;
                 const int col = PAR ? (hc & 1) * (kHX / 2) + (hc >> 1) : hc;
-                a[m] = *(const s4*)(hs + (size_t)(row + col) * 16 + q * 4);
+                a[m] = PAR ? *(const s4*)(hs + ((size_t)(q >> 1) * HALO + row + col) * 8 + (q & 1) * 4)
+                           : *(const s4*)(hs + (size_t)(row + col) * 16 + q * 4);
             }
 #pragma unroll
             for (int n = 0; n < NTMAX; ++n) {
                 if (n < NT) {
-                    const s4 bb = wq[(tt * g.CB + n * 16 + r16) * 4 + q];
+                    const s4 bb = PAR ? wq[((q >> 1) * 9 * g.CB + tt * g.CB + n * 16 + r16) * 2 + (q & 1)]
+                                      : wq[(tt * g.CB + n * 16 + r16) * 4 + q];
 #pragma unroll
                     for (int m = 0; m < MT; ++m)
                         acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bb, acc[m][n], 0, 0, 0);
