@@ -385,15 +385,6 @@ bool fft_length_supported(int L) {
     }
 }
 
-int row_rows_per_wg(int L) {
-    switch (L) {
-#define X(n) case n: return RowCfg<n>::RPW;
-        RIHVR_FOR_EACH_L(X)
-#undef X
-        default: return 1;
-    }
-}
-
 template <int L, int SIGN, int MODE>
 static hipError_t launch_rows_t(const RowArgs& a, int nrows_total_or_plane, int gy, hipStream_t s) {
     dim3 grid((nrows_total_or_plane + RowCfg<L>::RPW - 1) / RowCfg<L>::RPW, gy);

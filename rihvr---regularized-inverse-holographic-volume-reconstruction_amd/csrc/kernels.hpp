@@ -15,7 +15,6 @@ namespace rihvr {
 
 // ---- FFT (fft_kernels.hip) -------------------------------------------------
 bool fft_length_supported(int L);
-int row_rows_per_wg(int L);
 
 // Plain row FFT of nrows contiguous rows of length L.  tw: W_L^m for the sign.
 hipError_t fft_rows_plain(int L, int sign, const float2* in, float2* out, int nrows,
