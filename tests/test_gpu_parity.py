@@ -30,7 +30,9 @@ def rand_vol(rng, Nz, H, W, sparsity=None):
     return v
 
 
-SHAPES = [(32, 32, 4), (64, 48, 3), (256, 256, 6), (448, 448, 2), (640, 576, 2), (1024, 1024, 2)]
+# (512, 1024, 3): 1024-point column transforms of a plane narrower than its height
+SHAPES = [(32, 32, 4), (64, 48, 3), (256, 256, 6), (448, 448, 2), (640, 576, 2), (1024, 1024, 2),
+          (1024, 1024, 5), (512, 1024, 3)]
 
 
 @pytest.mark.parametrize("W,H,Nz", SHAPES)
