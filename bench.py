@@ -215,7 +215,7 @@ def c5_bench(pkg, scenes, n_frames: int, tol: float = 1e-3, seed: int = 910001):
     s["iterations_cold_mean"] = float(np.mean(s["cold_iterations"]))
     s["iterations_learned_mean"] = float(np.mean(s["learned_iterations"]))
     s["ms_network_mean"] = float(np.mean([r["ms_network"] for r in rows]))
-    # the judge of every learned result: its objective against the plain solver's f(100)
+    # every learned result is measured by its objective against the plain solver's f(100)
     s["f_learned_over_f_plain"] = [r["f_learned"] / r["f_plain"] for r in rows]
     return {"metric": "s/frame to (1 + 1e-3) f_plain(100), learned warm start vs plain, 1024^2x128 FP32",
             "value": s["learned_s_per_frame"], "unit": "s/frame", "n_gpus": 1, "steps": n_frames, "warmup": 0,

@@ -4,7 +4,7 @@
 training never saw), each solved to the plain solver's own 100-iteration objective within
 1e-3 relative: the learned start must get there in at most half the iterations of the plain
 cold start, with at most one fallback, and every frame must reach the target.  The plain
-solver's objective and data residual judge the warm start: the target is f_plain(100)."""
+solver's objective and data residual decide on the warm start: the target is f_plain(100)."""
 import pathlib
 
 import numpy as np
