@@ -357,7 +357,9 @@ rihvr_status rihvr_read_param_file(const char* path, rihvr_file_params* out);
  * cv::imread): uncompressed baseline TIFF (8/16-bit gray, 8-bit RGB converted with
  * BGR->gray weights, strips, either byte order) or binary PGM (P5, 8/16-bit).
  * Call with pixels == NULL to get the size.  RIHVR_E_BAD_FILENAME if unreadable,
- * RIHVR_E_UNKNOWN_TYPE for an unsupported encoding. */
+ * RIHVR_E_UNKNOWN_TYPE for an unsupported encoding or a size beyond cv::imread's default
+ * limits (width or height above 2^20, more than 2^30 pixels), RIHVR_E_INVALID_FILE for a
+ * raster shorter than its header says. */
 rihvr_status rihvr_read_image(const char* path, float* pixels, int32_t* width, int32_t* height);
 
 /* The input contract of main_sparse_inverse.cu:101-124: rescale to 0..1 (/255),

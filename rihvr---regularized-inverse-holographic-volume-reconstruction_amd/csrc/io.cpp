@@ -87,6 +87,15 @@ float bgr_to_gray(uint8_t b, uint8_t g, uint8_t r) {
     return (float)((b * 1868 + g * 9617 + r * 4899 + (1 << 13)) >> 14);
 }
 
+// The image sizes cv::imread accepts by default (OpenCV's validateInputImageSize:
+// CV_IO_MAX_IMAGE_WIDTH = CV_IO_MAX_IMAGE_HEIGHT = 2^20, CV_IO_MAX_IMAGE_PIXELS = 2^30);
+// a header describing a larger image is refused, so a reported size always fits int32 and
+// its pixel count fits the caller's buffer arithmetic.
+constexpr uint64_t kMaxImageSide = 1ull << 20, kMaxImagePixels = 1ull << 30;
+bool image_size_ok(uint64_t W, uint64_t H) {
+    return W >= 1 && H >= 1 && W <= kMaxImageSide && H <= kMaxImageSide && W * H <= kMaxImagePixels;
+}
+
 rihvr_status decode_tiff(Reader& r, float* pixels, int32_t* width, int32_t* height) {
     if (!r.ok(0, 8)) return RIHVR_E_UNKNOWN_TYPE;
     r.le = r.b[0] == 'I';
@@ -125,7 +134,7 @@ rihvr_status decode_tiff(Reader& r, float* pixels, int32_t* width, int32_t* heig
             default: break;
         }
     }
-    if (W == 0 || H == 0 || comp != 1 || planar != 1 || offsets.empty()) return RIHVR_E_UNKNOWN_TYPE;
+    if (!image_size_ok(W, H) || comp != 1 || planar != 1 || offsets.empty()) return RIHVR_E_UNKNOWN_TYPE;
     if (!((spp == 1 && (bps == 8 || bps == 16)) || (spp == 3 && bps == 8))) return RIHVR_E_UNKNOWN_TYPE;
     *width = (int32_t)W;
     *height = (int32_t)H;
@@ -171,15 +180,20 @@ rihvr_status decode_pgm(const std::vector<uint8_t>& b, float* pixels, int32_t* w
             }
         }
     };
+    // a decimal field; values beyond 2^40 stop growing (every such field is refused below)
     auto num = [&]() {
         skip();
         long v = 0;
-        while (p < b.size() && b[p] >= '0' && b[p] <= '9') v = v * 10 + (b[p++] - '0');
+        while (p < b.size() && b[p] >= '0' && b[p] <= '9') {
+            const long d = b[p++] - '0';
+            if (v <= (1L << 40)) v = v * 10 + d;
+        }
         return v;
     };
     const long W = num(), H = num(), maxv = num();
     ++p;  // one whitespace byte before the raster
-    if (W <= 0 || H <= 0 || maxv <= 0 || maxv > 65535) return RIHVR_E_UNKNOWN_TYPE;
+    if (!image_size_ok((uint64_t)std::max(W, 0L), (uint64_t)std::max(H, 0L)) || maxv <= 0 || maxv > 65535)
+        return RIHVR_E_UNKNOWN_TYPE;
     *width = (int32_t)W;
     *height = (int32_t)H;
     if (!pixels) return RIHVR_OK;

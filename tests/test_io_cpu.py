@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import pathlib
+import struct
 import subprocess
 
 import numpy as np
@@ -214,6 +215,46 @@ def test_read_image_errors(pkg, tmp_path):
     with pytest.raises(pkg.RihvrError) as e:
         pkg.read_image(trunc)
     assert e.value.status == 9
+
+
+def _tiff_with_size(W, H) -> bytes:
+    """An 8x8 TIFF whose ImageWidth / ImageLength tags say W x H (LONG values)."""
+    b = bytearray(tiff_bytes(np.ones((8, 8), np.uint8)))
+    n = struct.unpack_from("<H", b, 8)[0]
+    for i in range(n):
+        e = 10 + 12 * i
+        tag = struct.unpack_from("<H", b, e)[0]
+        if tag in (256, 257):
+            struct.pack_into("<HII", b, e + 2, 4, 1, W if tag == 256 else H)
+    return bytes(b)
+
+
+def _size_of(pkg, path):
+    w, h = ctypes.c_int32(0), ctypes.c_int32(0)
+    rc = pkg.lib().rihvr_read_image(str(path).encode(), None, ctypes.byref(w), ctypes.byref(h))
+    return rc, w.value, h.value
+
+
+def test_read_image_size_limits(pkg, tmp_path):
+    """Sizes beyond cv::imread's default limits (side 2^20, 2^30 pixels) are refused as an
+    unsupported image (status 1), so a reported size is always positive and fits int32: headers
+    of 2^31 (negative as int32), 2^20 + 1 and 2^20 x 2^11 (2^31 pixels); 2^20 x 2^10 is the
+    largest accepted and its missing raster is an invalid file (9).  PGM: a 23-digit width and
+    a width of 2^20 + 1 are refused the same way."""
+    p = tmp_path / "s.tif"
+    for W, H in [(1 << 31, 8), (8, 0xFFFFFFFF), ((1 << 20) + 1, 8), (1 << 20, 1 << 11)]:
+        p.write_bytes(_tiff_with_size(W, H))
+        assert _size_of(pkg, p)[0] == 1, (W, H)
+    p.write_bytes(_tiff_with_size(1 << 20, 1 << 10))
+    assert _size_of(pkg, p) == (0, 1 << 20, 1 << 10)
+    out = np.empty(8, np.float32)  # never written: the raster check comes first
+    w, h = ctypes.c_int32(0), ctypes.c_int32(0)
+    assert pkg.lib().rihvr_read_image(str(p).encode(), ctypes.c_void_p(out.ctypes.data), ctypes.byref(w),
+                                      ctypes.byref(h)) == 9
+    g = tmp_path / "s.pgm"
+    for head in (b"P5\n99999999999999999999999 10\n255\n", b"P5 1048577 1 255\n"):
+        g.write_bytes(head + bytes(64))
+        assert _size_of(pkg, g)[0] == 1, head
 
 
 # --------------------------------------------------------------------------
