@@ -405,7 +405,9 @@ rihvr_status rihvr_prepare_holograms_noalloc(const rihvr_file_params* fp, const 
                                              int32_t n_frames, const float* background, int32_t width,
                                              int32_t height, float* out, int32_t* out_width, int32_t* out_height);
 
-/* 8-bit grayscale uncompressed TIFF writer (the reference writes with cv::imwrite). */
+/* 8-bit grayscale uncompressed TIFF writer (the reference writes with cv::imwrite).  The
+ * writers, rihvr_write_hologram and rihvr_prepare_hologram take the sizes rihvr_read_image
+ * accepts (width and height up to 2^20, at most 2^30 pixels): RIHVR_E_INVALID_ARGUMENT beyond. */
 rihvr_status rihvr_write_tiff8(const char* path, const uint8_t* pixels, int32_t width,
                                int32_t height);
 

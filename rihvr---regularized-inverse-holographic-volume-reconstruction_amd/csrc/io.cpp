@@ -213,8 +213,10 @@ uint8_t to_u8(double v) {
 // convertTo(CV_8U, 255) of a float image: the product is formed in float
 uint8_t unit_to_u8(float v) { return to_u8((double)(v * 255.0f)); }
 
+// The writers take the sizes the readers accept (image_size_ok), which also keeps every
+// byte count within the 32-bit fields of the TIFF header and of a PNG chunk.
 rihvr_status write_tiff(const char* path, const uint8_t* px, int32_t W, int32_t H) {
-    if (!path || !px || W <= 0 || H <= 0) return RIHVR_E_INVALID_ARGUMENT;
+    if (!path || !px || W <= 0 || H <= 0 || !image_size_ok((uint64_t)W, (uint64_t)H)) return RIHVR_E_INVALID_ARGUMENT;
     FILE* f = std::fopen(path, "wb");
     if (!f) return RIHVR_E_BAD_FILENAME;
     const uint32_t n = (uint32_t)W * (uint32_t)H;
@@ -246,18 +248,20 @@ rihvr_status write_tiff(const char* path, const uint8_t* px, int32_t W, int32_t 
 // (uncompressed) deflate blocks (RFC 1950/1951), IEND; CRC-32 over each chunk's type
 // and data, Adler-32 over the filtered scanlines (filter byte 0 per row).
 // ---------------------------------------------------------------------------
-uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n) {
-    static uint32_t table[256];
-    static bool init = false;
-    if (!init) {
+struct Crc32Table {
+    uint32_t t[256];
+    Crc32Table() {
         for (uint32_t i = 0; i < 256; ++i) {
             uint32_t v = i;
             for (int k = 0; k < 8; ++k) v = (v & 1) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
-            table[i] = v;
+            t[i] = v;
         }
-        init = true;
     }
-    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xff] ^ (c >> 8);
+};
+
+uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n) {
+    static const Crc32Table table;  // initialised once, also when threads write PNGs at once
+    for (size_t i = 0; i < n; ++i) c = table.t[(c ^ p[i]) & 0xff] ^ (c >> 8);
     return c;
 }
 
@@ -275,7 +279,7 @@ void png_chunk(std::vector<uint8_t>& out, const char* type, const std::vector<ui
 }
 
 rihvr_status write_png(const char* path, const uint8_t* px, int32_t W, int32_t H) {
-    if (!path || !px || W <= 0 || H <= 0) return RIHVR_E_INVALID_ARGUMENT;
+    if (!path || !px || W <= 0 || H <= 0 || !image_size_ok((uint64_t)W, (uint64_t)H)) return RIHVR_E_INVALID_ARGUMENT;
     std::vector<uint8_t> raw;
     raw.reserve((size_t)(W + 1) * H);
     for (int y = 0; y < H; ++y) {
@@ -512,6 +516,7 @@ rihvr_status rihvr_prepare_hologram(const rihvr_file_params* fp, const float* im
                                     int32_t width, int32_t height, float* out, int32_t* out_width,
                                     int32_t* out_height) {
     if (!fp || !image || width <= 0 || height <= 0 || !out_width || !out_height) return RIHVR_E_INVALID_ARGUMENT;
+    if (!image_size_ok((uint64_t)width, (uint64_t)height)) return RIHVR_E_INVALID_ARGUMENT;  // the readers' limits
     if (fp->remove_background && !background) return RIHVR_E_MISSING_DATA;
     const long N = (long)width * height;
     std::vector<float> h(N);
@@ -560,7 +565,8 @@ rihvr_status rihvr_write_png8(const char* path, const uint8_t* pixels, int32_t w
 
 rihvr_status rihvr_write_hologram(const char* path, const float* holo, int32_t width, int32_t height,
                                   int32_t scale_to_minmax, double lo, double hi) {
-    if (!holo || width <= 0 || height <= 0) return RIHVR_E_INVALID_ARGUMENT;
+    if (!holo || width <= 0 || height <= 0 || !image_size_ok((uint64_t)width, (uint64_t)height))
+        return RIHVR_E_INVALID_ARGUMENT;
     const long N = (long)width * height;
     std::vector<uint8_t> px(N);
     if (scale_to_minmax) {

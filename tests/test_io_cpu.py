@@ -257,6 +257,23 @@ def test_read_image_size_limits(pkg, tmp_path):
         assert _size_of(pkg, g)[0] == 1, head
 
 
+def test_writers_and_preparation_refuse_sizes_beyond_the_readers(pkg, tmp_path):
+    """The writers, rihvr_write_hologram and rihvr_prepare_hologram refuse sizes the readers
+    refuse (status 7, invalid argument) before touching the pixels (a 1-element buffer stands for them)."""
+    L = pkg.lib()
+    one_u8 = np.zeros(1, np.uint8)
+    one_f = np.zeros(1, np.float32)
+    p = str(tmp_path / "big.tif").encode()
+    for W, H in [((1 << 20) + 1, 1), (1 << 16, 1 << 16), (0x7FFFFFFF, 0x7FFFFFFF)]:
+        assert L.rihvr_write_tiff8(p, ctypes.c_void_p(one_u8.ctypes.data), W, H) == 7
+        assert L.rihvr_write_png8(p, ctypes.c_void_p(one_u8.ctypes.data), W, H) == 7
+        assert L.rihvr_write_hologram(p, ctypes.c_void_p(one_f.ctypes.data), W, H, 1, 0.0, 1.0) == 7
+        ow, oh = ctypes.c_int32(0), ctypes.c_int32(0)
+        assert L.rihvr_prepare_hologram(ctypes.byref(_fp(pkg)), ctypes.c_void_p(one_f.ctypes.data), None, W, H, None,
+                                        ctypes.byref(ow), ctypes.byref(oh)) == 7
+    assert not (tmp_path / "big.tif").exists()
+
+
 # --------------------------------------------------------------------------
 # preprocessing (the executable's input contract)
 # --------------------------------------------------------------------------
