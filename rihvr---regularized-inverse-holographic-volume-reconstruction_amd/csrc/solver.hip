@@ -653,7 +653,7 @@ rihvr_status create_impl(const rihvr_params* params, const rihvr_schedule* sch, 
         if (const char* e = std::getenv("RIHVR_ZCHUNK")) zc = std::max(1, std::atoi(e));
         if (const char* e = std::getenv("RIHVR_ZCHUNK_ADJ")) za = std::max(1, std::atoi(e));
         if (const char* e = std::getenv("RIHVR_PROX_ROWS")) prow = std::max(0, std::atoi(e));
-        }
+    }
     if (prow >= 0)
         c->prox_rows = std::min(prow, c->H);
     else if (params->num_tv_iterations >= 0 && params->num_tv_iterations <= 10)
