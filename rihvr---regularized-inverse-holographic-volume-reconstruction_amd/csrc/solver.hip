@@ -640,7 +640,9 @@ rihvr_status create_impl(const rihvr_params* params, const rihvr_schedule* sch, 
     RIHVR_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
     RIHVR_HIP(hipEventCreateWithFlags(&c->ev_rhat, hipEventDisableTiming));
     RIHVR_HIP(hipEventCreateWithFlags(&c->ev_scal, hipEventDisableTiming));
-    int zc = 32, za = 16, prow = -1;
+    // defaults: A's partial spectra over 32 planes; A^T's row pass walks 8 planes per workgroup
+    // (at 1024^2 x 128: 588 against 584 iter/s with 16 over whole solves, the same bits)
+    int zc = 32, za = 8, prow = -1;
     bool force_gen = false;
     if (sch) {
         if (sch->zchunk_fwd > 0) zc = sch->zchunk_fwd;
@@ -1483,7 +1485,7 @@ rihvr_status autotune_impl(const rihvr_params* p, const float* holo, int holo_lo
     rihvr_schedule cur{};
     cur.max_radix = 16;
     cur.zchunk_fwd = std::min(32, p->num_planes);
-    cur.zchunk_adj = std::min(16, p->num_planes);
+    cur.zchunk_adj = std::min(8, p->num_planes);
     cur.graphs = 1;
     const bool fixed = fft_length_supported(p->width) && fft_length_supported(p->height);
     if (!fixed) cur.fft_path = 1;

@@ -56,7 +56,7 @@ def test_every_schedule_solves_to_the_oracle(pkg, orc, scenes, gpu_available, k,
     fixed = W == 128
     assert got["fft_path"] == (sd.get("fft_path", 0) if fixed else 1)
     assert got["graphs"] == sd.get("graphs", 1)
-    for key, default in (("zchunk_fwd", 32), ("zchunk_adj", 16)):
+    for key, default in (("zchunk_fwd", 32), ("zchunk_adj", 8)):
         assert got[key] == min(sd.get(key, default), Nz), key
     if "prox_rows" in sd:
         assert got["prox_rows"] == sd["prox_rows"]
